@@ -1,0 +1,163 @@
+/* C-ABI of the MI355X NIM time-step library (libnim_hip.so).
+ *
+ * The reference (ijarrah2/NIM-CONV22-SHARED) is a single Fortran program with
+ * no FFI: its time step is a sequence of argument-less module subroutines
+ * operating on global arrays (parameters.f90:222-292). Each entry point below
+ * replaces one of those subroutines, called at the same place in the time
+ * loop (solve_nse, nse.f90:47-75); the Fortran binding a host would add is in
+ * nim-conv22-shared_amd/fortran/nim_hip_api.f90.
+ *
+ * Conventions
+ *  - All entry points return 0 on success; nonzero values are NIM_E_* codes
+ *    below. nim_last_error() gives a message for the calling thread.
+ *  - Host arrays use the reference's layout and numbering: Fortran
+ *    column-major inner dimensions, cells numbered i_theta + n_theta*(j_r-1) +
+ *    n_theta*n_r*(k_z-1) (create_geometry.f90:449-473), faces numbered by
+ *    boundary group (create_geometry.f90:626-664). 1-based integer tables.
+ *  - The mesh is the reference's structured O-grid (create_hollow,
+ *    create_geometry.f90:321-622); connectivity is implied by the dimensions.
+ *  - One context = one case on the current HIP device and one HIP stream.
+ *    A context is used from one host thread at a time.
+ *  - Everything is FP64 (the reference compiles with -fdefault-real-8).
+ */
+#ifndef NIM_HIP_H
+#define NIM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    NIM_OK = 0,
+    NIM_E_ARG = 1,        /* invalid argument or unsupported mesh */
+    NIM_E_HIP = 2,        /* a HIP runtime call failed */
+    NIM_E_MAXITER = 3,    /* inner iteration cap reached before convergence */
+    NIM_E_NONFINITE = 4,  /* a non-finite increment was detected */
+    NIM_E_STATE = 5       /* call out of order (e.g. no geometry uploaded) */
+};
+
+/* O-grid size: n_theta (azimuthal, periodic, must be even), n_r (radial,
+ * >= 3) and n_z (axial, >= 3); NMESH_i, NMESH_j, NMESH_k of
+ * parameters.f90:87-89. */
+typedef struct {
+    int32_t n_theta, n_r, n_z;
+} nim_dims;
+
+/* Physical and solver constants, derived as parameters.f90:134-173 derives
+ * them (nu = 1/sqrt(Ra/(g_beta*th_tc*Pr)), Re = 1/nu, alpha = nu/Pr). */
+typedef struct {
+    double nu, Re, alpha, rho, g_beta, DT;
+    double w_u, w_v, w_w, w_p, w_p2, w_T; /* under-relaxation factors */
+    double v_limit;                       /* Taylor-branch switch, parameters.f90:128 */
+    double R_i, R_o;                      /* Nusselt length scales (nse.f90:2794) */
+} nim_params;
+
+/* Geometry from the host's mesh/metric setup (geometry.f90:14-32, 879-975,
+ * fill_global_arrays geometry.f90:978-1032), reference layout. */
+typedef struct {
+    const double *volume;        /* meshes_volume(NMESH) */
+    const double *xi_x;          /* meshes_xi_x(3,3,NMESH) */
+    const double *gij;           /* meshes_gij(3,3,NMESH) */
+    const double *gij_faces;     /* meshes_gij_faces(3,3,6,NMESH) */
+    const double *G1, *G2, *G3;  /* meshes_G1..3(NMESH) */
+    const double *faces_area;    /* faces_area(NFACE) */
+    const double *faces_normal;  /* faces_normal(3,NFACE) */
+    const double *faces_mid;     /* faces_mid_coordinates(3,NFACE) */
+    const int32_t *bc_points;    /* BCs_faces_points(5,NBC), 1-based face numbers */
+    const double *bc_a;          /* BCs_faces_a(20,NBC) */
+} nim_geometry;
+
+/* Solver state, reference layout (restart.f90:21-36 plus the time-centred
+ * and pressure cell averages). Any pointer may be NULL to skip that array. */
+typedef struct {
+    double *faces_u, *faces_v, *faces_w, *faces_p, *faces_T;     /* (NFACE) */
+    double *uxyz, *vxyz, *wxyz, *Txyz;                            /* (NMESH) */
+    double *uxyz_old, *vxyz_old, *wxyz_old, *Txyz_old;            /* (NMESH) */
+    double *uxyzt, *vxyzt, *wxyzt, *Txyzt;                        /* (NMESH) */
+    double *meshes_p, *meshes_dila;                               /* (NMESH) */
+} nim_state;
+
+typedef struct nim_ctx nim_ctx;
+
+/* counts implied by the dimensions: NMESH, NFACE (parameters.f90:117), NBC */
+int32_t nim_counts(const nim_dims *dims, int64_t *nmesh, int64_t *nface, int64_t *nbc);
+
+/* For each of the NFACE face slots of the device layout (radial faces, then
+ * azimuthal, then axial), the 0-based reference face number.
+ * Pure host function (no device use). */
+int32_t nim_face_map(const nim_dims *dims, int32_t *ref_face_of_slot);
+
+/* Context lifetime. Replaces the static allocation of parameters.f90:222-292. */
+int32_t nim_ctx_create(const nim_dims *dims, const nim_params *params, nim_ctx **out);
+int32_t nim_ctx_destroy(nim_ctx *ctx);
+
+/* Upload metrics and boundary tables (init, once). Also derives the
+ * wall-extrapolation weights of apply_zero_flux (bc.f90:353-387) and
+ * find_Nu2 (nse.f90:2722-2782) from faces_mid. */
+int32_t nim_upload_geometry(nim_ctx *ctx, const nim_geometry *geo);
+
+/* calculate_pressure_nim_constants (nim.f90:556-743), on the device. */
+int32_t nim_build_pressure_coeffs(nim_ctx *ctx);
+
+/* State transfer (init, restart, output; restart.f90:8-69). */
+int32_t nim_upload_state(nim_ctx *ctx, const nim_state *st);
+int32_t nim_download_state(nim_ctx *ctx, nim_state *st);
+
+/* calculate_velocity_nim_constants_formulation2 (nim.f90:160-330), once per
+ * time step. */
+int32_t nim_build_momentum_coeffs(nim_ctx *ctx);
+
+/* integrate_nse (nse.f90:117-176): outer iterations of momentum, pressure
+ * and energy sweeps until every maximum increment of u, v, w, p, T is at or
+ * below tol[] and at least min_iter passes ran. Sweeps run in a race-free
+ * 4-colour order (the reference sweeps in place in natural order). The
+ * increment of a pass is measured against the state at the end of the
+ * previous pass, as find_error does (nse.f90:260-325). Returns
+ * NIM_E_MAXITER after max_iter passes without convergence (the reference
+ * has no cap) and NIM_E_NONFINITE on NaN/Inf. */
+int32_t nim_inner_solve(nim_ctx *ctx, const double tol[5], int32_t max_iter, int32_t min_iter,
+                        int32_t *iters, double err[5]);
+
+/* check_steady_state (nse.f90:1316-1332) and evaluate_D (nse.f90:2504-2537),
+ * which integrate_nse runs after its loop. */
+int32_t nim_end_step(nim_ctx *ctx, double *dudt_max, double *dTdt_max, double *dilatation);
+
+/* update_time_step (nse.f90:1292-1313). */
+int32_t nim_advance_time(nim_ctx *ctx);
+
+/* find_Nu2 (nse.f90:2699-2798): out = {Nu_H, Nu_C, Average, Area1, Area2}
+ * exactly as the reference prints them (index 0 is the outer cylinder scaled
+ * by R_o, index 1 the hexagon scaled by R_i). */
+int32_t nim_nusselt(nim_ctx *ctx, double out[5]);
+
+/* Stage-level entry points for parity testing: run one stage of an outer
+ * iteration, and move named arrays in reference layout. Stages:
+ *   0 update_velocity_sources (nse.f90:199-209)
+ *   1 integrate_velocity_field (4-colour)
+ *   2 update_pressure_sources (nse.f90:228-235)
+ *   3 apply_pressure_bc + integrate_pressure_field (4-colour, per-cell walls)
+ *   4 calculate_temperature_nim_constants_formulation2 + apply_zero_flux
+ *   5 update_temperature_sources
+ *   6 integrate_temperature_field (4-colour)
+ * Array names: faces_u..faces_T, uxyz.., uxyz_old.., uxyzt.., meshes_p,
+ * meshes_dila, M_F1..M_F4, MT_F1..MT_F4, AA_px, AA_py, AA_pz, bc_ap,
+ * R_B_u, R_B_v, R_B_w (the sums R_C+R_D+R_B used by the sweep), R_b_p
+ * (R_D_p+R_b_p), R_D_T. */
+int32_t nim_run_stage(nim_ctx *ctx, int32_t stage);
+int32_t nim_get_array(nim_ctx *ctx, const char *name, double *out, int64_t count);
+int32_t nim_set_array(nim_ctx *ctx, const char *name, const double *in, int64_t count);
+
+/* Measurement support: the stream the context launches on (a hipStream_t),
+ * and synchronisation with it. */
+void *nim_stream(nim_ctx *ctx);
+int32_t nim_synchronize(nim_ctx *ctx);
+
+/* Message for the last failure on the calling thread. */
+const char *nim_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,635 @@
+// Host side of libnim_hip.so: the C-ABI of include/nim_hip.h, layout
+// conversion between the reference's numbering and the device layout, and
+// the launch sequence of one outer iteration.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nim_hip.h"
+#include "nim_kernels.h"
+
+namespace nim {
+__global__ void k_coef_A(Dev d, int energy);
+__global__ void k_coef_F(Dev d, int energy);
+__global__ void k_mom_src(Dev d);
+__global__ void k_ppe_src(Dev d);
+__global__ void k_T_src(Dev d);
+__global__ void k_mom_sweep(Dev d, int color);
+__global__ void k_T_sweep(Dev d, int color);
+__global__ void k_ppe_sweep(Dev d, int color);
+__global__ void k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4, int max_iter, int min_iter);
+__global__ void k_check(Dev d);
+__global__ void k_pressure_coeffs(Dev d);
+__global__ void k_wall_pressure_coeffs(Dev d);
+__global__ void k_advance(Dev d);
+__global__ void k_end_step_partial(Dev d);
+__global__ void k_end_step_final(Dev d, int nblocks, double *out);
+__global__ void k_nusselt(Dev d, double R_i, double R_o, double *out);
+}  // namespace nim
+
+using namespace nim;
+
+static thread_local std::string g_err;
+
+static int32_t fail(int32_t code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                       \
+    do {                                                                                                \
+        hipError_t e_ = (x);                                                                            \
+        if (e_ != hipSuccess)                                                                           \
+            return fail(NIM_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));                      \
+    } while (0)
+
+static constexpr int BS = 256;
+
+struct nim_ctx {
+    nim_dims dims;
+    nim_params prm;
+    int NT, NR, NZ, N, NF, NB, oT, oZ;
+    hipStream_t stream = nullptr;
+    Dev dev;
+    std::vector<int32_t> slot_ref, ref_slot;
+    bool has_geo = false, has_pcoef = false;
+    Ctl *h_ctl = nullptr;
+    double *h_out = nullptr, *d_out = nullptr;
+    std::vector<void *> allocs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int gbatch = 0;
+    int nred = 0;
+};
+
+extern "C" const char *nim_last_error(void) { return g_err.c_str(); }
+
+static bool dims_ok(const nim_dims *d)
+{
+    return d && d->n_theta >= 4 && d->n_theta % 2 == 0 && d->n_r >= 3 && d->n_z >= 3 &&
+           (int64_t)d->n_theta * d->n_r * d->n_z < (int64_t)1 << 30;
+}
+
+extern "C" int32_t nim_counts(const nim_dims *d, int64_t *nmesh, int64_t *nface, int64_t *nbc)
+{
+    if (!dims_ok(d)) return fail(NIM_E_ARG, "unsupported dimensions (n_theta even >= 4, n_r >= 3, n_z >= 3)");
+    int64_t NT = d->n_theta, NR = d->n_r, NZ = d->n_z;
+    if (nmesh) *nmesh = NT * NR * NZ;
+    if (nface) *nface = 3 * NT * NR * NZ + NT * NZ + NT * NR;  // parameters.f90:117
+    if (nbc) *nbc = 2 * NT * NR + 2 * NT * NZ;
+    return NIM_OK;
+}
+
+// reference face number (0-based) of each device face slot; the reference
+// numbers faces by boundary group (create_geometry.f90:626-664): interior
+// azimuthal faces, interior radial faces, interior axial faces, then
+// bottom, top, hexagon and cylinder walls
+extern "C" int32_t nim_face_map(const nim_dims *d, int32_t *out)
+{
+    int64_t N, NF, NBc;
+    int32_t rc = nim_counts(d, &N, &NF, &NBc);
+    if (rc) return rc;
+    const int64_t NT = d->n_theta, NR = d->n_r, NZ = d->n_z;
+    const int64_t o1 = N, o2 = o1 + NT * (NR - 1) * NZ, o3 = o2 + NT * NR * (NZ - 1), o4 = o3 + NT * NR;
+    const int64_t o5 = o4 + NT * NR, o6 = o5 + NT * NZ;
+    const int64_t oT = NT * (NR + 1) * NZ, oZ = oT + N;
+    for (int64_t k = 0; k < NZ; k++)
+        for (int64_t j = 0; j <= NR; j++)
+            for (int64_t i = 0; i < NT; i++) {
+                int64_t s = i + NT * (j + (NR + 1) * k), r;
+                if (j == 0) r = o5 + i + NT * k;
+                else if (j == NR) r = o6 + i + NT * k;
+                else r = o1 + i + NT * ((j - 1) + (NR - 1) * k);
+                out[s] = (int32_t)r;
+            }
+    for (int64_t c = 0; c < N; c++) out[oT + c] = (int32_t)c;
+    for (int64_t k = 0; k <= NZ; k++)
+        for (int64_t j = 0; j < NR; j++)
+            for (int64_t i = 0; i < NT; i++) {
+                int64_t s = oZ + i + NT * (j + NR * k), r;
+                if (k == 0) r = o3 + i + NT * j;
+                else if (k == NZ) r = o4 + i + NT * j;
+                else r = o2 + i + NT * (j + NR * (k - 1));
+                out[s] = (int32_t)r;
+            }
+    return NIM_OK;
+}
+
+template <class T>
+static int32_t dalloc(nim_ctx *c, T **p, size_t n)
+{
+    void *q = nullptr;
+    HIPCHK(hipMalloc(&q, n * sizeof(T) > 0 ? n * sizeof(T) : 8));
+    HIPCHK(hipMemset(q, 0, n * sizeof(T)));
+    c->allocs.push_back(q);
+    *p = (T *)q;
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, nim_ctx **out)
+{
+    if (!out || !prm) return fail(NIM_E_ARG, "null argument");
+    *out = nullptr;
+    int64_t N, NF, NB;
+    int32_t rc = nim_counts(dims, &N, &NF, &NB);
+    if (rc) return rc;
+    nim_ctx *c = new nim_ctx();
+    c->dims = *dims;
+    c->prm = *prm;
+    c->NT = dims->n_theta; c->NR = dims->n_r; c->NZ = dims->n_z;
+    c->N = (int)N; c->NF = (int)NF; c->NB = (int)NB;
+    c->oT = c->NT * (c->NR + 1) * c->NZ;
+    c->oZ = c->oT + c->N;
+    c->slot_ref.resize(NF);
+    nim_face_map(dims, c->slot_ref.data());
+    c->ref_slot.resize(NF);
+    for (int s = 0; s < c->NF; s++) c->ref_slot[c->slot_ref[s]] = s;
+    Dev &d = c->dev;
+    d.NT = c->NT; d.NR = c->NR; d.NZ = c->NZ; d.N = c->N; d.NF = c->NF; d.NB = c->NB; d.oT = c->oT; d.oZ = c->oZ;
+    d.nu = prm->nu; d.Rinv = 1 / prm->Re; d.alpha = prm->alpha; d.rho = prm->rho; d.rinv = 1 / prm->rho;
+    d.g_beta = prm->g_beta; d.DT = prm->DT;
+    d.w_u = prm->w_u; d.w_v = prm->w_v; d.w_w = prm->w_w; d.w_p = prm->w_p; d.w_p2 = prm->w_p2; d.w_T = prm->w_T;
+    d.v_limit = prm->v_limit;
+    c->nred = (c->N + BS - 1) / BS;
+    double *gc, *An, *wall;
+    int32_t *wpts;
+#define TRY(x)                                                                                          \
+    do {                                                                                                \
+        int32_t r_ = (x);                                                                               \
+        if (r_) { nim_ctx_destroy(c); return r_; }                                                      \
+    } while (0)
+    TRY(dalloc(c, &gc, (size_t)NGC * N));
+    TRY(dalloc(c, &An, (size_t)3 * NF));
+    TRY(dalloc(c, &d.AA, (size_t)45 * N));
+    TRY(dalloc(c, &wall, (size_t)NWS * NB));
+    TRY(dalloc(c, &wpts, (size_t)5 * NB));
+    TRY(dalloc(c, &d.sc, (size_t)NSC * N));
+    TRY(dalloc(c, &d.fc, (size_t)5 * NF));
+    TRY(dalloc(c, &d.MA, (size_t)18 * N));
+    TRY(dalloc(c, &d.TA, (size_t)18 * N));
+    TRY(dalloc(c, &d.MF, (size_t)NCF * N));
+    TRY(dalloc(c, &d.TF, (size_t)NCF * N));
+    TRY(dalloc(c, &d.src, (size_t)NSRC * N));
+    TRY(dalloc(c, &d.red, (size_t)6 * c->nred + 64));
+    TRY(dalloc(c, &d.ctl, (size_t)1));
+    TRY(dalloc(c, &d.errs, (size_t)2 * 5 * ERR_COPIES));
+    TRY(dalloc(c, &c->d_out, (size_t)16));
+    d.gc = gc; d.An = An; d.wall = wall; d.wpts = wpts;
+    if (hipHostMalloc((void **)&c->h_ctl, sizeof(Ctl), 0) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_out, 16 * sizeof(double), 0) != hipSuccess) {
+        nim_ctx_destroy(c);
+        return fail(NIM_E_HIP, "hipHostMalloc failed");
+    }
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        nim_ctx_destroy(c);
+        return fail(NIM_E_HIP, "hipStreamCreate failed");
+    }
+#undef TRY
+    *out = c;
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_ctx_destroy(nim_ctx *c)
+{
+    if (!c) return NIM_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->h_ctl) (void)hipHostFree(c->h_ctl);
+    if (c->h_out) (void)hipHostFree(c->h_out);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NIM_OK;
+}
+
+extern "C" void *nim_stream(nim_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+extern "C" int32_t nim_synchronize(nim_ctx *c)
+{
+    if (!c) return fail(NIM_E_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NIM_OK;
+}
+
+static int32_t h2d(nim_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NIM_OK;
+}
+
+static int32_t d2h(nim_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NIM_OK;
+}
+
+// wall face b -> (position, i, j, k) in the reference's boundary order
+static void wall_of(const nim_ctx *c, int b, int &pos, int &i, int &j, int &k)
+{
+    const int NT = c->NT, NR = c->NR, NZ = c->NZ;
+    if (b < NT * NR) { pos = 5; i = b % NT; j = b / NT; k = 0; }
+    else if (b < 2 * NT * NR) { pos = 6; i = (b - NT * NR) % NT; j = (b - NT * NR) / NT; k = NZ - 1; }
+    else if (b < 2 * NT * NR + NT * NZ) { pos = 1; i = (b - 2 * NT * NR) % NT; k = (b - 2 * NT * NR) / NT; j = 0; }
+    else { pos = 2; i = (b - 2 * NT * NR - NT * NZ) % NT; k = (b - 2 * NT * NR - NT * NZ) / NT; j = NR - 1; }
+}
+
+static int hslot(const nim_ctx *c, int pos, int i, int j, int k)
+{
+    const int NT = c->NT, NR = c->NR;
+    switch (pos) {
+    case 1: return i + NT * (j + (NR + 1) * k);
+    case 2: return i + NT * (j + 1 + (NR + 1) * k);
+    case 3: return c->oT + i + NT * (j + NR * k);
+    case 4: return c->oT + (i + 1) % NT + NT * (j + NR * k);
+    case 5: return c->oZ + i + NT * (j + NR * k);
+    default: return c->oZ + i + NT * (j + NR * (k + 1));
+    }
+}
+
+extern "C" int32_t nim_upload_geometry(nim_ctx *c, const nim_geometry *g)
+{
+    if (!c || !g || !g->volume || !g->xi_x || !g->gij || !g->gij_faces || !g->G1 || !g->G2 || !g->G3 ||
+        !g->faces_area || !g->faces_normal || !g->faces_mid || !g->bc_points || !g->bc_a)
+        return fail(NIM_E_ARG, "nim_upload_geometry: missing array");
+    const int N = c->N, NF = c->NF, NB = c->NB, NT = c->NT, NR = c->NR, NZ = c->NZ;
+    std::vector<double> gc((size_t)NGC * N, 0.), An((size_t)3 * NF), wall((size_t)NWS * NB, 0.);
+    std::vector<int32_t> wpts((size_t)5 * NB);
+    auto GF = [&](int n, int a, int b, int f) { return g->gij_faces[(size_t)n * 54 + (a - 1) + 3 * (b - 1) + 9 * (f - 1)]; };
+    auto GIJ = [&](int n, int a, int b) { return g->gij[(size_t)n * 9 + (a - 1) + 3 * (b - 1)]; };
+    for (int n = 0; n < N; n++) {
+        int i = n % NT, j = (n / NT) % NR, k = n / (NT * NR);
+        gc[(size_t)G_INVV * N + n] = 1. / g->volume[n];
+        for (int a = 1; a <= 3; a++)
+            for (int b = 1; b <= 3; b++)
+                gc[(size_t)(G_XI + 3 * (a - 1) + (b - 1)) * N + n] = g->xi_x[(size_t)n * 9 + (a - 1) + 3 * (b - 1)];
+        for (int a = 1; a <= 3; a++) gc[(size_t)(G_GD + a - 1) * N + n] = GIJ(n, a, a);
+        const int go[6][2] = {{1, 2}, {2, 1}, {1, 3}, {3, 1}, {2, 3}, {3, 2}};
+        for (int q = 0; q < 6; q++) gc[(size_t)(G_GO + q) * N + n] = GIJ(n, go[q][0], go[q][1]);
+        gc[(size_t)(G_G + 0) * N + n] = g->G1[n];
+        gc[(size_t)(G_G + 1) * N + n] = g->G2[n];
+        gc[(size_t)(G_G + 2) * N + n] = g->G3[n];
+        for (int p = 1; p <= 6; p++) gc[(size_t)(G_GF + p - 1) * N + n] = GF(n, (p + 1) / 2, (p + 1) / 2, p);
+        double *df = &gc[(size_t)G_DIFF * N];
+        if (j < NR - 1) {
+            int n2 = n + NT;
+            df[0 * (size_t)N + n] = GF(n2, 1, 2, 1) - GF(n, 1, 2, 2);
+            df[1 * (size_t)N + n] = GF(n2, 1, 3, 1) - GF(n, 1, 3, 2);
+        }
+        {
+            int n2 = (i + 1) % NT + NT * (j + NR * k);
+            df[2 * (size_t)N + n] = GF(n2, 2, 1, 3) - GF(n, 2, 1, 4);
+            df[3 * (size_t)N + n] = GF(n2, 2, 3, 3) - GF(n, 2, 3, 4);
+        }
+        if (k < NZ - 1) {
+            int n2 = n + NT * NR;
+            df[4 * (size_t)N + n] = GF(n2, 3, 1, 5) - GF(n, 3, 1, 6);
+            df[5 * (size_t)N + n] = GF(n2, 3, 2, 5) - GF(n, 3, 2, 6);
+        }
+    }
+    for (int s = 0; s < NF; s++) {
+        int r = c->slot_ref[s];
+        for (int q = 0; q < 3; q++) An[(size_t)q * NF + s] = g->faces_area[r] * g->faces_normal[(size_t)r * 3 + q];
+    }
+    auto mid = [&](int r, int q) { return g->faces_mid[(size_t)r * 3 + q]; };
+    auto dist = [&](int r1, int r2) {
+        double dx = mid(r1, 0) - mid(r2, 0), dy = mid(r1, 1) - mid(r2, 1), dz = mid(r1, 2) - mid(r2, 2);
+        return std::sqrt(dx * dx + dy * dy + dz * dz);
+    };
+    for (int b = 0; b < NB; b++) {
+        int pos, i, j, k;
+        wall_of(c, b, pos, i, j, k);
+        int n = i + NT * (j + NR * k);
+        int rw = c->slot_ref[hslot(c, pos, i, j, k)];
+        for (int m = 0; m < 5; m++) {
+            wall[(size_t)(W_A + m) * NB + b] = g->bc_a[(size_t)b * 20 + m];
+            int rf = g->bc_points[(size_t)b * 5 + m] - 1;
+            if (rf < 0 || rf >= NF) return fail(NIM_E_ARG, "bc_points out of range");
+            wpts[(size_t)m * NB + b] = c->ref_slot[rf];
+        }
+        for (int q = 0; q < 3; q++) wall[(size_t)(W_N + q) * NB + b] = g->faces_normal[(size_t)rw * 3 + q];
+        double gB2, gB2c, gB3;
+        if (pos <= 2) { gB2 = gB2c = GF(n, 1, 2, pos); gB3 = GF(n, 1, 3, pos); }
+        else {
+            gB2 = GF(n, 3, 1, pos);
+            gB2c = pos == 6 ? GF(n, 3, 2, pos) : gB2;  // bc.f90:318
+            gB3 = GF(n, 3, 2, pos);
+        }
+        wall[(size_t)W_GB2 * NB + b] = gB2;
+        wall[(size_t)W_GB2C * NB + b] = gB2c;
+        wall[(size_t)W_GB3 * NB + b] = gB3;
+        // one-sided 4-point weights from face mid-points (bc.f90:353-387,
+        // nse.f90:2722-2782): the wall face and the three next parallel faces
+        int r2, r3, r4;
+        if (pos == 5) { r2 = hslot(c, 6, i, j, 0); r3 = hslot(c, 6, i, j, 1); r4 = hslot(c, 6, i, j, 2); }
+        else if (pos == 6) { r2 = hslot(c, 5, i, j, k); r3 = hslot(c, 5, i, j, k - 1); r4 = hslot(c, 5, i, j, k - 2); }
+        else if (pos == 1) { r2 = hslot(c, 2, i, 0, k); r3 = hslot(c, 2, i, 1, k); r4 = hslot(c, 2, i, 2, k); }
+        else { r2 = hslot(c, 1, i, j, k); r3 = hslot(c, 1, i, j - 1, k); r4 = hslot(c, 1, i, j - 2, k); }
+        r2 = c->slot_ref[r2]; r3 = c->slot_ref[r3]; r4 = c->slot_ref[r4];
+        double h1 = dist(rw, r2), h2 = dist(r2, r3), h3 = dist(r3, r4);
+        double A = ((h1 + h2) * (h1 + h2 + h3)) / (h1 * h2 * (h2 + h3));
+        double Bw = (h1 * (h1 + h2 + h3)) / (h2 * h3 * (h1 + h2));
+        double Cw = (h1 * (h1 + h2)) / (h3 * (h2 + h3) * (h1 + h2 + h3));
+        double S = (1. / h1 + 1. / (h1 + h2) + 1. / (h1 + h2 + h3));
+        if (pos >= 5) {
+            wall[(size_t)(W_ZF + 0) * NB + b] = A;
+            wall[(size_t)(W_ZF + 1) * NB + b] = Bw;
+            wall[(size_t)(W_ZF + 2) * NB + b] = Cw;
+            wall[(size_t)(W_ZF + 3) * NB + b] = S;
+        } else {
+            wall[(size_t)(W_NU + 0) * NB + b] = S;
+            wall[(size_t)(W_NU + 1) * NB + b] = A;
+            wall[(size_t)(W_NU + 2) * NB + b] = Bw;
+            wall[(size_t)(W_NU + 3) * NB + b] = Cw;
+            wall[(size_t)(W_NU + 4) * NB + b] = g->faces_area[rw];
+        }
+    }
+    int32_t rc;
+    if ((rc = h2d(c, (void *)c->dev.gc, gc.data(), gc.size() * 8))) return rc;
+    if ((rc = h2d(c, (void *)c->dev.An, An.data(), An.size() * 8))) return rc;
+    // keep BCs_faces_ap slots if they were already built on the device
+    if (c->has_pcoef) {
+        std::vector<double> ap((size_t)8 * NB);
+        if ((rc = d2h(c, ap.data(), c->dev.wall + (size_t)W_AP * NB, ap.size() * 8))) return rc;
+        std::memcpy(&wall[(size_t)W_AP * NB], ap.data(), ap.size() * 8);
+    }
+    if ((rc = h2d(c, c->dev.wall, wall.data(), wall.size() * 8))) return rc;
+    if ((rc = h2d(c, (void *)c->dev.wpts, wpts.data(), wpts.size() * 4))) return rc;
+    c->has_geo = true;
+    return NIM_OK;
+}
+
+static inline int nblk(int n) { return (n + BS - 1) / BS; }
+
+extern "C" int32_t nim_build_pressure_coeffs(nim_ctx *c)
+{
+    if (!c) return fail(NIM_E_ARG, "null context");
+    if (!c->has_geo) return fail(NIM_E_STATE, "geometry not uploaded");
+    hipLaunchKernelGGL(k_pressure_coeffs, dim3(nblk(c->N)), dim3(BS), 0, c->stream, c->dev);
+    hipLaunchKernelGGL(k_wall_pressure_coeffs, dim3(nblk(c->NB)), dim3(BS), 0, c->stream, c->dev);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->has_pcoef = true;
+    return NIM_OK;
+}
+
+// ---- named arrays in reference layout -------------------------------
+struct ArrRef {
+    double *dev;   // first element
+    int kind;      // 0 cells (stride N), 1 faces (ref numbering), 2 walls (stride NB)
+    int mult;      // inner (Fortran) dimension
+};
+
+static bool lookup(nim_ctx *c, const char *name, ArrRef &a)
+{
+    const int N = c->N;
+    Dev &d = c->dev;
+    std::string s(name);
+    const char *ff[5] = {"faces_u", "faces_v", "faces_w", "faces_p", "faces_T"};
+    for (int q = 0; q < 5; q++)
+        if (s == ff[q]) { a = {d.fc + (size_t)q * c->NF, 1, 1}; return true; }
+    const char *cf[4] = {"u", "v", "w", "T"};
+    for (int q = 0; q < 4; q++) {
+        if (s == std::string(cf[q]) + "xyz") { a = {d.sc + (size_t)(S_CUR + q) * N, 0, 1}; return true; }
+        if (s == std::string(cf[q]) + "xyz_old") { a = {d.sc + (size_t)(S_OLD + q) * N, 0, 1}; return true; }
+        if (s == std::string(cf[q]) + "xyzt") { a = {d.sc + (size_t)(S_TC + q) * N, 0, 1}; return true; }
+    }
+    if (s == "meshes_p") { a = {d.sc + (size_t)S_P * N, 0, 1}; return true; }
+    if (s == "meshes_dila") { a = {d.sc + (size_t)S_DILA * N, 0, 1}; return true; }
+    if (s == "M_F4") { a = {d.MF + (size_t)C_F4 * N, 0, 8}; return true; }
+    if (s == "M_F1") { a = {d.MF + (size_t)C_F1 * N, 0, 5}; return true; }
+    if (s == "M_F2") { a = {d.MF + (size_t)C_F2 * N, 0, 5}; return true; }
+    if (s == "M_F3") { a = {d.MF + (size_t)C_F3 * N, 0, 5}; return true; }
+    if (s == "MT_F4") { a = {d.TF + (size_t)C_F4 * N, 0, 8}; return true; }
+    if (s == "MT_F1") { a = {d.TF + (size_t)C_F1 * N, 0, 5}; return true; }
+    if (s == "MT_F2") { a = {d.TF + (size_t)C_F2 * N, 0, 5}; return true; }
+    if (s == "MT_F3") { a = {d.TF + (size_t)C_F3 * N, 0, 5}; return true; }
+    if (s == "AA_px") { a = {d.AA, 0, 15}; return true; }
+    if (s == "AA_py") { a = {d.AA + (size_t)15 * N, 0, 15}; return true; }
+    if (s == "AA_pz") { a = {d.AA + (size_t)30 * N, 0, 15}; return true; }
+    if (s == "bc_ap") { a = {d.wall + (size_t)W_AP * c->NB, 2, 8}; return true; }
+    if (s == "R_B_u") { a = {d.src + (size_t)R_U * N, 0, 1}; return true; }
+    if (s == "R_B_v") { a = {d.src + (size_t)R_V * N, 0, 1}; return true; }
+    if (s == "R_B_w") { a = {d.src + (size_t)R_W * N, 0, 1}; return true; }
+    if (s == "R_b_p") { a = {d.src + (size_t)R_P * N, 0, 1}; return true; }
+    if (s == "R_D_T") { a = {d.src + (size_t)R_T * N, 0, 1}; return true; }
+    return false;
+}
+
+static int32_t move_array(nim_ctx *c, const char *name, double *host, int64_t count, bool to_host)
+{
+    if (!c || !name || !host) return fail(NIM_E_ARG, "null argument");
+    ArrRef a;
+    if (!lookup(c, name, a)) return fail(NIM_E_ARG, std::string("unknown array ") + name);
+    const int rows = a.kind == 1 ? c->NF : a.kind == 2 ? c->NB : c->N;
+    if (count != (int64_t)rows * a.mult) return fail(NIM_E_ARG, std::string("wrong element count for ") + name);
+    std::vector<double> buf((size_t)rows * a.mult);
+    int32_t rc;
+    if (to_host) {
+        if ((rc = d2h(c, buf.data(), a.dev, buf.size() * 8))) return rc;
+        if (a.kind == 1) {
+            for (int s = 0; s < rows; s++) host[c->slot_ref[s]] = buf[s];
+        } else {
+            for (int r = 0; r < rows; r++)
+                for (int m = 0; m < a.mult; m++) host[(size_t)r * a.mult + m] = buf[(size_t)m * rows + r];
+        }
+        return NIM_OK;
+    }
+    if (a.kind == 1) {
+        for (int s = 0; s < rows; s++) buf[s] = host[c->slot_ref[s]];
+    } else {
+        for (int r = 0; r < rows; r++)
+            for (int m = 0; m < a.mult; m++) buf[(size_t)m * rows + r] = host[(size_t)r * a.mult + m];
+    }
+    return h2d(c, a.dev, buf.data(), buf.size() * 8);
+}
+
+extern "C" int32_t nim_get_array(nim_ctx *c, const char *name, double *out, int64_t count)
+{
+    return move_array(c, name, out, count, true);
+}
+
+extern "C" int32_t nim_set_array(nim_ctx *c, const char *name, const double *in, int64_t count)
+{
+    return move_array(c, name, const_cast<double *>(in), count, false);
+}
+
+static int32_t state_io(nim_ctx *c, const nim_state *st, bool to_host)
+{
+    if (!c || !st) return fail(NIM_E_ARG, "null argument");
+    struct { const char *n; double *p; int64_t cnt; } tab[] = {
+        {"faces_u", st->faces_u, c->NF}, {"faces_v", st->faces_v, c->NF}, {"faces_w", st->faces_w, c->NF},
+        {"faces_p", st->faces_p, c->NF}, {"faces_T", st->faces_T, c->NF},
+        {"uxyz", st->uxyz, c->N}, {"vxyz", st->vxyz, c->N}, {"wxyz", st->wxyz, c->N}, {"Txyz", st->Txyz, c->N},
+        {"uxyz_old", st->uxyz_old, c->N}, {"vxyz_old", st->vxyz_old, c->N}, {"wxyz_old", st->wxyz_old, c->N},
+        {"Txyz_old", st->Txyz_old, c->N}, {"uxyzt", st->uxyzt, c->N}, {"vxyzt", st->vxyzt, c->N},
+        {"wxyzt", st->wxyzt, c->N}, {"Txyzt", st->Txyzt, c->N}, {"meshes_p", st->meshes_p, c->N},
+        {"meshes_dila", st->meshes_dila, c->N}};
+    for (auto &t : tab) {
+        if (!t.p) continue;
+        int32_t rc = move_array(c, t.n, t.p, t.cnt, to_host);
+        if (rc) return rc;
+    }
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_upload_state(nim_ctx *c, const nim_state *st) { return state_io(c, st, false); }
+extern "C" int32_t nim_download_state(nim_ctx *c, nim_state *st) { return state_io(c, st, true); }
+
+extern "C" int32_t nim_build_momentum_coeffs(nim_ctx *c)
+{
+    if (!c) return fail(NIM_E_ARG, "null context");
+    if (!c->has_geo) return fail(NIM_E_STATE, "geometry not uploaded");
+    hipLaunchKernelGGL(k_coef_A, dim3(nblk(c->N)), dim3(BS), 0, c->stream, c->dev, 0);
+    hipLaunchKernelGGL(k_coef_F, dim3(nblk(c->N)), dim3(BS), 0, c->stream, c->dev, 0);
+    HIPCHK(hipGetLastError());
+    return NIM_OK;
+}
+
+// ---- one outer iteration ---------------------------------------------
+static int color_blocks(const nim_ctx *c) { return nblk((c->NT / 2) * ((c->NR + 1) / 2) * c->NZ); }
+
+static void launch_stage(nim_ctx *c, int stage)
+{
+    const Dev &d = c->dev;
+    hipStream_t s = c->stream;
+    int gN = nblk(c->N), gC = color_blocks(c);
+    switch (stage) {
+    case 0: hipLaunchKernelGGL(k_mom_src, dim3(gN), dim3(BS), 0, s, d); break;
+    case 1: for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_mom_sweep, dim3(gC), dim3(BS), 0, s, d, q); break;
+    case 2: hipLaunchKernelGGL(k_ppe_src, dim3(gN), dim3(BS), 0, s, d); break;
+    case 3: for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_ppe_sweep, dim3(gC), dim3(BS), 0, s, d, q); break;
+    case 4:
+        hipLaunchKernelGGL(k_coef_A, dim3(gN), dim3(BS), 0, s, d, 1);
+        hipLaunchKernelGGL(k_coef_F, dim3(gN), dim3(BS), 0, s, d, 1);
+        break;
+    case 5: hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); break;
+    case 6: for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_T_sweep, dim3(gC), dim3(BS), 0, s, d, q); break;
+    }
+}
+
+static void launch_iteration(nim_ctx *c)
+{
+    for (int st = 0; st <= 6; st++) launch_stage(c, st);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, c->stream, c->dev);
+}
+
+static int32_t reset_ctl(nim_ctx *c, const double tol[5], int max_iter, int min_iter)
+{
+    hipLaunchKernelGGL(k_solve_reset, dim3(1), dim3(256), 0, c->stream, c->dev, tol[0], tol[1], tol[2], tol[3],
+                       tol[4], max_iter, min_iter);
+    HIPCHK(hipGetLastError());
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_run_stage(nim_ctx *c, int32_t stage)
+{
+    if (!c) return fail(NIM_E_ARG, "null context");
+    if (!c->has_geo || !c->has_pcoef) return fail(NIM_E_STATE, "geometry or pressure coefficients missing");
+    if (stage < 0 || stage > 6) return fail(NIM_E_ARG, "stage out of range");
+    const double tol[5] = {0, 0, 0, 0, 0};
+    int32_t rc = reset_ctl(c, tol, 1 << 30, 0);
+    if (rc) return rc;
+    launch_stage(c, stage);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return NIM_OK;
+}
+
+static int batch_size()
+{
+    const char *e = getenv("NIM_BATCH");
+    int b = e ? atoi(e) : 16;
+    return b < 1 ? 1 : b > 256 ? 256 : b;
+}
+
+static bool use_graph()
+{
+    const char *e = getenv("NIM_GRAPH");
+    return !(e && e[0] == '0');
+}
+
+static int32_t launch_batch(nim_ctx *c, int batch)
+{
+    if (!use_graph()) {
+        for (int q = 0; q < batch; q++) launch_iteration(c);
+        HIPCHK(hipGetLastError());
+        return NIM_OK;
+    }
+    if (!c->gexec || c->gbatch != batch) {
+        if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+        if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int q = 0; q < batch; q++) launch_iteration(c);
+        HIPCHK(hipStreamEndCapture(c->stream, &c->graph));
+        HIPCHK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->gbatch = batch;
+    }
+    HIPCHK(hipGraphLaunch(c->gexec, c->stream));
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_inner_solve(nim_ctx *c, const double tol[5], int32_t max_iter, int32_t min_iter,
+                                   int32_t *iters, double err[5])
+{
+    if (!c || !tol) return fail(NIM_E_ARG, "null argument");
+    if (!c->has_geo || !c->has_pcoef) return fail(NIM_E_STATE, "geometry or pressure coefficients missing");
+    if (max_iter < 1) return fail(NIM_E_ARG, "max_iter must be >= 1");
+    int32_t rc = reset_ctl(c, tol, max_iter, min_iter);
+    if (rc) return rc;
+    int batch = batch_size();
+    for (;;) {
+        if ((rc = launch_batch(c, batch))) return rc;
+        HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->h_ctl->done) break;
+    }
+    if (iters) *iters = c->h_ctl->iter;
+    if (err)
+        for (int q = 0; q < 5; q++) err[q] = c->h_ctl->err[q];
+    int st = c->h_ctl->status;
+    if (st == 3) return fail(NIM_E_MAXITER, "inner iteration cap reached");
+    if (st == 4) return fail(NIM_E_NONFINITE, "non-finite increment");
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_end_step(nim_ctx *c, double *dudt, double *dTdt, double *dil)
+{
+    if (!c) return fail(NIM_E_ARG, "null context");
+    hipLaunchKernelGGL(k_end_step_partial, dim3(c->nred), dim3(BS), 0, c->stream, c->dev);
+    hipLaunchKernelGGL(k_end_step_final, dim3(1), dim3(BS), 0, c->stream, c->dev, c->nred, c->d_out);
+    HIPCHK(hipGetLastError());
+    int32_t rc = d2h(c, c->h_out, c->d_out, 3 * sizeof(double));
+    if (rc) return rc;
+    if (dudt) *dudt = c->h_out[0];
+    if (dTdt) *dTdt = c->h_out[1];
+    if (dil) *dil = c->h_out[2];
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_advance_time(nim_ctx *c)
+{
+    if (!c) return fail(NIM_E_ARG, "null context");
+    hipLaunchKernelGGL(k_advance, dim3(nblk(c->N)), dim3(BS), 0, c->stream, c->dev);
+    HIPCHK(hipGetLastError());
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_nusselt(nim_ctx *c, double out[5])
+{
+    if (!c || !out) return fail(NIM_E_ARG, "null argument");
+    hipLaunchKernelGGL(k_nusselt, dim3(1), dim3(BS), 0, c->stream, c->dev, c->prm.R_i, c->prm.R_o, c->d_out);
+    HIPCHK(hipGetLastError());
+    int32_t rc = d2h(c, c->h_out, c->d_out, 5 * sizeof(double));
+    if (rc) return rc;
+    for (int q = 0; q < 5; q++) out[q] = c->h_out[q];
+    return NIM_OK;
+}

@@ -1,0 +1,106 @@
+// Device-side data layout and helpers of the NIM time-step kernels (gfx950).
+//
+// Layout in HBM (structure of arrays, FP64):
+//  * cells: c = i + NT*(j + NR*k), i azimuthal (periodic), j radial, k axial
+//    -- the reference's natural cell order (create_geometry.f90:449-473).
+//  * faces of one field: three families in one buffer of NF doubles
+//      radial  R(i,j,k), j in [0,NR]   at i + NT*(j + (NR+1)*k)
+//      azimuthal T(i,j,k)              at oT + i + NT*(j + NR*k)
+//      axial   Z(i,j,k), k in [0,NZ]   at oZ + i + NT*(j + NR*k)
+//    Cell-local face p (reference frame, mesh.f90:65-69): 1 = R(i,j), 2 =
+//    R(i,j+1), 3 = T(i), 4 = T(i+1 mod NT), 5 = Z(k), 6 = Z(k+1).
+//  * per-cell arrays grouped as [slot][N] blocks (one allocation per group).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nim {
+
+// per-cell geometry slots in Dev::gc
+enum {
+    G_INVV = 0,
+    G_XI = 1,     // xi_x(a,b) at G_XI + 3*(a-1) + (b-1)
+    G_GD = 10,    // g11, g22, g33
+    G_GO = 13,    // g12, g21, g13, g31, g23, g32
+    G_G = 19,     // G1, G2, G3
+    G_GF = 22,    // gij_faces(1,1,1), (1,1,2), (2,2,3), (2,2,4), (3,3,5), (3,3,6)
+    G_DIFF = 28,  // d2a, d2b, d4a, d4b, d6a, d6b (neighbour-minus-own face metric differences)
+    NGC = 34
+};
+
+// per-wall-face slots in Dev::wall (boundary faces in the reference's BC group
+// order: bottom, top, hexagon, cylinder)
+enum {
+    W_A = 0,     // BCs_faces_a(1..5)
+    W_AP = 5,    // BCs_faces_ap(1..8)
+    W_N = 13,    // face normal
+    W_GB2 = 16,  // metric factor of the first tangential derivative, one-sided branches
+    W_GB2C = 17, // same, central branch (differs on the top wall, bc.f90:318)
+    W_GB3 = 18,  // metric factor of the second tangential derivative
+    W_ZF = 19,   // zero-flux extrapolation: c2, c3, c4, den
+    W_NU = 23,   // Nusselt one-sided derivative: S, A, B, C, wall area
+    NWS = 28
+};
+
+// per-cell state slots in Dev::sc
+enum {
+    S_CUR = 0,   // u, v, w, T
+    S_OLD = 4,   // previous time level
+    S_TC = 8,    // time-centred
+    S_P = 12,
+    S_DILA = 13,
+    S_DILAO = 14,
+    NSC = 15
+};
+
+// coefficient slots in Dev::MF / Dev::TF: F4(1..8), F1(1..5), F2(1..5), F3(1..5)
+enum { C_F4 = 0, C_F1 = 8, C_F2 = 13, C_F3 = 18, NCF = 23 };
+// source slots in Dev::src: R sums for u, v, w; pressure R_D_p + R_b_p; R_D_T
+enum { R_U = 0, R_V = 1, R_W = 2, R_P = 3, R_T = 4, NSRC = 5 };
+// face field slots in Dev::fc
+enum { F_U = 0, F_V = 1, F_W = 2, F_P = 3, F_T = 4 };
+
+constexpr int ERR_COPIES = 64;  // spread of the per-iteration max-increment atomics
+
+struct Ctl {
+    int32_t done, iter, status, max_iter, min_iter, pad[3];
+    double tol[5];
+    double err[5];
+};
+
+struct Dev {
+    int NT, NR, NZ, N, NF, NB, oT, oZ;
+    double nu, Rinv, alpha, rinv, rho, g_beta, DT;
+    double w_u, w_v, w_w, w_p, w_p2, w_T, v_limit;
+    const double *gc;   // [NGC][N]
+    const double *An;   // [3][NF] face area times unit normal
+    double *AA;         // [3][15][N] pressure coefficients AA_px, AA_py, AA_pz
+    double *wall;       // [NWS][NB]
+    const int32_t *wpts;// [5][NB] face slots of the 5-point normal stencil
+    double *sc;         // [NSC][N]
+    double *fc;         // [5][NF]
+    double *MA, *TA;    // [18][N] exponential-fit constants (momentum, energy)
+    double *MF, *TF;    // [NCF][N]
+    double *src;        // [NSRC][N]
+    double *red;        // reduction scratch
+    Ctl *ctl;
+    unsigned long long *errs;  // [2][5][ERR_COPIES]
+};
+
+__device__ __forceinline__ int ip1(const Dev &d, int i) { return i + 1 == d.NT ? 0 : i + 1; }
+__device__ __forceinline__ int im1(const Dev &d, int i) { return i == 0 ? d.NT - 1 : i - 1; }
+__device__ __forceinline__ int cidx(const Dev &d, int i, int j, int k) { return i + d.NT * (j + d.NR * k); }
+
+__device__ __forceinline__ int fidx(const Dev &d, int pos, int i, int j, int k)
+{
+    switch (pos) {
+    case 1: return i + d.NT * (j + (d.NR + 1) * k);
+    case 2: return i + d.NT * (j + 1 + (d.NR + 1) * k);
+    case 3: return d.oT + i + d.NT * (j + d.NR * k);
+    case 4: return d.oT + ip1(d, i) + d.NT * (j + d.NR * k);
+    case 5: return d.oZ + i + d.NT * (j + d.NR * k);
+    default: return d.oZ + i + d.NT * (j + d.NR * (k + 1));
+    }
+}
+
+}  // namespace nim

@@ -1,0 +1,866 @@
+// HIP kernels of the NIM time step for gfx950 (FP64).
+//
+// One thread handles one cell (or one wall face) and performs the same
+// per-cell arithmetic as the reference subroutine named in each comment.
+// Relaxation sweeps run in four colour phases, colour = (2 i + j + 3 k) mod 4:
+// within a colour no cell reads a value another cell of that colour writes
+// (the stencils reach +-1 in every direction, +-2 in r and z at walls, and
+// the diagonal neighbours of the face-tangential corrections), so a phase is
+// race-free and needs no atomics. Convergence increments are reduced inside
+// the sweeps: each value is written once per outer iteration, so
+// |new - old| at the write is exactly the find_error increment
+// (nse.f90:260-325); waves reduce with shuffles and one lane per wave does an
+// atomic max on the bit pattern (non-negative doubles order like uint64).
+#include "nim_kernels.h"
+
+namespace nim {
+
+// ---------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ void err_commit(const Dev &d, int field, double e)
+{
+    e = wave_max(e);
+    if ((threadIdx.x & 63) == 0) {
+        int slot = d.ctl->iter & 1;
+        unsigned long long *p = d.errs + (size_t)(slot * 5 + field) * ERR_COPIES + (blockIdx.x & (ERR_COPIES - 1));
+        atomicMax(p, (unsigned long long)__double_as_longlong(e));
+    }
+}
+
+// cell of colour `color` handled by thread `tid`
+__device__ __forceinline__ bool color_cell(const Dev &d, int color, int tid, int &i, int &j, int &k)
+{
+    int nth = d.NT >> 1, nrh = (d.NR + 1) >> 1;
+    int ih = tid % nth, rest = tid / nth;
+    int jj = rest % nrh;
+    k = rest / nrh;
+    if (k >= d.NZ) return false;
+    j = ((color + k) & 1) + 2 * jj;
+    if (j >= d.NR) return false;
+    int r = (color - j - 3 * k) & 3;
+    i = 2 * ih + (r >> 1);
+    return true;
+}
+
+// ---------------------------------------------------------------------
+// finite differences shared by sources, sweeps and wall updates
+// ---------------------------------------------------------------------
+// derivative across cells of the face value at position fpos, along
+// direction dir (0 theta, 1 r, 2 z): central (f(+1) - f(-1))/4, 2nd-order
+// one-sided at walls (nse.f90:1643-1653 and its repetitions)
+__device__ __forceinline__ double tder(const Dev &d, const double *F, int fpos, int i, int j, int k, int dir)
+{
+    if (dir == 0) return (F[fidx(d, fpos, ip1(d, i), j, k)] - F[fidx(d, fpos, im1(d, i), j, k)]) / 4.;
+    if (dir == 1) {
+        if (j == 0)
+            return (-3 * F[fidx(d, fpos, i, 0, k)] + 4 * F[fidx(d, fpos, i, 1, k)] - F[fidx(d, fpos, i, 2, k)]) / 4.;
+        if (j == d.NR - 1)
+            return (3 * F[fidx(d, fpos, i, j, k)] - 4 * F[fidx(d, fpos, i, j - 1, k)] + F[fidx(d, fpos, i, j - 2, k)]) / 4.;
+        return (F[fidx(d, fpos, i, j + 1, k)] - F[fidx(d, fpos, i, j - 1, k)]) / 4.;
+    }
+    if (k == 0)
+        return (-3 * F[fidx(d, fpos, i, j, 0)] + 4 * F[fidx(d, fpos, i, j, 1)] - F[fidx(d, fpos, i, j, 2)]) / 4.;
+    if (k == d.NZ - 1)
+        return (3 * F[fidx(d, fpos, i, j, k)] - 4 * F[fidx(d, fpos, i, j, k - 1)] + F[fidx(d, fpos, i, j, k - 2)]) / 4.;
+    return (F[fidx(d, fpos, i, j, k + 1)] - F[fidx(d, fpos, i, j, k - 1)]) / 4.;
+}
+
+// cross-diffusion operator of calculate_R_D2 / _R_D_p2 / _R_D_T
+// (nse.f90:1628-2134)
+__device__ double cross_diffusion(const Dev &d, const double *F, int i, int j, int k, int c, double coef)
+{
+    const int N = d.N;
+    const double *g = d.gc;
+    double fxy = 0.5 * (tder(d, F, 2, i, j, k, 0) - tder(d, F, 1, i, j, k, 0));
+    double fzy = 0.5 * (tder(d, F, 6, i, j, k, 0) - tder(d, F, 5, i, j, k, 0));
+    double fyx = 0.5 * (tder(d, F, 4, i, j, k, 1) - tder(d, F, 3, i, j, k, 1));
+    double fzx = 0.5 * (tder(d, F, 6, i, j, k, 1) - tder(d, F, 5, i, j, k, 1));
+    double fxz = 0.5 * (tder(d, F, 2, i, j, k, 2) - tder(d, F, 1, i, j, k, 2));
+    double fyz = 0.5 * (tder(d, F, 4, i, j, k, 2) - tder(d, F, 3, i, j, k, 2));
+    double fx = (F[fidx(d, 2, i, j, k)] - F[fidx(d, 1, i, j, k)]) / 2.;
+    double fy = (F[fidx(d, 4, i, j, k)] - F[fidx(d, 3, i, j, k)]) / 2.;
+    double fz = (F[fidx(d, 6, i, j, k)] - F[fidx(d, 5, i, j, k)]) / 2.;
+    return coef * (g[(G_G + 0) * N + c] * fx + g[(G_G + 1) * N + c] * fy + g[(G_G + 2) * N + c] * fz
+                   - g[(G_GO + 0) * N + c] * fxy - g[(G_GO + 1) * N + c] * fyx - g[(G_GO + 2) * N + c] * fxz
+                   - g[(G_GO + 3) * N + c] * fzx - g[(G_GO + 4) * N + c] * fyz - g[(G_GO + 5) * N + c] * fzy);
+}
+
+// gradient component of a face field by the divergence theorem over the six
+// faces (dphi_dx2..dphi_dz2, nse.f90:2866-2903)
+__device__ __forceinline__ double dphi(const Dev &d, const double phi[6], const int f[6], int comp, double invV)
+{
+    const double *A = d.An + (size_t)comp * d.NF;
+    double s = phi[1] * A[f[1]] - phi[0] * A[f[0]] + phi[3] * A[f[3]] - phi[2] * A[f[2]] + phi[5] * A[f[5]]
+               - phi[4] * A[f[4]];
+    return invV * s;
+}
+
+__device__ __forceinline__ void cell_faces(const Dev &d, int i, int j, int k, int f[6])
+{
+    for (int p = 0; p < 6; p++) f[p] = fidx(d, p + 1, i, j, k);
+}
+
+// ---------------------------------------------------------------------
+// NIM exponential-fit constants of one direction (nim.f90:184-287)
+// ---------------------------------------------------------------------
+__device__ void nim_direction(double u, double k, double gp, double gm, double vlim, double A[6])
+{
+    double Re = 2 * u / k;
+    if (fabs(Re) > vlim) {
+        double e = exp(Re), ie = 1. / e;
+        A[0] = (2 / (k * (ie - 1.)) + 1. / u) * gp;
+        A[1] = (-(2 / (k * (1. - e))) - 1. / u) * gm;
+        A[2] = (u / (k * (ie - 1.))) * gp;
+        A[3] = (u / (k * (1. - e))) * gm;
+        if (Re > 0) {
+            A[4] = (-k + (u * (ie + 1.) / (-ie + 1.))) / (u * u);
+            A[5] = 1. / (-ie + 1.) - 1. / Re;
+        } else {
+            A[4] = (-k + (u * (1. + e) / (-1. + e))) / (u * u);
+            A[5] = e / (-1. + e) - 1. / Re;
+        }
+    } else {
+        double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u, u6 = u4 * u2, u7 = u6 * u, u8 = u4 * u4;
+        double u9 = u8 * u, u10 = u8 * u2;
+        double k2 = k * k, k3 = k2 * k, k4 = k2 * k2, k5 = k4 * k, k6 = k4 * k2, k7 = k6 * k, k8 = k4 * k4;
+        double k9 = k8 * k, k10 = k8 * k2, k11 = k10 * k;
+        A[0] = (-(1 / k) - u / (3. * k2) + u3 / (45. * k4) - 2 * u5 / (945. * k6) - u7 / (4725. * k8)
+                - 2 * u9 / (93555. * k10)) * gp;
+        A[1] = (-(1 / k) + u / (3. * k2) - u3 / (45. * k4) + 2 * u5 / (945. * k6) + u7 / (4725. * k8)
+                + 2 * u9 / (93555. * k10)) * gm;
+        A[2] = (-0.5 - u / (2. * k) - u2 / (6. * k2) + u4 / (90. * k4) + u6 / (945. * k6) + u8 / (9450. * k8)
+                - u10 / (93555. * k10)) * gp;
+        A[3] = (-0.5 + u / (2. * k) - u2 / (6. * k2) + u4 / (90. * k4) + u6 / (945. * k6) + u8 / (9450. * k8)
+                - u10 / (93555. * k10)) * gm;
+        A[4] = 1 / (3. * k) - u2 / (45. * k3) + 2 * u4 / (945. * k5) + u6 / (4725. * k7) + 2 * u8 / (93555. * k9)
+               + 1382 * u10 / (6.38512875e8 * k11);
+        A[5] = 0.5 + u / (6. * k) - u3 / (90. * k3) + u5 / (945. * k5) + u7 / (9450. * k7) + u9 / (93555. * k9);
+    }
+}
+
+// exponential-fit constants of every cell: momentum (time-centred velocity,
+// nu; nim.f90:166-290) or energy (mean of current iterate and previous
+// level, alpha; nim.f90:339-463)
+__global__ void k_coef_A(Dev d, int energy)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (energy && d.ctl->done) return;
+    if (c >= d.N) return;
+    const int N = d.N;
+    const double *g = d.gc;
+    double ux, uy, uz, kd;
+    if (energy) {
+        ux = 0.5 * (d.sc[(S_CUR + 0) * N + c] + d.sc[(S_OLD + 0) * N + c]);
+        uy = 0.5 * (d.sc[(S_CUR + 1) * N + c] + d.sc[(S_OLD + 1) * N + c]);
+        uz = 0.5 * (d.sc[(S_CUR + 2) * N + c] + d.sc[(S_OLD + 2) * N + c]);
+        kd = d.alpha;
+    } else {
+        ux = d.sc[(S_TC + 0) * N + c];
+        uy = d.sc[(S_TC + 1) * N + c];
+        uz = d.sc[(S_TC + 2) * N + c];
+        kd = d.nu;
+    }
+    double *out = energy ? d.TA : d.MA;
+    for (int dir = 0; dir < 3; dir++) {
+        double vel = ux * g[(G_XI + 3 * dir + 0) * N + c] + uy * g[(G_XI + 3 * dir + 1) * N + c]
+                     + uz * g[(G_XI + 3 * dir + 2) * N + c];
+        double A[6];
+        nim_direction(vel, g[(G_GD + dir) * N + c] * kd, g[(G_GF + 2 * dir + 1) * N + c],
+                      g[(G_GF + 2 * dir) * N + c], d.v_limit, A);
+        for (int m = 0; m < 6; m++) out[(dir * 6 + m) * N + c] = A[m];
+    }
+}
+
+// F4, F1..F3 from the constants (nim.f90:292-329 / 465-502); for the energy
+// equation the same kernel also applies the adiabatic top/bottom wall
+// extrapolation apply_zero_flux (bc.f90:345-390), which reads only interior
+// axial faces
+__global__ void k_coef_F(Dev d, int energy)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (energy && d.ctl->done) return;
+    bool act = c < d.N;
+    double eT = 0.;
+    if (act) {
+        const int N = d.N;
+        const double *A = energy ? d.TA : d.MA;
+        double *F = energy ? d.TF : d.MF;
+        int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+#define AV(n, dir, m) A[((dir - 1) * 6 + (m) - 1) * N + (n)]
+        double f2 = 0.5 * (1. / AV(c, 1, 5) + 1. / AV(c, 2, 5) + 1. / AV(c, 3, 5) - 1. / d.DT);
+        double f3 = -(AV(c, 1, 6) / AV(c, 1, 5)), f4 = (AV(c, 1, 6) - 1.) / AV(c, 1, 5);
+        double f5 = -(AV(c, 2, 6) / AV(c, 2, 5)), f6 = (AV(c, 2, 6) - 1.) / AV(c, 2, 5);
+        double f7 = -(AV(c, 3, 6) / AV(c, 3, 5)), f8 = (AV(c, 3, 6) - 1.) / AV(c, 3, 5);
+        F[(C_F4 + 0) * N + c] = f2 + f3 + f4 + f5 + f6 + f7 + f8;
+        F[(C_F4 + 1) * N + c] = f2;
+        F[(C_F4 + 2) * N + c] = f3;
+        F[(C_F4 + 3) * N + c] = f4;
+        F[(C_F4 + 4) * N + c] = f5;
+        F[(C_F4 + 5) * N + c] = f6;
+        F[(C_F4 + 6) * N + c] = f7;
+        F[(C_F4 + 7) * N + c] = f8;
+        const int base[3] = {C_F1, C_F2, C_F3};
+        for (int dir = 1; dir <= 3; dir++) {
+            int c2;
+            if (dir == 1) { if (j == d.NR - 1) continue; c2 = c + d.NT; }
+            else if (dir == 2) c2 = cidx(d, ip1(d, i), j, k);
+            else { if (k == d.NZ - 1) continue; c2 = c + d.NT * d.NR; }
+            double b = AV(c, dir, 3) - (AV(c, dir, 1) * AV(c, dir, 6) / AV(c, dir, 5));
+            double e = AV(c2, dir, 4) - AV(c2, dir, 2) / AV(c2, dir, 5) + AV(c2, dir, 2) * AV(c2, dir, 6) / AV(c2, dir, 5);
+            double h = AV(c, dir, 1) / (2. * AV(c, dir, 5));
+            double q = AV(c2, dir, 2) / (2. * AV(c2, dir, 5));
+            int s = base[dir - 1];
+            F[(s + 0) * N + c] = b + e + 2. * h + 2. * q;
+            F[(s + 1) * N + c] = b;
+            F[(s + 2) * N + c] = e;
+            F[(s + 3) * N + c] = h;
+            F[(s + 4) * N + c] = q;
+        }
+#undef AV
+        if (energy && (k == 0 || k == d.NZ - 1)) {
+            double *T = d.fc + (size_t)F_T * d.NF;
+            int b, fw, fa, fb, fcc;
+            if (k == 0) {
+                b = i + d.NT * j;
+                fw = fidx(d, 5, i, j, 0); fa = fidx(d, 6, i, j, 0); fb = fidx(d, 6, i, j, 1); fcc = fidx(d, 6, i, j, 2);
+            } else {
+                b = d.NT * d.NR + i + d.NT * j;
+                fw = fidx(d, 6, i, j, k); fa = fidx(d, 5, i, j, k); fb = fidx(d, 5, i, j, k - 1); fcc = fidx(d, 5, i, j, k - 2);
+            }
+            const double *W = d.wall;
+            const int NB = d.NB;
+            double nv = (W[(W_ZF + 0) * NB + b] * T[fa] - W[(W_ZF + 1) * NB + b] * T[fb] + W[(W_ZF + 2) * NB + b] * T[fcc])
+                        / W[(W_ZF + 3) * NB + b];
+            eT = fabs(nv - T[fw]);
+            T[fw] = nv;
+        }
+    }
+    if (energy) err_commit(d, F_T, eT);
+}
+
+// ---------------------------------------------------------------------
+// sources (computed from the state at the start of each stage)
+// ---------------------------------------------------------------------
+// update_velocity_sources (nse.f90:199-209): calculate_gforces, calculate_R_b,
+// calculate_R_D2. The lagged-convection term calculate_R_c_new
+// (nse.f90:1393-1434) vanishes identically: meshes_u and uc are the same
+// expression of the same time-centred velocity (nim.f90:169-171 and
+// nse.f90:1422-1428), so it is not evaluated.
+__global__ void k_mom_src(Dev d)
+{
+    if (d.ctl->done) return;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.N) return;
+    const int N = d.N;
+    int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    int f[6];
+    cell_faces(d, i, j, k, f);
+    const double *P = d.fc + (size_t)F_P * d.NF;
+    double pf[6];
+    for (int p = 0; p < 6; p++) pf[p] = P[f[p]];
+    double invV = d.gc[G_INVV * N + c];
+    double bz = -(d.g_beta * 0.5 * (d.sc[(S_CUR + 3) * N + c] + d.sc[(S_OLD + 3) * N + c]));
+    double RB[3] = {0. + d.rinv * dphi(d, pf, f, 0, invV), 0. + d.rinv * dphi(d, pf, f, 1, invV),
+                    bz + d.rinv * dphi(d, pf, f, 2, invV)};
+    for (int q = 0; q < 3; q++) {
+        double RD = cross_diffusion(d, d.fc + (size_t)q * d.NF, i, j, k, c, d.Rinv);
+        d.src[q * N + c] = RD + RB[q];
+    }
+}
+
+// update_pressure_sources (nse.f90:228-235): calculate_R_b_p
+// (nse.f90:2362-2407; faces_bx = faces_by = 0, faces_bz = -g_beta*faces_T)
+// and calculate_R_D_p2; stores R_D_p + R_b_p, the combination the pressure
+// sweep and wall update use, and the dilatation
+__global__ void k_ppe_src(Dev d)
+{
+    if (d.ctl->done) return;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.N) return;
+    const int N = d.N;
+    int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    int f[6];
+    cell_faces(d, i, j, k, f);
+    double uf[6], vf[6], wf[6], bzf[6];
+    const double *U = d.fc, *V = d.fc + d.NF, *W = d.fc + 2 * (size_t)d.NF, *T = d.fc + 4 * (size_t)d.NF;
+    for (int p = 0; p < 6; p++) {
+        uf[p] = U[f[p]];
+        vf[p] = V[f[p]];
+        wf[p] = W[f[p]];
+        bzf[p] = -(d.g_beta * T[f[p]]);
+    }
+    double invV = d.gc[G_INVV * N + c];
+    double ux = dphi(d, uf, f, 0, invV), uy = dphi(d, uf, f, 1, invV), uz = dphi(d, uf, f, 2, invV);
+    double vx = dphi(d, vf, f, 0, invV), vy = dphi(d, vf, f, 1, invV), vz = dphi(d, vf, f, 2, invV);
+    double wx = dphi(d, wf, f, 0, invV), wy = dphi(d, wf, f, 1, invV), wz = dphi(d, wf, f, 2, invV);
+    double bzz = dphi(d, bzf, f, 2, invV);
+    double Dila = ux + vy + wz;
+    d.sc[S_DILA * N + c] = Dila;
+    double Rbp = -(ux * ux + vy * vy + wz * wz) - 2.0 * (uy * vx + vz * wy + wx * uz) - bzz + Dila / (2 * d.DT);
+    double RDp = cross_diffusion(d, d.fc + (size_t)F_P * d.NF, i, j, k, c, d.rinv);
+    d.src[R_P * N + c] = RDp + Rbp;
+}
+
+// update_temperature_sources (nse.f90:249-256): R_b_T = q = 0, R_D_T
+__global__ void k_T_src(Dev d)
+{
+    if (d.ctl->done) return;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.N) return;
+    int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    d.src[R_T * d.N + c] = cross_diffusion(d, d.fc + (size_t)F_T * d.NF, i, j, k, c, d.alpha) + 0.;
+}
+
+// ---------------------------------------------------------------------
+// relaxation sweeps
+// ---------------------------------------------------------------------
+// one transported field of one cell: the cell average, then owned faces
+// 2, 4, 6 (integrate_velocity_field_formulation2_i nse.f90:831-1006 and
+// integrate_temperature_field_formulation2_i nse.f90:1009-1089)
+__device__ void transport_unit(const Dev &d, double *Fc, double *C, const double *CO, const double *coef, double B,
+                               double w, int i, int j, int k, int c, double &ecell, double &eface)
+{
+    const int N = d.N;
+    int f[6];
+    cell_faces(d, i, j, k, f);
+    double old = C[c];
+    double nv = (1 - w) * old + (w / coef[(C_F4 + 0) * N + c])
+                                    * (coef[(C_F4 + 1) * N + c] * CO[c] + coef[(C_F4 + 2) * N + c] * Fc[f[0]]
+                                       + coef[(C_F4 + 3) * N + c] * Fc[f[1]] + coef[(C_F4 + 4) * N + c] * Fc[f[2]]
+                                       + coef[(C_F4 + 5) * N + c] * Fc[f[3]] + coef[(C_F4 + 6) * N + c] * Fc[f[4]]
+                                       + coef[(C_F4 + 7) * N + c] * Fc[f[5]] + B);
+    C[c] = nv;
+    ecell = fmax(ecell, fabs(nv - old));
+    const double *gdf = d.gc + (size_t)G_DIFF * N;
+    double ccur = nv + CO[c];
+    // face 2: radial neighbour (i, j+1, k)
+    if (j < d.NR - 1) {
+        int c2 = c + d.NT;
+        int g3 = fidx(d, 3, i, j + 1, k), g4 = fidx(d, 4, i, j + 1, k), g5 = fidx(d, 5, i, j + 1, k);
+        int g6 = fidx(d, 6, i, j + 1, k), g2 = fidx(d, 2, i, j + 1, k);
+        double B3 = gdf[0 * N + c] * (Fc[f[3]] - Fc[f[2]] + Fc[g4] - Fc[g3]) / 4.0;
+        double B4 = gdf[1 * N + c] * (Fc[f[5]] - Fc[f[4]] + Fc[g6] - Fc[g5]) / 4.0;
+        double o = Fc[f[1]];
+        double v = (1 - w) * o + (w / coef[(C_F1 + 0) * N + c])
+                                     * (coef[(C_F1 + 1) * N + c] * Fc[f[0]] + coef[(C_F1 + 2) * N + c] * Fc[g2]
+                                        + coef[(C_F1 + 3) * N + c] * ccur + coef[(C_F1 + 4) * N + c] * (C[c2] + CO[c2])
+                                        - B3 - B4);
+        Fc[f[1]] = v;
+        eface = fmax(eface, fabs(v - o));
+    }
+    // face 4: azimuthal neighbour (i+1, j, k), periodic
+    {
+        int ip = ip1(d, i);
+        int c2 = cidx(d, ip, j, k);
+        int g1 = fidx(d, 1, ip, j, k), g2 = fidx(d, 2, ip, j, k), g4 = fidx(d, 4, ip, j, k);
+        int g5 = fidx(d, 5, ip, j, k), g6 = fidx(d, 6, ip, j, k);
+        double B3 = gdf[2 * N + c] * (Fc[f[1]] - Fc[f[0]] + Fc[g2] - Fc[g1]) / 4.0;
+        double B4 = gdf[3 * N + c] * (Fc[f[5]] - Fc[f[4]] + Fc[g6] - Fc[g5]) / 4.0;
+        double o = Fc[f[3]];
+        double v = (1 - w) * o + (w / coef[(C_F2 + 0) * N + c])
+                                     * (coef[(C_F2 + 1) * N + c] * Fc[f[2]] + coef[(C_F2 + 2) * N + c] * Fc[g4]
+                                        + coef[(C_F2 + 3) * N + c] * ccur + coef[(C_F2 + 4) * N + c] * (C[c2] + CO[c2])
+                                        - B3 - B4);
+        Fc[f[3]] = v;
+        eface = fmax(eface, fabs(v - o));
+    }
+    // face 6: axial neighbour (i, j, k+1)
+    if (k < d.NZ - 1) {
+        int c2 = c + d.NT * d.NR;
+        int g1 = fidx(d, 1, i, j, k + 1), g2 = fidx(d, 2, i, j, k + 1), g3 = fidx(d, 3, i, j, k + 1);
+        int g4 = fidx(d, 4, i, j, k + 1), g6 = fidx(d, 6, i, j, k + 1);
+        double B3 = gdf[4 * N + c] * (Fc[f[1]] - Fc[f[0]] + Fc[g2] - Fc[g1]) / 4.0;
+        double B4 = gdf[5 * N + c] * (Fc[f[3]] - Fc[f[2]] + Fc[g4] - Fc[g3]) / 4.0;
+        double o = Fc[f[5]];
+        double v = (1 - w) * o + (w / coef[(C_F3 + 0) * N + c])
+                                     * (coef[(C_F3 + 1) * N + c] * Fc[f[4]] + coef[(C_F3 + 2) * N + c] * Fc[g6]
+                                        + coef[(C_F3 + 3) * N + c] * ccur + coef[(C_F3 + 4) * N + c] * (C[c2] + CO[c2])
+                                        - B3 - B4);
+        Fc[f[5]] = v;
+        eface = fmax(eface, fabs(v - o));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_mom_sweep(Dev d, int color)
+{
+    if (d.ctl->done) return;
+    int i, j, k;
+    bool act = color_cell(d, color, blockIdx.x * blockDim.x + threadIdx.x, i, j, k);
+    double e[3] = {0., 0., 0.};
+    if (act) {
+        const int N = d.N;
+        int c = cidx(d, i, j, k);
+        const double wq[3] = {d.w_u, d.w_v, d.w_w};
+        for (int q = 0; q < 3; q++) {
+            double ec = 0., ef = 0.;
+            transport_unit(d, d.fc + (size_t)q * d.NF, d.sc + (size_t)(S_CUR + q) * N, d.sc + (size_t)(S_OLD + q) * N,
+                           d.MF, d.src[q * N + c], wq[q], i, j, k, c, ec, ef);
+            e[q] = fmax(ec, ef);
+        }
+    }
+    err_commit(d, F_U, e[0]);
+    err_commit(d, F_V, e[1]);
+    err_commit(d, F_W, e[2]);
+}
+
+__global__ void __launch_bounds__(256) k_T_sweep(Dev d, int color)
+{
+    if (d.ctl->done) return;
+    int i, j, k;
+    bool act = color_cell(d, color, blockIdx.x * blockDim.x + threadIdx.x, i, j, k);
+    double e = 0.;
+    if (act) {
+        const int N = d.N;
+        int c = cidx(d, i, j, k);
+        double ec = 0., ef = 0.;
+        transport_unit(d, d.fc + (size_t)F_T * d.NF, d.sc + (size_t)(S_CUR + 3) * N, d.sc + (size_t)(S_OLD + 3) * N,
+                       d.TF, d.src[R_T * N + c], d.w_T, i, j, k, c, ec, ef);
+        e = fmax(ec, ef);
+    }
+    err_commit(d, F_T, e);
+}
+
+// apply_pressure_bc for the wall face at position pos of cell (i,j,k)
+// (bc.f90:147-341)
+__device__ double pressure_wall(const Dev &d, int b, int pos, int i, int j, int k, int c)
+{
+    const int NB = d.NB;
+    const double *W = d.wall;
+    double *P = d.fc + (size_t)F_P * d.NF;
+    const double *U = d.fc, *V = d.fc + d.NF, *Wv = d.fc + 2 * (size_t)d.NF, *T = d.fc + 4 * (size_t)d.NF;
+    double n1 = W[(W_N + 0) * NB + b], n2 = W[(W_N + 1) * NB + b], n3 = W[(W_N + 2) * NB + b];
+    int fw = fidx(d, pos, i, j, k);
+    double b_n = 0. * n1 + 0. * n2 + (-(d.g_beta * T[fw])) * n3;
+    double d2 = 0.;
+    for (int m = 0; m < 5; m++) {
+        int q = d.wpts[m * NB + b];
+        double vn = U[q] * n1 + V[q] * n2 + Wv[q] * n3;
+        d2 = m == 0 ? -(W[(W_A + 0) * NB + b] * vn) : d2 - W[(W_A + m) * NB + b] * vn;
+    }
+    double gB2 = W[W_GB2 * NB + b], gB2c = W[W_GB2C * NB + b], gB3 = W[W_GB3 * NB + b];
+    double B2, B3;
+    if (pos <= 2) {
+        // tangential directions theta (always central) and z
+        B2 = (P[fidx(d, pos, ip1(d, i), j, k)] - P[fidx(d, pos, im1(d, i), j, k)]) / 4.0 * gB2c;
+        if (k == 0)
+            B3 = (-3. * P[fw] + 4. * P[fidx(d, pos, i, j, 1)] - P[fidx(d, pos, i, j, 2)]) / 4.0 * gB3;
+        else if (k == d.NZ - 1)
+            B3 = (3. * P[fw] - 4. * P[fidx(d, pos, i, j, k - 1)] + P[fidx(d, pos, i, j, k - 2)]) / 4.0 * gB3;
+        else
+            B3 = (P[fidx(d, pos, i, j, k + 1)] - P[fidx(d, pos, i, j, k - 1)]) / 4.0 * gB3;
+    } else {
+        // tangential directions r and theta (always central)
+        if (j == 0)
+            B2 = (-3. * P[fw] + 4. * P[fidx(d, pos, i, 1, k)] - P[fidx(d, pos, i, 2, k)]) / 4.0 * gB2;
+        else if (j == d.NR - 1)
+            B2 = (3. * P[fw] - 4. * P[fidx(d, pos, i, j - 1, k)] + P[fidx(d, pos, i, j - 2, k)]) / 4.0 * gB2;
+        else
+            B2 = (P[fidx(d, pos, i, j + 1, k)] - P[fidx(d, pos, i, j - 1, k)]) / 4.0 * gB2c;
+        B3 = (P[fidx(d, pos, ip1(d, i), j, k)] - P[fidx(d, pos, im1(d, i), j, k)]) / 4.0 * gB3;
+    }
+    double B1 = W[(W_AP + 7) * NB + b] * d.src[R_P * d.N + c];
+    double s = 0.;
+    bool first = true;
+    for (int q = 1; q <= 6; q++) {
+        if (q == pos) continue;
+        double t = W[(W_AP + q - 1) * NB + b] * P[fidx(d, q, i, j, k)];
+        s = first ? -t : s - t;
+        first = false;
+    }
+    double o = P[fw];
+    double v = (1 - d.w_p2) * o + d.w_p2 / W[(W_AP + pos - 1) * NB + b]
+                                      * (s + B1 - B2 - B3 + (d.nu * d.rho * d2 - b_n * d.rho));
+    P[fw] = v;
+    return fabs(v - o);
+}
+
+// integrate_pressure_field_i (nse.f90:1091-1289) for one cell, preceded by
+// the cell's own wall faces in the reference's boundary-group order
+__global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, int color)
+{
+    if (d.ctl->done) return;
+    int i, j, k;
+    bool act = color_cell(d, color, blockIdx.x * blockDim.x + threadIdx.x, i, j, k);
+    double e = 0.;
+    if (act) {
+        const int N = d.N, NT = d.NT, NR = d.NR, NZ = d.NZ;
+        int c = cidx(d, i, j, k);
+        if (k == 0) e = fmax(e, pressure_wall(d, i + NT * j, 5, i, j, k, c));
+        if (k == NZ - 1) e = fmax(e, pressure_wall(d, NT * NR + i + NT * j, 6, i, j, k, c));
+        if (j == 0) e = fmax(e, pressure_wall(d, 2 * NT * NR + i + NT * k, 1, i, j, k, c));
+        if (j == NR - 1) e = fmax(e, pressure_wall(d, 2 * NT * NR + NT * NZ + i + NT * k, 2, i, j, k, c));
+        double *P = d.fc + (size_t)F_P * d.NF;
+        const double *gdf = d.gc + (size_t)G_DIFF * N;
+        const double S1 = d.src[R_P * N + c];
+        const double w = d.w_p;
+        int f[6];
+        cell_faces(d, i, j, k, f);
+        if (j < NR - 1) {
+            int n2 = c + NT;
+            const double *A = d.AA + (size_t)(0 * 15) * N;
+#define AX(m) A[((m) - 1) * N + c]
+            double B1 = S1 * AX(13), B2 = d.src[R_P * N + n2] * AX(15);
+            double B3 = gdf[0 * N + c] * tder(d, P, 2, i, j, k, 0);
+            double B4 = gdf[1 * N + c] * tder(d, P, 2, i, j, k, 2);
+            double o = P[f[1]];
+            double v = (1 - w) * o + w / AX(2) * (-AX(1) * P[f[0]] - AX(3) * P[fidx(d, 2, i, j + 1, k)] - AX(4) * P[f[2]]
+                                                  - AX(5) * P[f[3]] - AX(6) * P[fidx(d, 3, i, j + 1, k)]
+                                                  - AX(7) * P[fidx(d, 4, i, j + 1, k)] - AX(8) * P[f[4]] - AX(9) * P[f[5]]
+                                                  - AX(10) * P[fidx(d, 5, i, j + 1, k)]
+                                                  - AX(11) * P[fidx(d, 6, i, j + 1, k)] + B1 + B2 + B3 + B4);
+            P[f[1]] = v;
+            e = fmax(e, fabs(v - o));
+#undef AX
+        }
+        {
+            int ip = ip1(d, i);
+            int n2 = cidx(d, ip, j, k);
+            const double *A = d.AA + (size_t)(1 * 15) * N;
+#define AY(m) A[((m) - 1) * N + c]
+            double B1 = S1 * AY(13), B2 = d.src[R_P * N + n2] * AY(15);
+            double B3 = gdf[2 * N + c] * tder(d, P, 4, i, j, k, 1);
+            double B4 = gdf[3 * N + c] * tder(d, P, 4, i, j, k, 2);
+            double o = P[f[3]];
+            double v = (1 - w) * o + w / AY(6) * (-AY(1) * P[f[0]] - AY(2) * P[f[1]] - AY(3) * P[fidx(d, 1, ip, j, k)]
+                                                  - AY(4) * P[fidx(d, 2, ip, j, k)] - AY(5) * P[f[2]]
+                                                  - AY(7) * P[fidx(d, 4, ip, j, k)] - AY(8) * P[f[4]] - AY(9) * P[f[5]]
+                                                  - AY(10) * P[fidx(d, 5, ip, j, k)] - AY(11) * P[fidx(d, 6, ip, j, k)]
+                                                  + B1 + B2 + B3 + B4);
+            P[f[3]] = v;
+            e = fmax(e, fabs(v - o));
+#undef AY
+        }
+        if (k < NZ - 1) {
+            int n2 = c + NT * NR;
+            const double *A = d.AA + (size_t)(2 * 15) * N;
+#define AZ(m) A[((m) - 1) * N + c]
+            double B1 = S1 * AZ(13), B2 = d.src[R_P * N + n2] * AZ(15);
+            double B3 = gdf[4 * N + c] * tder(d, P, 6, i, j, k, 1);
+            double B4 = gdf[5 * N + c] * tder(d, P, 6, i, j, k, 0);
+            double o = P[f[5]];
+            double v = (1 - w) * o + w / AZ(10) * (-AZ(1) * P[f[0]] - AZ(2) * P[f[1]] - AZ(3) * P[fidx(d, 1, i, j, k + 1)]
+                                                   - AZ(4) * P[fidx(d, 2, i, j, k + 1)] - AZ(5) * P[f[2]] - AZ(6) * P[f[3]]
+                                                   - AZ(7) * P[fidx(d, 3, i, j, k + 1)]
+                                                   - AZ(8) * P[fidx(d, 4, i, j, k + 1)] - AZ(9) * P[f[4]]
+                                                   - AZ(11) * P[fidx(d, 6, i, j, k + 1)] + B1 + B2 + B3 + B4);
+            P[f[5]] = v;
+            e = fmax(e, fabs(v - o));
+#undef AZ
+        }
+    }
+    err_commit(d, F_P, e);
+}
+
+// ---------------------------------------------------------------------
+// iteration control
+// ---------------------------------------------------------------------
+__global__ void k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4, int max_iter, int min_iter)
+{
+    int l = threadIdx.x;
+    for (int q = l; q < 2 * 5 * ERR_COPIES; q += blockDim.x) d.errs[q] = 0ull;
+    if (l == 0) {
+        Ctl *c = d.ctl;
+        c->done = 0;
+        c->iter = 0;
+        c->status = 0;
+        c->max_iter = max_iter;
+        c->min_iter = min_iter;
+        c->tol[0] = t0; c->tol[1] = t1; c->tol[2] = t2; c->tol[3] = t3; c->tol[4] = t4;
+        for (int q = 0; q < 5; q++) c->err[q] = 1.;
+    }
+}
+
+// end of one outer iteration: reduce the increments, test convergence
+// (the while condition of integrate_nse, nse.f90:129), clear the slot the
+// next iteration uses
+__global__ void k_check(Dev d)
+{
+    Ctl *ctl = d.ctl;
+    if (ctl->done) return;
+    int l = threadIdx.x;  // one wave
+    int s = ctl->iter & 1;
+    double e[5];
+    for (int q = 0; q < 5; q++) {
+        e[q] = wave_max(__longlong_as_double((long long)d.errs[(size_t)(s * 5 + q) * ERR_COPIES + l]));
+        d.errs[(size_t)((s ^ 1) * 5 + q) * ERR_COPIES + l] = 0ull;
+    }
+    if (l == 0) {
+        int it = ctl->iter + 1;
+        ctl->iter = it;
+        bool fin = true, conv = it >= ctl->min_iter;
+        for (int q = 0; q < 5; q++) {
+            ctl->err[q] = e[q];
+            fin = fin && isfinite(e[q]);
+            conv = conv && !(e[q] > ctl->tol[q]);
+        }
+        if (!fin) { ctl->status = 4; ctl->done = 1; }
+        else if (conv) { ctl->status = 0; ctl->done = 1; }
+        else if (it >= ctl->max_iter) { ctl->status = 3; ctl->done = 1; }
+    }
+}
+
+// ---------------------------------------------------------------------
+// once-per-step and init kernels
+// ---------------------------------------------------------------------
+// calculate_pressure_nim_constants (nim.f90:556-743) at u = 0 (Taylor
+// branch of B_nim..J_nim): C = -1/(3K), E = F = Det/2, G = I = 1/2, H = 1/K,
+// J = -1/K, K = g^ii/rho
+struct PCoef { double C[3], E[3], F[3], H[3], J[3], Det; };
+
+__device__ PCoef pcoef(const Dev &d, int c)
+{
+    PCoef p;
+    const int N = d.N;
+    for (int q = 0; q < 3; q++) {
+        double K = d.gc[(G_GD + q) * N + c] / d.rho;
+        p.C[q] = -(1. / (3. * K));
+        p.H[q] = 1. / K;
+        p.J[q] = -(1. / K);
+    }
+    p.Det = 1. / (p.C[0] * p.C[1] + p.C[1] * p.C[2] + p.C[0] * p.C[2]);
+    for (int q = 0; q < 3; q++) {
+        p.E[q] = p.Det / 2.;
+        p.F[q] = p.Det / 2.;
+    }
+    return p;
+}
+
+__global__ void k_pressure_coeffs(Dev d)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.N) return;
+    const int N = d.N;
+    const double GG = 0.5, II = 0.5;
+    int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    PCoef a = pcoef(d, c);
+    for (int dir = 0; dir < 3; dir++) {
+        int c2;
+        if (dir == 0) { if (j == d.NR - 1) continue; c2 = c + d.NT; }
+        else if (dir == 1) c2 = cidx(d, ip1(d, i), j, k);
+        else { if (k == d.NZ - 1) continue; c2 = c + d.NT * d.NR; }
+        PCoef b = pcoef(d, c2);
+        int e1 = dir == 0 ? 1 : 0, e2 = dir == 2 ? 1 : 2;  // tangential directions (0-based)
+        double gp = d.gc[(G_GF + 2 * dir + 1) * N + c], gm = d.gc[(G_GF + 2 * dir) * N + c2];
+        double A[16];
+        double Cea = a.C[e1] + a.C[e2], Ceb = b.C[e1] + b.C[e2];
+        // normal pair 2dir+1, 2dir+2 (1-based slots), tangential pairs
+        int n0 = 2 * dir + 1;
+        A[n0] = -(GG + Cea * a.E[dir] * a.H[dir]) * gp;
+        A[n0 + 1] = (GG - Cea * a.F[dir] * a.H[dir]) * gp + (II + Ceb * b.E[dir] * b.J[dir]) * gm;
+        A[n0 + 2] = -(II - Ceb * b.F[dir] * b.J[dir]) * gm;
+        // slot layout per direction (nim.f90:622-697): the 11 face slots are
+        // ordered by face position 1..6 of cell and neighbour
+        int te[2] = {e1, e2};
+        double tv[2][4];
+        for (int t = 0; t < 2; t++) {
+            int oth = te[1 - t];
+            tv[t][0] = a.C[oth] * a.E[te[t]] * a.H[dir] * gp;
+            tv[t][1] = a.C[oth] * a.F[te[t]] * a.H[dir] * gp;
+            tv[t][2] = -(b.C[oth] * b.E[te[t]] * b.J[dir] * gm);
+            tv[t][3] = -(b.C[oth] * b.F[te[t]] * b.J[dir] * gm);
+        }
+        double *out = d.AA + (size_t)(dir * 15) * N;
+        if (dir == 0) {
+            // AA_px: 1 self-, 2 diag, 3 far, 4..7 theta pair (e1), 8..11 z pair (e2)
+            out[0 * N + c] = A[1]; out[1 * N + c] = A[2]; out[2 * N + c] = A[3];
+            out[3 * N + c] = tv[0][0]; out[4 * N + c] = tv[0][1]; out[5 * N + c] = tv[0][2]; out[6 * N + c] = tv[0][3];
+            out[7 * N + c] = tv[1][0]; out[8 * N + c] = tv[1][1]; out[9 * N + c] = tv[1][2]; out[10 * N + c] = tv[1][3];
+        } else if (dir == 1) {
+            // AA_py: 1..4 r pair, 5 self-, 6 diag, 7 far, 8..11 z pair
+            out[0 * N + c] = tv[0][0]; out[1 * N + c] = tv[0][1]; out[2 * N + c] = tv[0][2]; out[3 * N + c] = tv[0][3];
+            out[4 * N + c] = A[3]; out[5 * N + c] = A[4]; out[6 * N + c] = A[5];
+            out[7 * N + c] = tv[1][0]; out[8 * N + c] = tv[1][1]; out[9 * N + c] = tv[1][2]; out[10 * N + c] = tv[1][3];
+        } else {
+            // AA_pz: 1..4 r pair, 5..8 theta pair, 9 self-, 10 diag, 11 far
+            out[0 * N + c] = tv[0][0]; out[1 * N + c] = tv[0][1]; out[2 * N + c] = tv[0][2]; out[3 * N + c] = tv[0][3];
+            out[4 * N + c] = tv[1][0]; out[5 * N + c] = tv[1][1]; out[6 * N + c] = tv[1][2]; out[7 * N + c] = tv[1][3];
+            out[8 * N + c] = A[5]; out[9 * N + c] = A[6]; out[10 * N + c] = A[7];
+        }
+        double s = a.C[e1] * a.C[e2] * a.Det * a.H[dir] * gp;
+        double s2 = b.C[e1] * b.C[e2] * b.Det * b.J[dir] * gm;
+        out[11 * N + c] = s;
+        out[12 * N + c] = -s;
+        out[13 * N + c] = -s2;
+        out[14 * N + c] = s2;
+    }
+}
+
+// BCs_faces_ap (nim.f90:682-742) for every wall face
+__global__ void k_wall_pressure_coeffs(Dev d)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.NB) return;
+    const int NT = d.NT, NR = d.NR, NZ = d.NZ, NB = d.NB, N = d.N;
+    int i, j, k, pos;
+    if (b < NT * NR) { pos = 5; i = b % NT; j = b / NT; k = 0; }
+    else if (b < 2 * NT * NR) { pos = 6; i = (b - NT * NR) % NT; j = (b - NT * NR) / NT; k = NZ - 1; }
+    else if (b < 2 * NT * NR + NT * NZ) { pos = 1; i = (b - 2 * NT * NR) % NT; k = (b - 2 * NT * NR) / NT; j = 0; }
+    else { pos = 2; i = (b - 2 * NT * NR - NT * NZ) % NT; k = (b - 2 * NT * NR - NT * NZ) / NT; j = NR - 1; }
+    int c = cidx(d, i, j, k);
+    PCoef a = pcoef(d, c);
+    int dir = (pos - 1) / 2;
+    int e1 = dir == 0 ? 1 : 0, e2 = dir == 2 ? 1 : 2;
+    bool plus = (pos % 2) == 0;
+    double g = d.gc[(G_GF + pos - 1) * N + c];
+    double L = plus ? a.H[dir] : a.J[dir];
+    double Ce = a.C[e1] + a.C[e2];
+    double ap[7];
+    ap[2 * dir + 1] = -(0.5 + Ce * a.E[dir] * L) * g;
+    ap[2 * dir + 2] = (0.5 - Ce * a.F[dir] * L) * g;
+    ap[2 * e1 + 1] = a.C[e2] * a.E[e1] * L * g;
+    ap[2 * e1 + 2] = a.C[e2] * a.F[e1] * L * g;
+    ap[2 * e2 + 1] = a.C[e1] * a.E[e2] * L * g;
+    ap[2 * e2 + 2] = a.C[e1] * a.F[e2] * L * g;
+    double s7 = a.C[e1] * a.C[e2] * a.Det * L * g;
+    for (int m = 1; m <= 6; m++) d.wall[(W_AP + m - 1) * NB + b] = ap[m];
+    d.wall[(W_AP + 6) * NB + b] = s7;
+    d.wall[(W_AP + 7) * NB + b] = -s7;
+}
+
+// update_time_step (nse.f90:1292-1313)
+__global__ void k_advance(Dev d)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.N) return;
+    const int N = d.N;
+    double *s = d.sc;
+    for (int q = 0; q < 4; q++) {
+        double cur = s[(S_CUR + q) * N + c], old = s[(S_OLD + q) * N + c];
+        s[(S_TC + q) * N + c] = (old + cur) / 2.;
+        s[(S_OLD + q) * N + c] = cur;
+    }
+    int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    const double *P = d.fc + (size_t)F_P * d.NF;
+    int f[6];
+    cell_faces(d, i, j, k, f);
+    s[S_P * N + c] = (P[f[0]] + P[f[1]] + P[f[2]] + P[f[3]] + P[f[4]] + P[f[5]]) / 6.0;
+    s[S_DILAO * N + c] = s[S_DILA * N + c];
+}
+
+// per-block partial results of check_steady_state (nse.f90:1316-1332) and
+// evaluate_D (nse.f90:2504-2537): red[b*6 + {du, dv, dw, dT, sumD2}]
+__global__ void __launch_bounds__(256) k_end_step_partial(Dev d)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double m[4] = {0., 0., 0., 0.}, s = 0.;
+    if (c < d.N) {
+        const int N = d.N;
+        double den = 2 * d.DT;
+        for (int q = 0; q < 4; q++) m[q] = fabs(d.sc[(S_CUR + q) * N + c] - d.sc[(S_OLD + q) * N + c]) / den;
+        int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+        int f[6];
+        cell_faces(d, i, j, k, f);
+        double uf[6], vf[6], wf[6];
+        for (int p = 0; p < 6; p++) {
+            uf[p] = d.fc[f[p]];
+            vf[p] = d.fc[d.NF + f[p]];
+            wf[p] = d.fc[2 * (size_t)d.NF + f[p]];
+        }
+        double invV = d.gc[G_INVV * N + c];
+        double D = dphi(d, uf, f, 0, invV) + dphi(d, vf, f, 1, invV) + dphi(d, wf, f, 2, invV);
+        d.sc[S_DILA * N + c] = D;
+        s = D * D;
+    }
+    __shared__ double sh[4][5];
+    int l = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = 0; q < 4; q++) m[q] = wave_max(m[q]);
+    s = wave_sum(s);
+    if (l == 0) {
+        for (int q = 0; q < 4; q++) sh[wv][q] = m[q];
+        sh[wv][4] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 5; q++) {
+            double v = sh[0][q];
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = q < 4 ? fmax(v, sh[w][q]) : v + sh[w][q];
+            d.red[(size_t)blockIdx.x * 6 + q] = v;
+        }
+    }
+}
+
+// combine the partials in a fixed order; out = {dudt_max, dTdt_max, Dilatation}
+__global__ void k_end_step_final(Dev d, int nblocks, double *out)
+{
+    int l = threadIdx.x;  // 256 threads
+    double m[4] = {0., 0., 0., 0.}, s = 0.;
+    for (int b = l; b < nblocks; b += blockDim.x) {
+        for (int q = 0; q < 4; q++) m[q] = fmax(m[q], d.red[(size_t)b * 6 + q]);
+        s += d.red[(size_t)b * 6 + 4];
+    }
+    __shared__ double sh[4][5];
+    int wv = l >> 6;
+    for (int q = 0; q < 4; q++) m[q] = wave_max(m[q]);
+    s = wave_sum(s);
+    if ((l & 63) == 0) {
+        for (int q = 0; q < 4; q++) sh[wv][q] = m[q];
+        sh[wv][4] = s;
+    }
+    __syncthreads();
+    if (l == 0) {
+        double r[5];
+        for (int q = 0; q < 5; q++) {
+            r[q] = sh[0][q];
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) r[q] = q < 4 ? fmax(r[q], sh[w][q]) : r[q] + sh[w][q];
+        }
+        out[0] = fmax(r[2], fmax(r[0], r[1]));
+        out[1] = r[3];
+        out[2] = sqrt(r[4] / d.N);
+    }
+}
+
+// find_Nu2 (nse.f90:2699-2798): one block, fixed summation order.
+// out = {Nu_H, Nu_C, Average, Area1, Area2}
+__global__ void k_nusselt(Dev d, double R_i, double R_o, double *out)
+{
+    const int NT = d.NT, NR = d.NR, NZ = d.NZ, NB = d.NB;
+    const double *T = d.fc + (size_t)F_T * d.NF, *W = d.wall;
+    int n = NT * NZ;
+    double s1 = 0., a1 = 0., s2 = 0., a2 = 0.;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        int i = q % NT, k = q / NT;
+        // hexagon wall, position 1
+        int b = 2 * NT * NR + q;
+        double dd = -W[(W_NU + 0) * NB + b] * T[fidx(d, 1, i, 0, k)] + W[(W_NU + 1) * NB + b] * T[fidx(d, 2, i, 0, k)]
+                    - W[(W_NU + 2) * NB + b] * T[fidx(d, 2, i, 1, k)] + W[(W_NU + 3) * NB + b] * T[fidx(d, 2, i, 2, k)];
+        s2 += W[(W_NU + 4) * NB + b] * (-dd);
+        a2 += W[(W_NU + 4) * NB + b];
+        // cylinder wall, position 2
+        b = 2 * NT * NR + NT * NZ + q;
+        dd = -W[(W_NU + 0) * NB + b] * T[fidx(d, 2, i, NR - 1, k)] + W[(W_NU + 1) * NB + b] * T[fidx(d, 1, i, NR - 1, k)]
+             - W[(W_NU + 2) * NB + b] * T[fidx(d, 1, i, NR - 2, k)] + W[(W_NU + 3) * NB + b] * T[fidx(d, 1, i, NR - 3, k)];
+        s1 += W[(W_NU + 4) * NB + b] * dd;
+        a1 += W[(W_NU + 4) * NB + b];
+    }
+    __shared__ double sh[4][4];
+    int wv = threadIdx.x >> 6;
+    s1 = wave_sum(s1); a1 = wave_sum(a1); s2 = wave_sum(s2); a2 = wave_sum(a2);
+    if ((threadIdx.x & 63) == 0) { sh[wv][0] = s1; sh[wv][1] = a1; sh[wv][2] = s2; sh[wv][3] = a2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s1 = sh[0][0]; a1 = sh[0][1]; s2 = sh[0][2]; a2 = sh[0][3];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            s1 += sh[w][0]; a1 += sh[w][1]; s2 += sh[w][2]; a2 += sh[w][3];
+        }
+        out[0] = -(s1 / a1 * R_o);
+        out[1] = -(s2 / a2 * R_i);
+        out[2] = -((s1 / a1 * R_o) + (s2 / a2 * R_i)) / 2.;
+        out[3] = a1;
+        out[4] = a2;
+    }
+}
+
+}  // namespace nim

@@ -1,0 +1,268 @@
+"""Python binding of libnim_hip.so (the C-ABI in include/nim_hip.h).
+
+Arrays cross the boundary in the reference's layout and numbering (see the
+header). This module loads the in-tree HIP library and raises if it is
+missing; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "libnim_hip.so")
+
+NIM_OK, NIM_E_ARG, NIM_E_HIP, NIM_E_MAXITER, NIM_E_NONFINITE, NIM_E_STATE = range(6)
+
+STAGES = {
+    "velocity_sources": 0, "velocity_sweep": 1, "pressure_sources": 2, "pressure_stage": 3,
+    "temperature_coeffs": 4, "temperature_sources": 5, "temperature_sweep": 6,
+}
+
+
+class NimError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("nim error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [("n_theta", ctypes.c_int32), ("n_r", ctypes.c_int32), ("n_z", ctypes.c_int32)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "nu", "Re", "alpha", "rho", "g_beta", "DT", "w_u", "w_v", "w_w", "w_p", "w_p2", "w_T",
+        "v_limit", "R_i", "R_o")]
+
+
+_P = ctypes.POINTER(ctypes.c_double)
+_PI = ctypes.POINTER(ctypes.c_int32)
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("volume", _P), ("xi_x", _P), ("gij", _P), ("gij_faces", _P), ("G1", _P), ("G2", _P),
+                ("G3", _P), ("faces_area", _P), ("faces_normal", _P), ("faces_mid", _P), ("bc_points", _PI),
+                ("bc_a", _P)]
+
+
+STATE_FIELDS = ("faces_u", "faces_v", "faces_w", "faces_p", "faces_T", "uxyz", "vxyz", "wxyz", "Txyz",
+                "uxyz_old", "vxyz_old", "wxyz_old", "Txyz_old", "uxyzt", "vxyzt", "wxyzt", "Txyzt",
+                "meshes_p", "meshes_dila")
+
+
+class State(ctypes.Structure):
+    _fields_ = [(n, _P) for n in STATE_FIELDS]
+
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree libnim_hip.so; raise if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libnim_hip.so is not built: run __graft_entry__.build() "
+                               "(make -C nim-conv22-shared_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        i32 = ctypes.c_int32
+        L.nim_counts.restype = i32
+        L.nim_counts.argtypes = [ctypes.POINTER(Dims), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                 ctypes.POINTER(ctypes.c_int64)]
+        L.nim_face_map.restype = i32
+        L.nim_face_map.argtypes = [ctypes.POINTER(Dims), _PI]
+        L.nim_ctx_create.restype = i32
+        L.nim_ctx_create.argtypes = [ctypes.POINTER(Dims), ctypes.POINTER(Params), ctypes.POINTER(vp)]
+        L.nim_ctx_destroy.restype = i32
+        L.nim_ctx_destroy.argtypes = [vp]
+        L.nim_upload_geometry.restype = i32
+        L.nim_upload_geometry.argtypes = [vp, ctypes.POINTER(Geometry)]
+        for n in ("nim_build_pressure_coeffs", "nim_build_momentum_coeffs", "nim_advance_time", "nim_synchronize"):
+            getattr(L, n).restype = i32
+            getattr(L, n).argtypes = [vp]
+        L.nim_upload_state.restype = i32
+        L.nim_upload_state.argtypes = [vp, ctypes.POINTER(State)]
+        L.nim_download_state.restype = i32
+        L.nim_download_state.argtypes = [vp, ctypes.POINTER(State)]
+        L.nim_inner_solve.restype = i32
+        L.nim_inner_solve.argtypes = [vp, _P, i32, i32, _PI, _P]
+        L.nim_end_step.restype = i32
+        L.nim_end_step.argtypes = [vp, _P, _P, _P]
+        L.nim_nusselt.restype = i32
+        L.nim_nusselt.argtypes = [vp, _P]
+        L.nim_run_stage.restype = i32
+        L.nim_run_stage.argtypes = [vp, i32]
+        L.nim_get_array.restype = i32
+        L.nim_get_array.argtypes = [vp, ctypes.c_char_p, _P, ctypes.c_int64]
+        L.nim_set_array.restype = i32
+        L.nim_set_array.argtypes = [vp, ctypes.c_char_p, _P, ctypes.c_int64]
+        L.nim_stream.restype = vp
+        L.nim_stream.argtypes = [vp]
+        L.nim_last_error.restype = ctypes.c_char_p
+        L.nim_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points declared in include/nim_hip.h."""
+    return ["nim_counts", "nim_face_map", "nim_ctx_create", "nim_ctx_destroy", "nim_upload_geometry",
+            "nim_build_pressure_coeffs", "nim_upload_state", "nim_download_state", "nim_build_momentum_coeffs",
+            "nim_inner_solve", "nim_end_step", "nim_advance_time", "nim_nusselt", "nim_run_stage",
+            "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_last_error"]
+
+
+def _check(rc):
+    if rc != NIM_OK:
+        raise NimError(rc, lib().nim_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(_P)
+
+
+def counts(nt, nr, nz):
+    d = Dims(nt, nr, nz)
+    a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _check(lib().nim_counts(ctypes.byref(d), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+    return a.value, b.value, c.value
+
+
+def face_map(nt, nr, nz):
+    _, nf, _ = counts(nt, nr, nz)
+    out = np.zeros(nf, dtype=np.int32)
+    _check(lib().nim_face_map(ctypes.byref(Dims(nt, nr, nz)), out.ctypes.data_as(_PI)))
+    return out
+
+
+def make_params(p):
+    """Params from any object with the nim_params field names."""
+    q = Params()
+    for n, _ in Params._fields_:
+        setattr(q, n, float(getattr(p, n)))
+    return q
+
+
+class Context:
+    """One case on the current HIP device (nim_ctx)."""
+
+    def __init__(self, nt, nr, nz, params):
+        self.L = lib()
+        self.dims = Dims(nt, nr, nz)
+        self.params = make_params(params)
+        self.nmesh, self.nface, self.nbc = counts(nt, nr, nz)
+        h = ctypes.c_void_p()
+        _check(self.L.nim_ctx_create(ctypes.byref(self.dims), ctypes.byref(self.params), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.nim_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_geometry(self, geo):
+        """geo: mapping with the reference-layout arrays named as the
+        nim_geometry fields (bc_points 1-based)."""
+        keep = {}
+        g = Geometry()
+        for n, t in Geometry._fields_:
+            if n == "bc_points":
+                a = np.ascontiguousarray(geo[n], dtype=np.int32)
+                setattr(g, n, a.ctypes.data_as(_PI))
+            else:
+                a = np.ascontiguousarray(geo[n], dtype=np.float64)
+                setattr(g, n, _dp(a))
+            keep[n] = a
+        _check(self.L.nim_upload_geometry(self.h, ctypes.byref(g)))
+
+    def build_pressure_coeffs(self):
+        _check(self.L.nim_build_pressure_coeffs(self.h))
+
+    def build_momentum_coeffs(self):
+        _check(self.L.nim_build_momentum_coeffs(self.h))
+
+    def _len(self, name):
+        if name.startswith("faces_"):
+            return self.nface
+        mult = {"M_F1": 5, "M_F2": 5, "M_F3": 5, "M_F4": 8, "MT_F1": 5, "MT_F2": 5, "MT_F3": 5, "MT_F4": 8,
+                "AA_px": 15, "AA_py": 15, "AA_pz": 15}
+        if name == "bc_ap":
+            return 8 * self.nbc
+        return mult.get(name, 1) * self.nmesh
+
+    def get(self, name):
+        out = np.zeros(self._len(name))
+        _check(self.L.nim_get_array(self.h, name.encode(), _dp(out), out.size))
+        return out
+
+    def set(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        _check(self.L.nim_set_array(self.h, name.encode(), _dp(a), a.size))
+
+    def upload_state(self, st):
+        """st: mapping from STATE_FIELDS names to reference-layout arrays."""
+        s = State()
+        keep = []
+        for n in STATE_FIELDS:
+            if n in st:
+                a = np.ascontiguousarray(st[n], dtype=np.float64)
+                keep.append(a)
+                setattr(s, n, _dp(a))
+        _check(self.L.nim_upload_state(self.h, ctypes.byref(s)))
+
+    def download_state(self):
+        out = {n: np.zeros(self.nface if n.startswith("faces_") else self.nmesh) for n in STATE_FIELDS}
+        s = State()
+        for n in STATE_FIELDS:
+            setattr(s, n, _dp(out[n]))
+        _check(self.L.nim_download_state(self.h, ctypes.byref(s)))
+        return out
+
+    def run_stage(self, stage):
+        _check(self.L.nim_run_stage(self.h, STAGES[stage] if isinstance(stage, str) else int(stage)))
+
+    def inner_solve(self, tol, max_iter=100000, min_iter=0, raise_on_cap=True):
+        t = np.ascontiguousarray(tol, dtype=np.float64)
+        it = ctypes.c_int32()
+        err = np.zeros(5)
+        rc = self.L.nim_inner_solve(self.h, _dp(t), max_iter, min_iter, ctypes.byref(it), _dp(err))
+        if rc != NIM_OK and (raise_on_cap or rc != NIM_E_MAXITER):
+            _check(rc)
+        return rc, it.value, err
+
+    def end_step(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(self.L.nim_end_step(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return dict(dudt_max=a.value, dTdt_max=b.value, Dilatation=c.value)
+
+    def advance_time(self):
+        _check(self.L.nim_advance_time(self.h))
+
+    def nusselt(self):
+        out = np.zeros(5)
+        _check(self.L.nim_nusselt(self.h, _dp(out)))
+        return out
+
+    def synchronize(self):
+        _check(self.L.nim_synchronize(self.h))
+
+    def stream(self):
+        return self.L.nim_stream(self.h)
+
+    def time_step(self, tol, max_iter=100000, min_iter=0):
+        """One step of solve_nse (nse.f90:49-73) through the C-ABI."""
+        self.build_momentum_coeffs()
+        rc, iters, err = self.inner_solve(tol, max_iter, min_iter)
+        scal = self.end_step()
+        self.advance_time()
+        nu = self.nusselt()
+        return iters, err, scal, nu

@@ -1,0 +1,160 @@
+"""Python access to the Fortran host library (libnim_host.so): case geometry
+and initial state in the reference's layout, and the case definitions used
+by the tests and the benchmark."""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+from . import PKG
+
+HOST_LIB = os.path.join(PKG, "libnim_host.so")
+MATH_PI = 3.14159265359  # parameters.f90:31
+
+
+class GeoCase(ctypes.Structure):
+    _fields_ = [("n_theta", ctypes.c_int32), ("n_r", ctypes.c_int32), ("n_z", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("R_i", ctypes.c_double), ("R_o", ctypes.c_double),
+                ("height", ctypes.c_double), ("blend_in", ctypes.c_double), ("blend_out", ctypes.c_double),
+                ("grading", ctypes.c_double), ("hex_order", ctypes.c_double), ("pi_const", ctypes.c_double)]
+
+
+class Case:
+    """One NIM case: O-grid, geometry and physical constants.
+
+    dims: (n_theta, n_r, n_z). R_i is the hexagon circumradius, R_o the
+    cylinder radius, height the axial length. blend_in/blend_out are the
+    absolute radii of the circle-to-hexagon transition; by default they sit
+    at the same fractions of the gap as the reference's 1.25 and 1.75 on its
+    pi/3..2 annulus (general.f90:433).
+    """
+
+    def __init__(self, dims, R_i, R_o, height, Ra, Pr, DT, blend=None, grading=1.5, hex_order=-1.0, rho=1.0,
+                 g_beta=1.0, th_tc=1.0, relax=0.4, v_limit=1e-2, T_hot=1.0, T_cold=0.0):
+        self.dims = tuple(int(x) for x in dims)
+        self.R_i, self.R_o, self.height = float(R_i), float(R_o), float(height)
+        if blend is None:
+            ri_ref, ro_ref = 1.0 * (MATH_PI / 3.0), 2.0
+            f1 = (1.25 - ri_ref) / (ro_ref - ri_ref)
+            f2 = (1.75 - ri_ref) / (ro_ref - ri_ref)
+            blend = (self.R_i + f1 * (self.R_o - self.R_i), self.R_i + f2 * (self.R_o - self.R_i))
+        self.blend = tuple(float(b) for b in blend)
+        self.grading, self.hex_order = float(grading), float(hex_order)
+        # derived constants as parameters.f90:166-173
+        self.Ra, self.Pr, self.DT = float(Ra), float(Pr), float(DT)
+        self.nu = 1. / math.sqrt(self.Ra / (g_beta * th_tc * self.Pr))
+        self.Re = 1. / self.nu
+        self.alpha = self.nu / self.Pr
+        self.rho, self.g_beta = float(rho), float(g_beta)
+        for n in ("w_u", "w_v", "w_w", "w_p", "w_p2", "w_T"):
+            setattr(self, n, float(relax))
+        self.v_limit = float(v_limit)
+        self.T_hot, self.T_cold = float(T_hot), float(T_cold)
+
+    @property
+    def ncells(self):
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+    def geo_case(self):
+        g = GeoCase()
+        g.n_theta, g.n_r, g.n_z = self.dims
+        g.R_i, g.R_o, g.height = self.R_i, self.R_o, self.height
+        g.blend_in, g.blend_out = self.blend
+        g.grading, g.hex_order, g.pi_const = self.grading, self.hex_order, MATH_PI
+        return g
+
+
+def reference_case():
+    """The reference's built-in case (parameters.f90:84-173): 36 x 6 x 12,
+    annulus pi/3..2, L = 2, blend radii 1.25/1.75, Ra = 1e5, Pr = 0.7,
+    DT = 0.01."""
+    return Case((36, 6, 12), 1.0 * (MATH_PI / 3.0), 2.0, 2.0, Ra=1.0e5, Pr=0.7, DT=0.01, blend=(1.25, 1.75))
+
+
+def baseline_case(name):
+    """Cases of the specification: 'C1' 24 x 48 x 16 (theta 48), 'C2'
+    128 x 256 x 32, 'C4' 48 x 96 x 24, 'C5' 256 x 512 x 64 (n_r x n_theta x
+    n_z), in a cylinder R = 1, H = 1 around a hexagon of circumradius 0.3."""
+    table = {
+        "C1": ((48, 24, 16), 1e4, 1e-3),
+        "C2": ((256, 128, 32), 1e5, 5e-4),
+        "C4": ((96, 48, 24), 5e4, 1e-3),
+        "C5": ((512, 256, 64), 1e6, 2.5e-4),
+    }
+    dims, Ra, DT = table[name]
+    return Case(dims, 0.3, 1.0, 1.0, Ra=Ra, Pr=0.71, DT=DT)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError("libnim_host.so is not built (make -C nim-conv22-shared_amd/fortran)")
+        L = ctypes.CDLL(HOST_LIB)
+        P = ctypes.POINTER(ctypes.c_double)
+        PI = ctypes.POINTER(ctypes.c_int32)
+        L.nimh_build_geometry.restype = ctypes.c_int32
+        L.nimh_build_geometry.argtypes = [ctypes.POINTER(GeoCase)] + [P] * 12 + [PI] * 6 + [P, P]
+        L.nimh_initial_state.restype = ctypes.c_int32
+        L.nimh_initial_state.argtypes = [ctypes.POINTER(GeoCase), ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int64] + [P] * 9
+        L.nimh_noise.restype = ctypes.c_double
+        L.nimh_noise.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]
+        _lib = L
+    return _lib
+
+
+def build_geometry(case):
+    """Every geometry array of the case, reference layout (1-based integer
+    tables)."""
+    nt, nr, nz = case.dims
+    N = nt * nr * nz
+    NF = 3 * N + nt * nz + nt * nr
+    NB = 2 * nt * nr + 2 * nt * nz
+    NN = nt * (nr + 1) * (nz + 1)
+    f = lambda n: np.zeros(n)
+    i = lambda n: np.zeros(n, dtype=np.int32)
+    g = dict(volume=f(N), xi_x=f(9 * N), gij=f(9 * N), gij_faces=f(54 * N), G1=f(N), G2=f(N), G3=f(N),
+             faces_area=f(NF), faces_normal=f(3 * NF), faces_tangential=f(3 * NF), faces_bitangential=f(3 * NF),
+             faces_mid=f(3 * NF), meshes_faces=i(6 * N), meshes_neighbors=i(6 * N), bc_faces=i(NB),
+             bc_position=i(NB), bc_mesh=i(NB), bc_points=i(5 * NB), bc_a=f(20 * NB), node_xyz=f(3 * NN))
+    order = ["volume", "xi_x", "gij", "gij_faces", "G1", "G2", "G3", "faces_area", "faces_normal",
+             "faces_tangential", "faces_bitangential", "faces_mid", "meshes_faces", "meshes_neighbors", "bc_faces",
+             "bc_position", "bc_mesh", "bc_points", "bc_a", "node_xyz"]
+    args = []
+    for n in order:
+        a = g[n]
+        args.append(a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32 if a.dtype == np.int32 else ctypes.c_double)))
+    gc = case.geo_case()
+    rc = lib().nimh_build_geometry(ctypes.byref(gc), *args)
+    if rc != 0:
+        raise ValueError("nimh_build_geometry rejected the case")
+    return g
+
+
+def initial_state(case, amp=1e-3, seed=20221):
+    """Faces at rest with the Dirichlet wall temperatures, cells perturbed by
+    amp*noise(seed) on all three time levels."""
+    nt, nr, nz = case.dims
+    N = nt * nr * nz
+    NF = 3 * N + nt * nz + nt * nr
+    st = {n: np.zeros(NF) for n in ("faces_u", "faces_v", "faces_w", "faces_p", "faces_T")}
+    for n in ("uxyz", "vxyz", "wxyz", "Txyz"):
+        st[n] = np.zeros(N)
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    gc = case.geo_case()
+    rc = lib().nimh_initial_state(ctypes.byref(gc), case.T_hot, case.T_cold, amp, seed,
+                                  *[P(st[n]) for n in ("faces_u", "faces_v", "faces_w", "faces_p", "faces_T",
+                                                       "uxyz", "vxyz", "wxyz", "Txyz")])
+    if rc != 0:
+        raise ValueError("nimh_initial_state failed")
+    for f in "uvwT":
+        st[f + "xyz_old"] = st[f + "xyz"].copy()
+        st[f + "xyzt"] = st[f + "xyz"].copy()
+    st["meshes_p"] = np.zeros(N)
+    st["meshes_dila"] = np.zeros(N)
+    return st

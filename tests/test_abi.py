@@ -1,0 +1,50 @@
+"""C-ABI library: loads, exports every declared entry point, and its host-side
+layout mapping agrees with the reference's connectivity (no device use)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+import nim_amd
+import refdata as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_entry_points():
+    text = open(os.path.join(ROOT, "include", "nim_hip.h")).read()
+    return sorted(set(re.findall(r"\b(nim_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_all_declared_symbols():
+    L = ctypes.CDLL(nim_amd.LIB_PATH)
+    names = declared_entry_points()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(nim_amd.exported_symbols()) == names
+
+
+def test_counts_and_face_map_match_reference_numbering():
+    geo = R.geometry()
+    nt, nr, nz = R.REF_DIMS
+    nmesh, nface, nbc = nim_amd.counts(nt, nr, nz)
+    assert (nmesh, nface, nbc) == (int(geo["dims"][3]), int(geo["dims"][4]), int(geo["dims"][5]))
+    fm = nim_amd.face_map(nt, nr, nz)
+    assert sorted(fm.tolist()) == list(range(nface))
+    # device slot of each cell-local face must be the reference's meshes_faces
+    mf = geo["meshes_faces"].reshape(nmesh, 6) - 1
+    oT = nt * (nr + 1) * nz
+    oZ = oT + nmesh
+    for c in range(nmesh):
+        i, j, k = c % nt, (c // nt) % nr, c // (nt * nr)
+        slots = [i + nt * (j + (nr + 1) * k), i + nt * (j + 1 + (nr + 1) * k), oT + c,
+                 oT + (i + 1) % nt + nt * (j + nr * k), oZ + c, oZ + c + nt * nr]
+        assert list(fm[slots]) == list(mf[c]), c
+
+
+def test_unsupported_dimensions_rejected():
+    import pytest
+    with pytest.raises(nim_amd.NimError):
+        nim_amd.counts(35, 6, 12)  # odd azimuthal count cannot be 4-coloured

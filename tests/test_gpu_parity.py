@@ -1,0 +1,159 @@
+"""HIP path against the CPU oracle on the reference's 36 x 6 x 12 case.
+
+Tolerances: the GPU contracts multiply-adds into FMA and evaluates a few
+sums in another order, so per-stage agreement is at rounding level
+(relative max error 1e-12 for single stages). Converged fields are held to
+the 1e-8 relative L2 bar of the specification and Nusselt numbers to 1e-6.
+"""
+import numpy as np
+import pytest
+
+import gpucase as G
+import oracle as O
+import refdata as R
+
+pytestmark = pytest.mark.gpu
+
+STAGE_TOL = 1e-12
+# The exponential-fit constants (nim.f90:184-287) cancel catastrophically just
+# above |Re| = v_limit: -k + u*coth(Re/2) ~ k*Re^2/12 and 1 - exp(-Re) ~ Re,
+# so one ulp of exp() (the device's exp is not glibc's) or one fused
+# multiply-add moves them by up to ~1e7 ulp. Measured on the CPU alone, FMA
+# contraction changes M_F1 by 6e-11 relative. Coefficients are therefore held
+# to 1e-8 and the oracle's coefficients are copied into the device before the
+# kernels that consume them, so every later stage is checked at rounding level.
+COEF_TOL = 1e-8
+
+
+@pytest.fixture(scope="module")
+def geo():
+    return R.geometry()
+
+
+@pytest.fixture(scope="module")
+def steps():
+    return R.steps()
+
+
+def test_pressure_coefficients_match_reference(geo):
+    ctx = G.gpu_context(geo)
+    for n in ("AA_px", "AA_py", "AA_pz", "bc_ap"):
+        got = ctx.get(n)
+        ref = geo[n]
+        # slots the reference never assigns (walls without a neighbour) stay 0
+        mask = ref != 0
+        assert G.rel_max(got[mask], ref[mask]) < 1e-14, n
+    ctx.close()
+
+
+def test_stages_match_oracle_colored(geo, steps):
+    ctx = G.gpu_context(geo)
+    o = G.oracle_context(geo)
+    st = G.state_from_steps(steps, "s3")
+    ctx.upload_state(st)
+    R.load_state(o, steps, "s3")
+
+    ctx.build_momentum_coeffs()
+    o.velocity_constants()
+    for n in ("M_F1", "M_F2", "M_F3", "M_F4"):
+        ref = o.get(n)
+        got = ctx.get(n)
+        mask = ref != 0
+        assert G.rel_max(got[mask], ref[mask]) < COEF_TOL, n
+        ctx.set(n, ref)
+
+    ctx.run_stage("velocity_sources")
+    o.velocity_sources()
+    for f in "uvw":
+        ref = o.get("R_C_" + f) + o.get("R_D_" + f) + o.get("R_B_" + f)
+        assert G.rel_max(ctx.get("R_B_" + f), ref) < STAGE_TOL, f
+
+    ctx.run_stage("velocity_sweep")
+    o.velocity_sweep(O.COLORED)
+    for f in "uvw":
+        assert G.rel_max(ctx.get("faces_" + f), o.get("faces_" + f)) < STAGE_TOL, f
+        assert G.rel_max(ctx.get(f + "xyz"), o.get(f + "xyz")) < STAGE_TOL, f
+
+    ctx.run_stage("pressure_sources")
+    o.pressure_sources()
+    assert G.rel_max(ctx.get("R_b_p"), o.get("R_D_p") + o.get("R_B_p")) < STAGE_TOL
+    assert G.rel_max(ctx.get("meshes_dila"), o.get("dila")) < STAGE_TOL
+
+    ctx.run_stage("pressure_stage")
+    o.pressure_stage(O.COLORED)
+    assert G.rel_max(ctx.get("faces_p"), o.get("faces_p")) < STAGE_TOL
+
+    ctx.run_stage("temperature_coeffs")
+    o.temperature_constants()
+    o.zero_flux()
+    for n in ("MT_F1", "MT_F2", "MT_F3", "MT_F4"):
+        ref = o.get(n)
+        mask = ref != 0
+        assert G.rel_max(ctx.get(n)[mask], ref[mask]) < COEF_TOL, n
+        ctx.set(n, ref)
+    assert G.rel_max(ctx.get("faces_T"), o.get("faces_T")) < STAGE_TOL
+
+    ctx.run_stage("temperature_sources")
+    o.temperature_sources()
+    assert G.rel_max(ctx.get("R_D_T"), o.get("R_D_T")) < STAGE_TOL
+
+    ctx.run_stage("temperature_sweep")
+    o.temperature_sweep(O.COLORED)
+    assert G.rel_max(ctx.get("faces_T"), o.get("faces_T")) < STAGE_TOL
+    assert G.rel_max(ctx.get("Txyz"), o.get("Txyz")) < STAGE_TOL
+    ctx.close()
+
+
+@pytest.mark.parametrize("graph", ["1", "0"])
+def test_inner_solve_matches_oracle_colored(geo, steps, graph, monkeypatch):
+    monkeypatch.setenv("NIM_GRAPH", graph)
+    ctx = G.gpu_context(geo)
+    o = G.oracle_context(geo)
+    ctx.upload_state(G.state_from_steps(steps, "s3"))
+    R.load_state(o, steps, "s3")
+    tol = np.full(5, 1e-9)
+    ctx.build_momentum_coeffs()
+    o.velocity_constants()
+    rc, it_g, err_g = ctx.inner_solve(tol)
+    rc_o, it_o, err_o = o.inner_solve(tol, O.COLORED)
+    assert rc == 0 and rc_o == 0
+    assert abs(it_g - it_o) <= 1, (it_g, it_o)
+    for f in "uvwpT":
+        assert G.rel_l2(ctx.get("faces_" + f), o.get("faces_" + f)) < 1e-9, f
+    for f in "uvwT":
+        assert G.rel_l2(ctx.get(f + "xyz"), o.get(f + "xyz")) < 1e-9, f
+    ctx.close()
+
+
+def test_three_steps_match_reference_order(geo, steps):
+    """GPU (4-colour order) vs the oracle in the reference's own order, both
+    converged to 1e-12 per pass: fields within 1e-8 relative L2 and the
+    Nusselt numbers within 1e-6 (the parity bar of the specification)."""
+    ctx = G.gpu_context(geo)
+    o = G.oracle_context(geo)
+    ctx.upload_state(G.state_from_steps(steps, "s0"))
+    R.load_state(o, steps, "s0")
+    tol = np.full(5, 1e-12)
+    for k in range(3):
+        it_g, err_g, scal_g, nu_g = ctx.time_step(tol, min_iter=2 if k == 0 else 0)
+        rc, scal_o, nu_o = o.time_step(tol, O.NATURAL)
+        assert rc == 0
+        np.testing.assert_allclose(nu_g, nu_o, rtol=1e-6)
+        np.testing.assert_allclose(scal_g["Dilatation"], scal_o[6], rtol=1e-6)
+    for f in "uvwpT":
+        assert G.rel_l2(ctx.get("faces_" + f), o.get("faces_" + f)) < 1e-8, f
+    for f in "uvwT":
+        for suf in ("xyz", "xyz_old", "xyzt"):
+            assert G.rel_l2(ctx.get(f + suf), o.get(f + suf)) < 1e-8, (f, suf)
+    assert G.rel_l2(ctx.get("meshes_p"), o.get("meshes_p")) < 1e-8
+    ctx.close()
+
+
+def test_iteration_cap_reports_error(geo, steps):
+    ctx = G.gpu_context(geo)
+    ctx.upload_state(G.state_from_steps(steps, "s0"))
+    ctx.build_momentum_coeffs()
+    import nim_amd
+    rc, it, err = ctx.inner_solve(np.full(5, 1e-14), max_iter=3, raise_on_cap=False)
+    assert rc == nim_amd.NIM_E_MAXITER and it == 3
+    ctx.close()

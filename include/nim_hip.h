@@ -85,7 +85,8 @@ typedef struct nim_ctx nim_ctx;
 int32_t nim_counts(const nim_dims *dims, int64_t *nmesh, int64_t *nface, int64_t *nbc);
 
 /* For each of the NFACE face slots of the device layout (radial faces, then
- * azimuthal, then axial), the 0-based reference face number.
+ * azimuthal, then axial, as laid out in csrc/nim_kernels.h), the 0-based
+ * reference face number.
  * Pure host function (no device use). */
 int32_t nim_face_map(const nim_dims *dims, int32_t *ref_face_of_slot);
 
@@ -149,8 +150,20 @@ int32_t nim_run_stage(nim_ctx *ctx, int32_t stage);
 int32_t nim_get_array(nim_ctx *ctx, const char *name, double *out, int64_t count);
 int32_t nim_set_array(nim_ctx *ctx, const char *name, const double *in, int64_t count);
 
-/* Measurement support: the stream the context launches on (a hipStream_t),
- * and synchronisation with it. */
+/* Measurement support. nim_kernel_times runs `iters` outer iterations from
+ * the current state (state is advanced; convergence is not tested) with a
+ * HIP event around every launch on the context's stream, and returns for
+ * each kernel class the mean time per iteration in ms (ms_per_iter) and the
+ * launches per iteration (launches). Classes, in launch order:
+ *   0 velocity sources   1 velocity sweep (4 colour launches)
+ *   2 pressure sources   3 pressure walls + sweep (4 colour launches)
+ *   4 energy constants   5 energy coefficients + adiabatic walls
+ *   6 energy sources     7 energy sweep (4 colour launches)
+ *   8 convergence check
+ * nim_stream returns the hipStream_t the context launches on. */
+#define NIM_KERNEL_CLASSES 9
+int32_t nim_kernel_times(nim_ctx *ctx, int32_t iters, double ms_per_iter[NIM_KERNEL_CLASSES],
+                         int32_t launches[NIM_KERNEL_CLASSES]);
 void *nim_stream(nim_ctx *ctx);
 int32_t nim_synchronize(nim_ctx *ctx);
 

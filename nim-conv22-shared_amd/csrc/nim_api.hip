@@ -633,3 +633,48 @@ extern "C" int32_t nim_nusselt(nim_ctx *c, double out[5])
     for (int q = 0; q < 5; q++) out[q] = c->h_out[q];
     return NIM_OK;
 }
+
+extern "C" int32_t nim_kernel_times(nim_ctx *c, int32_t iters, double *ms, int32_t *launches)
+{
+    if (!c || !ms || iters < 1) return fail(NIM_E_ARG, "bad argument");
+    if (!c->has_geo || !c->has_pcoef) return fail(NIM_E_STATE, "geometry or pressure coefficients missing");
+    const Dev &d = c->dev;
+    hipStream_t s = c->stream;
+    const int gN = nblk(c->N), gC = color_blocks(c);
+    // launch list of one iteration: (class, kernel index)
+    const int cls[18] = {0, 1, 1, 1, 1, 2, 3, 3, 3, 3, 4, 5, 6, 7, 7, 7, 7, 8};
+    const double tol[5] = {0, 0, 0, 0, 0};
+    int32_t rc = reset_ctl(c, tol, 1 << 30, 1 << 30);
+    if (rc) return rc;
+    std::vector<hipEvent_t> ev(19);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    for (int q = 0; q < NIM_KERNEL_CLASSES; q++) {
+        ms[q] = 0.;
+        if (launches) launches[q] = 0;
+    }
+    for (int it = 0; it < iters; it++) {
+        int n = 0;
+        HIPCHK(hipEventRecord(ev[0], s));
+        auto mark = [&]() { (void)hipEventRecord(ev[++n], s); };
+        hipLaunchKernelGGL(k_mom_src, dim3(gN), dim3(BS), 0, s, d); mark();
+        for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_mom_sweep, dim3(gC), dim3(BS), 0, s, d, q); mark(); }
+        hipLaunchKernelGGL(k_ppe_src, dim3(gN), dim3(BS), 0, s, d); mark();
+        for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_ppe_sweep, dim3(gC), dim3(BS), 0, s, d, q); mark(); }
+        hipLaunchKernelGGL(k_coef_A, dim3(gN), dim3(BS), 0, s, d, 1); mark();
+        hipLaunchKernelGGL(k_coef_F, dim3(gN), dim3(BS), 0, s, d, 1); mark();
+        hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); mark();
+        for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_T_sweep, dim3(gC), dim3(BS), 0, s, d, q); mark(); }
+        hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, d); mark();
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventSynchronize(ev[18]));
+        for (int q = 0; q < 18; q++) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
+            ms[cls[q]] += t;
+            if (launches && it == 0) launches[cls[q]]++;
+        }
+    }
+    for (int q = 0; q < NIM_KERNEL_CLASSES; q++) ms[q] /= iters;
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return NIM_OK;
+}

@@ -1,0 +1,221 @@
+! Fortran host time loop over the MI355X library: the module a reference
+! build would use in place of class_nse's solve_nse (nse.f90:23-85). Mesh and
+! metrics come from nim_host_geometry, every per-step routine is a call
+! through nim_hip_api, and the restart file keeps the reference's record
+! layout (restart.f90:8-69).
+module nim_nse
+  use iso_c_binding
+  use nim_host_geometry
+  use nim_hip_api
+  implicit none
+  private
+  public :: nim_run_config, nim_setup, nim_solve, nim_write_restart, nim_read_restart, nim_finish
+
+  type :: nim_run_config
+    type(nim_geo_case) :: geo
+    real(c_double) :: Ra = 1.0d5, Pr = 0.7d0, DT = 0.01d0, rho = 1.0d0, g_beta = 1.0d0, th_tc = 1.0d0
+    real(c_double) :: relax = 0.4d0, v_limit = 1.0d-2, T_hot = 1.0d0, T_cold = 0.0d0
+    real(c_double) :: noise = 1.0d-3
+    integer(c_int64_t) :: seed = 20221
+    real(c_double) :: tol(5) = (/ 1.0d-6, 1.0d-6, 1.0d-6, 5.0d-3, 1.0d-6 /)  ! parameters.f90:146-150
+    real(c_double) :: dudt_epsilon = 1.0d-5                                  ! parameters.f90:151
+    logical :: check_steady = .true.
+    integer :: max_iter = 200000
+  end type
+
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer, save :: N, NF, NB
+  real(c_double), save :: TI, DTs
+  integer, save :: STEP
+  ! host copies of the state in the reference's layout
+  real(c_double), allocatable, target, save :: faces_u(:), faces_v(:), faces_w(:), faces_p(:), faces_T(:)
+  real(c_double), allocatable, target, save :: uxyz(:), vxyz(:), wxyz(:), Txyz(:)
+  real(c_double), allocatable, target, save :: uxyz_old(:), vxyz_old(:), wxyz_old(:), Txyz_old(:)
+  real(c_double), allocatable, target, save :: uxyzt(:), vxyzt(:), wxyzt(:), Txyzt(:), meshes_p(:), dila(:)
+
+  interface
+    integer(c_int32_t) function nimh_initial_state(cs, T_hot, T_cold, amp, seed, fu, fv, fw, fp, fT, &
+        u, v, w, T) bind(C, name="nimh_initial_state")
+      import :: c_int32_t, c_double, c_int64_t, nim_geo_case
+      type(nim_geo_case), intent(in) :: cs
+      real(c_double), value :: T_hot, T_cold, amp
+      integer(c_int64_t), value :: seed
+      real(c_double), intent(out) :: fu(*), fv(*), fw(*), fp(*), fT(*), u(*), v(*), w(*), T(*)
+    end function
+  end interface
+
+contains
+
+  subroutine check(rc, what)
+    integer(c_int32_t), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc /= NIM_OK) then
+      print '(A, A, I0, A, A)', what, ' failed with code ', rc, ': ', nim_error_message()
+      error stop 1
+    endif
+  end subroutine
+
+  function state_ptrs() result(st)
+    type(nim_state) :: st
+    st%faces_u = c_loc(faces_u); st%faces_v = c_loc(faces_v); st%faces_w = c_loc(faces_w)
+    st%faces_p = c_loc(faces_p); st%faces_T = c_loc(faces_T)
+    st%uxyz = c_loc(uxyz); st%vxyz = c_loc(vxyz); st%wxyz = c_loc(wxyz); st%Txyz = c_loc(Txyz)
+    st%uxyz_old = c_loc(uxyz_old); st%vxyz_old = c_loc(vxyz_old); st%wxyz_old = c_loc(wxyz_old)
+    st%Txyz_old = c_loc(Txyz_old); st%uxyzt = c_loc(uxyzt); st%vxyzt = c_loc(vxyzt); st%wxyzt = c_loc(wxyzt)
+    st%Txyzt = c_loc(Txyzt); st%meshes_p = c_loc(meshes_p); st%meshes_dila = c_loc(dila)
+  end function
+
+  ! initialize_nse (nse.f90:89-112) + apply_all_dirichlet_bcs (nse.f90:38)
+  subroutine nim_setup(cfg)
+    type(nim_run_config), intent(in) :: cfg
+    type(nim_dims) :: dims
+    type(nim_params) :: prm
+    type(nim_geometry) :: g
+    real(c_double), allocatable, target :: volume(:), xi_x(:,:,:), gij(:,:,:), gij_faces(:,:,:,:), G1(:), G2(:), G3(:)
+    real(c_double), allocatable, target :: area(:), nrm(:,:), tan1(:,:), tan2(:,:), mid(:,:), bc_a(:,:), nodes(:,:)
+    integer(c_int32_t), allocatable, target :: mfaces(:,:), nbrs(:,:), bcf(:), bcp(:), bcm(:), bcpts(:,:)
+    integer :: nt, nr, nz
+    nt = cfg%geo%n_theta; nr = cfg%geo%n_r; nz = cfg%geo%n_z
+    N = nt*nr*nz; NF = 3*N + nt*nz + nt*nr; NB = 2*nt*nr + 2*nt*nz
+    allocate(volume(N), xi_x(3,3,N), gij(3,3,N), gij_faces(3,3,6,N), G1(N), G2(N), G3(N))
+    allocate(area(NF), nrm(3,NF), tan1(3,NF), tan2(3,NF), mid(3,NF), bc_a(20,NB), nodes(3,nt*(nr+1)*(nz+1)))
+    allocate(mfaces(6,N), nbrs(6,N), bcf(NB), bcp(NB), bcm(NB), bcpts(5,NB))
+    call build_geometry(cfg%geo, volume, xi_x, gij, gij_faces, G1, G2, G3, area, nrm, tan1, tan2, mid, &
+                        mfaces, nbrs, bcf, bcp, bcm, bcpts, bc_a, nodes)
+    dims = nim_dims(nt, nr, nz)
+    ! parameters.f90:166-173
+    prm%nu = 1./sqrt(cfg%Ra/(cfg%g_beta*cfg%th_tc*cfg%Pr))
+    prm%Re = 1./prm%nu
+    prm%alpha = prm%nu/cfg%Pr
+    prm%rho = cfg%rho; prm%g_beta = cfg%g_beta; prm%DT = cfg%DT
+    prm%w_u = cfg%relax; prm%w_v = cfg%relax; prm%w_w = cfg%relax; prm%w_p = cfg%relax
+    prm%w_p2 = cfg%relax; prm%w_T = cfg%relax
+    prm%v_limit = cfg%v_limit; prm%R_i = cfg%geo%R_i; prm%R_o = cfg%geo%R_o
+    call check(nim_ctx_create(dims, prm, ctx), 'nim_ctx_create')
+    g%volume = c_loc(volume); g%xi_x = c_loc(xi_x); g%gij = c_loc(gij); g%gij_faces = c_loc(gij_faces)
+    g%G1 = c_loc(G1); g%G2 = c_loc(G2); g%G3 = c_loc(G3); g%faces_area = c_loc(area)
+    g%faces_normal = c_loc(nrm); g%faces_mid = c_loc(mid); g%bc_points = c_loc(bcpts); g%bc_a = c_loc(bc_a)
+    call check(nim_upload_geometry(ctx, g), 'nim_upload_geometry')
+    call check(nim_build_pressure_coeffs(ctx), 'nim_build_pressure_coeffs')
+    allocate(faces_u(NF), faces_v(NF), faces_w(NF), faces_p(NF), faces_T(NF))
+    allocate(uxyz(N), vxyz(N), wxyz(N), Txyz(N), uxyz_old(N), vxyz_old(N), wxyz_old(N), Txyz_old(N))
+    allocate(uxyzt(N), vxyzt(N), wxyzt(N), Txyzt(N), meshes_p(N), dila(N))
+    call check(nimh_initial_state(cfg%geo, cfg%T_hot, cfg%T_cold, cfg%noise, cfg%seed, faces_u, faces_v, &
+                                  faces_w, faces_p, faces_T, uxyz, vxyz, wxyz, Txyz), 'nimh_initial_state')
+    uxyz_old = uxyz; vxyz_old = vxyz; wxyz_old = wxyz; Txyz_old = Txyz
+    uxyzt = uxyz; vxyzt = vxyz; wxyzt = wxyz; Txyzt = Txyz
+    meshes_p = 0; dila = 0
+    call check(nim_upload_state(ctx, state_ptrs()), 'nim_upload_state')
+    TI = cfg%DT   ! nse.f90:31-32
+    STEP = 1
+    DTs = cfg%DT
+  end subroutine
+
+  ! the time loop of solve_nse (nse.f90:47-75); returns the steps run
+  integer function nim_solve(cfg, nsteps, restart_every, restart_prefix) result(done)
+    type(nim_run_config), intent(in) :: cfg
+    integer, intent(in) :: nsteps, restart_every
+    character(len=*), intent(in) :: restart_prefix
+    integer :: l
+    integer(c_int32_t) :: iters, rc
+    real(c_double) :: err(5), dudt, dTdt, dil, nu(5)
+    character(len=32) :: tag
+    done = 0
+    do l = 1, nsteps
+      call check(nim_build_momentum_coeffs(ctx), 'nim_build_momentum_coeffs')
+      ! the reference's *_old arrays start at zero, so its first pass always
+      ! sees the Dirichlet wall values as an increment: at least 2 passes
+      rc = nim_inner_solve(ctx, cfg%tol, int(cfg%max_iter, c_int32_t), merge(2_c_int32_t, 0_c_int32_t, l == 1), &
+                           iters, err)
+      call check(rc, 'nim_inner_solve')
+      call check(nim_end_step(ctx, dudt, dTdt, dil), 'nim_end_step')
+      print '(A, I6, A, I6, A, 1p, E11.4, A, E11.4, A, E11.4, A, E11.4, A, E11.4, A, E11.4)', 'step #', STEP, &
+            '  | # iters. ', iters, '  | err V=', max(err(1), err(2), err(3)), '  | err p=', err(4), &
+            '  | err T=', err(5), '  | D=', dil, '  | dudt=', dudt, '  | dTdt=', dTdt
+      if (restart_every > 0) then
+        if (mod(STEP, restart_every) == 0) then
+          write(tag, '(I0)') STEP
+          call nim_write_restart(trim(restart_prefix)//trim(tag))
+        endif
+      endif
+      if (cfg%check_steady .and. dudt < cfg%dudt_epsilon .and. dTdt < cfg%dudt_epsilon) then
+        print '(A, 1p, E11.4)', 'Steady state is reached, dudt = ', dudt
+        if (restart_every > 0) then
+          write(tag, '(I0)') STEP
+          call nim_write_restart(trim(restart_prefix)//trim(tag))
+        endif
+        done = done + 1
+        exit
+      endif
+      ! update_time_step (nse.f90:1294-1296) and find_Nu2
+      STEP = STEP + 1
+      TI = TI + 2*DTs
+      call check(nim_advance_time(ctx), 'nim_advance_time')
+      call check(nim_nusselt(ctx, nu), 'nim_nusselt')
+      print '(A, 1p, E24.16, A, E24.16, A, E24.16)', 'Nu_H=', nu(1), ' |  Nu_C=', nu(2), ' | Average = ', nu(3)
+      done = done + 1
+    enddo
+  end function
+
+  ! write_restart (restart.f90:8-39): same records, same order
+  subroutine nim_write_restart(path)
+    character(len=*), intent(in) :: path
+    integer :: u
+    call check(nim_download_state(ctx, state_ptrs()), 'nim_download_state')
+    open(newunit=u, file=path, form='unformatted', status='replace')
+    write(u) TI
+    write(u) STEP
+    write(u) faces_u
+    write(u) faces_v
+    write(u) faces_w
+    write(u) faces_p
+    write(u) faces_T
+    write(u) uxyz
+    write(u) vxyz
+    write(u) wxyz
+    write(u) Txyz
+    write(u) uxyz_old
+    write(u) vxyz_old
+    write(u) wxyz_old
+    write(u) Txyz_old
+    write(u) dila
+    close(u)
+  end subroutine
+
+  ! read_restart (restart.f90:42-69) followed by update_time_step as
+  ! initialize_nse does (nse.f90:104-108)
+  subroutine nim_read_restart(path)
+    character(len=*), intent(in) :: path
+    integer :: u
+    call check(nim_download_state(ctx, state_ptrs()), 'nim_download_state')
+    open(newunit=u, file=path, form='unformatted', status='old')
+    read(u) TI
+    read(u) STEP
+    read(u) faces_u
+    read(u) faces_v
+    read(u) faces_w
+    read(u) faces_p
+    read(u) faces_T
+    read(u) uxyz
+    read(u) vxyz
+    read(u) wxyz
+    read(u) Txyz
+    read(u) uxyz_old
+    read(u) vxyz_old
+    read(u) wxyz_old
+    read(u) Txyz_old
+    read(u) dila
+    close(u)
+    call check(nim_upload_state(ctx, state_ptrs()), 'nim_upload_state')
+    STEP = STEP + 1
+    TI = TI + 2*DTs
+    call check(nim_advance_time(ctx), 'nim_advance_time')
+  end subroutine
+
+  subroutine nim_finish()
+    integer(c_int32_t) :: rc
+    if (c_associated(ctx)) rc = nim_ctx_destroy(ctx)
+    ctx = c_null_ptr
+  end subroutine
+
+end module nim_nse

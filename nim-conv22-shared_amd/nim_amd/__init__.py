@@ -101,6 +101,8 @@ def lib():
         L.nim_set_array.argtypes = [vp, ctypes.c_char_p, _P, ctypes.c_int64]
         L.nim_stream.restype = vp
         L.nim_stream.argtypes = [vp]
+        L.nim_kernel_times.restype = i32
+        L.nim_kernel_times.argtypes = [vp, i32, _P, _PI]
         L.nim_last_error.restype = ctypes.c_char_p
         L.nim_last_error.argtypes = []
         _lib = L
@@ -112,7 +114,8 @@ def exported_symbols():
     return ["nim_counts", "nim_face_map", "nim_ctx_create", "nim_ctx_destroy", "nim_upload_geometry",
             "nim_build_pressure_coeffs", "nim_upload_state", "nim_download_state", "nim_build_momentum_coeffs",
             "nim_inner_solve", "nim_end_step", "nim_advance_time", "nim_nusselt", "nim_run_stage",
-            "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_last_error"]
+            "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_kernel_times",
+            "nim_last_error"]
 
 
 def _check(rc):
@@ -257,6 +260,17 @@ class Context:
 
     def stream(self):
         return self.L.nim_stream(self.h)
+
+    KERNEL_CLASSES = ("velocity_sources", "velocity_sweep", "pressure_sources", "pressure_sweep",
+                      "energy_constants", "energy_coefficients", "energy_sources", "energy_sweep", "check")
+
+    def kernel_times(self, iters):
+        """Per-class mean ms per outer iteration and launches per iteration,
+        from HIP events around every launch (advances the state)."""
+        ms = np.zeros(9)
+        nl = np.zeros(9, dtype=np.int32)
+        _check(self.L.nim_kernel_times(self.h, iters, _dp(ms), nl.ctypes.data_as(_PI)))
+        return {k: (float(ms[q]), int(nl[q])) for q, k in enumerate(self.KERNEL_CLASSES)}
 
     def time_step(self, tol, max_iter=100000, min_iter=0):
         """One step of solve_nse (nse.f90:49-73) through the C-ABI."""

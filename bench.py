@@ -31,11 +31,11 @@ from nim_amd import host  # noqa: E402
 
 HBM_PEAK = 8.0e12
 # Algorithmic FP64 bytes per cell moved by each kernel class (each array
-# touched once, neighbour reuse on chip); derivation in DESIGN.md.
+# touched once, neighbour reuse on chip): the per-cell arrays each kernel
+# reads and writes, counted from its source.
 BYTES_PER_CELL = {
     "velocity_sources": 36 * 8, "velocity_sweep": 59 * 8, "pressure_sources": 36 * 8, "pressure_sweep": 52 * 8,
-    "energy_constants": 42 * 8, "energy_coefficients": 41 * 8, "energy_sources": 13 * 8, "energy_sweep": 39 * 8,
-    "check": 0,
+    "energy_coefficients": 47 * 8, "unused": 0, "energy_sources": 13 * 8, "energy_sweep": 39 * 8, "check": 0,
 }
 PATH_BYTES_PER_CELL_UPDATE = 1200  # 150 doubles per cell and outer iteration
 
@@ -105,7 +105,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs one case per device and has no multi-GPU "
-              "path (see DESIGN.md for how a case would shard)", file=sys.stderr)
+              "path (one case lives on one device)", file=sys.stderr)
         return 2
 
     case = host.baseline_case(args.case)
@@ -140,7 +140,7 @@ def main():
     # per-kernel HIP-event timing of the same outer iteration, on the
     # context's stream, continuing from the benchmarked state
     kt = ctx.kernel_times(args.profile_iters)
-    dom = max((k for k in kt if k != "check"), key=lambda k: kt[k][0])
+    dom = max((k for k in kt if kt[k][1] > 0 and k != "check"), key=lambda k: kt[k][0])
     ms_iter, launches = kt[dom]
     per_launch_s = ms_iter / launches * 1e-3
     cells_per_launch = N / launches
@@ -161,8 +161,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded 1e-3 noise on cell averages)",
         "config": {"workload": "%s: %d x %d x %d (theta x r x z) = %d cells, Ra=%g, Pr=%g, DT=%g, FP64, "
-                               "tol %g" % (args.case, case.dims[0], case.dims[1], case.dims[2], N, case.Ra, case.Pr,
-                                           case.DT, args.tol),
+                               "tol %g (p %g)" % (args.case, case.dims[0], case.dims[1], case.dims[2], N, case.Ra,
+                                                  case.Pr, case.DT, args.tol, args.tol_p),
                    "cells": N, "precision": "fp64", "sweep_order": "4-colour"},
         "steps_per_s": args.steps / wall,
         "inner_iterations": total_iters,

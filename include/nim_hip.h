@@ -157,7 +157,8 @@ int32_t nim_set_array(nim_ctx *ctx, const char *name, const double *in, int64_t 
  * launches per iteration (launches). Classes, in launch order:
  *   0 velocity sources   1 velocity sweep (4 colour launches)
  *   2 pressure sources   3 pressure walls + sweep (4 colour launches)
- *   4 energy constants   5 energy coefficients + adiabatic walls
+ *   4 energy coefficients and adiabatic walls (one fused launch)
+ *   5 unused (kept so class numbers stay stable)
  *   6 energy sources     7 energy sweep (4 colour launches)
  *   8 convergence check
  * nim_stream returns the hipStream_t the context launches on. */

@@ -16,6 +16,7 @@
 namespace nim {
 __global__ void k_coef_A(Dev d, int energy);
 __global__ void k_coef_F(Dev d, int energy);
+__global__ void k_T_coef(Dev d);
 __global__ void k_mom_src(Dev d);
 __global__ void k_ppe_src(Dev d);
 __global__ void k_T_src(Dev d);
@@ -99,20 +100,21 @@ extern "C" int32_t nim_face_map(const nim_dims *d, int32_t *out)
     const int64_t o1 = N, o2 = o1 + NT * (NR - 1) * NZ, o3 = o2 + NT * NR * (NZ - 1), o4 = o3 + NT * NR;
     const int64_t o5 = o4 + NT * NR, o6 = o5 + NT * NZ;
     const int64_t oT = NT * (NR + 1) * NZ, oZ = oT + N;
+    auto P = [&](int64_t i) { return (i & 1) * (NT / 2) + (i >> 1); };
     for (int64_t k = 0; k < NZ; k++)
         for (int64_t j = 0; j <= NR; j++)
             for (int64_t i = 0; i < NT; i++) {
-                int64_t s = i + NT * (j + (NR + 1) * k), r;
+                int64_t s = P(i) + NT * (j + (NR + 1) * k), r;
                 if (j == 0) r = o5 + i + NT * k;
                 else if (j == NR) r = o6 + i + NT * k;
                 else r = o1 + i + NT * ((j - 1) + (NR - 1) * k);
                 out[s] = (int32_t)r;
             }
-    for (int64_t c = 0; c < N; c++) out[oT + c] = (int32_t)c;
+    for (int64_t c = 0; c < N; c++) out[oT + P(c % NT) + (c / NT) * NT] = (int32_t)c;
     for (int64_t k = 0; k <= NZ; k++)
         for (int64_t j = 0; j < NR; j++)
             for (int64_t i = 0; i < NT; i++) {
-                int64_t s = oZ + i + NT * (j + NR * k), r;
+                int64_t s = oZ + P(i) + NT * (j + NR * k), r;
                 if (k == 0) r = o3 + i + NT * j;
                 else if (k == NZ) r = o4 + i + NT * j;
                 else r = o2 + i + NT * (j + NR * (k - 1));
@@ -242,18 +244,25 @@ static void wall_of(const nim_ctx *c, int b, int &pos, int &i, int &j, int &k)
     else { pos = 2; i = (b - 2 * NT * NR - NT * NZ) % NT; k = (b - 2 * NT * NR - NT * NZ) / NT; j = NR - 1; }
 }
 
+// storage position of azimuthal index i (even i first, then odd i)
+static inline int hpos(const nim_ctx *c, int i) { return (i & 1) * (c->NT / 2) + (i >> 1); }
+
+// device slot of face `pos` of cell (i,j,k); must agree with fidx()
 static int hslot(const nim_ctx *c, int pos, int i, int j, int k)
 {
     const int NT = c->NT, NR = c->NR;
     switch (pos) {
-    case 1: return i + NT * (j + (NR + 1) * k);
-    case 2: return i + NT * (j + 1 + (NR + 1) * k);
-    case 3: return c->oT + i + NT * (j + NR * k);
-    case 4: return c->oT + (i + 1) % NT + NT * (j + NR * k);
-    case 5: return c->oZ + i + NT * (j + NR * k);
-    default: return c->oZ + i + NT * (j + NR * (k + 1));
+    case 1: return hpos(c, i) + NT * (j + (NR + 1) * k);
+    case 2: return hpos(c, i) + NT * (j + 1 + (NR + 1) * k);
+    case 3: return c->oT + hpos(c, i) + NT * (j + NR * k);
+    case 4: return c->oT + hpos(c, (i + 1) % NT) + NT * (j + NR * k);
+    case 5: return c->oZ + hpos(c, i) + NT * (j + NR * k);
+    default: return c->oZ + hpos(c, i) + NT * (j + NR * (k + 1));
     }
 }
+
+// device index of the reference's (natural-order) cell n
+static inline int hcell(const nim_ctx *c, int n) { return hpos(c, n % c->NT) + (n / c->NT) * c->NT; }
 
 extern "C" int32_t nim_upload_geometry(nim_ctx *c, const nim_geometry *g)
 {
@@ -267,32 +276,33 @@ extern "C" int32_t nim_upload_geometry(nim_ctx *c, const nim_geometry *g)
     auto GIJ = [&](int n, int a, int b) { return g->gij[(size_t)n * 9 + (a - 1) + 3 * (b - 1)]; };
     for (int n = 0; n < N; n++) {
         int i = n % NT, j = (n / NT) % NR, k = n / (NT * NR);
-        gc[(size_t)G_INVV * N + n] = 1. / g->volume[n];
+        const int dn = hcell(c, n);
+        gc[(size_t)G_INVV * N + dn] = 1. / g->volume[n];
         for (int a = 1; a <= 3; a++)
             for (int b = 1; b <= 3; b++)
-                gc[(size_t)(G_XI + 3 * (a - 1) + (b - 1)) * N + n] = g->xi_x[(size_t)n * 9 + (a - 1) + 3 * (b - 1)];
-        for (int a = 1; a <= 3; a++) gc[(size_t)(G_GD + a - 1) * N + n] = GIJ(n, a, a);
+                gc[(size_t)(G_XI + 3 * (a - 1) + (b - 1)) * N + dn] = g->xi_x[(size_t)n * 9 + (a - 1) + 3 * (b - 1)];
+        for (int a = 1; a <= 3; a++) gc[(size_t)(G_GD + a - 1) * N + dn] = GIJ(n, a, a);
         const int go[6][2] = {{1, 2}, {2, 1}, {1, 3}, {3, 1}, {2, 3}, {3, 2}};
-        for (int q = 0; q < 6; q++) gc[(size_t)(G_GO + q) * N + n] = GIJ(n, go[q][0], go[q][1]);
-        gc[(size_t)(G_G + 0) * N + n] = g->G1[n];
-        gc[(size_t)(G_G + 1) * N + n] = g->G2[n];
-        gc[(size_t)(G_G + 2) * N + n] = g->G3[n];
-        for (int p = 1; p <= 6; p++) gc[(size_t)(G_GF + p - 1) * N + n] = GF(n, (p + 1) / 2, (p + 1) / 2, p);
+        for (int q = 0; q < 6; q++) gc[(size_t)(G_GO + q) * N + dn] = GIJ(n, go[q][0], go[q][1]);
+        gc[(size_t)(G_G + 0) * N + dn] = g->G1[n];
+        gc[(size_t)(G_G + 1) * N + dn] = g->G2[n];
+        gc[(size_t)(G_G + 2) * N + dn] = g->G3[n];
+        for (int p = 1; p <= 6; p++) gc[(size_t)(G_GF + p - 1) * N + dn] = GF(n, (p + 1) / 2, (p + 1) / 2, p);
         double *df = &gc[(size_t)G_DIFF * N];
         if (j < NR - 1) {
             int n2 = n + NT;
-            df[0 * (size_t)N + n] = GF(n2, 1, 2, 1) - GF(n, 1, 2, 2);
-            df[1 * (size_t)N + n] = GF(n2, 1, 3, 1) - GF(n, 1, 3, 2);
+            df[0 * (size_t)N + dn] = GF(n2, 1, 2, 1) - GF(n, 1, 2, 2);
+            df[1 * (size_t)N + dn] = GF(n2, 1, 3, 1) - GF(n, 1, 3, 2);
         }
         {
             int n2 = (i + 1) % NT + NT * (j + NR * k);
-            df[2 * (size_t)N + n] = GF(n2, 2, 1, 3) - GF(n, 2, 1, 4);
-            df[3 * (size_t)N + n] = GF(n2, 2, 3, 3) - GF(n, 2, 3, 4);
+            df[2 * (size_t)N + dn] = GF(n2, 2, 1, 3) - GF(n, 2, 1, 4);
+            df[3 * (size_t)N + dn] = GF(n2, 2, 3, 3) - GF(n, 2, 3, 4);
         }
         if (k < NZ - 1) {
             int n2 = n + NT * NR;
-            df[4 * (size_t)N + n] = GF(n2, 3, 1, 5) - GF(n, 3, 1, 6);
-            df[5 * (size_t)N + n] = GF(n2, 3, 2, 5) - GF(n, 3, 2, 6);
+            df[4 * (size_t)N + dn] = GF(n2, 3, 1, 5) - GF(n, 3, 1, 6);
+            df[5 * (size_t)N + dn] = GF(n2, 3, 2, 5) - GF(n, 3, 2, 6);
         }
     }
     for (int s = 0; s < NF; s++) {
@@ -438,16 +448,20 @@ static int32_t move_array(nim_ctx *c, const char *name, double *host, int64_t co
         if (a.kind == 1) {
             for (int s = 0; s < rows; s++) host[c->slot_ref[s]] = buf[s];
         } else {
-            for (int r = 0; r < rows; r++)
-                for (int m = 0; m < a.mult; m++) host[(size_t)r * a.mult + m] = buf[(size_t)m * rows + r];
+            for (int r = 0; r < rows; r++) {
+                int dr = a.kind == 0 ? hcell(c, r) : r;
+                for (int m = 0; m < a.mult; m++) host[(size_t)r * a.mult + m] = buf[(size_t)m * rows + dr];
+            }
         }
         return NIM_OK;
     }
     if (a.kind == 1) {
         for (int s = 0; s < rows; s++) buf[s] = host[c->slot_ref[s]];
     } else {
-        for (int r = 0; r < rows; r++)
-            for (int m = 0; m < a.mult; m++) buf[(size_t)m * rows + r] = host[(size_t)r * a.mult + m];
+        for (int r = 0; r < rows; r++) {
+            int dr = a.kind == 0 ? hcell(c, r) : r;
+            for (int m = 0; m < a.mult; m++) buf[(size_t)m * rows + dr] = host[(size_t)r * a.mult + m];
+        }
     }
     return h2d(c, a.dev, buf.data(), buf.size() * 8);
 }
@@ -495,7 +509,12 @@ extern "C" int32_t nim_build_momentum_coeffs(nim_ctx *c)
 }
 
 // ---- one outer iteration ---------------------------------------------
-static int color_blocks(const nim_ctx *c) { return nblk((c->NT / 2) * ((c->NR + 1) / 2) * c->NZ); }
+static int color_cells(const nim_ctx *c) { return (c->NT / 2) * ((c->NR + 1) / 2) * c->NZ; }
+static int color_blocks(const nim_ctx *c) { return nblk(color_cells(c)); }
+// momentum kernels: 64 cells per block, one wave per velocity component
+static const dim3 BS3(64, 3);
+static int color_blocks3(const nim_ctx *c) { return (color_cells(c) + 63) / 64; }
+static int cell_blocks3(const nim_ctx *c) { return (c->N + 63) / 64; }
 
 static void launch_stage(nim_ctx *c, int stage)
 {
@@ -503,14 +522,13 @@ static void launch_stage(nim_ctx *c, int stage)
     hipStream_t s = c->stream;
     int gN = nblk(c->N), gC = color_blocks(c);
     switch (stage) {
-    case 0: hipLaunchKernelGGL(k_mom_src, dim3(gN), dim3(BS), 0, s, d); break;
-    case 1: for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_mom_sweep, dim3(gC), dim3(BS), 0, s, d, q); break;
+    case 0: hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); break;
+    case 1:
+        for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_mom_sweep, dim3(color_blocks3(c)), BS3, 0, s, d, q);
+        break;
     case 2: hipLaunchKernelGGL(k_ppe_src, dim3(gN), dim3(BS), 0, s, d); break;
     case 3: for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_ppe_sweep, dim3(gC), dim3(BS), 0, s, d, q); break;
-    case 4:
-        hipLaunchKernelGGL(k_coef_A, dim3(gN), dim3(BS), 0, s, d, 1);
-        hipLaunchKernelGGL(k_coef_F, dim3(gN), dim3(BS), 0, s, d, 1);
-        break;
+    case 4: hipLaunchKernelGGL(k_T_coef, dim3(gN), dim3(BS), 0, s, d); break;
     case 5: hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); break;
     case 6: for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_T_sweep, dim3(gC), dim3(BS), 0, s, d, q); break;
     }
@@ -642,11 +660,11 @@ extern "C" int32_t nim_kernel_times(nim_ctx *c, int32_t iters, double *ms, int32
     hipStream_t s = c->stream;
     const int gN = nblk(c->N), gC = color_blocks(c);
     // launch list of one iteration: (class, kernel index)
-    const int cls[18] = {0, 1, 1, 1, 1, 2, 3, 3, 3, 3, 4, 5, 6, 7, 7, 7, 7, 8};
+    const int cls[17] = {0, 1, 1, 1, 1, 2, 3, 3, 3, 3, 4, 6, 7, 7, 7, 7, 8};
     const double tol[5] = {0, 0, 0, 0, 0};
     int32_t rc = reset_ctl(c, tol, 1 << 30, 1 << 30);
     if (rc) return rc;
-    std::vector<hipEvent_t> ev(19);
+    std::vector<hipEvent_t> ev(18);
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     for (int q = 0; q < NIM_KERNEL_CLASSES; q++) {
         ms[q] = 0.;
@@ -656,18 +674,20 @@ extern "C" int32_t nim_kernel_times(nim_ctx *c, int32_t iters, double *ms, int32
         int n = 0;
         HIPCHK(hipEventRecord(ev[0], s));
         auto mark = [&]() { (void)hipEventRecord(ev[++n], s); };
-        hipLaunchKernelGGL(k_mom_src, dim3(gN), dim3(BS), 0, s, d); mark();
-        for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_mom_sweep, dim3(gC), dim3(BS), 0, s, d, q); mark(); }
+        hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); mark();
+        for (int q = 0; q < 4; q++) {
+            hipLaunchKernelGGL(k_mom_sweep, dim3(color_blocks3(c)), BS3, 0, s, d, q);
+            mark();
+        }
         hipLaunchKernelGGL(k_ppe_src, dim3(gN), dim3(BS), 0, s, d); mark();
         for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_ppe_sweep, dim3(gC), dim3(BS), 0, s, d, q); mark(); }
-        hipLaunchKernelGGL(k_coef_A, dim3(gN), dim3(BS), 0, s, d, 1); mark();
-        hipLaunchKernelGGL(k_coef_F, dim3(gN), dim3(BS), 0, s, d, 1); mark();
+        hipLaunchKernelGGL(k_T_coef, dim3(gN), dim3(BS), 0, s, d); mark();
         hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); mark();
         for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_T_sweep, dim3(gC), dim3(BS), 0, s, d, q); mark(); }
         hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, d); mark();
         HIPCHK(hipGetLastError());
-        HIPCHK(hipEventSynchronize(ev[18]));
-        for (int q = 0; q < 18; q++) {
+        HIPCHK(hipEventSynchronize(ev[17]));
+        for (int q = 0; q < 17; q++) {
             float t = 0.f;
             HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
             ms[cls[q]] += t;

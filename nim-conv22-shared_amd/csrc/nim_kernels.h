@@ -1,12 +1,15 @@
 // Device-side data layout and helpers of the NIM time-step kernels (gfx950).
 //
 // Layout in HBM (structure of arrays, FP64):
-//  * cells: c = i + NT*(j + NR*k), i azimuthal (periodic), j radial, k axial
-//    -- the reference's natural cell order (create_geometry.f90:449-473).
+//  * cells: c = P(i) + NT*(j + NR*k), i azimuthal (periodic), j radial, k
+//    axial, P(i) = (i mod 2)*NT/2 + i/2: the reference's natural order
+//    (create_geometry.f90:449-473) with each azimuthal row split into its
+//    even and odd cells, so that every colour phase of a sweep reads and
+//    writes contiguous half-rows.
 //  * faces of one field: three families in one buffer of NF doubles
-//      radial  R(i,j,k), j in [0,NR]   at i + NT*(j + (NR+1)*k)
-//      azimuthal T(i,j,k)              at oT + i + NT*(j + NR*k)
-//      axial   Z(i,j,k), k in [0,NZ]   at oZ + i + NT*(j + NR*k)
+//      radial  R(i,j,k), j in [0,NR]   at P(i) + NT*(j + (NR+1)*k)
+//      azimuthal T(i,j,k)              at oT + P(i) + NT*(j + NR*k)
+//      axial   Z(i,j,k), k in [0,NZ]   at oZ + P(i) + NT*(j + NR*k)
 //    Cell-local face p (reference frame, mesh.f90:65-69): 1 = R(i,j), 2 =
 //    R(i,j+1), 3 = T(i), 4 = T(i+1 mod NT), 5 = Z(k), 6 = Z(k+1).
 //  * per-cell arrays grouped as [slot][N] blocks (one allocation per group).
@@ -89,17 +92,25 @@ struct Dev {
 
 __device__ __forceinline__ int ip1(const Dev &d, int i) { return i + 1 == d.NT ? 0 : i + 1; }
 __device__ __forceinline__ int im1(const Dev &d, int i) { return i == 0 ? d.NT - 1 : i - 1; }
-__device__ __forceinline__ int cidx(const Dev &d, int i, int j, int k) { return i + d.NT * (j + d.NR * k); }
+// storage position of azimuthal index i within its row: even i first, then
+// odd i, so that the cells (and faces) a colour phase touches are contiguous
+__device__ __forceinline__ int ipos(const Dev &d, int i) { return (i & 1) * (d.NT >> 1) + (i >> 1); }
+__device__ __forceinline__ int iof(const Dev &d, int p)
+{
+    int h = d.NT >> 1;
+    return p < h ? 2 * p : 2 * (p - h) + 1;
+}
+__device__ __forceinline__ int cidx(const Dev &d, int i, int j, int k) { return ipos(d, i) + d.NT * (j + d.NR * k); }
 
 __device__ __forceinline__ int fidx(const Dev &d, int pos, int i, int j, int k)
 {
     switch (pos) {
-    case 1: return i + d.NT * (j + (d.NR + 1) * k);
-    case 2: return i + d.NT * (j + 1 + (d.NR + 1) * k);
-    case 3: return d.oT + i + d.NT * (j + d.NR * k);
-    case 4: return d.oT + ip1(d, i) + d.NT * (j + d.NR * k);
-    case 5: return d.oZ + i + d.NT * (j + d.NR * k);
-    default: return d.oZ + i + d.NT * (j + d.NR * (k + 1));
+    case 1: return ipos(d, i) + d.NT * (j + (d.NR + 1) * k);
+    case 2: return ipos(d, i) + d.NT * (j + 1 + (d.NR + 1) * k);
+    case 3: return d.oT + ipos(d, i) + d.NT * (j + d.NR * k);
+    case 4: return d.oT + ipos(d, ip1(d, i)) + d.NT * (j + d.NR * k);
+    case 5: return d.oZ + ipos(d, i) + d.NT * (j + d.NR * k);
+    default: return d.oZ + ipos(d, i) + d.NT * (j + d.NR * (k + 1));
     }
 }
 

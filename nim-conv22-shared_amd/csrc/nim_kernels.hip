@@ -154,7 +154,7 @@ __device__ void nim_direction(double u, double k, double gp, double gm, double v
 // exponential-fit constants of every cell: momentum (time-centred velocity,
 // nu; nim.f90:166-290) or energy (mean of current iterate and previous
 // level, alpha; nim.f90:339-463)
-__global__ void k_coef_A(Dev d, int energy)
+__global__ void __launch_bounds__(256) k_coef_A(Dev d, int energy)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (energy && d.ctl->done) return;
@@ -188,7 +188,7 @@ __global__ void k_coef_A(Dev d, int energy)
 // equation the same kernel also applies the adiabatic top/bottom wall
 // extrapolation apply_zero_flux (bc.f90:345-390), which reads only interior
 // axial faces
-__global__ void k_coef_F(Dev d, int energy)
+__global__ void __launch_bounds__(256) k_coef_F(Dev d, int energy)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (energy && d.ctl->done) return;
@@ -198,7 +198,7 @@ __global__ void k_coef_F(Dev d, int energy)
         const int N = d.N;
         const double *A = energy ? d.TA : d.MA;
         double *F = energy ? d.TF : d.MF;
-        int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+        int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
 #define AV(n, dir, m) A[((dir - 1) * 6 + (m) - 1) * N + (n)]
         double f2 = 0.5 * (1. / AV(c, 1, 5) + 1. / AV(c, 2, 5) + 1. / AV(c, 3, 5) - 1. / d.DT);
         double f3 = -(AV(c, 1, 6) / AV(c, 1, 5)), f4 = (AV(c, 1, 6) - 1.) / AV(c, 1, 5);
@@ -251,6 +251,90 @@ __global__ void k_coef_F(Dev d, int energy)
     if (energy) err_commit(d, F_T, eT);
 }
 
+// Energy coefficients of one outer iteration in a single pass
+// (calculate_temperature_nim_constants_formulation2, nim.f90:334-503, then
+// apply_zero_flux, bc.f90:345-390): each cell evaluates its own three
+// directions and, for the face coefficients F1..F3, the one direction it
+// needs of each + neighbour, so the 18 exponential-fit constants per cell
+// never leave registers.
+__device__ __forceinline__ void energy_direction(const Dev &d, int c, int dir, double A[6])
+{
+    const int N = d.N;
+    const double *g = d.gc;
+    double ux = 0.5 * (d.sc[(S_CUR + 0) * N + c] + d.sc[(S_OLD + 0) * N + c]);
+    double uy = 0.5 * (d.sc[(S_CUR + 1) * N + c] + d.sc[(S_OLD + 1) * N + c]);
+    double uz = 0.5 * (d.sc[(S_CUR + 2) * N + c] + d.sc[(S_OLD + 2) * N + c]);
+    double vel = ux * g[(G_XI + 3 * dir + 0) * N + c] + uy * g[(G_XI + 3 * dir + 1) * N + c]
+                 + uz * g[(G_XI + 3 * dir + 2) * N + c];
+    nim_direction(vel, g[(G_GD + dir) * N + c] * d.alpha, g[(G_GF + 2 * dir + 1) * N + c], g[(G_GF + 2 * dir) * N + c],
+                  d.v_limit, A);
+}
+
+__global__ void __launch_bounds__(256) k_T_coef(Dev d)
+{
+    if (d.ctl->done) return;
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = c < d.N;
+    double eT = 0.;
+    if (act) {
+        const int N = d.N;
+        double *F = d.TF;
+        int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+        double A[3][6];
+#pragma unroll
+        for (int dir = 0; dir < 3; dir++) energy_direction(d, c, dir, A[dir]);
+        double f2 = 0.5 * (1. / A[0][4] + 1. / A[1][4] + 1. / A[2][4] - 1. / d.DT);
+        double f3 = -(A[0][5] / A[0][4]), f4 = (A[0][5] - 1.) / A[0][4];
+        double f5 = -(A[1][5] / A[1][4]), f6 = (A[1][5] - 1.) / A[1][4];
+        double f7 = -(A[2][5] / A[2][4]), f8 = (A[2][5] - 1.) / A[2][4];
+        F[(C_F4 + 0) * N + c] = f2 + f3 + f4 + f5 + f6 + f7 + f8;
+        F[(C_F4 + 1) * N + c] = f2;
+        F[(C_F4 + 2) * N + c] = f3;
+        F[(C_F4 + 3) * N + c] = f4;
+        F[(C_F4 + 4) * N + c] = f5;
+        F[(C_F4 + 5) * N + c] = f6;
+        F[(C_F4 + 6) * N + c] = f7;
+        F[(C_F4 + 7) * N + c] = f8;
+#pragma unroll
+        for (int dir = 0; dir < 3; dir++) {
+            int c2;
+            if (dir == 0) { if (j == d.NR - 1) continue; c2 = c + d.NT; }
+            else if (dir == 1) c2 = cidx(d, ip1(d, i), j, k);
+            else { if (k == d.NZ - 1) continue; c2 = c + d.NT * d.NR; }
+            double B[6];
+            energy_direction(d, c2, dir, B);
+            double b = A[dir][2] - (A[dir][0] * A[dir][5] / A[dir][4]);
+            double e = B[3] - B[1] / B[4] + B[1] * B[5] / B[4];
+            double h = A[dir][0] / (2. * A[dir][4]);
+            double q = B[1] / (2. * B[4]);
+            const int s = dir == 0 ? C_F1 : dir == 1 ? C_F2 : C_F3;
+            F[(s + 0) * N + c] = b + e + 2. * h + 2. * q;
+            F[(s + 1) * N + c] = b;
+            F[(s + 2) * N + c] = e;
+            F[(s + 3) * N + c] = h;
+            F[(s + 4) * N + c] = q;
+        }
+        if (k == 0 || k == d.NZ - 1) {
+            double *T = d.fc + (size_t)F_T * d.NF;
+            int b, fw, fa, fb, fcc;
+            if (k == 0) {
+                b = i + d.NT * j;
+                fw = fidx(d, 5, i, j, 0); fa = fidx(d, 6, i, j, 0); fb = fidx(d, 6, i, j, 1); fcc = fidx(d, 6, i, j, 2);
+            } else {
+                b = d.NT * d.NR + i + d.NT * j;
+                fw = fidx(d, 6, i, j, k); fa = fidx(d, 5, i, j, k); fb = fidx(d, 5, i, j, k - 1); fcc = fidx(d, 5, i, j, k - 2);
+            }
+            const double *W = d.wall;
+            const int NB = d.NB;
+            double nv = (W[(W_ZF + 0) * NB + b] * T[fa] - W[(W_ZF + 1) * NB + b] * T[fb] + W[(W_ZF + 2) * NB + b] * T[fcc])
+                        / W[(W_ZF + 3) * NB + b];
+            eT = fabs(nv - T[fw]);
+            T[fw] = nv;
+        }
+    }
+    err_commit(d, F_T, eT);
+}
+
 // ---------------------------------------------------------------------
 // sources (computed from the state at the start of each stage)
 // ---------------------------------------------------------------------
@@ -259,39 +343,41 @@ __global__ void k_coef_F(Dev d, int energy)
 // (nse.f90:1393-1434) vanishes identically: meshes_u and uc are the same
 // expression of the same time-centred velocity (nim.f90:169-171 and
 // nse.f90:1422-1428), so it is not evaluated.
-__global__ void k_mom_src(Dev d)
+// One wave per field: threadIdx.y selects u, v or w, so the three waves of
+// a block work on the same 64 cells and share their metric loads in cache.
+__global__ void __launch_bounds__(192) k_mom_src(Dev d)
 {
     if (d.ctl->done) return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.x * 64 + threadIdx.x;
+    const int q = threadIdx.y;
     if (c >= d.N) return;
     const int N = d.N;
-    int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     int f[6];
     cell_faces(d, i, j, k, f);
     const double *P = d.fc + (size_t)F_P * d.NF;
     double pf[6];
+#pragma unroll
     for (int p = 0; p < 6; p++) pf[p] = P[f[p]];
     double invV = d.gc[G_INVV * N + c];
-    double bz = -(d.g_beta * 0.5 * (d.sc[(S_CUR + 3) * N + c] + d.sc[(S_OLD + 3) * N + c]));
-    double RB[3] = {0. + d.rinv * dphi(d, pf, f, 0, invV), 0. + d.rinv * dphi(d, pf, f, 1, invV),
-                    bz + d.rinv * dphi(d, pf, f, 2, invV)};
-    for (int q = 0; q < 3; q++) {
-        double RD = cross_diffusion(d, d.fc + (size_t)q * d.NF, i, j, k, c, d.Rinv);
-        d.src[q * N + c] = RD + RB[q];
-    }
+    double b = 0.;
+    if (q == 2) b = -(d.g_beta * 0.5 * (d.sc[(S_CUR + 3) * N + c] + d.sc[(S_OLD + 3) * N + c]));
+    double RB = b + d.rinv * dphi(d, pf, f, q, invV);
+    double RD = cross_diffusion(d, d.fc + (size_t)q * d.NF, i, j, k, c, d.Rinv);
+    d.src[q * N + c] = RD + RB;
 }
 
 // update_pressure_sources (nse.f90:228-235): calculate_R_b_p
 // (nse.f90:2362-2407; faces_bx = faces_by = 0, faces_bz = -g_beta*faces_T)
 // and calculate_R_D_p2; stores R_D_p + R_b_p, the combination the pressure
 // sweep and wall update use, and the dilatation
-__global__ void k_ppe_src(Dev d)
+__global__ void __launch_bounds__(256) k_ppe_src(Dev d)
 {
     if (d.ctl->done) return;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
     const int N = d.N;
-    int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     int f[6];
     cell_faces(d, i, j, k, f);
     double uf[6], vf[6], wf[6], bzf[6];
@@ -315,12 +401,12 @@ __global__ void k_ppe_src(Dev d)
 }
 
 // update_temperature_sources (nse.f90:249-256): R_b_T = q = 0, R_D_T
-__global__ void k_T_src(Dev d)
+__global__ void __launch_bounds__(256) k_T_src(Dev d)
 {
     if (d.ctl->done) return;
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
-    int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     d.src[R_T * d.N + c] = cross_diffusion(d, d.fc + (size_t)F_T * d.NF, i, j, k, c, d.alpha) + 0.;
 }
 
@@ -394,26 +480,24 @@ __device__ void transport_unit(const Dev &d, double *Fc, double *C, const double
     }
 }
 
-__global__ void __launch_bounds__(256) k_mom_sweep(Dev d, int color)
+// threadIdx.y selects the field (u, v, w are independent within the sweep)
+__global__ void __launch_bounds__(192) k_mom_sweep(Dev d, int color)
 {
     if (d.ctl->done) return;
     int i, j, k;
-    bool act = color_cell(d, color, blockIdx.x * blockDim.x + threadIdx.x, i, j, k);
-    double e[3] = {0., 0., 0.};
+    const int q = threadIdx.y;
+    bool act = color_cell(d, color, blockIdx.x * 64 + threadIdx.x, i, j, k);
+    double e = 0.;
     if (act) {
         const int N = d.N;
         int c = cidx(d, i, j, k);
-        const double wq[3] = {d.w_u, d.w_v, d.w_w};
-        for (int q = 0; q < 3; q++) {
-            double ec = 0., ef = 0.;
-            transport_unit(d, d.fc + (size_t)q * d.NF, d.sc + (size_t)(S_CUR + q) * N, d.sc + (size_t)(S_OLD + q) * N,
-                           d.MF, d.src[q * N + c], wq[q], i, j, k, c, ec, ef);
-            e[q] = fmax(ec, ef);
-        }
+        const double w = q == 0 ? d.w_u : q == 1 ? d.w_v : d.w_w;
+        double ec = 0., ef = 0.;
+        transport_unit(d, d.fc + (size_t)q * d.NF, d.sc + (size_t)(S_CUR + q) * N, d.sc + (size_t)(S_OLD + q) * N,
+                       d.MF, d.src[q * N + c], w, i, j, k, c, ec, ef);
+        e = fmax(ec, ef);
     }
-    err_commit(d, F_U, e[0]);
-    err_commit(d, F_V, e[1]);
-    err_commit(d, F_W, e[2]);
+    err_commit(d, q, e);
 }
 
 __global__ void __launch_bounds__(256) k_T_sweep(Dev d, int color)
@@ -567,7 +651,7 @@ __global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, int color)
 // ---------------------------------------------------------------------
 // iteration control
 // ---------------------------------------------------------------------
-__global__ void k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4, int max_iter, int min_iter)
+__global__ void __launch_bounds__(256) k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4, int max_iter, int min_iter)
 {
     int l = threadIdx.x;
     for (int q = l; q < 2 * 5 * ERR_COPIES; q += blockDim.x) d.errs[q] = 0ull;
@@ -586,7 +670,7 @@ __global__ void k_solve_reset(Dev d, double t0, double t1, double t2, double t3,
 // end of one outer iteration: reduce the increments, test convergence
 // (the while condition of integrate_nse, nse.f90:129), clear the slot the
 // next iteration uses
-__global__ void k_check(Dev d)
+__global__ void __launch_bounds__(64) k_check(Dev d)
 {
     Ctl *ctl = d.ctl;
     if (ctl->done) return;
@@ -638,13 +722,13 @@ __device__ PCoef pcoef(const Dev &d, int c)
     return p;
 }
 
-__global__ void k_pressure_coeffs(Dev d)
+__global__ void __launch_bounds__(256) k_pressure_coeffs(Dev d)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
     const int N = d.N;
     const double GG = 0.5, II = 0.5;
-    int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     PCoef a = pcoef(d, c);
     for (int dir = 0; dir < 3; dir++) {
         int c2;
@@ -699,7 +783,7 @@ __global__ void k_pressure_coeffs(Dev d)
 }
 
 // BCs_faces_ap (nim.f90:682-742) for every wall face
-__global__ void k_wall_pressure_coeffs(Dev d)
+__global__ void __launch_bounds__(256) k_wall_pressure_coeffs(Dev d)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= d.NB) return;
@@ -731,7 +815,7 @@ __global__ void k_wall_pressure_coeffs(Dev d)
 }
 
 // update_time_step (nse.f90:1292-1313)
-__global__ void k_advance(Dev d)
+__global__ void __launch_bounds__(256) k_advance(Dev d)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
@@ -742,7 +826,7 @@ __global__ void k_advance(Dev d)
         s[(S_TC + q) * N + c] = (old + cur) / 2.;
         s[(S_OLD + q) * N + c] = cur;
     }
-    int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     const double *P = d.fc + (size_t)F_P * d.NF;
     int f[6];
     cell_faces(d, i, j, k, f);
@@ -760,7 +844,7 @@ __global__ void __launch_bounds__(256) k_end_step_partial(Dev d)
         const int N = d.N;
         double den = 2 * d.DT;
         for (int q = 0; q < 4; q++) m[q] = fabs(d.sc[(S_CUR + q) * N + c] - d.sc[(S_OLD + q) * N + c]) / den;
-        int i = c % d.NT, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+        int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
         int f[6];
         cell_faces(d, i, j, k, f);
         double uf[6], vf[6], wf[6];
@@ -793,7 +877,7 @@ __global__ void __launch_bounds__(256) k_end_step_partial(Dev d)
 }
 
 // combine the partials in a fixed order; out = {dudt_max, dTdt_max, Dilatation}
-__global__ void k_end_step_final(Dev d, int nblocks, double *out)
+__global__ void __launch_bounds__(256) k_end_step_final(Dev d, int nblocks, double *out)
 {
     int l = threadIdx.x;  // 256 threads
     double m[4] = {0., 0., 0., 0.}, s = 0.;
@@ -824,7 +908,7 @@ __global__ void k_end_step_final(Dev d, int nblocks, double *out)
 
 // find_Nu2 (nse.f90:2699-2798): one block, fixed summation order.
 // out = {Nu_H, Nu_C, Average, Area1, Area2}
-__global__ void k_nusselt(Dev d, double R_i, double R_o, double *out)
+__global__ void __launch_bounds__(256) k_nusselt(Dev d, double R_i, double R_o, double *out)
 {
     const int NT = d.NT, NR = d.NR, NZ = d.NZ, NB = d.NB;
     const double *T = d.fc + (size_t)F_T * d.NF, *W = d.wall;

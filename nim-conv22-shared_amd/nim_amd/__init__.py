@@ -262,7 +262,7 @@ class Context:
         return self.L.nim_stream(self.h)
 
     KERNEL_CLASSES = ("velocity_sources", "velocity_sweep", "pressure_sources", "pressure_sweep",
-                      "energy_constants", "energy_coefficients", "energy_sources", "energy_sweep", "check")
+                      "energy_coefficients", "unused", "energy_sources", "energy_sweep", "check")
 
     def kernel_times(self, iters):
         """Per-class mean ms per outer iteration and launches per iteration,

@@ -37,10 +37,15 @@ def test_counts_and_face_map_match_reference_numbering():
     mf = geo["meshes_faces"].reshape(nmesh, 6) - 1
     oT = nt * (nr + 1) * nz
     oZ = oT + nmesh
+
+    def P(i):  # storage position of azimuthal index i: even cells, then odd
+        return (i % 2) * (nt // 2) + i // 2
+
     for c in range(nmesh):
         i, j, k = c % nt, (c // nt) % nr, c // (nt * nr)
-        slots = [i + nt * (j + (nr + 1) * k), i + nt * (j + 1 + (nr + 1) * k), oT + c,
-                 oT + (i + 1) % nt + nt * (j + nr * k), oZ + c, oZ + c + nt * nr]
+        slots = [P(i) + nt * (j + (nr + 1) * k), P(i) + nt * (j + 1 + (nr + 1) * k), oT + P(i) + nt * (j + nr * k),
+                 oT + P((i + 1) % nt) + nt * (j + nr * k), oZ + P(i) + nt * (j + nr * k),
+                 oZ + P(i) + nt * (j + nr * (k + 1))]
         assert list(fm[slots]) == list(mf[c]), c
 
 

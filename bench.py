@@ -88,6 +88,85 @@ def cpu_baseline(case, geo, st, iters):
     return case.ncells * iters / dt, dt
 
 
+# kernel symbol -> kernel class of nim_kernel_times
+KERNEL_CLASS = {"k_mom_src": "velocity_sources", "k_mom_sweep": "velocity_sweep", "k_ppe_src": "pressure_sources",
+                "k_ppe_sweep": "pressure_sweep", "k_T_coef": "energy_coefficients", "k_T_src": "energy_sources",
+                "k_T_sweep": "energy_sweep", "k_check": "check"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "%s_fp64_hbm_traffic.md")
+
+
+def _pmc_rows(d):
+    """Per-dispatch (kernel, counter value KiB, duration us) of one rocprofv3
+    --pmc run (the *_counter_collection.csv under d)."""
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(path)):
+            n = r["Kernel_Name"]
+            n = "copy_probe" if "neg_kernel" in n else n.split("(")[0].replace("nim::", "")
+            rows.append((n, float(r["Counter_Value"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
+    return rows
+
+
+def pmc_summary(d, case, copy_mib):
+    """Markdown summary of four --pmc runs under d: fetch/ and write/
+    (FETCH_SIZE, WRITE_SIZE of this bench in --profile-only mode) and
+    copy_fetch/, copy_write/ (the same counters for --copy-probe MIB). Each
+    counter is scaled by known bytes / counter value of the copy."""
+    def mean_of(rows, name):
+        v = [x for n, x, _ in rows if n == name]
+        return sum(v) / len(v), len(v)
+    known_kib = copy_mib * 1024.
+    sf = known_kib / mean_of(_pmc_rows(os.path.join(d, "copy_fetch")), "copy_probe")[0]
+    sw = known_kib / mean_of(_pmc_rows(os.path.join(d, "copy_write")), "copy_probe")[0]
+    fr, wr = _pmc_rows(os.path.join(d, "fetch")), _pmc_rows(os.path.join(d, "write"))
+    out = ["| kernel | class | dispatches | read MB/launch | write MB/launch | HBM MB/launch | us/launch (under --pmc) |",
+           "|---|---|---|---|---|---|---|"]
+    traffic = {}
+    for k, cls in KERNEL_CLASS.items():
+        if not any(n == k for n, _, _ in fr):
+            continue
+        f, nf = mean_of(fr, k)
+        w, _ = mean_of(wr, k)
+        us = sum(t for n, _, t in fr if n == k) / nf
+        rb, wb = f * sf * 1024, w * sw * 1024
+        traffic[cls] = rb + wb
+        out.append("| %s | %s | %d | %.1f | %.1f | %.1f | %.1f |" % (k, cls, nf, rb / 1e6, wb / 1e6, (rb + wb) / 1e6, us))
+    out.append("")
+    out.append("Copy calibration (%d MiB read and written per launch): FETCH_SIZE scale %.4f, WRITE_SIZE scale %.4f."
+               % (copy_mib, sf, sw))
+    out.append("")
+    out.append("bench-traffic: " + json.dumps({k: round(v) for k, v in traffic.items()}))
+    return "\n".join(out)
+
+
+def measured_traffic(case):
+    """HBM bytes per launch of each kernel class from the committed counter
+    summary of this case, or None."""
+    path = TRAFFIC_FILE % case
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bench-traffic:"):
+            return json.loads(line.split(":", 1)[1])
+    return None
+
+
+def copy_probe(mib):
+    """Three launches of an elementwise FP64 negate-copy of MIB MiB: it reads
+    and writes MIB MiB each, the known size that scales the HBM counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,16 +181,34 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
+    # counter passes: only the per-kernel event pass (few dispatches), or a
+    # plain copy of known size that calibrates FETCH_SIZE / WRITE_SIZE
+    ap.add_argument("--profile-only", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
+    ap.add_argument("--pmc-summary", metavar="DIR", help="print the HBM traffic summary of counter runs under DIR")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(pmc_summary(args.pmc_summary, args.case, args.copy_probe or 1024))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs one case per device and has no multi-GPU "
               "path (one case lives on one device)", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
 
     case = host.baseline_case(args.case)
     t_setup = time.perf_counter()
     ctx, geo, st = setup(case)
     t_setup = time.perf_counter() - t_setup
+    if args.profile_only:
+        ctx.build_momentum_coeffs()
+        kt = ctx.kernel_times(args.profile_iters)
+        ctx.close()
+        print(json.dumps({"case": args.case, "cells": case.ncells, "profile_iters": args.profile_iters,
+                          "kernel_ms_per_iteration": {k: v[0] for k, v in kt.items()},
+                          "launches_per_iteration": {k: v[1] for k, v in kt.items()}}))
+        return 0
     tol = np.array([args.tol, args.tol, args.tol, args.tol_p, args.tol])
 
     def step(first):
@@ -175,7 +272,7 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": (measured_traffic(args.case) or {}).get(dom),
             "bytes_per_cell": BYTES_PER_CELL[dom],
             "launch_ms": per_launch_s * 1e3,
         },

@@ -154,7 +154,8 @@ int32_t nim_set_array(nim_ctx *ctx, const char *name, const double *in, int64_t 
  * the current state (state is advanced; convergence is not tested) with a
  * HIP event around every launch on the context's stream, and returns for
  * each kernel class the mean time per iteration in ms (ms_per_iter) and the
- * launches per iteration (launches). Classes, in launch order:
+ * launches per iteration (launches); NIM_E_NONFINITE if an increment became
+ * non-finite (the kernels then stop early). Classes, in launch order:
  *   0 velocity sources   1 velocity sweep (4 colour launches)
  *   2 pressure sources   3 pressure walls + sweep (4 colour launches)
  *   4 energy coefficients and adiabatic walls (one fused launch)

@@ -180,7 +180,7 @@ extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, n
     TRY(dalloc(c, &d.src, (size_t)NSRC * N));
     TRY(dalloc(c, &d.red, (size_t)6 * c->nred + 64));
     TRY(dalloc(c, &d.ctl, (size_t)1));
-    TRY(dalloc(c, &d.errs, (size_t)2 * 5 * ERR_COPIES));
+    TRY(dalloc(c, &d.errs, (size_t)2 * 5 * ERR_COPIES * ERR_STRIDE));
     TRY(dalloc(c, &c->d_out, (size_t)16));
     d.gc = gc; d.An = An; d.wall = wall; d.wpts = wpts;
     if (hipHostMalloc((void **)&c->h_ctl, sizeof(Ctl), 0) != hipSuccess ||
@@ -696,5 +696,10 @@ extern "C" int32_t nim_kernel_times(nim_ctx *c, int32_t iters, double *ms, int32
     }
     for (int q = 0; q < NIM_KERNEL_CLASSES; q++) ms[q] /= iters;
     for (auto &e : ev) (void)hipEventDestroy(e);
+    // a non-finite increment stops the kernels early, which would time
+    // nothing: report it
+    HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (c->h_ctl->status == 4) return fail(NIM_E_NONFINITE, "non-finite increment while timing");
     return NIM_OK;
 }

@@ -64,6 +64,11 @@ enum { R_U = 0, R_V = 1, R_W = 2, R_P = 3, R_T = 4, NSRC = 5 };
 enum { F_U = 0, F_V = 1, F_W = 2, F_P = 3, F_T = 4 };
 
 constexpr int ERR_COPIES = 64;  // spread of the per-iteration max-increment atomics
+constexpr int ERR_STRIDE = 16;  // one 128-byte cache line per copy
+__device__ __host__ inline size_t err_slot(int slot, int field, int copy)
+{
+    return ((size_t)(slot * 5 + field) * ERR_COPIES + copy) * ERR_STRIDE;
+}
 
 struct Ctl {
     int32_t done, iter, status, max_iter, min_iter, pad[3];
@@ -87,8 +92,20 @@ struct Dev {
     double *src;        // [NSRC][N]
     double *red;        // reduction scratch
     Ctl *ctl;
-    unsigned long long *errs;  // [2][5][ERR_COPIES]
+    unsigned long long *errs;  // [2][5][ERR_COPIES] at err_slot()
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2. The per-iteration kernels renumber them so that XCD x works through
+// one contiguous range of cells (a bijection whatever the dispatch order):
+// neighbour half-rows, rows and planes are then read through the same L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_block()
+{
+    int b = blockIdx.x, nb = gridDim.x;
+    int q = nb / NXCD, rem = nb % NXCD, x = b % NXCD, r = b / NXCD;
+    return x * q + (x < rem ? x : rem) + r;
+}
 
 __device__ __forceinline__ int ip1(const Dev &d, int i) { return i + 1 == d.NT ? 0 : i + 1; }
 __device__ __forceinline__ int im1(const Dev &d, int i) { return i == 0 ? d.NT - 1 : i - 1; }

@@ -35,7 +35,7 @@ __device__ __forceinline__ void err_commit(const Dev &d, int field, double e)
     e = wave_max(e);
     if ((threadIdx.x & 63) == 0) {
         int slot = d.ctl->iter & 1;
-        unsigned long long *p = d.errs + (size_t)(slot * 5 + field) * ERR_COPIES + (blockIdx.x & (ERR_COPIES - 1));
+        unsigned long long *p = d.errs + err_slot(slot, field, blockIdx.x & (ERR_COPIES - 1));
         atomicMax(p, (unsigned long long)__double_as_longlong(e));
     }
 }
@@ -61,21 +61,33 @@ __device__ __forceinline__ bool color_cell(const Dev &d, int color, int tid, int
 // derivative across cells of the face value at position fpos, along
 // direction dir (0 theta, 1 r, 2 z): central (f(+1) - f(-1))/4, 2nd-order
 // one-sided at walls (nse.f90:1643-1653 and its repetitions)
+//
+// The three stencil points and their weights are chosen by selects and the
+// points are always loaded, so the loads of all derivatives of a cell issue
+// together instead of one branch (and one memory round trip) at a time.
+// Weights (1, -1, 0) give the central difference exactly (the products by
+// +-1 and 0 are exact); (-3, 4, -1) and (3, -4, 1) the one-sided forms in
+// the reference's operation order.
+__device__ __forceinline__ double three_point(double f0, double f1, double f2, bool lo, bool hi)
+{
+    double w0 = lo ? -3. : hi ? 3. : 1.;
+    double w1 = lo ? 4. : hi ? -4. : -1.;
+    double w2 = lo ? -1. : hi ? 1. : 0.;
+    return (w0 * f0 + w1 * f1 + w2 * f2) / 4.;
+}
+
 __device__ __forceinline__ double tder(const Dev &d, const double *F, int fpos, int i, int j, int k, int dir)
 {
     if (dir == 0) return (F[fidx(d, fpos, ip1(d, i), j, k)] - F[fidx(d, fpos, im1(d, i), j, k)]) / 4.;
     if (dir == 1) {
-        if (j == 0)
-            return (-3 * F[fidx(d, fpos, i, 0, k)] + 4 * F[fidx(d, fpos, i, 1, k)] - F[fidx(d, fpos, i, 2, k)]) / 4.;
-        if (j == d.NR - 1)
-            return (3 * F[fidx(d, fpos, i, j, k)] - 4 * F[fidx(d, fpos, i, j - 1, k)] + F[fidx(d, fpos, i, j - 2, k)]) / 4.;
-        return (F[fidx(d, fpos, i, j + 1, k)] - F[fidx(d, fpos, i, j - 1, k)]) / 4.;
+        bool lo = j == 0, hi = j == d.NR - 1;
+        // central: (j+1, j-1); low wall: (0, 1, 2); high wall: (j, j-1, j-2)
+        int j0 = lo ? 0 : hi ? j : j + 1, j1 = lo ? 1 : j - 1, j2 = lo ? 2 : hi ? j - 2 : j1;
+        return three_point(F[fidx(d, fpos, i, j0, k)], F[fidx(d, fpos, i, j1, k)], F[fidx(d, fpos, i, j2, k)], lo, hi);
     }
-    if (k == 0)
-        return (-3 * F[fidx(d, fpos, i, j, 0)] + 4 * F[fidx(d, fpos, i, j, 1)] - F[fidx(d, fpos, i, j, 2)]) / 4.;
-    if (k == d.NZ - 1)
-        return (3 * F[fidx(d, fpos, i, j, k)] - 4 * F[fidx(d, fpos, i, j, k - 1)] + F[fidx(d, fpos, i, j, k - 2)]) / 4.;
-    return (F[fidx(d, fpos, i, j, k + 1)] - F[fidx(d, fpos, i, j, k - 1)]) / 4.;
+    bool lo = k == 0, hi = k == d.NZ - 1;
+    int k0 = lo ? 0 : hi ? k : k + 1, k1 = lo ? 1 : k - 1, k2 = lo ? 2 : hi ? k - 2 : k1;
+    return three_point(F[fidx(d, fpos, i, j, k0)], F[fidx(d, fpos, i, j, k1)], F[fidx(d, fpos, i, j, k2)], lo, hi);
 }
 
 // cross-diffusion operator of calculate_R_D2 / _R_D_p2 / _R_D_T
@@ -273,7 +285,7 @@ __device__ __forceinline__ void energy_direction(const Dev &d, int c, int dir, d
 __global__ void __launch_bounds__(256) k_T_coef(Dev d)
 {
     if (d.ctl->done) return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = xcd_block() * blockDim.x + threadIdx.x;
     bool act = c < d.N;
     double eT = 0.;
     if (act) {
@@ -348,7 +360,7 @@ __global__ void __launch_bounds__(256) k_T_coef(Dev d)
 __global__ void __launch_bounds__(192) k_mom_src(Dev d)
 {
     if (d.ctl->done) return;
-    int c = blockIdx.x * 64 + threadIdx.x;
+    int c = xcd_block() * 64 + threadIdx.x;
     const int q = threadIdx.y;
     if (c >= d.N) return;
     const int N = d.N;
@@ -374,7 +386,7 @@ __global__ void __launch_bounds__(192) k_mom_src(Dev d)
 __global__ void __launch_bounds__(256) k_ppe_src(Dev d)
 {
     if (d.ctl->done) return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
     const int N = d.N;
     int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
@@ -404,7 +416,7 @@ __global__ void __launch_bounds__(256) k_ppe_src(Dev d)
 __global__ void __launch_bounds__(256) k_T_src(Dev d)
 {
     if (d.ctl->done) return;
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
     int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     d.src[R_T * d.N + c] = cross_diffusion(d, d.fc + (size_t)F_T * d.NF, i, j, k, c, d.alpha) + 0.;
@@ -415,69 +427,89 @@ __global__ void __launch_bounds__(256) k_T_src(Dev d)
 // ---------------------------------------------------------------------
 // one transported field of one cell: the cell average, then owned faces
 // 2, 4, 6 (integrate_velocity_field_formulation2_i nse.f90:831-1006 and
-// integrate_temperature_field_formulation2_i nse.f90:1009-1089)
-__device__ void transport_unit(const Dev &d, double *Fc, double *C, const double *CO, const double *coef, double B,
-                               double w, int i, int j, int k, int c, double &ecell, double &eface)
+// integrate_temperature_field_formulation2_i nse.f90:1009-1089).
+// Gather, compute, scatter: every input is loaded before anything is
+// stored, and the unit's own updated values (cell, face 2, face 4) are
+// carried in registers to the later updates that read them, as the
+// reference's in-place order does. No other cell of the colour writes any
+// location read here, so loading early changes no value.
+__device__ void transport_unit(const Dev &d, double *__restrict__ Fc, double *__restrict__ C,
+                               const double *__restrict__ CO, const double *__restrict__ coef, double B, double w, int i,
+                               int j, int k, int c, double &ecell, double &eface)
 {
     const int N = d.N;
+    const bool has2 = j < d.NR - 1, has6 = k < d.NZ - 1;
+    // neighbour rows clamped at the walls: the loads stay in bounds and the
+    // results are discarded there
+    const int jn = has2 ? j + 1 : j, kn = has6 ? k + 1 : k, ip = ip1(d, i);
+    const double *gdf = d.gc + (size_t)G_DIFF * N;
     int f[6];
     cell_faces(d, i, j, k, f);
-    double old = C[c];
-    double nv = (1 - w) * old + (w / coef[(C_F4 + 0) * N + c])
-                                    * (coef[(C_F4 + 1) * N + c] * CO[c] + coef[(C_F4 + 2) * N + c] * Fc[f[0]]
-                                       + coef[(C_F4 + 3) * N + c] * Fc[f[1]] + coef[(C_F4 + 4) * N + c] * Fc[f[2]]
-                                       + coef[(C_F4 + 5) * N + c] * Fc[f[3]] + coef[(C_F4 + 6) * N + c] * Fc[f[4]]
-                                       + coef[(C_F4 + 7) * N + c] * Fc[f[5]] + B);
-    C[c] = nv;
+    // ---- gather
+    const double old = C[c], oc = CO[c];
+    double a[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) a[m] = coef[(C_F4 + m) * N + c];
+    double F0 = Fc[f[0]], F1 = Fc[f[1]], F2 = Fc[f[2]], F3 = Fc[f[3]], F4 = Fc[f[4]], F5 = Fc[f[5]];
+    double b[5], e[5], h[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+        b[m] = coef[(C_F1 + m) * N + c];
+        e[m] = coef[(C_F2 + m) * N + c];
+        h[m] = coef[(C_F3 + m) * N + c];
+    }
+    const double gd0 = gdf[0 * N + c], gd1 = gdf[1 * N + c], gd2 = gdf[2 * N + c];
+    const double gd3 = gdf[3 * N + c], gd4 = gdf[4 * N + c], gd5 = gdf[5 * N + c];
+    // face 2 partner: cell (i, j+1, k)
+    const int cr = cidx(d, i, jn, k);
+    const double r2 = Fc[fidx(d, 2, i, jn, k)], r3 = Fc[fidx(d, 3, i, jn, k)], r4 = Fc[fidx(d, 4, i, jn, k)];
+    const double r5 = Fc[fidx(d, 5, i, jn, k)], r6 = Fc[fidx(d, 6, i, jn, k)];
+    const double sr = C[cr] + CO[cr];
+    // face 4 partner: cell (i+1, j, k), periodic
+    const int ct = cidx(d, ip, j, k);
+    const double t1 = Fc[fidx(d, 1, ip, j, k)], t2 = Fc[fidx(d, 2, ip, j, k)], t4 = Fc[fidx(d, 4, ip, j, k)];
+    const double t5 = Fc[fidx(d, 5, ip, j, k)], t6 = Fc[fidx(d, 6, ip, j, k)];
+    const double st = C[ct] + CO[ct];
+    // face 6 partner: cell (i, j, k+1)
+    const int cz = cidx(d, i, j, kn);
+    const double z1 = Fc[fidx(d, 1, i, j, kn)], z2 = Fc[fidx(d, 2, i, j, kn)], z3 = Fc[fidx(d, 3, i, j, kn)];
+    const double z4 = Fc[fidx(d, 4, i, j, kn)], z6 = Fc[fidx(d, 6, i, j, kn)];
+    const double sz = C[cz] + CO[cz];
+    // ---- cell average
+    const double nv = (1 - w) * old + (w / a[0])
+                                          * (a[1] * oc + a[2] * F0 + a[3] * F1 + a[4] * F2 + a[5] * F3 + a[6] * F4
+                                             + a[7] * F5 + B);
     ecell = fmax(ecell, fabs(nv - old));
-    const double *gdf = d.gc + (size_t)G_DIFF * N;
-    double ccur = nv + CO[c];
-    // face 2: radial neighbour (i, j+1, k)
-    if (j < d.NR - 1) {
-        int c2 = c + d.NT;
-        int g3 = fidx(d, 3, i, j + 1, k), g4 = fidx(d, 4, i, j + 1, k), g5 = fidx(d, 5, i, j + 1, k);
-        int g6 = fidx(d, 6, i, j + 1, k), g2 = fidx(d, 2, i, j + 1, k);
-        double B3 = gdf[0 * N + c] * (Fc[f[3]] - Fc[f[2]] + Fc[g4] - Fc[g3]) / 4.0;
-        double B4 = gdf[1 * N + c] * (Fc[f[5]] - Fc[f[4]] + Fc[g6] - Fc[g5]) / 4.0;
-        double o = Fc[f[1]];
-        double v = (1 - w) * o + (w / coef[(C_F1 + 0) * N + c])
-                                     * (coef[(C_F1 + 1) * N + c] * Fc[f[0]] + coef[(C_F1 + 2) * N + c] * Fc[g2]
-                                        + coef[(C_F1 + 3) * N + c] * ccur + coef[(C_F1 + 4) * N + c] * (C[c2] + CO[c2])
-                                        - B3 - B4);
-        Fc[f[1]] = v;
-        eface = fmax(eface, fabs(v - o));
+    const double ccur = nv + oc;
+    // ---- face 2
+    if (has2) {
+        double B3 = gd0 * (F3 - F2 + r4 - r3) / 4.0;
+        double B4 = gd1 * (F5 - F4 + r6 - r5) / 4.0;
+        double v = (1 - w) * F1 + (w / b[0]) * (b[1] * F0 + b[2] * r2 + b[3] * ccur + b[4] * sr - B3 - B4);
+        eface = fmax(eface, fabs(v - F1));
+        F1 = v;
     }
-    // face 4: azimuthal neighbour (i+1, j, k), periodic
+    // ---- face 4
     {
-        int ip = ip1(d, i);
-        int c2 = cidx(d, ip, j, k);
-        int g1 = fidx(d, 1, ip, j, k), g2 = fidx(d, 2, ip, j, k), g4 = fidx(d, 4, ip, j, k);
-        int g5 = fidx(d, 5, ip, j, k), g6 = fidx(d, 6, ip, j, k);
-        double B3 = gdf[2 * N + c] * (Fc[f[1]] - Fc[f[0]] + Fc[g2] - Fc[g1]) / 4.0;
-        double B4 = gdf[3 * N + c] * (Fc[f[5]] - Fc[f[4]] + Fc[g6] - Fc[g5]) / 4.0;
-        double o = Fc[f[3]];
-        double v = (1 - w) * o + (w / coef[(C_F2 + 0) * N + c])
-                                     * (coef[(C_F2 + 1) * N + c] * Fc[f[2]] + coef[(C_F2 + 2) * N + c] * Fc[g4]
-                                        + coef[(C_F2 + 3) * N + c] * ccur + coef[(C_F2 + 4) * N + c] * (C[c2] + CO[c2])
-                                        - B3 - B4);
-        Fc[f[3]] = v;
-        eface = fmax(eface, fabs(v - o));
+        double B3 = gd2 * (F1 - F0 + t2 - t1) / 4.0;
+        double B4 = gd3 * (F5 - F4 + t6 - t5) / 4.0;
+        double v = (1 - w) * F3 + (w / e[0]) * (e[1] * F2 + e[2] * t4 + e[3] * ccur + e[4] * st - B3 - B4);
+        eface = fmax(eface, fabs(v - F3));
+        F3 = v;
     }
-    // face 6: axial neighbour (i, j, k+1)
-    if (k < d.NZ - 1) {
-        int c2 = c + d.NT * d.NR;
-        int g1 = fidx(d, 1, i, j, k + 1), g2 = fidx(d, 2, i, j, k + 1), g3 = fidx(d, 3, i, j, k + 1);
-        int g4 = fidx(d, 4, i, j, k + 1), g6 = fidx(d, 6, i, j, k + 1);
-        double B3 = gdf[4 * N + c] * (Fc[f[1]] - Fc[f[0]] + Fc[g2] - Fc[g1]) / 4.0;
-        double B4 = gdf[5 * N + c] * (Fc[f[3]] - Fc[f[2]] + Fc[g4] - Fc[g3]) / 4.0;
-        double o = Fc[f[5]];
-        double v = (1 - w) * o + (w / coef[(C_F3 + 0) * N + c])
-                                     * (coef[(C_F3 + 1) * N + c] * Fc[f[4]] + coef[(C_F3 + 2) * N + c] * Fc[g6]
-                                        + coef[(C_F3 + 3) * N + c] * ccur + coef[(C_F3 + 4) * N + c] * (C[c2] + CO[c2])
-                                        - B3 - B4);
-        Fc[f[5]] = v;
-        eface = fmax(eface, fabs(v - o));
+    // ---- face 6
+    if (has6) {
+        double B3 = gd4 * (F1 - F0 + z2 - z1) / 4.0;
+        double B4 = gd5 * (F3 - F2 + z4 - z3) / 4.0;
+        double v = (1 - w) * F5 + (w / h[0]) * (h[1] * F4 + h[2] * z6 + h[3] * ccur + h[4] * sz - B3 - B4);
+        eface = fmax(eface, fabs(v - F5));
+        F5 = v;
     }
+    // ---- scatter
+    C[c] = nv;
+    if (has2) Fc[f[1]] = F1;
+    Fc[f[3]] = F3;
+    if (has6) Fc[f[5]] = F5;
 }
 
 // threadIdx.y selects the field (u, v, w are independent within the sweep)
@@ -486,7 +518,7 @@ __global__ void __launch_bounds__(192) k_mom_sweep(Dev d, int color)
     if (d.ctl->done) return;
     int i, j, k;
     const int q = threadIdx.y;
-    bool act = color_cell(d, color, blockIdx.x * 64 + threadIdx.x, i, j, k);
+    bool act = color_cell(d, color, xcd_block() * 64 + threadIdx.x, i, j, k);
     double e = 0.;
     if (act) {
         const int N = d.N;
@@ -504,7 +536,7 @@ __global__ void __launch_bounds__(256) k_T_sweep(Dev d, int color)
 {
     if (d.ctl->done) return;
     int i, j, k;
-    bool act = color_cell(d, color, blockIdx.x * blockDim.x + threadIdx.x, i, j, k);
+    bool act = color_cell(d, color, xcd_block() * blockDim.x + threadIdx.x, i, j, k);
     double e = 0.;
     if (act) {
         const int N = d.N;
@@ -538,22 +570,13 @@ __device__ double pressure_wall(const Dev &d, int b, int pos, int i, int j, int 
     double B2, B3;
     if (pos <= 2) {
         // tangential directions theta (always central) and z
-        B2 = (P[fidx(d, pos, ip1(d, i), j, k)] - P[fidx(d, pos, im1(d, i), j, k)]) / 4.0 * gB2c;
-        if (k == 0)
-            B3 = (-3. * P[fw] + 4. * P[fidx(d, pos, i, j, 1)] - P[fidx(d, pos, i, j, 2)]) / 4.0 * gB3;
-        else if (k == d.NZ - 1)
-            B3 = (3. * P[fw] - 4. * P[fidx(d, pos, i, j, k - 1)] + P[fidx(d, pos, i, j, k - 2)]) / 4.0 * gB3;
-        else
-            B3 = (P[fidx(d, pos, i, j, k + 1)] - P[fidx(d, pos, i, j, k - 1)]) / 4.0 * gB3;
+        B2 = tder(d, P, pos, i, j, k, 0) * gB2c;
+        B3 = tder(d, P, pos, i, j, k, 2) * gB3;
     } else {
-        // tangential directions r and theta (always central)
-        if (j == 0)
-            B2 = (-3. * P[fw] + 4. * P[fidx(d, pos, i, 1, k)] - P[fidx(d, pos, i, 2, k)]) / 4.0 * gB2;
-        else if (j == d.NR - 1)
-            B2 = (3. * P[fw] - 4. * P[fidx(d, pos, i, j - 1, k)] + P[fidx(d, pos, i, j - 2, k)]) / 4.0 * gB2;
-        else
-            B2 = (P[fidx(d, pos, i, j + 1, k)] - P[fidx(d, pos, i, j - 1, k)]) / 4.0 * gB2c;
-        B3 = (P[fidx(d, pos, ip1(d, i), j, k)] - P[fidx(d, pos, im1(d, i), j, k)]) / 4.0 * gB3;
+        // tangential directions r and theta (always central); the r
+        // derivative takes a different metric factor in its central branch
+        B2 = tder(d, P, pos, i, j, k, 1) * (j == 0 || j == d.NR - 1 ? gB2 : gB2c);
+        B3 = tder(d, P, pos, i, j, k, 0) * gB3;
     }
     double B1 = W[(W_AP + 7) * NB + b] * d.src[R_P * d.N + c];
     double s = 0.;
@@ -572,12 +595,15 @@ __device__ double pressure_wall(const Dev &d, int b, int pos, int i, int j, int 
 }
 
 // integrate_pressure_field_i (nse.f90:1091-1289) for one cell, preceded by
-// the cell's own wall faces in the reference's boundary-group order
+// the cell's own wall faces in the reference's boundary-group order.
+// After the walls, every input of the three interior face updates is
+// gathered before the first store; the cell's own faces are carried in
+// registers, so face 4 sees the new face 2 and face 6 the new faces 2 and 4.
 __global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, int color)
 {
     if (d.ctl->done) return;
     int i, j, k;
-    bool act = color_cell(d, color, blockIdx.x * blockDim.x + threadIdx.x, i, j, k);
+    bool act = color_cell(d, color, xcd_block() * blockDim.x + threadIdx.x, i, j, k);
     double e = 0.;
     if (act) {
         const int N = d.N, NT = d.NT, NR = d.NR, NZ = d.NZ;
@@ -586,64 +612,72 @@ __global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, int color)
         if (k == NZ - 1) e = fmax(e, pressure_wall(d, NT * NR + i + NT * j, 6, i, j, k, c));
         if (j == 0) e = fmax(e, pressure_wall(d, 2 * NT * NR + i + NT * k, 1, i, j, k, c));
         if (j == NR - 1) e = fmax(e, pressure_wall(d, 2 * NT * NR + NT * NZ + i + NT * k, 2, i, j, k, c));
-        double *P = d.fc + (size_t)F_P * d.NF;
-        const double *gdf = d.gc + (size_t)G_DIFF * N;
-        const double S1 = d.src[R_P * N + c];
+        double *__restrict__ P = d.fc + (size_t)F_P * d.NF;
+        const double *__restrict__ gdf = d.gc + (size_t)G_DIFF * N;
+        const double *__restrict__ R = d.src + (size_t)R_P * N;
         const double w = d.w_p;
+        const bool hx = j < NR - 1, hz = k < NZ - 1;
+        const int jn = hx ? j + 1 : j, kn = hz ? k + 1 : k, ip = ip1(d, i);
         int f[6];
         cell_faces(d, i, j, k, f);
-        if (j < NR - 1) {
-            int n2 = c + NT;
-            const double *A = d.AA + (size_t)(0 * 15) * N;
-#define AX(m) A[((m) - 1) * N + c]
-            double B1 = S1 * AX(13), B2 = d.src[R_P * N + n2] * AX(15);
-            double B3 = gdf[0 * N + c] * tder(d, P, 2, i, j, k, 0);
-            double B4 = gdf[1 * N + c] * tder(d, P, 2, i, j, k, 2);
-            double o = P[f[1]];
-            double v = (1 - w) * o + w / AX(2) * (-AX(1) * P[f[0]] - AX(3) * P[fidx(d, 2, i, j + 1, k)] - AX(4) * P[f[2]]
-                                                  - AX(5) * P[f[3]] - AX(6) * P[fidx(d, 3, i, j + 1, k)]
-                                                  - AX(7) * P[fidx(d, 4, i, j + 1, k)] - AX(8) * P[f[4]] - AX(9) * P[f[5]]
-                                                  - AX(10) * P[fidx(d, 5, i, j + 1, k)]
-                                                  - AX(11) * P[fidx(d, 6, i, j + 1, k)] + B1 + B2 + B3 + B4);
-            P[f[1]] = v;
-            e = fmax(e, fabs(v - o));
-#undef AX
+        // ---- gather
+        double pf[6];
+#pragma unroll
+        for (int p = 0; p < 6; p++) pf[p] = P[f[p]];
+        const double S1 = R[c];
+        const double *__restrict__ Ax = d.AA, *__restrict__ Ay = d.AA + (size_t)15 * N, *__restrict__ Az = d.AA + (size_t)30 * N;
+        double ax[15], ay[15], az[15];
+#pragma unroll
+        for (int m = 0; m < 15; m++) {
+            if (m == 11 || m == 13) continue;  // AA(12) and AA(14) are not used by the sweep
+            ax[m] = Ax[m * N + c];
+            ay[m] = Ay[m * N + c];
+            az[m] = Az[m * N + c];
+        }
+        // x: faces of cell (i, j+1, k)
+        const double Sx = R[cidx(d, i, jn, k)];
+        const double x2 = P[fidx(d, 2, i, jn, k)], x3 = P[fidx(d, 3, i, jn, k)], x4 = P[fidx(d, 4, i, jn, k)];
+        const double x5 = P[fidx(d, 5, i, jn, k)], x6 = P[fidx(d, 6, i, jn, k)];
+        const double Bx3 = gdf[0 * N + c] * tder(d, P, 2, i, j, k, 0), Bx4 = gdf[1 * N + c] * tder(d, P, 2, i, j, k, 2);
+        // y: faces of cell (i+1, j, k)
+        const double Sy = R[cidx(d, ip, j, k)];
+        const double y1 = P[fidx(d, 1, ip, j, k)], y2 = P[fidx(d, 2, ip, j, k)], y4 = P[fidx(d, 4, ip, j, k)];
+        const double y5 = P[fidx(d, 5, ip, j, k)], y6 = P[fidx(d, 6, ip, j, k)];
+        const double By3 = gdf[2 * N + c] * tder(d, P, 4, i, j, k, 1), By4 = gdf[3 * N + c] * tder(d, P, 4, i, j, k, 2);
+        // z: faces of cell (i, j, k+1)
+        const double Sz = R[cidx(d, i, j, kn)];
+        const double z1 = P[fidx(d, 1, i, j, kn)], z2 = P[fidx(d, 2, i, j, kn)], z3 = P[fidx(d, 3, i, j, kn)];
+        const double z4 = P[fidx(d, 4, i, j, kn)], z6 = P[fidx(d, 6, i, j, kn)];
+        const double Bz3 = gdf[4 * N + c] * tder(d, P, 6, i, j, k, 1), Bz4 = gdf[5 * N + c] * tder(d, P, 6, i, j, k, 0);
+        // ---- compute (AA(m) is a?[m-1])
+        if (hx) {
+            double B1 = S1 * ax[12], B2 = Sx * ax[14];
+            double v = (1 - w) * pf[1] + w / ax[1] * (-ax[0] * pf[0] - ax[2] * x2 - ax[3] * pf[2] - ax[4] * pf[3]
+                                                      - ax[5] * x3 - ax[6] * x4 - ax[7] * pf[4] - ax[8] * pf[5]
+                                                      - ax[9] * x5 - ax[10] * x6 + B1 + B2 + Bx3 + Bx4);
+            e = fmax(e, fabs(v - pf[1]));
+            pf[1] = v;
         }
         {
-            int ip = ip1(d, i);
-            int n2 = cidx(d, ip, j, k);
-            const double *A = d.AA + (size_t)(1 * 15) * N;
-#define AY(m) A[((m) - 1) * N + c]
-            double B1 = S1 * AY(13), B2 = d.src[R_P * N + n2] * AY(15);
-            double B3 = gdf[2 * N + c] * tder(d, P, 4, i, j, k, 1);
-            double B4 = gdf[3 * N + c] * tder(d, P, 4, i, j, k, 2);
-            double o = P[f[3]];
-            double v = (1 - w) * o + w / AY(6) * (-AY(1) * P[f[0]] - AY(2) * P[f[1]] - AY(3) * P[fidx(d, 1, ip, j, k)]
-                                                  - AY(4) * P[fidx(d, 2, ip, j, k)] - AY(5) * P[f[2]]
-                                                  - AY(7) * P[fidx(d, 4, ip, j, k)] - AY(8) * P[f[4]] - AY(9) * P[f[5]]
-                                                  - AY(10) * P[fidx(d, 5, ip, j, k)] - AY(11) * P[fidx(d, 6, ip, j, k)]
-                                                  + B1 + B2 + B3 + B4);
-            P[f[3]] = v;
-            e = fmax(e, fabs(v - o));
-#undef AY
+            double B1 = S1 * ay[12], B2 = Sy * ay[14];
+            double v = (1 - w) * pf[3] + w / ay[5] * (-ay[0] * pf[0] - ay[1] * pf[1] - ay[2] * y1 - ay[3] * y2
+                                                      - ay[4] * pf[2] - ay[6] * y4 - ay[7] * pf[4] - ay[8] * pf[5]
+                                                      - ay[9] * y5 - ay[10] * y6 + B1 + B2 + By3 + By4);
+            e = fmax(e, fabs(v - pf[3]));
+            pf[3] = v;
         }
-        if (k < NZ - 1) {
-            int n2 = c + NT * NR;
-            const double *A = d.AA + (size_t)(2 * 15) * N;
-#define AZ(m) A[((m) - 1) * N + c]
-            double B1 = S1 * AZ(13), B2 = d.src[R_P * N + n2] * AZ(15);
-            double B3 = gdf[4 * N + c] * tder(d, P, 6, i, j, k, 1);
-            double B4 = gdf[5 * N + c] * tder(d, P, 6, i, j, k, 0);
-            double o = P[f[5]];
-            double v = (1 - w) * o + w / AZ(10) * (-AZ(1) * P[f[0]] - AZ(2) * P[f[1]] - AZ(3) * P[fidx(d, 1, i, j, k + 1)]
-                                                   - AZ(4) * P[fidx(d, 2, i, j, k + 1)] - AZ(5) * P[f[2]] - AZ(6) * P[f[3]]
-                                                   - AZ(7) * P[fidx(d, 3, i, j, k + 1)]
-                                                   - AZ(8) * P[fidx(d, 4, i, j, k + 1)] - AZ(9) * P[f[4]]
-                                                   - AZ(11) * P[fidx(d, 6, i, j, k + 1)] + B1 + B2 + B3 + B4);
-            P[f[5]] = v;
-            e = fmax(e, fabs(v - o));
-#undef AZ
+        if (hz) {
+            double B1 = S1 * az[12], B2 = Sz * az[14];
+            double v = (1 - w) * pf[5] + w / az[9] * (-az[0] * pf[0] - az[1] * pf[1] - az[2] * z1 - az[3] * z2
+                                                      - az[4] * pf[2] - az[5] * pf[3] - az[6] * z3 - az[7] * z4
+                                                      - az[8] * pf[4] - az[10] * z6 + B1 + B2 + Bz3 + Bz4);
+            e = fmax(e, fabs(v - pf[5]));
+            pf[5] = v;
         }
+        // ---- scatter
+        if (hx) P[f[1]] = pf[1];
+        P[f[3]] = pf[3];
+        if (hz) P[f[5]] = pf[5];
     }
     err_commit(d, F_P, e);
 }
@@ -654,7 +688,7 @@ __global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, int color)
 __global__ void __launch_bounds__(256) k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4, int max_iter, int min_iter)
 {
     int l = threadIdx.x;
-    for (int q = l; q < 2 * 5 * ERR_COPIES; q += blockDim.x) d.errs[q] = 0ull;
+    for (int q = l; q < 2 * 5 * ERR_COPIES; q += blockDim.x) d.errs[(size_t)q * ERR_STRIDE] = 0ull;
     if (l == 0) {
         Ctl *c = d.ctl;
         c->done = 0;
@@ -678,8 +712,8 @@ __global__ void __launch_bounds__(64) k_check(Dev d)
     int s = ctl->iter & 1;
     double e[5];
     for (int q = 0; q < 5; q++) {
-        e[q] = wave_max(__longlong_as_double((long long)d.errs[(size_t)(s * 5 + q) * ERR_COPIES + l]));
-        d.errs[(size_t)((s ^ 1) * 5 + q) * ERR_COPIES + l] = 0ull;
+        e[q] = wave_max(__longlong_as_double((long long)d.errs[err_slot(s, q, l)]));
+        d.errs[err_slot(s ^ 1, q, l)] = 0ull;
     }
     if (l == 0) {
         int it = ctl->iter + 1;

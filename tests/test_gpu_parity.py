@@ -136,13 +136,44 @@ def test_inner_solve_matches_oracle_colored(geo, steps, graph, monkeypatch):
     ctx.close()
 
 
+def test_fixed_passes_match_oracle_colored(geo, steps):
+    """A whole time step's worth of passes (4000) in the same order on both
+    sides, without a stopping test: every field agrees to rounding. (Where
+    a tolerance stops the loop, the pass it stops at is itself sensitive to
+    rounding near u's floor, see test_converged_step_vs_reference_order.)"""
+    ctx = G.gpu_context(geo)
+    o = G.oracle_context(geo)
+    ctx.upload_state(G.state_from_steps(steps, "s3"))
+    R.load_state(o, steps, "s3")
+    ctx.build_momentum_coeffs()
+    o.velocity_constants()
+    zero = np.zeros(5)
+    rc, it_g, _ = ctx.inner_solve(zero, max_iter=4000, raise_on_cap=False)
+    rc_o, it_o, _ = o.inner_solve(zero, O.COLORED, max_iter=4000)
+    assert it_g == it_o == 4000
+    for f in "uvwpT":
+        assert G.rel_l2(ctx.get("faces_" + f), o.get("faces_" + f)) < 1e-9, f
+    for f in "uvwT":
+        assert G.rel_l2(ctx.get(f + "xyz"), o.get(f + "xyz")) < 1e-9, f
+    ctx.close()
+
+
 def test_converged_step_vs_reference_order(geo, steps):
     """GPU (4-colour order) against the oracle in the reference's own order,
     one time step from the reference's state after step 3, both converged
     to TOL_CONVERGED. Temperature and Nusselt numbers meet the parity bar
-    (1e-8 relative L2, 1e-6); velocity and pressure differ by how far the
-    reference's iteration is from its fixed point at that tolerance, which
-    the oracle alone shows between its two orders (u 7e-7, p 6e-6)."""
+    (1e-8 relative L2, 1e-6).
+
+    Velocity and pressure are only determined by the reference's iteration
+    to the level its stopping pass is: p's constant mode never converges
+    (its increment stays ~1.5e-8 per pass) and reaches u through the
+    discrete gradient of a constant, which the face-centre normals make
+    nonzero, so u's increment decreases very slowly near 1e-10 and the pass
+    where it crosses the tolerance moves by tens of passes under rounding.
+    The oracle alone shows the spread: a 1e-15 relative perturbation of the
+    initial p moves its converged u by 7e-7, v by 1.2e-6, p by 1.2e-5
+    (test_oracle_sensitivity.py), and its two orders differ by u 7e-7,
+    p 6e-6. The bounds below allow a few times that spread."""
     ctx = G.gpu_context(geo)
     o = G.oracle_context(geo)
     ctx.upload_state(G.state_from_steps(steps, "s3"))
@@ -156,10 +187,10 @@ def test_converged_step_vs_reference_order(geo, steps):
     for suf in ("xyz", "xyz_old", "xyzt"):
         assert G.rel_l2(ctx.get("T" + suf), o.get("T" + suf)) < 1e-8, suf
     for f in "uvw":
-        assert G.rel_l2(ctx.get("faces_" + f), o.get("faces_" + f)) < 1e-5, f
-        assert G.rel_l2(ctx.get(f + "xyz"), o.get(f + "xyz")) < 1e-5, f
+        assert G.rel_l2(ctx.get("faces_" + f), o.get("faces_" + f)) < 2e-5, f
+        assert G.rel_l2(ctx.get(f + "xyz"), o.get(f + "xyz")) < 2e-5, f
     pg, po = ctx.get("faces_p"), o.get("faces_p")
-    assert G.rel_l2(pg - pg.mean(), po - po.mean()) < 3e-5
+    assert G.rel_l2(pg - pg.mean(), po - po.mean()) < 2e-4
     ctx.close()
 
 

@@ -18,7 +18,8 @@
  *    create_geometry.f90:321-622); connectivity is implied by the dimensions.
  *  - One context = one case on the current HIP device and one HIP stream.
  *    A context is used from one host thread at a time.
- *  - Everything is FP64 (the reference compiles with -fdefault-real-8).
+ *  - Everything is FP64 (the reference compiles with -fdefault-real-8), except
+ *    the FP32 correction passes of nim_inner_solve_mixed.
  */
 #ifndef NIM_HIP_H
 #define NIM_HIP_H
@@ -121,6 +122,25 @@ int32_t nim_build_momentum_coeffs(nim_ctx *ctx);
 int32_t nim_inner_solve(nim_ctx *ctx, const double tol[5], int32_t max_iter, int32_t min_iter,
                         int32_t *iters, double err[5]);
 
+/* Mixed-precision form of the same inner loop (integrate_nse,
+ * nse.f90:117-176); the reference has no counterpart. Each cycle computes in
+ * FP64 the residual r = S(x) - a x of every unknown's equation (the equation
+ * an FP64 pass relaxes), stops when max |w r / a| per field is within tol,
+ * otherwise runs `inner` FP32 relaxation passes on the correction dx with r
+ * as its source (energy coefficients and the quadratic part of the pressure
+ * source held at x) and adds dx to x in FP64. The fixed point is the FP64
+ * one; FP32 limits only how fast it is reached. `cycles` returns the number
+ * of FP64 residual evaluations (FP32 passes = (cycles - 1) * inner).
+ * Returns NIM_E_MAXITER after max_cycles. */
+int32_t nim_inner_solve_mixed(nim_ctx *ctx, const double tol[5], int32_t max_cycles, int32_t min_cycles,
+                              int32_t inner, int32_t *cycles, double err[5]);
+
+/* FP64 residual measure of the current state, max |w r / a| per field
+ * (u, v, w, p, T), as nim_inner_solve_mixed tests it; no reference
+ * counterpart. The state is not changed (the derived sources, energy
+ * coefficients and dilatation are rebuilt at the state). */
+int32_t nim_residual(nim_ctx *ctx, double err[5]);
+
 /* check_steady_state (nse.f90:1316-1332) and evaluate_D (nse.f90:2504-2537),
  * which integrate_nse runs after its loop. */
 int32_t nim_end_step(nim_ctx *ctx, double *dudt_max, double *dTdt_max, double *dilatation);
@@ -166,6 +186,18 @@ int32_t nim_set_array(nim_ctx *ctx, const char *name, const double *in, int64_t 
 #define NIM_KERNEL_CLASSES 9
 int32_t nim_kernel_times(nim_ctx *ctx, int32_t iters, double ms_per_iter[NIM_KERNEL_CLASSES],
                          int32_t launches[NIM_KERNEL_CLASSES]);
+/* The same for one cycle of nim_inner_solve_mixed (`cycles` cycles of
+ * `inner` FP32 passes each; ms per cycle). Classes:
+ *   0 FP64 velocity residual (sources + equations)
+ *   1 FP64 pressure residual   2 FP64 energy residual (coefficients, sources,
+ *   equations)                 3 convergence check
+ *   4 FP32 velocity sources    5 FP32 velocity sweep (4 colour launches)
+ *   6 FP32 pressure sources    7 FP32 pressure walls + sweep (4 launches)
+ *   8 FP32 adiabatic walls     9 FP32 energy sources
+ *  10 FP32 energy sweep (4 launches)   11 FP64 update */
+#define NIM_MIXED_CLASSES 12
+int32_t nim_kernel_times_mixed(nim_ctx *ctx, int32_t cycles, int32_t inner, double ms_per_cycle[NIM_MIXED_CLASSES],
+                               int32_t launches[NIM_MIXED_CLASSES]);
 void *nim_stream(nim_ctx *ctx);
 int32_t nim_synchronize(nim_ctx *ctx);
 

@@ -3,6 +3,7 @@
 // the launch sequence of one outer iteration.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -31,6 +32,20 @@ __global__ void k_advance(Dev d);
 __global__ void k_end_step_partial(Dev d);
 __global__ void k_end_step_final(Dev d, int nblocks, double *out);
 __global__ void k_nusselt(Dev d, double R_i, double R_o, double *out);
+__global__ void k_res_mom(Dev d, View<double> v);
+__global__ void k_res_ppe_src(Dev d, View<double> v);
+__global__ void k_res_ppe(Dev d, View<double> v);
+__global__ void k_res_T_coef(Dev d, View<double> v);
+__global__ void k_res_T(Dev d, View<double> v);
+__global__ void k_d_mom_src(Dev d, View<float> v);
+__global__ void k_d_mom_sweep(Dev d, View<float> v, int color);
+__global__ void k_d_ppe_src(Dev d, View<float> v);
+__global__ void k_d_ppe_sweep(Dev d, View<float> v, int color);
+__global__ void k_d_zero_flux(Dev d, View<float> v);
+__global__ void k_d_T_src(Dev d, View<float> v);
+__global__ void k_d_T_sweep(Dev d, View<float> v, int color);
+__global__ void k_mix_update(Dev d, View<double> x, View<float> dx);
+__global__ void k_cast(const double *in, float *out, long n);
 }  // namespace nim
 
 using namespace nim;
@@ -67,6 +82,17 @@ struct nim_ctx {
     hipGraphExec_t gexec = nullptr;
     int gbatch = 0;
     int nred = 0;
+    // mixed precision: FP32 corrections and constants, residual buffers
+    int const_version = 0;        // bumped by every geometry or pressure-coefficient build
+    struct {
+        bool alloc = false;
+        int version = -1;         // const_version the FP32 constants were cast from
+        View<float> v32;          // corrections (sc, fc, src) and FP32 constants
+        View<double> v64;         // FP64 state with the residual buffers
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t gexec = nullptr;
+        int gbatch = 0, ginner = 0;
+    } mx;
 };
 
 extern "C" const char *nim_last_error(void) { return g_err.c_str(); }
@@ -203,6 +229,8 @@ extern "C" int32_t nim_ctx_destroy(nim_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->mx.gexec) (void)hipGraphExecDestroy(c->mx.gexec);
+    if (c->mx.graph) (void)hipGraphDestroy(c->mx.graph);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->h_out) (void)hipHostFree(c->h_out);
@@ -374,6 +402,7 @@ extern "C" int32_t nim_upload_geometry(nim_ctx *c, const nim_geometry *g)
     if ((rc = h2d(c, c->dev.wall, wall.data(), wall.size() * 8))) return rc;
     if ((rc = h2d(c, (void *)c->dev.wpts, wpts.data(), wpts.size() * 4))) return rc;
     c->has_geo = true;
+    c->const_version++;
     return NIM_OK;
 }
 
@@ -388,6 +417,7 @@ extern "C" int32_t nim_build_pressure_coeffs(nim_ctx *c)
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->has_pcoef = true;
+    c->const_version++;
     return NIM_OK;
 }
 
@@ -619,6 +649,162 @@ extern "C" int32_t nim_inner_solve(nim_ctx *c, const double tol[5], int32_t max_
     return NIM_OK;
 }
 
+// ---- mixed precision ----------------------------------------------------
+// FP32 buffers of the correction form (allocated on first use) and FP32
+// copies of the constant arrays, recast when geometry or pressure
+// coefficients were rebuilt
+static int32_t mixed_prepare(nim_ctx *c)
+{
+    auto &m = c->mx;
+    const size_t N = c->N, NF = c->NF, NB = c->NB;
+    if (!m.alloc) {
+        float *gc, *An, *AA, *wall, *MF, *TF, *sc, *fc, *src, *rc, *rf, *grad;
+        int32_t rc_;
+        if ((rc_ = dalloc(c, &gc, NGC * N)) || (rc_ = dalloc(c, &An, 3 * NF)) || (rc_ = dalloc(c, &AA, 45 * N)) ||
+            (rc_ = dalloc(c, &wall, NWS * NB)) || (rc_ = dalloc(c, &MF, NCF * N)) || (rc_ = dalloc(c, &TF, NCF * N)) ||
+            (rc_ = dalloc(c, &sc, 4 * N)) || (rc_ = dalloc(c, &fc, 5 * NF)) || (rc_ = dalloc(c, &src, NSRC * N)) ||
+            (rc_ = dalloc(c, &rc, 4 * N)) || (rc_ = dalloc(c, &rf, 5 * NF)) || (rc_ = dalloc(c, &grad, 9 * N)))
+            return rc_;
+        View<float> &v = m.v32;
+        v.gc = gc; v.An = An; v.AA = AA; v.wall = wall; v.MF = MF; v.TF = TF;
+        v.sc = sc; v.old = nullptr; v.fc = fc; v.src = src; v.rc = rc; v.rf = rf; v.TF32 = nullptr; v.grad = grad;
+        m.v64 = view64(c->dev);
+        m.v64.rc = rc; m.v64.rf = rf; m.v64.TF32 = TF; m.v64.grad = grad;
+        m.alloc = true;
+    }
+    hipStream_t s = c->stream;
+    auto cast = [&](const double *in, const float *out, size_t n) {
+        int g = (int)std::min<size_t>((n + BS - 1) / BS, 4096);
+        hipLaunchKernelGGL(k_cast, dim3(g), dim3(BS), 0, s, in, (float *)out, (long)n);
+    };
+    if (m.version != c->const_version) {
+        const Dev &d = c->dev;
+        cast(d.gc, m.v32.gc, NGC * N);
+        cast(d.An, m.v32.An, 3 * NF);
+        cast(d.AA, m.v32.AA, 45 * N);
+        cast(d.wall, m.v32.wall, NWS * NB);
+        m.version = c->const_version;
+    }
+    cast(c->dev.MF, m.v32.MF, NCF * N);  // momentum coefficients change every step
+    HIPCHK(hipGetLastError());
+    return NIM_OK;
+}
+
+// FP64 residual of every unknown's equation at the current state, the
+// maximum of |w r / a| per field reduced into the iteration control
+static void launch_residual(nim_ctx *c)
+{
+    const Dev &d = c->dev;
+    const View<double> &r = c->mx.v64;
+    hipStream_t s = c->stream;
+    const int gN = nblk(c->N);
+    hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d);
+    hipLaunchKernelGGL(k_res_mom, dim3(cell_blocks3(c)), BS3, 0, s, d, r);
+    hipLaunchKernelGGL(k_res_ppe_src, dim3(gN), dim3(BS), 0, s, d, r);
+    hipLaunchKernelGGL(k_res_ppe, dim3(gN), dim3(BS), 0, s, d, r);
+    hipLaunchKernelGGL(k_res_T_coef, dim3(gN), dim3(BS), 0, s, d, r);
+    hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d);
+    hipLaunchKernelGGL(k_res_T, dim3(gN), dim3(BS), 0, s, d, r);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, d);
+}
+
+// one FP32 pass of the correction form, stages in the FP64 pass's order
+static void launch_delta(nim_ctx *c)
+{
+    const Dev &d = c->dev;
+    const View<float> &v = c->mx.v32;
+    hipStream_t s = c->stream;
+    const int gN = nblk(c->N), gC = color_blocks(c);
+    hipLaunchKernelGGL(k_d_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d, v);
+    for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_d_mom_sweep, dim3(color_blocks3(c)), BS3, 0, s, d, v, q);
+    hipLaunchKernelGGL(k_d_ppe_src, dim3(gN), dim3(BS), 0, s, d, v);
+    for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_d_ppe_sweep, dim3(gC), dim3(BS), 0, s, d, v, q);
+    hipLaunchKernelGGL(k_d_zero_flux, dim3(nblk(2 * c->NT * c->NR)), dim3(BS), 0, s, d, v);
+    hipLaunchKernelGGL(k_d_T_src, dim3(gN), dim3(BS), 0, s, d, v);
+    for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_d_T_sweep, dim3(gC), dim3(BS), 0, s, d, v, q);
+}
+
+static void launch_update(nim_ctx *c)
+{
+    hipLaunchKernelGGL(k_mix_update, dim3(nblk(std::max(c->N, c->NF))), dim3(BS), 0, c->stream, c->dev, c->mx.v64,
+                       c->mx.v32);
+}
+
+static void launch_cycle(nim_ctx *c, int inner)
+{
+    launch_residual(c);
+    for (int q = 0; q < inner; q++) launch_delta(c);
+    launch_update(c);
+}
+
+static int32_t launch_mixed_batch(nim_ctx *c, int batch, int inner)
+{
+    auto &m = c->mx;
+    if (!use_graph()) {
+        for (int q = 0; q < batch; q++) launch_cycle(c, inner);
+        HIPCHK(hipGetLastError());
+        return NIM_OK;
+    }
+    if (!m.gexec || m.gbatch != batch || m.ginner != inner) {
+        if (m.gexec) { (void)hipGraphExecDestroy(m.gexec); m.gexec = nullptr; }
+        if (m.graph) { (void)hipGraphDestroy(m.graph); m.graph = nullptr; }
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int q = 0; q < batch; q++) launch_cycle(c, inner);
+        HIPCHK(hipStreamEndCapture(c->stream, &m.graph));
+        HIPCHK(hipGraphInstantiate(&m.gexec, m.graph, nullptr, nullptr, 0));
+        m.gbatch = batch;
+        m.ginner = inner;
+    }
+    HIPCHK(hipGraphLaunch(m.gexec, c->stream));
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_inner_solve_mixed(nim_ctx *c, const double tol[5], int32_t max_cycles, int32_t min_cycles,
+                                         int32_t inner, int32_t *cycles, double err[5])
+{
+    if (!c || !tol) return fail(NIM_E_ARG, "null argument");
+    if (!c->has_geo || !c->has_pcoef) return fail(NIM_E_STATE, "geometry or pressure coefficients missing");
+    if (max_cycles < 1) return fail(NIM_E_ARG, "max_cycles must be >= 1");
+    if (inner < 1 || inner > 1024) return fail(NIM_E_ARG, "inner passes must be in [1, 1024]");
+    int32_t rc = mixed_prepare(c);
+    if (rc) return rc;
+    if ((rc = reset_ctl(c, tol, max_cycles, min_cycles))) return rc;
+    // cycles between host reads of the control block: about 16 passes
+    const char *e = getenv("NIM_MIXED_BATCH");
+    int batch = e ? atoi(e) : (16 + inner) / (inner + 1);
+    batch = batch < 1 ? 1 : batch > 64 ? 64 : batch;
+    for (;;) {
+        if ((rc = launch_mixed_batch(c, batch, inner))) return rc;
+        HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->h_ctl->done) break;
+    }
+    if (cycles) *cycles = c->h_ctl->iter;
+    if (err)
+        for (int q = 0; q < 5; q++) err[q] = c->h_ctl->err[q];
+    int st = c->h_ctl->status;
+    if (st == 3) return fail(NIM_E_MAXITER, "cycle cap reached");
+    if (st == 4) return fail(NIM_E_NONFINITE, "non-finite residual");
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_residual(nim_ctx *c, double err[5])
+{
+    if (!c || !err) return fail(NIM_E_ARG, "null argument");
+    if (!c->has_geo || !c->has_pcoef) return fail(NIM_E_STATE, "geometry or pressure coefficients missing");
+    int32_t rc = mixed_prepare(c);
+    if (rc) return rc;
+    const double big[5] = {1e300, 1e300, 1e300, 1e300, 1e300};
+    if ((rc = reset_ctl(c, big, 1, 0))) return rc;
+    launch_residual(c);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 5; q++) err[q] = c->h_ctl->err[q];
+    if (c->h_ctl->status == 4) return fail(NIM_E_NONFINITE, "non-finite residual");
+    return NIM_OK;
+}
+
 extern "C" int32_t nim_end_step(nim_ctx *c, double *dudt, double *dTdt, double *dil)
 {
     if (!c) return fail(NIM_E_ARG, "null context");
@@ -701,5 +887,74 @@ extern "C" int32_t nim_kernel_times(nim_ctx *c, int32_t iters, double *ms, int32
     HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (c->h_ctl->status == 4) return fail(NIM_E_NONFINITE, "non-finite increment while timing");
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_kernel_times_mixed(nim_ctx *c, int32_t cycles, int32_t inner, double *ms, int32_t *launches)
+{
+    if (!c || !ms || cycles < 1 || inner < 1 || inner > 1024) return fail(NIM_E_ARG, "bad argument");
+    if (!c->has_geo || !c->has_pcoef) return fail(NIM_E_STATE, "geometry or pressure coefficients missing");
+    int32_t rc = mixed_prepare(c);
+    if (rc) return rc;
+    const double tol[5] = {0, 0, 0, 0, 0};
+    if ((rc = reset_ctl(c, tol, 1 << 30, 1 << 30))) return rc;
+    const Dev &d = c->dev;
+    const View<double> &r = c->mx.v64;
+    const View<float> &v = c->mx.v32;
+    hipStream_t s = c->stream;
+    const int gN = nblk(c->N), gC = color_blocks(c);
+    std::vector<hipEvent_t> ev;
+    std::vector<int> cls;
+    auto mark = [&](int k) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        (void)hipEventRecord(e, s);
+        ev.push_back(e);
+        cls.push_back(k);
+    };
+    for (int q = 0; q < NIM_MIXED_CLASSES; q++) {
+        ms[q] = 0.;
+        if (launches) launches[q] = 0;
+    }
+    for (int it = 0; it < cycles; it++) {
+        ev.clear();
+        cls.clear();
+        mark(-1);
+        hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); mark(0);
+        hipLaunchKernelGGL(k_res_mom, dim3(cell_blocks3(c)), BS3, 0, s, d, r); mark(0);
+        hipLaunchKernelGGL(k_res_ppe_src, dim3(gN), dim3(BS), 0, s, d, r); mark(1);
+        hipLaunchKernelGGL(k_res_ppe, dim3(gN), dim3(BS), 0, s, d, r); mark(1);
+        hipLaunchKernelGGL(k_res_T_coef, dim3(gN), dim3(BS), 0, s, d, r); mark(2);
+        hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); mark(2);
+        hipLaunchKernelGGL(k_res_T, dim3(gN), dim3(BS), 0, s, d, r); mark(2);
+        hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, d); mark(3);
+        for (int p = 0; p < inner; p++) {
+            hipLaunchKernelGGL(k_d_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d, v); mark(4);
+            for (int q = 0; q < 4; q++) {
+                hipLaunchKernelGGL(k_d_mom_sweep, dim3(color_blocks3(c)), BS3, 0, s, d, v, q);
+                mark(5);
+            }
+            hipLaunchKernelGGL(k_d_ppe_src, dim3(gN), dim3(BS), 0, s, d, v); mark(6);
+            for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_d_ppe_sweep, dim3(gC), dim3(BS), 0, s, d, v, q); mark(7); }
+            hipLaunchKernelGGL(k_d_zero_flux, dim3(nblk(2 * c->NT * c->NR)), dim3(BS), 0, s, d, v); mark(8);
+            hipLaunchKernelGGL(k_d_T_src, dim3(gN), dim3(BS), 0, s, d, v); mark(9);
+            for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_d_T_sweep, dim3(gC), dim3(BS), 0, s, d, v, q); mark(10); }
+        }
+        launch_update(c); mark(11);
+        HIPCHK(hipGetLastError());
+        if (ev.size() != cls.size() || ev.empty()) return fail(NIM_E_HIP, "hipEventCreate failed");
+        HIPCHK(hipEventSynchronize(ev.back()));
+        for (size_t q = 1; q < ev.size(); q++) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, ev[q - 1], ev[q]));
+            ms[cls[q]] += t;
+            if (launches && it == 0) launches[cls[q]]++;
+        }
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    for (int q = 0; q < NIM_MIXED_CLASSES; q++) ms[q] /= cycles;
+    HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (c->h_ctl->status == 4) return fail(NIM_E_NONFINITE, "non-finite residual while timing");
     return NIM_OK;
 }

@@ -95,6 +95,43 @@ struct Dev {
     unsigned long long *errs;  // [2][5][ERR_COPIES] at err_slot()
 };
 
+// What a sweep routine does with the equation of one unknown, a phi = S(x):
+//  GS     relax in place, phi <- (1 - w) phi + (w / a) S(x)   (FP64 pass)
+//  RES    write the residual r = S(x0) - a phi0 without moving x0 (FP64)
+//  DELTA  relax the correction, dphi <- (1 - w) dphi + (w / a)(S_lin(dx) + r)
+//         where S_lin drops the terms that do not depend on the unknowns
+//         (previous time level) and r is the residual of the last RES pass
+//         (FP32). x0 + dx then follows the FP64 iteration's sequence, except
+//         that the energy coefficients (exponential fits of the velocity)
+//         stay at x0 until the next RES pass.
+enum { MODE_GS = 0, MODE_RES = 1, MODE_DELTA = 2 };
+
+// The arrays a kernel works on: the FP64 state and constants, or the FP32
+// corrections with FP32 copies of the constants. Slot layouts are those of
+// Dev (gc [NGC][N], AA [45][N], wall [NWS][NB], MF/TF [NCF][N], src
+// [NSRC][N], cells [4][N] at S_CUR, faces [5][NF]).
+template <typename T> struct View {
+    const T *gc, *An, *AA, *wall, *MF, *TF;
+    T *sc;          // u, v, w, T cell values (corrections in DELTA)
+    const T *old;   // previous time level of u, v, w, T; null in DELTA
+    T *fc;          // faces of u, v, w, p, T
+    T *src;
+    float *rc;      // [4][N] residuals of the cell equations (RES writes, DELTA reads)
+    float *rf;      // [5][NF] residuals of the face equations
+    float *TF32;    // RES: FP32 copy of the energy coefficients it builds
+    float *grad;    // [9][N] velocity gradients at x0 (RES writes, DELTA reads)
+};
+
+// the FP64 state of a context as a View (no residual buffers)
+__device__ __host__ inline View<double> view64(const Dev &d)
+{
+    View<double> v;
+    v.gc = d.gc; v.An = d.An; v.AA = d.AA; v.wall = d.wall; v.MF = d.MF; v.TF = d.TF;
+    v.sc = d.sc; v.old = d.sc + (size_t)S_OLD * d.N; v.fc = d.fc; v.src = d.src;
+    v.rc = nullptr; v.rf = nullptr; v.TF32 = nullptr; v.grad = nullptr;
+    return v;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2. The per-iteration kernels renumber them so that XCD x works through
 // one contiguous range of cells (a bijection whatever the dispatch order):

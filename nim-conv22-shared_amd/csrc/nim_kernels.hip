@@ -68,17 +68,19 @@ __device__ __forceinline__ bool color_cell(const Dev &d, int color, int tid, int
 // Weights (1, -1, 0) give the central difference exactly (the products by
 // +-1 and 0 are exact); (-3, 4, -1) and (3, -4, 1) the one-sided forms in
 // the reference's operation order.
-__device__ __forceinline__ double three_point(double f0, double f1, double f2, bool lo, bool hi)
+template <typename T>
+__device__ __forceinline__ T three_point(T f0, T f1, T f2, bool lo, bool hi)
 {
-    double w0 = lo ? -3. : hi ? 3. : 1.;
-    double w1 = lo ? 4. : hi ? -4. : -1.;
-    double w2 = lo ? -1. : hi ? 1. : 0.;
-    return (w0 * f0 + w1 * f1 + w2 * f2) / 4.;
+    T w0 = lo ? T(-3) : hi ? T(3) : T(1);
+    T w1 = lo ? T(4) : hi ? T(-4) : T(-1);
+    T w2 = lo ? T(-1) : hi ? T(1) : T(0);
+    return (w0 * f0 + w1 * f1 + w2 * f2) / T(4);
 }
 
-__device__ __forceinline__ double tder(const Dev &d, const double *F, int fpos, int i, int j, int k, int dir)
+template <typename T>
+__device__ __forceinline__ T tder(const Dev &d, const T *F, int fpos, int i, int j, int k, int dir)
 {
-    if (dir == 0) return (F[fidx(d, fpos, ip1(d, i), j, k)] - F[fidx(d, fpos, im1(d, i), j, k)]) / 4.;
+    if (dir == 0) return (F[fidx(d, fpos, ip1(d, i), j, k)] - F[fidx(d, fpos, im1(d, i), j, k)]) / T(4);
     if (dir == 1) {
         bool lo = j == 0, hi = j == d.NR - 1;
         // central: (j+1, j-1); low wall: (0, 1, 2); high wall: (j, j-1, j-2)
@@ -91,32 +93,33 @@ __device__ __forceinline__ double tder(const Dev &d, const double *F, int fpos, 
 }
 
 // cross-diffusion operator of calculate_R_D2 / _R_D_p2 / _R_D_T
-// (nse.f90:1628-2134)
-__device__ double cross_diffusion(const Dev &d, const double *F, int i, int j, int k, int c, double coef)
+// (nse.f90:1628-2134); g is the per-cell geometry block [NGC][N]
+template <typename T>
+__device__ T cross_diffusion(const Dev &d, const T *g, const T *F, int i, int j, int k, int c, T coef)
 {
     const int N = d.N;
-    const double *g = d.gc;
-    double fxy = 0.5 * (tder(d, F, 2, i, j, k, 0) - tder(d, F, 1, i, j, k, 0));
-    double fzy = 0.5 * (tder(d, F, 6, i, j, k, 0) - tder(d, F, 5, i, j, k, 0));
-    double fyx = 0.5 * (tder(d, F, 4, i, j, k, 1) - tder(d, F, 3, i, j, k, 1));
-    double fzx = 0.5 * (tder(d, F, 6, i, j, k, 1) - tder(d, F, 5, i, j, k, 1));
-    double fxz = 0.5 * (tder(d, F, 2, i, j, k, 2) - tder(d, F, 1, i, j, k, 2));
-    double fyz = 0.5 * (tder(d, F, 4, i, j, k, 2) - tder(d, F, 3, i, j, k, 2));
-    double fx = (F[fidx(d, 2, i, j, k)] - F[fidx(d, 1, i, j, k)]) / 2.;
-    double fy = (F[fidx(d, 4, i, j, k)] - F[fidx(d, 3, i, j, k)]) / 2.;
-    double fz = (F[fidx(d, 6, i, j, k)] - F[fidx(d, 5, i, j, k)]) / 2.;
+    T fxy = T(0.5) * (tder(d, F, 2, i, j, k, 0) - tder(d, F, 1, i, j, k, 0));
+    T fzy = T(0.5) * (tder(d, F, 6, i, j, k, 0) - tder(d, F, 5, i, j, k, 0));
+    T fyx = T(0.5) * (tder(d, F, 4, i, j, k, 1) - tder(d, F, 3, i, j, k, 1));
+    T fzx = T(0.5) * (tder(d, F, 6, i, j, k, 1) - tder(d, F, 5, i, j, k, 1));
+    T fxz = T(0.5) * (tder(d, F, 2, i, j, k, 2) - tder(d, F, 1, i, j, k, 2));
+    T fyz = T(0.5) * (tder(d, F, 4, i, j, k, 2) - tder(d, F, 3, i, j, k, 2));
+    T fx = (F[fidx(d, 2, i, j, k)] - F[fidx(d, 1, i, j, k)]) / T(2);
+    T fy = (F[fidx(d, 4, i, j, k)] - F[fidx(d, 3, i, j, k)]) / T(2);
+    T fz = (F[fidx(d, 6, i, j, k)] - F[fidx(d, 5, i, j, k)]) / T(2);
     return coef * (g[(G_G + 0) * N + c] * fx + g[(G_G + 1) * N + c] * fy + g[(G_G + 2) * N + c] * fz
                    - g[(G_GO + 0) * N + c] * fxy - g[(G_GO + 1) * N + c] * fyx - g[(G_GO + 2) * N + c] * fxz
                    - g[(G_GO + 3) * N + c] * fzx - g[(G_GO + 4) * N + c] * fyz - g[(G_GO + 5) * N + c] * fzy);
 }
 
 // gradient component of a face field by the divergence theorem over the six
-// faces (dphi_dx2..dphi_dz2, nse.f90:2866-2903)
-__device__ __forceinline__ double dphi(const Dev &d, const double phi[6], const int f[6], int comp, double invV)
+// faces (dphi_dx2..dphi_dz2, nse.f90:2866-2903); An is [3][NF]
+template <typename T>
+__device__ __forceinline__ T dphi(const Dev &d, const T *An, const T phi[6], const int f[6], int comp, T invV)
 {
-    const double *A = d.An + (size_t)comp * d.NF;
-    double s = phi[1] * A[f[1]] - phi[0] * A[f[0]] + phi[3] * A[f[3]] - phi[2] * A[f[2]] + phi[5] * A[f[5]]
-               - phi[4] * A[f[4]];
+    const T *A = An + (size_t)comp * d.NF;
+    T s = phi[1] * A[f[1]] - phi[0] * A[f[0]] + phi[3] * A[f[3]] - phi[2] * A[f[2]] + phi[5] * A[f[5]]
+          - phi[4] * A[f[4]];
     return invV * s;
 }
 
@@ -282,9 +285,9 @@ __device__ __forceinline__ void energy_direction(const Dev &d, int c, int dir, d
                   d.v_limit, A);
 }
 
-__global__ void __launch_bounds__(256) k_T_coef(Dev d)
+template <int MODE>
+__device__ __forceinline__ void T_coef(const Dev &d, const View<double> &v)
 {
-    if (d.ctl->done) return;
     int c = xcd_block() * blockDim.x + threadIdx.x;
     bool act = c < d.N;
     double eT = 0.;
@@ -295,18 +298,23 @@ __global__ void __launch_bounds__(256) k_T_coef(Dev d)
         double A[3][6];
 #pragma unroll
         for (int dir = 0; dir < 3; dir++) energy_direction(d, c, dir, A[dir]);
+        // F is the FP64 block; RES also keeps an FP32 copy for the corrections
+        auto put = [&](int slot, double x) {
+            F[slot * N + c] = x;
+            if (MODE == MODE_RES) v.TF32[slot * N + c] = (float)x;
+        };
         double f2 = 0.5 * (1. / A[0][4] + 1. / A[1][4] + 1. / A[2][4] - 1. / d.DT);
         double f3 = -(A[0][5] / A[0][4]), f4 = (A[0][5] - 1.) / A[0][4];
         double f5 = -(A[1][5] / A[1][4]), f6 = (A[1][5] - 1.) / A[1][4];
         double f7 = -(A[2][5] / A[2][4]), f8 = (A[2][5] - 1.) / A[2][4];
-        F[(C_F4 + 0) * N + c] = f2 + f3 + f4 + f5 + f6 + f7 + f8;
-        F[(C_F4 + 1) * N + c] = f2;
-        F[(C_F4 + 2) * N + c] = f3;
-        F[(C_F4 + 3) * N + c] = f4;
-        F[(C_F4 + 4) * N + c] = f5;
-        F[(C_F4 + 5) * N + c] = f6;
-        F[(C_F4 + 6) * N + c] = f7;
-        F[(C_F4 + 7) * N + c] = f8;
+        put(C_F4 + 0, f2 + f3 + f4 + f5 + f6 + f7 + f8);
+        put(C_F4 + 1, f2);
+        put(C_F4 + 2, f3);
+        put(C_F4 + 3, f4);
+        put(C_F4 + 4, f5);
+        put(C_F4 + 5, f6);
+        put(C_F4 + 6, f7);
+        put(C_F4 + 7, f8);
 #pragma unroll
         for (int dir = 0; dir < 3; dir++) {
             int c2;
@@ -320,11 +328,11 @@ __global__ void __launch_bounds__(256) k_T_coef(Dev d)
             double h = A[dir][0] / (2. * A[dir][4]);
             double q = B[1] / (2. * B[4]);
             const int s = dir == 0 ? C_F1 : dir == 1 ? C_F2 : C_F3;
-            F[(s + 0) * N + c] = b + e + 2. * h + 2. * q;
-            F[(s + 1) * N + c] = b;
-            F[(s + 2) * N + c] = e;
-            F[(s + 3) * N + c] = h;
-            F[(s + 4) * N + c] = q;
+            put(s + 0, b + e + 2. * h + 2. * q);
+            put(s + 1, b);
+            put(s + 2, e);
+            put(s + 3, h);
+            put(s + 4, q);
         }
         if (k == 0 || k == d.NZ - 1) {
             double *T = d.fc + (size_t)F_T * d.NF;
@@ -338,13 +346,34 @@ __global__ void __launch_bounds__(256) k_T_coef(Dev d)
             }
             const double *W = d.wall;
             const int NB = d.NB;
-            double nv = (W[(W_ZF + 0) * NB + b] * T[fa] - W[(W_ZF + 1) * NB + b] * T[fb] + W[(W_ZF + 2) * NB + b] * T[fcc])
-                        / W[(W_ZF + 3) * NB + b];
-            eT = fabs(nv - T[fw]);
-            T[fw] = nv;
+            double S = W[(W_ZF + 0) * NB + b] * T[fa] - W[(W_ZF + 1) * NB + b] * T[fb] + W[(W_ZF + 2) * NB + b] * T[fcc];
+            double den = W[(W_ZF + 3) * NB + b];
+            if (MODE == MODE_RES) {
+                // residual of den T_w = S, measured as the change the
+                // assignment would make
+                double r = S - den * T[fw];
+                v.rf[(size_t)F_T * d.NF + fw] = (float)r;
+                eT = fabs(r / den);
+            } else {
+                double nv = S / den;
+                eT = fabs(nv - T[fw]);
+                T[fw] = nv;
+            }
         }
     }
     err_commit(d, F_T, eT);
+}
+
+__global__ void __launch_bounds__(256) k_T_coef(Dev d)
+{
+    if (d.ctl->done) return;
+    T_coef<MODE_GS>(d, view64(d));
+}
+
+__global__ void __launch_bounds__(256) k_res_T_coef(Dev d, View<double> v)
+{
+    if (d.ctl->done) return;
+    T_coef<MODE_RES>(d, v);
 }
 
 // ---------------------------------------------------------------------
@@ -357,9 +386,11 @@ __global__ void __launch_bounds__(256) k_T_coef(Dev d)
 // nse.f90:1422-1428), so it is not evaluated.
 // One wave per field: threadIdx.y selects u, v or w, so the three waves of
 // a block work on the same 64 cells and share their metric loads in cache.
-__global__ void __launch_bounds__(192) k_mom_src(Dev d)
+// For corrections (v.old null) the previous-level part of the buoyancy is
+// dropped: every other term is linear in the unknowns.
+template <typename T>
+__device__ __forceinline__ void mom_src(const Dev &d, const View<T> &v)
 {
-    if (d.ctl->done) return;
     int c = xcd_block() * 64 + threadIdx.x;
     const int q = threadIdx.y;
     if (c >= d.N) return;
@@ -367,59 +398,121 @@ __global__ void __launch_bounds__(192) k_mom_src(Dev d)
     int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     int f[6];
     cell_faces(d, i, j, k, f);
-    const double *P = d.fc + (size_t)F_P * d.NF;
-    double pf[6];
+    const T *P = v.fc + (size_t)F_P * d.NF;
+    T pf[6];
 #pragma unroll
     for (int p = 0; p < 6; p++) pf[p] = P[f[p]];
-    double invV = d.gc[G_INVV * N + c];
-    double b = 0.;
-    if (q == 2) b = -(d.g_beta * 0.5 * (d.sc[(S_CUR + 3) * N + c] + d.sc[(S_OLD + 3) * N + c]));
-    double RB = b + d.rinv * dphi(d, pf, f, q, invV);
-    double RD = cross_diffusion(d, d.fc + (size_t)q * d.NF, i, j, k, c, d.Rinv);
-    d.src[q * N + c] = RD + RB;
+    T invV = v.gc[G_INVV * N + c];
+    T b = T(0);
+    if (q == 2) b = -(T(d.g_beta) * T(0.5) * (v.sc[3 * N + c] + (v.old ? v.old[3 * N + c] : T(0))));
+    T RB = b + T(d.rinv) * dphi(d, v.An, pf, f, q, invV);
+    T RD = cross_diffusion(d, v.gc, v.fc + (size_t)q * d.NF, i, j, k, c, T(d.Rinv));
+    v.src[q * N + c] = RD + RB;
+}
+
+__global__ void __launch_bounds__(192) k_mom_src(Dev d)
+{
+    if (d.ctl->done) return;
+    mom_src(d, view64(d));
+}
+
+__global__ void __launch_bounds__(192) k_d_mom_src(Dev d, View<float> v)
+{
+    if (d.ctl->done) return;
+    mom_src(d, v);
 }
 
 // update_pressure_sources (nse.f90:228-235): calculate_R_b_p
 // (nse.f90:2362-2407; faces_bx = faces_by = 0, faces_bz = -g_beta*faces_T)
 // and calculate_R_D_p2; stores R_D_p + R_b_p, the combination the pressure
-// sweep and wall update use, and the dilatation
-__global__ void __launch_bounds__(256) k_ppe_src(Dev d)
+// sweep and wall update use, and (FP64) the dilatation. R_b_p is quadratic
+// in the velocity gradients: the residual pass keeps the gradients at x0
+// and the corrections use the exact expansion R_b_p(x0 + dx) - R_b_p(x0)
+// (e.g. (a0 + da)^2 - a0^2 = 2 a0 da + da^2), which is small in dx.
+template <int MODE, typename T>
+__device__ __forceinline__ void ppe_src(const Dev &d, const View<T> &v)
 {
-    if (d.ctl->done) return;
     int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
     const int N = d.N;
     int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     int f[6];
     cell_faces(d, i, j, k, f);
-    double uf[6], vf[6], wf[6], bzf[6];
-    const double *U = d.fc, *V = d.fc + d.NF, *W = d.fc + 2 * (size_t)d.NF, *T = d.fc + 4 * (size_t)d.NF;
+    T uf[6], vf[6], wf[6], bzf[6];
+    const T *U = v.fc, *V = v.fc + d.NF, *W = v.fc + 2 * (size_t)d.NF, *Tf = v.fc + 4 * (size_t)d.NF;
     for (int p = 0; p < 6; p++) {
         uf[p] = U[f[p]];
         vf[p] = V[f[p]];
         wf[p] = W[f[p]];
-        bzf[p] = -(d.g_beta * T[f[p]]);
+        bzf[p] = -(T(d.g_beta) * Tf[f[p]]);
     }
-    double invV = d.gc[G_INVV * N + c];
-    double ux = dphi(d, uf, f, 0, invV), uy = dphi(d, uf, f, 1, invV), uz = dphi(d, uf, f, 2, invV);
-    double vx = dphi(d, vf, f, 0, invV), vy = dphi(d, vf, f, 1, invV), vz = dphi(d, vf, f, 2, invV);
-    double wx = dphi(d, wf, f, 0, invV), wy = dphi(d, wf, f, 1, invV), wz = dphi(d, wf, f, 2, invV);
-    double bzz = dphi(d, bzf, f, 2, invV);
-    double Dila = ux + vy + wz;
-    d.sc[S_DILA * N + c] = Dila;
-    double Rbp = -(ux * ux + vy * vy + wz * wz) - 2.0 * (uy * vx + vz * wy + wx * uz) - bzz + Dila / (2 * d.DT);
-    double RDp = cross_diffusion(d, d.fc + (size_t)F_P * d.NF, i, j, k, c, d.rinv);
-    d.src[R_P * N + c] = RDp + Rbp;
+    T invV = v.gc[G_INVV * N + c];
+    T ux = dphi(d, v.An, uf, f, 0, invV), uy = dphi(d, v.An, uf, f, 1, invV), uz = dphi(d, v.An, uf, f, 2, invV);
+    T vx = dphi(d, v.An, vf, f, 0, invV), vy = dphi(d, v.An, vf, f, 1, invV), vz = dphi(d, v.An, vf, f, 2, invV);
+    T wx = dphi(d, v.An, wf, f, 0, invV), wy = dphi(d, v.An, wf, f, 1, invV), wz = dphi(d, v.An, wf, f, 2, invV);
+    T bzz = dphi(d, v.An, bzf, f, 2, invV);
+    T Dila = ux + vy + wz;
+    T Rbp;
+    if (MODE == MODE_DELTA) {
+        const float *g0 = v.grad;
+        T ux0 = g0[0 * N + c], uy0 = g0[1 * N + c], uz0 = g0[2 * N + c];
+        T vx0 = g0[3 * N + c], vy0 = g0[4 * N + c], vz0 = g0[5 * N + c];
+        T wx0 = g0[6 * N + c], wy0 = g0[7 * N + c], wz0 = g0[8 * N + c];
+        T sq = (T(2) * ux0 + ux) * ux + (T(2) * vy0 + vy) * vy + (T(2) * wz0 + wz) * wz;
+        T cr = uy0 * vx + uy * vx0 + uy * vx + vz0 * wy + vz * wy0 + vz * wy + wx0 * uz + wx * uz0 + wx * uz;
+        Rbp = -sq - T(2) * cr - bzz + Dila / (T(2) * T(d.DT));
+    } else {
+        if (MODE == MODE_GS) v.sc[S_DILA * N + c] = Dila;
+        if (MODE == MODE_RES) {
+            float *g0 = v.grad;
+            g0[0 * N + c] = (float)ux; g0[1 * N + c] = (float)uy; g0[2 * N + c] = (float)uz;
+            g0[3 * N + c] = (float)vx; g0[4 * N + c] = (float)vy; g0[5 * N + c] = (float)vz;
+            g0[6 * N + c] = (float)wx; g0[7 * N + c] = (float)wy; g0[8 * N + c] = (float)wz;
+        }
+        Rbp = -(ux * ux + vy * vy + wz * wz) - T(2) * (uy * vx + vz * wy + wx * uz) - bzz + Dila / (T(2) * T(d.DT));
+    }
+    T RDp = cross_diffusion(d, v.gc, v.fc + (size_t)F_P * d.NF, i, j, k, c, T(d.rinv));
+    v.src[R_P * N + c] = RDp + Rbp;
+}
+
+__global__ void __launch_bounds__(256) k_ppe_src(Dev d)
+{
+    if (d.ctl->done) return;
+    ppe_src<MODE_GS>(d, view64(d));
+}
+
+__global__ void __launch_bounds__(256) k_res_ppe_src(Dev d, View<double> v)
+{
+    if (d.ctl->done) return;
+    ppe_src<MODE_RES>(d, v);
+}
+
+__global__ void __launch_bounds__(256) k_d_ppe_src(Dev d, View<float> v)
+{
+    if (d.ctl->done) return;
+    ppe_src<MODE_DELTA>(d, v);
 }
 
 // update_temperature_sources (nse.f90:249-256): R_b_T = q = 0, R_D_T
-__global__ void __launch_bounds__(256) k_T_src(Dev d)
+template <typename T>
+__device__ __forceinline__ void T_src(const Dev &d, const View<T> &v)
 {
-    if (d.ctl->done) return;
     int c = xcd_block() * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
     int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
-    d.src[R_T * d.N + c] = cross_diffusion(d, d.fc + (size_t)F_T * d.NF, i, j, k, c, d.alpha) + 0.;
+    v.src[R_T * d.N + c] = cross_diffusion(d, v.gc, v.fc + (size_t)F_T * d.NF, i, j, k, c, T(d.alpha)) + T(0);
+}
+
+__global__ void __launch_bounds__(256) k_T_src(Dev d)
+{
+    if (d.ctl->done) return;
+    T_src(d, view64(d));
+}
+
+__global__ void __launch_bounds__(256) k_d_T_src(Dev d, View<float> v)
+{
+    if (d.ctl->done) return;
+    T_src(d, v);
 }
 
 // ---------------------------------------------------------------------
@@ -433,165 +526,265 @@ __global__ void __launch_bounds__(256) k_T_src(Dev d)
 // carried in registers to the later updates that read them, as the
 // reference's in-place order does. No other cell of the colour writes any
 // location read here, so loading early changes no value.
-__device__ void transport_unit(const Dev &d, double *__restrict__ Fc, double *__restrict__ C,
-                               const double *__restrict__ CO, const double *__restrict__ coef, double B, double w, int i,
-                               int j, int k, int c, double &ecell, double &eface)
+// ff is the face field (F_U..F_T), cs the cell slot (0..3).
+template <int MODE, typename T>
+__device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, const T *__restrict__ coef, T B, T w,
+                               int i, int j, int k, int c, double &ecell, double &eface)
 {
     const int N = d.N;
+    T *__restrict__ Fc = vw.fc + (size_t)ff * d.NF;
+    T *__restrict__ C = vw.sc + (size_t)cs * N;
+    const T *__restrict__ CO = MODE == MODE_DELTA ? nullptr : vw.old + (size_t)cs * N;
     const bool has2 = j < d.NR - 1, has6 = k < d.NZ - 1;
     // neighbour rows clamped at the walls: the loads stay in bounds and the
     // results are discarded there
     const int jn = has2 ? j + 1 : j, kn = has6 ? k + 1 : k, ip = ip1(d, i);
-    const double *gdf = d.gc + (size_t)G_DIFF * N;
+    const T *gdf = vw.gc + (size_t)G_DIFF * N;
     int f[6];
     cell_faces(d, i, j, k, f);
     // ---- gather
-    const double old = C[c], oc = CO[c];
-    double a[8];
+    const T old = C[c], oc = MODE == MODE_DELTA ? T(0) : CO[c];
+    T a[8];
 #pragma unroll
     for (int m = 0; m < 8; m++) a[m] = coef[(C_F4 + m) * N + c];
-    double F0 = Fc[f[0]], F1 = Fc[f[1]], F2 = Fc[f[2]], F3 = Fc[f[3]], F4 = Fc[f[4]], F5 = Fc[f[5]];
-    double b[5], e[5], h[5];
+    T F0 = Fc[f[0]], F1 = Fc[f[1]], F2 = Fc[f[2]], F3 = Fc[f[3]], F4 = Fc[f[4]], F5 = Fc[f[5]];
+    T b[5], e[5], h[5];
 #pragma unroll
     for (int m = 0; m < 5; m++) {
         b[m] = coef[(C_F1 + m) * N + c];
         e[m] = coef[(C_F2 + m) * N + c];
         h[m] = coef[(C_F3 + m) * N + c];
     }
-    const double gd0 = gdf[0 * N + c], gd1 = gdf[1 * N + c], gd2 = gdf[2 * N + c];
-    const double gd3 = gdf[3 * N + c], gd4 = gdf[4 * N + c], gd5 = gdf[5 * N + c];
+    const T gd0 = gdf[0 * N + c], gd1 = gdf[1 * N + c], gd2 = gdf[2 * N + c];
+    const T gd3 = gdf[3 * N + c], gd4 = gdf[4 * N + c], gd5 = gdf[5 * N + c];
     // face 2 partner: cell (i, j+1, k)
     const int cr = cidx(d, i, jn, k);
-    const double r2 = Fc[fidx(d, 2, i, jn, k)], r3 = Fc[fidx(d, 3, i, jn, k)], r4 = Fc[fidx(d, 4, i, jn, k)];
-    const double r5 = Fc[fidx(d, 5, i, jn, k)], r6 = Fc[fidx(d, 6, i, jn, k)];
-    const double sr = C[cr] + CO[cr];
+    const T r2 = Fc[fidx(d, 2, i, jn, k)], r3 = Fc[fidx(d, 3, i, jn, k)], r4 = Fc[fidx(d, 4, i, jn, k)];
+    const T r5 = Fc[fidx(d, 5, i, jn, k)], r6 = Fc[fidx(d, 6, i, jn, k)];
+    const T sr = MODE == MODE_DELTA ? C[cr] : C[cr] + CO[cr];
     // face 4 partner: cell (i+1, j, k), periodic
     const int ct = cidx(d, ip, j, k);
-    const double t1 = Fc[fidx(d, 1, ip, j, k)], t2 = Fc[fidx(d, 2, ip, j, k)], t4 = Fc[fidx(d, 4, ip, j, k)];
-    const double t5 = Fc[fidx(d, 5, ip, j, k)], t6 = Fc[fidx(d, 6, ip, j, k)];
-    const double st = C[ct] + CO[ct];
+    const T t1 = Fc[fidx(d, 1, ip, j, k)], t2 = Fc[fidx(d, 2, ip, j, k)], t4 = Fc[fidx(d, 4, ip, j, k)];
+    const T t5 = Fc[fidx(d, 5, ip, j, k)], t6 = Fc[fidx(d, 6, ip, j, k)];
+    const T st = MODE == MODE_DELTA ? C[ct] : C[ct] + CO[ct];
     // face 6 partner: cell (i, j, k+1)
     const int cz = cidx(d, i, j, kn);
-    const double z1 = Fc[fidx(d, 1, i, j, kn)], z2 = Fc[fidx(d, 2, i, j, kn)], z3 = Fc[fidx(d, 3, i, j, kn)];
-    const double z4 = Fc[fidx(d, 4, i, j, kn)], z6 = Fc[fidx(d, 6, i, j, kn)];
-    const double sz = C[cz] + CO[cz];
+    const T z1 = Fc[fidx(d, 1, i, j, kn)], z2 = Fc[fidx(d, 2, i, j, kn)], z3 = Fc[fidx(d, 3, i, j, kn)];
+    const T z4 = Fc[fidx(d, 4, i, j, kn)], z6 = Fc[fidx(d, 6, i, j, kn)];
+    const T sz = MODE == MODE_DELTA ? C[cz] : C[cz] + CO[cz];
+    float *rF = MODE == MODE_GS ? nullptr : vw.rf + (size_t)ff * d.NF;
+    float *rC = MODE == MODE_GS ? nullptr : vw.rc + (size_t)cs * N;
+    T q0 = T(0), q2 = T(0), q4 = T(0), q6 = T(0);
+    if (MODE == MODE_DELTA) {
+        q0 = rC[c];
+        q2 = rF[f[1]];
+        q4 = rF[f[3]];
+        q6 = rF[f[5]];
+    }
     // ---- cell average
-    const double nv = (1 - w) * old + (w / a[0])
-                                          * (a[1] * oc + a[2] * F0 + a[3] * F1 + a[4] * F2 + a[5] * F3 + a[6] * F4
-                                             + a[7] * F5 + B);
-    ecell = fmax(ecell, fabs(nv - old));
-    const double ccur = nv + oc;
+    const T S0 = a[1] * oc + a[2] * F0 + a[3] * F1 + a[4] * F2 + a[5] * F3 + a[6] * F4 + a[7] * F5 + B;
+    T nv = old;
+    if (MODE == MODE_RES) {
+        T r = S0 - a[0] * old;
+        rC[c] = (float)r;
+        ecell = fmax(ecell, fabs((double)(w * r / a[0])));
+    } else {
+        nv = (1 - w) * old + (w / a[0]) * (MODE == MODE_DELTA ? S0 + q0 : S0);
+        ecell = fmax(ecell, fabs((double)(nv - old)));
+    }
+    const T ccur = nv + oc;
     // ---- face 2
     if (has2) {
-        double B3 = gd0 * (F3 - F2 + r4 - r3) / 4.0;
-        double B4 = gd1 * (F5 - F4 + r6 - r5) / 4.0;
-        double v = (1 - w) * F1 + (w / b[0]) * (b[1] * F0 + b[2] * r2 + b[3] * ccur + b[4] * sr - B3 - B4);
-        eface = fmax(eface, fabs(v - F1));
-        F1 = v;
+        T B3 = gd0 * (F3 - F2 + r4 - r3) / T(4);
+        T B4 = gd1 * (F5 - F4 + r6 - r5) / T(4);
+        T S = b[1] * F0 + b[2] * r2 + b[3] * ccur + b[4] * sr - B3 - B4;
+        if (MODE == MODE_RES) {
+            T r = S - b[0] * F1;
+            rF[f[1]] = (float)r;
+            eface = fmax(eface, fabs((double)(w * r / b[0])));
+        } else {
+            T v = (1 - w) * F1 + (w / b[0]) * (MODE == MODE_DELTA ? S + q2 : S);
+            eface = fmax(eface, fabs((double)(v - F1)));
+            F1 = v;
+        }
     }
     // ---- face 4
     {
-        double B3 = gd2 * (F1 - F0 + t2 - t1) / 4.0;
-        double B4 = gd3 * (F5 - F4 + t6 - t5) / 4.0;
-        double v = (1 - w) * F3 + (w / e[0]) * (e[1] * F2 + e[2] * t4 + e[3] * ccur + e[4] * st - B3 - B4);
-        eface = fmax(eface, fabs(v - F3));
-        F3 = v;
+        T B3 = gd2 * (F1 - F0 + t2 - t1) / T(4);
+        T B4 = gd3 * (F5 - F4 + t6 - t5) / T(4);
+        T S = e[1] * F2 + e[2] * t4 + e[3] * ccur + e[4] * st - B3 - B4;
+        if (MODE == MODE_RES) {
+            T r = S - e[0] * F3;
+            rF[f[3]] = (float)r;
+            eface = fmax(eface, fabs((double)(w * r / e[0])));
+        } else {
+            T v = (1 - w) * F3 + (w / e[0]) * (MODE == MODE_DELTA ? S + q4 : S);
+            eface = fmax(eface, fabs((double)(v - F3)));
+            F3 = v;
+        }
     }
     // ---- face 6
     if (has6) {
-        double B3 = gd4 * (F1 - F0 + z2 - z1) / 4.0;
-        double B4 = gd5 * (F3 - F2 + z4 - z3) / 4.0;
-        double v = (1 - w) * F5 + (w / h[0]) * (h[1] * F4 + h[2] * z6 + h[3] * ccur + h[4] * sz - B3 - B4);
-        eface = fmax(eface, fabs(v - F5));
-        F5 = v;
+        T B3 = gd4 * (F1 - F0 + z2 - z1) / T(4);
+        T B4 = gd5 * (F3 - F2 + z4 - z3) / T(4);
+        T S = h[1] * F4 + h[2] * z6 + h[3] * ccur + h[4] * sz - B3 - B4;
+        if (MODE == MODE_RES) {
+            T r = S - h[0] * F5;
+            rF[f[5]] = (float)r;
+            eface = fmax(eface, fabs((double)(w * r / h[0])));
+        } else {
+            T v = (1 - w) * F5 + (w / h[0]) * (MODE == MODE_DELTA ? S + q6 : S);
+            eface = fmax(eface, fabs((double)(v - F5)));
+            F5 = v;
+        }
     }
     // ---- scatter
-    C[c] = nv;
-    if (has2) Fc[f[1]] = F1;
-    Fc[f[3]] = F3;
-    if (has6) Fc[f[5]] = F5;
+    if (MODE != MODE_RES) {
+        C[c] = nv;
+        if (has2) Fc[f[1]] = F1;
+        Fc[f[3]] = F3;
+        if (has6) Fc[f[5]] = F5;
+    }
+}
+
+// cell of this thread: one colour of a relaxation phase, or (RES, color < 0)
+// every cell
+template <int MODE>
+__device__ __forceinline__ bool sweep_cell(const Dev &d, int color, int tid, int &i, int &j, int &k)
+{
+    if (MODE != MODE_RES) return color_cell(d, color, tid, i, j, k);
+    if (tid >= d.N) return false;
+    i = iof(d, tid % d.NT);
+    j = (tid / d.NT) % d.NR;
+    k = tid / (d.NT * d.NR);
+    return true;
 }
 
 // threadIdx.y selects the field (u, v, w are independent within the sweep)
-__global__ void __launch_bounds__(192) k_mom_sweep(Dev d, int color)
+template <int MODE, typename T>
+__device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, int color)
 {
-    if (d.ctl->done) return;
     int i, j, k;
     const int q = threadIdx.y;
-    bool act = color_cell(d, color, xcd_block() * 64 + threadIdx.x, i, j, k);
+    bool act = sweep_cell<MODE>(d, color, xcd_block() * 64 + threadIdx.x, i, j, k);
     double e = 0.;
     if (act) {
         const int N = d.N;
         int c = cidx(d, i, j, k);
-        const double w = q == 0 ? d.w_u : q == 1 ? d.w_v : d.w_w;
+        const T w = T(q == 0 ? d.w_u : q == 1 ? d.w_v : d.w_w);
         double ec = 0., ef = 0.;
-        transport_unit(d, d.fc + (size_t)q * d.NF, d.sc + (size_t)(S_CUR + q) * N, d.sc + (size_t)(S_OLD + q) * N,
-                       d.MF, d.src[q * N + c], w, i, j, k, c, ec, ef);
+        transport_unit<MODE>(d, v, q, q, v.MF, v.src[q * N + c], w, i, j, k, c, ec, ef);
         e = fmax(ec, ef);
     }
-    err_commit(d, q, e);
+    if (MODE != MODE_DELTA) err_commit(d, q, e);
+}
+
+__global__ void __launch_bounds__(192) k_mom_sweep(Dev d, int color)
+{
+    if (d.ctl->done) return;
+    mom_sweep<MODE_GS>(d, view64(d), color);
+}
+
+__global__ void __launch_bounds__(192) k_res_mom(Dev d, View<double> v)
+{
+    if (d.ctl->done) return;
+    mom_sweep<MODE_RES>(d, v, -1);
+}
+
+__global__ void __launch_bounds__(192) k_d_mom_sweep(Dev d, View<float> v, int color)
+{
+    if (d.ctl->done) return;
+    mom_sweep<MODE_DELTA>(d, v, color);
+}
+
+template <int MODE, typename T>
+__device__ __forceinline__ void T_sweep(const Dev &d, const View<T> &v, const T *coef, int color)
+{
+    int i, j, k;
+    bool act = sweep_cell<MODE>(d, color, xcd_block() * blockDim.x + threadIdx.x, i, j, k);
+    double e = 0.;
+    if (act) {
+        const int N = d.N;
+        int c = cidx(d, i, j, k);
+        double ec = 0., ef = 0.;
+        transport_unit<MODE>(d, v, F_T, 3, coef, v.src[R_T * N + c], T(d.w_T), i, j, k, c, ec, ef);
+        e = fmax(ec, ef);
+    }
+    if (MODE != MODE_DELTA) err_commit(d, F_T, e);
 }
 
 __global__ void __launch_bounds__(256) k_T_sweep(Dev d, int color)
 {
     if (d.ctl->done) return;
-    int i, j, k;
-    bool act = color_cell(d, color, xcd_block() * blockDim.x + threadIdx.x, i, j, k);
-    double e = 0.;
-    if (act) {
-        const int N = d.N;
-        int c = cidx(d, i, j, k);
-        double ec = 0., ef = 0.;
-        transport_unit(d, d.fc + (size_t)F_T * d.NF, d.sc + (size_t)(S_CUR + 3) * N, d.sc + (size_t)(S_OLD + 3) * N,
-                       d.TF, d.src[R_T * N + c], d.w_T, i, j, k, c, ec, ef);
-        e = fmax(ec, ef);
-    }
-    err_commit(d, F_T, e);
+    View<double> v = view64(d);
+    T_sweep<MODE_GS>(d, v, v.TF, color);
+}
+
+__global__ void __launch_bounds__(256) k_res_T(Dev d, View<double> v)
+{
+    if (d.ctl->done) return;
+    T_sweep<MODE_RES>(d, v, v.TF, -1);
+}
+
+__global__ void __launch_bounds__(256) k_d_T_sweep(Dev d, View<float> v, int color)
+{
+    if (d.ctl->done) return;
+    T_sweep<MODE_DELTA>(d, v, v.TF, color);
 }
 
 // apply_pressure_bc for the wall face at position pos of cell (i,j,k)
-// (bc.f90:147-341)
-__device__ double pressure_wall(const Dev &d, int b, int pos, int i, int j, int k, int c)
+// (bc.f90:147-341). Every term is linear in the unknowns (p, u, v, w, T and
+// the pressure source), so the correction form is the same formula on the
+// corrections plus the residual.
+template <int MODE, typename T>
+__device__ void pressure_wall(const Dev &d, const View<T> &vw, int b, int pos, int i, int j, int k, int c, double &e)
 {
     const int NB = d.NB;
-    const double *W = d.wall;
-    double *P = d.fc + (size_t)F_P * d.NF;
-    const double *U = d.fc, *V = d.fc + d.NF, *Wv = d.fc + 2 * (size_t)d.NF, *T = d.fc + 4 * (size_t)d.NF;
-    double n1 = W[(W_N + 0) * NB + b], n2 = W[(W_N + 1) * NB + b], n3 = W[(W_N + 2) * NB + b];
+    const T *W = vw.wall;
+    T *P = vw.fc + (size_t)F_P * d.NF;
+    const T *U = vw.fc, *V = vw.fc + d.NF, *Wv = vw.fc + 2 * (size_t)d.NF, *Tf = vw.fc + 4 * (size_t)d.NF;
+    T n1 = W[(W_N + 0) * NB + b], n2 = W[(W_N + 1) * NB + b], n3 = W[(W_N + 2) * NB + b];
     int fw = fidx(d, pos, i, j, k);
-    double b_n = 0. * n1 + 0. * n2 + (-(d.g_beta * T[fw])) * n3;
-    double d2 = 0.;
+    T b_n = T(0) * n1 + T(0) * n2 + (-(T(d.g_beta) * Tf[fw])) * n3;
+    T d2 = T(0);
     for (int m = 0; m < 5; m++) {
         int q = d.wpts[m * NB + b];
-        double vn = U[q] * n1 + V[q] * n2 + Wv[q] * n3;
+        T vn = U[q] * n1 + V[q] * n2 + Wv[q] * n3;
         d2 = m == 0 ? -(W[(W_A + 0) * NB + b] * vn) : d2 - W[(W_A + m) * NB + b] * vn;
     }
-    double gB2 = W[W_GB2 * NB + b], gB2c = W[W_GB2C * NB + b], gB3 = W[W_GB3 * NB + b];
-    double B2, B3;
+    T gB2 = W[W_GB2 * NB + b], gB2c = W[W_GB2C * NB + b], gB3 = W[W_GB3 * NB + b];
+    T B2, B3;
     if (pos <= 2) {
         // tangential directions theta (always central) and z
-        B2 = tder(d, P, pos, i, j, k, 0) * gB2c;
-        B3 = tder(d, P, pos, i, j, k, 2) * gB3;
+        B2 = tder(d, (const T *)P, pos, i, j, k, 0) * gB2c;
+        B3 = tder(d, (const T *)P, pos, i, j, k, 2) * gB3;
     } else {
         // tangential directions r and theta (always central); the r
         // derivative takes a different metric factor in its central branch
-        B2 = tder(d, P, pos, i, j, k, 1) * (j == 0 || j == d.NR - 1 ? gB2 : gB2c);
-        B3 = tder(d, P, pos, i, j, k, 0) * gB3;
+        B2 = tder(d, (const T *)P, pos, i, j, k, 1) * (j == 0 || j == d.NR - 1 ? gB2 : gB2c);
+        B3 = tder(d, (const T *)P, pos, i, j, k, 0) * gB3;
     }
-    double B1 = W[(W_AP + 7) * NB + b] * d.src[R_P * d.N + c];
-    double s = 0.;
+    T B1 = W[(W_AP + 7) * NB + b] * vw.src[R_P * d.N + c];
+    T s = T(0);
     bool first = true;
     for (int q = 1; q <= 6; q++) {
         if (q == pos) continue;
-        double t = W[(W_AP + q - 1) * NB + b] * P[fidx(d, q, i, j, k)];
+        T t = W[(W_AP + q - 1) * NB + b] * P[fidx(d, q, i, j, k)];
         s = first ? -t : s - t;
         first = false;
     }
-    double o = P[fw];
-    double v = (1 - d.w_p2) * o + d.w_p2 / W[(W_AP + pos - 1) * NB + b]
-                                      * (s + B1 - B2 - B3 + (d.nu * d.rho * d2 - b_n * d.rho));
-    P[fw] = v;
-    return fabs(v - o);
+    const T o = P[fw], ap = W[(W_AP + pos - 1) * NB + b], w2 = T(d.w_p2);
+    const T S = s + B1 - B2 - B3 + (T(d.nu) * T(d.rho) * d2 - b_n * T(d.rho));
+    float *rF = MODE == MODE_GS ? nullptr : vw.rf + (size_t)F_P * d.NF;
+    if (MODE == MODE_RES) {
+        T r = S - ap * o;
+        rF[fw] = (float)r;
+        e = fmax(e, fabs((double)(w2 * r / ap)));
+    } else {
+        T v = (1 - w2) * o + w2 / ap * (MODE == MODE_DELTA ? S + rF[fw] : S);
+        P[fw] = v;
+        e = fmax(e, fabs((double)(v - o)));
+    }
 }
 
 // integrate_pressure_field_i (nse.f90:1091-1289) for one cell, preceded by
@@ -599,34 +792,34 @@ __device__ double pressure_wall(const Dev &d, int b, int pos, int i, int j, int 
 // After the walls, every input of the three interior face updates is
 // gathered before the first store; the cell's own faces are carried in
 // registers, so face 4 sees the new face 2 and face 6 the new faces 2 and 4.
-__global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, int color)
+template <int MODE, typename T>
+__device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, int color)
 {
-    if (d.ctl->done) return;
     int i, j, k;
-    bool act = color_cell(d, color, xcd_block() * blockDim.x + threadIdx.x, i, j, k);
+    bool act = sweep_cell<MODE>(d, color, xcd_block() * blockDim.x + threadIdx.x, i, j, k);
     double e = 0.;
     if (act) {
         const int N = d.N, NT = d.NT, NR = d.NR, NZ = d.NZ;
         int c = cidx(d, i, j, k);
-        if (k == 0) e = fmax(e, pressure_wall(d, i + NT * j, 5, i, j, k, c));
-        if (k == NZ - 1) e = fmax(e, pressure_wall(d, NT * NR + i + NT * j, 6, i, j, k, c));
-        if (j == 0) e = fmax(e, pressure_wall(d, 2 * NT * NR + i + NT * k, 1, i, j, k, c));
-        if (j == NR - 1) e = fmax(e, pressure_wall(d, 2 * NT * NR + NT * NZ + i + NT * k, 2, i, j, k, c));
-        double *__restrict__ P = d.fc + (size_t)F_P * d.NF;
-        const double *__restrict__ gdf = d.gc + (size_t)G_DIFF * N;
-        const double *__restrict__ R = d.src + (size_t)R_P * N;
-        const double w = d.w_p;
+        if (k == 0) pressure_wall<MODE>(d, vw, i + NT * j, 5, i, j, k, c, e);
+        if (k == NZ - 1) pressure_wall<MODE>(d, vw, NT * NR + i + NT * j, 6, i, j, k, c, e);
+        if (j == 0) pressure_wall<MODE>(d, vw, 2 * NT * NR + i + NT * k, 1, i, j, k, c, e);
+        if (j == NR - 1) pressure_wall<MODE>(d, vw, 2 * NT * NR + NT * NZ + i + NT * k, 2, i, j, k, c, e);
+        T *__restrict__ P = vw.fc + (size_t)F_P * d.NF;
+        const T *__restrict__ gdf = vw.gc + (size_t)G_DIFF * N;
+        const T *__restrict__ R = vw.src + (size_t)R_P * N;
+        const T w = T(d.w_p);
         const bool hx = j < NR - 1, hz = k < NZ - 1;
         const int jn = hx ? j + 1 : j, kn = hz ? k + 1 : k, ip = ip1(d, i);
         int f[6];
         cell_faces(d, i, j, k, f);
         // ---- gather
-        double pf[6];
+        T pf[6];
 #pragma unroll
         for (int p = 0; p < 6; p++) pf[p] = P[f[p]];
-        const double S1 = R[c];
-        const double *__restrict__ Ax = d.AA, *__restrict__ Ay = d.AA + (size_t)15 * N, *__restrict__ Az = d.AA + (size_t)30 * N;
-        double ax[15], ay[15], az[15];
+        const T S1 = R[c];
+        const T *__restrict__ Ax = vw.AA, *__restrict__ Ay = vw.AA + (size_t)15 * N, *__restrict__ Az = vw.AA + (size_t)30 * N;
+        T ax[15], ay[15], az[15];
 #pragma unroll
         for (int m = 0; m < 15; m++) {
             if (m == 11 || m == 13) continue;  // AA(12) and AA(14) are not used by the sweep
@@ -635,51 +828,140 @@ __global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, int color)
             az[m] = Az[m * N + c];
         }
         // x: faces of cell (i, j+1, k)
-        const double Sx = R[cidx(d, i, jn, k)];
-        const double x2 = P[fidx(d, 2, i, jn, k)], x3 = P[fidx(d, 3, i, jn, k)], x4 = P[fidx(d, 4, i, jn, k)];
-        const double x5 = P[fidx(d, 5, i, jn, k)], x6 = P[fidx(d, 6, i, jn, k)];
-        const double Bx3 = gdf[0 * N + c] * tder(d, P, 2, i, j, k, 0), Bx4 = gdf[1 * N + c] * tder(d, P, 2, i, j, k, 2);
+        const T Sx = R[cidx(d, i, jn, k)];
+        const T x2 = P[fidx(d, 2, i, jn, k)], x3 = P[fidx(d, 3, i, jn, k)], x4 = P[fidx(d, 4, i, jn, k)];
+        const T x5 = P[fidx(d, 5, i, jn, k)], x6 = P[fidx(d, 6, i, jn, k)];
+        const T Bx3 = gdf[0 * N + c] * tder(d, (const T *)P, 2, i, j, k, 0);
+        const T Bx4 = gdf[1 * N + c] * tder(d, (const T *)P, 2, i, j, k, 2);
         // y: faces of cell (i+1, j, k)
-        const double Sy = R[cidx(d, ip, j, k)];
-        const double y1 = P[fidx(d, 1, ip, j, k)], y2 = P[fidx(d, 2, ip, j, k)], y4 = P[fidx(d, 4, ip, j, k)];
-        const double y5 = P[fidx(d, 5, ip, j, k)], y6 = P[fidx(d, 6, ip, j, k)];
-        const double By3 = gdf[2 * N + c] * tder(d, P, 4, i, j, k, 1), By4 = gdf[3 * N + c] * tder(d, P, 4, i, j, k, 2);
+        const T Sy = R[cidx(d, ip, j, k)];
+        const T y1 = P[fidx(d, 1, ip, j, k)], y2 = P[fidx(d, 2, ip, j, k)], y4 = P[fidx(d, 4, ip, j, k)];
+        const T y5 = P[fidx(d, 5, ip, j, k)], y6 = P[fidx(d, 6, ip, j, k)];
+        const T By3 = gdf[2 * N + c] * tder(d, (const T *)P, 4, i, j, k, 1);
+        const T By4 = gdf[3 * N + c] * tder(d, (const T *)P, 4, i, j, k, 2);
         // z: faces of cell (i, j, k+1)
-        const double Sz = R[cidx(d, i, j, kn)];
-        const double z1 = P[fidx(d, 1, i, j, kn)], z2 = P[fidx(d, 2, i, j, kn)], z3 = P[fidx(d, 3, i, j, kn)];
-        const double z4 = P[fidx(d, 4, i, j, kn)], z6 = P[fidx(d, 6, i, j, kn)];
-        const double Bz3 = gdf[4 * N + c] * tder(d, P, 6, i, j, k, 1), Bz4 = gdf[5 * N + c] * tder(d, P, 6, i, j, k, 0);
+        const T Sz = R[cidx(d, i, j, kn)];
+        const T z1 = P[fidx(d, 1, i, j, kn)], z2 = P[fidx(d, 2, i, j, kn)], z3 = P[fidx(d, 3, i, j, kn)];
+        const T z4 = P[fidx(d, 4, i, j, kn)], z6 = P[fidx(d, 6, i, j, kn)];
+        const T Bz3 = gdf[4 * N + c] * tder(d, (const T *)P, 6, i, j, k, 1);
+        const T Bz4 = gdf[5 * N + c] * tder(d, (const T *)P, 6, i, j, k, 0);
+        float *rF = MODE == MODE_GS ? nullptr : vw.rf + (size_t)F_P * d.NF;
+        T q2 = T(0), q4 = T(0), q6 = T(0);
+        if (MODE == MODE_DELTA) {
+            q2 = rF[f[1]];
+            q4 = rF[f[3]];
+            q6 = rF[f[5]];
+        }
+        // one face equation: relax in place, or write its residual
+        auto face = [&](T &pv, int slot, T diag, T S, T q) {
+            if (MODE == MODE_RES) {
+                T r = S - diag * pv;
+                rF[slot] = (float)r;
+                e = fmax(e, fabs((double)(w * r / diag)));
+            } else {
+                T v = (1 - w) * pv + w / diag * (MODE == MODE_DELTA ? S + q : S);
+                e = fmax(e, fabs((double)(v - pv)));
+                pv = v;
+            }
+        };
         // ---- compute (AA(m) is a?[m-1])
         if (hx) {
-            double B1 = S1 * ax[12], B2 = Sx * ax[14];
-            double v = (1 - w) * pf[1] + w / ax[1] * (-ax[0] * pf[0] - ax[2] * x2 - ax[3] * pf[2] - ax[4] * pf[3]
-                                                      - ax[5] * x3 - ax[6] * x4 - ax[7] * pf[4] - ax[8] * pf[5]
-                                                      - ax[9] * x5 - ax[10] * x6 + B1 + B2 + Bx3 + Bx4);
-            e = fmax(e, fabs(v - pf[1]));
-            pf[1] = v;
+            T B1 = S1 * ax[12], B2 = Sx * ax[14];
+            T S = -ax[0] * pf[0] - ax[2] * x2 - ax[3] * pf[2] - ax[4] * pf[3] - ax[5] * x3 - ax[6] * x4
+                  - ax[7] * pf[4] - ax[8] * pf[5] - ax[9] * x5 - ax[10] * x6 + B1 + B2 + Bx3 + Bx4;
+            face(pf[1], f[1], ax[1], S, q2);
         }
         {
-            double B1 = S1 * ay[12], B2 = Sy * ay[14];
-            double v = (1 - w) * pf[3] + w / ay[5] * (-ay[0] * pf[0] - ay[1] * pf[1] - ay[2] * y1 - ay[3] * y2
-                                                      - ay[4] * pf[2] - ay[6] * y4 - ay[7] * pf[4] - ay[8] * pf[5]
-                                                      - ay[9] * y5 - ay[10] * y6 + B1 + B2 + By3 + By4);
-            e = fmax(e, fabs(v - pf[3]));
-            pf[3] = v;
+            T B1 = S1 * ay[12], B2 = Sy * ay[14];
+            T S = -ay[0] * pf[0] - ay[1] * pf[1] - ay[2] * y1 - ay[3] * y2 - ay[4] * pf[2] - ay[6] * y4
+                  - ay[7] * pf[4] - ay[8] * pf[5] - ay[9] * y5 - ay[10] * y6 + B1 + B2 + By3 + By4;
+            face(pf[3], f[3], ay[5], S, q4);
         }
         if (hz) {
-            double B1 = S1 * az[12], B2 = Sz * az[14];
-            double v = (1 - w) * pf[5] + w / az[9] * (-az[0] * pf[0] - az[1] * pf[1] - az[2] * z1 - az[3] * z2
-                                                      - az[4] * pf[2] - az[5] * pf[3] - az[6] * z3 - az[7] * z4
-                                                      - az[8] * pf[4] - az[10] * z6 + B1 + B2 + Bz3 + Bz4);
-            e = fmax(e, fabs(v - pf[5]));
-            pf[5] = v;
+            T B1 = S1 * az[12], B2 = Sz * az[14];
+            T S = -az[0] * pf[0] - az[1] * pf[1] - az[2] * z1 - az[3] * z2 - az[4] * pf[2] - az[5] * pf[3]
+                  - az[6] * z3 - az[7] * z4 - az[8] * pf[4] - az[10] * z6 + B1 + B2 + Bz3 + Bz4;
+            face(pf[5], f[5], az[9], S, q6);
         }
         // ---- scatter
-        if (hx) P[f[1]] = pf[1];
-        P[f[3]] = pf[3];
-        if (hz) P[f[5]] = pf[5];
+        if (MODE != MODE_RES) {
+            if (hx) P[f[1]] = pf[1];
+            P[f[3]] = pf[3];
+            if (hz) P[f[5]] = pf[5];
+        }
     }
-    err_commit(d, F_P, e);
+    if (MODE != MODE_DELTA) err_commit(d, F_P, e);
+}
+
+__global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, int color)
+{
+    if (d.ctl->done) return;
+    ppe_sweep<MODE_GS>(d, view64(d), color);
+}
+
+__global__ void __launch_bounds__(256) k_res_ppe(Dev d, View<double> v)
+{
+    if (d.ctl->done) return;
+    ppe_sweep<MODE_RES>(d, v, -1);
+}
+
+__global__ void __launch_bounds__(256) k_d_ppe_sweep(Dev d, View<float> v, int color)
+{
+    if (d.ctl->done) return;
+    ppe_sweep<MODE_DELTA>(d, v, color);
+}
+
+// ---------------------------------------------------------------------
+// mixed precision: FP32 corrections between FP64 residuals
+// ---------------------------------------------------------------------
+// apply_zero_flux (bc.f90:345-390) on the corrections: the FP64 residual r
+// of den T_w = S(T) closes it, dT_w = (S(dT) + r) / den
+__global__ void __launch_bounds__(256) k_d_zero_flux(Dev d, View<float> v)
+{
+    if (d.ctl->done) return;
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int NT = d.NT, NR = d.NR, NZ = d.NZ, NB = d.NB;
+    if (b >= 2 * NT * NR) return;
+    float *T = v.fc + (size_t)F_T * d.NF;
+    const float *rT = v.rf + (size_t)F_T * d.NF;
+    int fw, fa, fb, fcc;
+    if (b < NT * NR) {
+        int i = b % NT, j = b / NT;
+        fw = fidx(d, 5, i, j, 0); fa = fidx(d, 6, i, j, 0); fb = fidx(d, 6, i, j, 1); fcc = fidx(d, 6, i, j, 2);
+    } else {
+        int i = (b - NT * NR) % NT, j = (b - NT * NR) / NT, k = NZ - 1;
+        fw = fidx(d, 6, i, j, k); fa = fidx(d, 5, i, j, k); fb = fidx(d, 5, i, j, k - 1); fcc = fidx(d, 5, i, j, k - 2);
+    }
+    const float *W = v.wall;
+    float S = W[(W_ZF + 0) * NB + b] * T[fa] - W[(W_ZF + 1) * NB + b] * T[fb] + W[(W_ZF + 2) * NB + b] * T[fcc];
+    T[fw] = (S + rT[fw]) / W[(W_ZF + 3) * NB + b];
+}
+
+// outer update of the defect correction in FP64: x0 += dx, dx = 0, for the
+// u, v, w, T cells and the u, v, w, p, T faces
+__global__ void __launch_bounds__(256) k_mix_update(Dev d, View<double> x, View<float> dx)
+{
+    if (d.ctl->done) return;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < d.N)
+        for (int q = 0; q < 4; q++) {
+            size_t s = (size_t)q * d.N + t;
+            x.sc[s] += (double)dx.sc[s];
+            dx.sc[s] = 0.f;
+        }
+    if (t < d.NF)
+        for (int q = 0; q < 5; q++) {
+            size_t s = (size_t)q * d.NF + t;
+            x.fc[s] += (double)dx.fc[s];
+            dx.fc[s] = 0.f;
+        }
+}
+
+// FP32 copy of an FP64 block
+__global__ void __launch_bounds__(256) k_cast(const double *in, float *out, long n)
+{
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        out[t] = (float)in[t];
 }
 
 // ---------------------------------------------------------------------
@@ -888,7 +1170,7 @@ __global__ void __launch_bounds__(256) k_end_step_partial(Dev d)
             wf[p] = d.fc[2 * (size_t)d.NF + f[p]];
         }
         double invV = d.gc[G_INVV * N + c];
-        double D = dphi(d, uf, f, 0, invV) + dphi(d, vf, f, 1, invV) + dphi(d, wf, f, 2, invV);
+        double D = dphi(d, d.An, uf, f, 0, invV) + dphi(d, d.An, vf, f, 1, invV) + dphi(d, d.An, wf, f, 2, invV);
         d.sc[S_DILA * N + c] = D;
         s = D * D;
     }

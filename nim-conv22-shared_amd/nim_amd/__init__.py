@@ -103,6 +103,12 @@ def lib():
         L.nim_stream.argtypes = [vp]
         L.nim_kernel_times.restype = i32
         L.nim_kernel_times.argtypes = [vp, i32, _P, _PI]
+        L.nim_inner_solve_mixed.restype = i32
+        L.nim_inner_solve_mixed.argtypes = [vp, _P, i32, i32, i32, ctypes.POINTER(i32), _P]
+        L.nim_residual.restype = i32
+        L.nim_residual.argtypes = [vp, _P]
+        L.nim_kernel_times_mixed.restype = i32
+        L.nim_kernel_times_mixed.argtypes = [vp, i32, i32, _P, _PI]
         L.nim_last_error.restype = ctypes.c_char_p
         L.nim_last_error.argtypes = []
         _lib = L
@@ -115,7 +121,7 @@ def exported_symbols():
             "nim_build_pressure_coeffs", "nim_upload_state", "nim_download_state", "nim_build_momentum_coeffs",
             "nim_inner_solve", "nim_end_step", "nim_advance_time", "nim_nusselt", "nim_run_stage",
             "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_kernel_times",
-            "nim_last_error"]
+            "nim_inner_solve_mixed", "nim_residual", "nim_kernel_times_mixed", "nim_last_error"]
 
 
 def _check(rc):
@@ -271,6 +277,46 @@ class Context:
         nl = np.zeros(9, dtype=np.int32)
         _check(self.L.nim_kernel_times(self.h, iters, _dp(ms), nl.ctypes.data_as(_PI)))
         return {k: (float(ms[q]), int(nl[q])) for q, k in enumerate(self.KERNEL_CLASSES)}
+
+    def inner_solve_mixed(self, tol, inner=8, max_cycles=100000, min_cycles=0, raise_on_cap=True):
+        """Mixed-precision inner loop: FP64 residual, `inner` FP32 correction
+        passes, FP64 update, until max |w r / a| <= tol per field. Returns
+        (rc, cycles, err); FP32 passes = (cycles - 1) * inner."""
+        t = np.ascontiguousarray(tol, dtype=np.float64)
+        cy = ctypes.c_int32()
+        err = np.zeros(5)
+        rc = self.L.nim_inner_solve_mixed(self.h, _dp(t), max_cycles, min_cycles, inner, ctypes.byref(cy), _dp(err))
+        if rc != NIM_OK and (raise_on_cap or rc != NIM_E_MAXITER):
+            _check(rc)
+        return rc, cy.value, err
+
+    def residual(self):
+        """FP64 residual measure max |w r / a| of the current state per
+        field (u, v, w, p, T)."""
+        err = np.zeros(5)
+        _check(self.L.nim_residual(self.h, _dp(err)))
+        return err
+
+    MIXED_CLASSES = ("residual_velocity", "residual_pressure", "residual_energy", "check", "d_velocity_sources",
+                     "d_velocity_sweep", "d_pressure_sources", "d_pressure_sweep", "d_adiabatic_walls",
+                     "d_energy_sources", "d_energy_sweep", "update")
+
+    def kernel_times_mixed(self, cycles, inner):
+        """Per-class mean ms per mixed cycle (one FP64 residual, `inner` FP32
+        passes, one update) and launches per cycle (advances the state)."""
+        ms = np.zeros(12)
+        nl = np.zeros(12, dtype=np.int32)
+        _check(self.L.nim_kernel_times_mixed(self.h, cycles, inner, _dp(ms), nl.ctypes.data_as(_PI)))
+        return {k: (float(ms[q]), int(nl[q])) for q, k in enumerate(self.MIXED_CLASSES)}
+
+    def time_step_mixed(self, tol, inner=8, max_cycles=100000, min_cycles=0):
+        """One step of solve_nse with the mixed-precision inner loop."""
+        self.build_momentum_coeffs()
+        rc, cycles, err = self.inner_solve_mixed(tol, inner, max_cycles, min_cycles)
+        scal = self.end_step()
+        self.advance_time()
+        nu = self.nusselt()
+        return cycles, err, scal, nu
 
     def time_step(self, tol, max_iter=100000, min_iter=0):
         """One step of solve_nse (nse.f90:49-73) through the C-ABI."""

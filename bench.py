@@ -9,8 +9,12 @@ inner iterations to convergence, the end-of-step reductions, the time shift
 and the Nusselt numbers. Inputs are resident on the device before the timed
 region.
 
-Prints one JSON line. value = cell-updates/s = cells x inner iterations /
-wall time of the timed steps.
+Prints one JSON line. value = cell-updates/s of the FP64 path = cells x
+inner passes / wall time of the timed steps. The `mixed` object runs the
+same steps from the same initial state with the mixed-precision loop (FP64
+residual, FP32 correction passes, FP64 update) to the same tolerances on the
+FP64 residual: its steps/s, speedup, final residuals of both loops and the
+per-field difference from the FP64 result after the same steps.
 """
 import argparse
 import json
@@ -38,6 +42,15 @@ BYTES_PER_CELL = {
     "energy_coefficients": 47 * 8, "unused": 0, "energy_sources": 13 * 8, "energy_sweep": 39 * 8, "check": 0,
 }
 PATH_BYTES_PER_CELL_UPDATE = 1200  # 150 doubles per cell and outer iteration
+# Algorithmic FP32 bytes per cell of the correction-pass kernels of the
+# mixed loop (the FP64 counts without the previous time level, plus the
+# residual of each relaxed unknown and, for the pressure sources, the nine
+# velocity gradients of the residual state)
+BYTES_PER_CELL_MIXED = {
+    "d_velocity_sources": 35 * 4, "d_velocity_sweep": 68 * 4, "d_pressure_sources": 44 * 4,
+    "d_pressure_sweep": 55 * 4, "d_energy_sources": 13 * 4, "d_energy_sweep": 42 * 4,
+}
+FACE_STATE = ("faces_u", "faces_v", "faces_w", "faces_T")
 
 
 def setup(case, amp=1e-3, seed=20221):
@@ -181,6 +194,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
+    ap.add_argument("--inner", type=int, default=12, help="FP32 passes per FP64 residual of the mixed loop")
+    ap.add_argument("--no-mixed", action="store_true")
     # counter passes: only the per-kernel event pass (few dispatches), or a
     # plain copy of known size that calibrates FETCH_SIZE / WRITE_SIZE
     ap.add_argument("--profile-only", action="store_true")
@@ -210,27 +225,43 @@ def main():
                           "launches_per_iteration": {k: v[1] for k, v in kt.items()}}))
         return 0
     tol = np.array([args.tol, args.tol, args.tol, args.tol_p, args.tol])
+    s_init = ctx.download_state()
 
-    def step(first):
-        ctx.build_momentum_coeffs()
-        rc, iters, err = ctx.inner_solve(tol, max_iter=20000, min_iter=2 if first else 0)
-        print("step: %d inner iterations, err %s" % (iters, " ".join("%.2e" % e for e in err)),
-              file=sys.stderr, flush=True)
-        ctx.end_step()
-        ctx.advance_time()
-        nu = ctx.nusselt()
-        return iters, nu
+    def run(mixed):
+        """warm-up and timed steps from the initial state; returns wall time
+        of the timed steps, their inner passes, the FP64 residual measure
+        after the last inner solve (not timed) and the last Nusselt numbers"""
+        ctx.upload_state(s_init)
+        passes, wall, t_res, res = 0, 0., 0., None
+        for s in range(args.warmup + args.steps):
+            timed, first, last = s >= args.warmup, s == 0, s == args.warmup + args.steps - 1
+            if timed and s == args.warmup:
+                ctx.synchronize()
+                t0 = time.perf_counter()
+            ctx.build_momentum_coeffs()
+            if mixed:
+                rc, n, err = ctx.inner_solve_mixed(tol, inner=args.inner, max_cycles=20000)
+                n = (n - 1) * args.inner
+            else:
+                rc, n, err = ctx.inner_solve(tol, max_iter=20000, min_iter=2 if first else 0)
+            print("%s step: %d passes, err %s" % ("mixed" if mixed else "fp64", n, " ".join("%.2e" % e for e in err)),
+                  file=sys.stderr, flush=True)
+            if last:
+                ctx.synchronize()
+                t1 = time.perf_counter()
+                res = ctx.residual()
+                t_res = time.perf_counter() - t1
+            ctx.end_step()
+            ctx.advance_time()
+            nu = ctx.nusselt()
+            if timed:
+                passes += n
+        ctx.synchronize()
+        wall = time.perf_counter() - t0 - t_res
+        return wall, passes, res, nu
 
-    for w in range(args.warmup):
-        step(w == 0)
-    ctx.synchronize()
-    total_iters = 0
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        it, nu = step(args.warmup == 0 and s == 0)
-        total_iters += it
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
+    wall, total_iters, res64, nu = run(False)
+    final64 = {n: ctx.get(n) for n in FACE_STATE + ("faces_p",)}
 
     N = case.ncells
     cups = N * total_iters / wall
@@ -243,6 +274,35 @@ def main():
     cells_per_launch = N / launches
     achieved = BYTES_PER_CELL[dom] * cells_per_launch / per_launch_s
     iter_ms = sum(v[0] for v in kt.values())
+
+    mixed = None
+    if not args.no_mixed:
+        wall_m, passes_m, res_m, nu_m = run(True)
+        diff = {n[6:]: float(np.linalg.norm(ctx.get(n) - final64[n]) / np.linalg.norm(final64[n])) for n in FACE_STATE}
+        pm, p64 = ctx.get("faces_p"), final64["faces_p"]
+        diff["p (mean removed)"] = float(np.linalg.norm((pm - pm.mean()) - (p64 - p64.mean()))
+                                         / np.linalg.norm(p64 - p64.mean()))
+        ktm = ctx.kernel_times_mixed(3, args.inner)
+        cyc_ms = sum(v[0] for v in ktm.values())
+        domm = max(BYTES_PER_CELL_MIXED, key=lambda k: ktm[k][0])
+        per_launch_m = ktm[domm][0] / ktm[domm][1] * 1e-3
+        ach_m = BYTES_PER_CELL_MIXED[domm] * (N * args.inner / ktm[domm][1]) / per_launch_m
+        mixed = {
+            "steps_per_s": args.steps / wall_m,
+            "speedup_vs_fp64": (args.steps / wall_m) / (args.steps / wall),
+            "cell_updates_per_s": N * passes_m / wall_m,
+            "fp32_passes_per_step": passes_m / args.steps,
+            "inner_passes_per_fp64_residual": args.inner,
+            "ms_per_step": wall_m / args.steps * 1e3,
+            "residual_last_step": {"mixed": [float(x) for x in res_m], "fp64": [float(x) for x in res64]},
+            "rel_l2_vs_fp64_after_same_steps": diff,
+            "nusselt_last": [float(x) for x in nu_m[:3]],
+            "roofline": {"bound": "hbm", "kernel": domm, "achieved": ach_m / 1e9, "peak": HBM_PEAK / 1e9,
+                         "unit": "GB/s", "frac": ach_m / HBM_PEAK, "bytes_per_cell": BYTES_PER_CELL_MIXED[domm],
+                         "launch_ms": per_launch_m * 1e3},
+            "kernel_ms_per_cycle": {k: v[0] for k, v in ktm.items()},
+            "event_cycle_ms": cyc_ms,
+        }
 
     line = {
         "metric": "cell-updates/s & steps/s at 1M cells, FP64 vs mixed at 1e-8 residual; % HBM",
@@ -283,6 +343,8 @@ def main():
             "event_iteration_ms": iter_ms,
         },
         "kernel_ms_per_iteration": {k: v[0] for k, v in kt.items()},
+        "residual_last_step": [float(x) for x in res64],
+        "mixed": mixed,
         "setup_s": t_setup,
     }
     if not args.no_cpu:

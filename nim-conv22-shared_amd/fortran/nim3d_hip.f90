@@ -6,14 +6,17 @@
 ! &case n_theta=36, n_r=6, n_z=12, R_i=1.0471975511966, R_o=2., height=2.,
 !       blend_in=1.25, blend_out=1.75, Ra=1e5, Pr=0.7, DT=0.01, nsteps=3,
 !       tol=1e-6,1e-6,1e-6,5e-3,1e-6, noise=1e-3, seed=20221,
-!       restart_every=0, restart_prefix='restart/rest.step', restart_in='' /
+!       restart_every=0, restart_prefix='restart/rest.step', restart_in='',
+!       final_out='', mixed_inner=0 /
+! mixed_inner > 0 selects the mixed-precision inner loop (FP64 residuals,
+! that many FP32 correction passes per residual).
 program nim3d_hip
   use iso_c_binding
   use nim_host_geometry
   use nim_nse
   implicit none
   type(nim_run_config) :: cfg
-  integer :: n_theta = 36, n_r = 6, n_z = 12, nsteps = 3, restart_every = 0, done, u, ios
+  integer :: n_theta = 36, n_r = 6, n_z = 12, nsteps = 3, restart_every = 0, done, u, ios, mixed_inner = 0
   real(c_double) :: R_i = 1.0d0*(3.14159265359d0/3.0d0), R_o = 2.0d0, height = 2.0d0
   real(c_double) :: blend_in = 1.25d0, blend_out = 1.75d0, grading = 1.5d0, hex_order = -1.0d0
   real(c_double) :: Ra = 1.0d5, Pr = 0.7d0, DT = 0.01d0, noise = 1.0d-3, tol(5) = (/1d-6, 1d-6, 1d-6, 5d-3, 1d-6/)
@@ -23,7 +26,7 @@ program nim3d_hip
   character(len=256) :: nml = 'nim3d.nml', restart_prefix = 'restart/rest.step', restart_in = '', final_out = ''
   namelist /case/ n_theta, n_r, n_z, R_i, R_o, height, blend_in, blend_out, grading, hex_order, Ra, Pr, DT, &
                   nsteps, tol, noise, seed, T_hot, T_cold, restart_every, restart_prefix, restart_in, &
-                  check_steady, final_out
+                  check_steady, final_out, mixed_inner
 
   if (command_argument_count() >= 1) call get_command_argument(1, nml)
   open(newunit=u, file=trim(nml), status='old', iostat=ios)
@@ -34,7 +37,7 @@ program nim3d_hip
   cfg%geo = nim_geo_case(n_theta, n_r, n_z, 0, R_i, R_o, height, blend_in, blend_out, grading, hex_order, &
                          3.14159265359d0)
   cfg%Ra = Ra; cfg%Pr = Pr; cfg%DT = DT; cfg%tol = tol; cfg%noise = noise; cfg%seed = seed
-  cfg%T_hot = T_hot; cfg%T_cold = T_cold; cfg%check_steady = check_steady
+  cfg%T_hot = T_hot; cfg%T_cold = T_cold; cfg%check_steady = check_steady; cfg%mixed_inner = mixed_inner
   call nim_setup(cfg)
   if (len_trim(restart_in) > 0) call nim_read_restart(trim(restart_in))
   done = nim_solve(cfg, nsteps, restart_every, trim(restart_prefix))

@@ -81,6 +81,23 @@ module nim_hip_api
       integer(c_int32_t), intent(out) :: iters
       real(c_double), intent(out) :: err(5)
     end function
+    ! mixed-precision form of the same loop: FP64 residuals, `inner` FP32
+    ! correction passes per residual, FP64 update (no reference counterpart)
+    integer(c_int32_t) function nim_inner_solve_mixed(ctx, tol, max_cycles, min_cycles, inner, cycles, err) &
+        bind(C, name="nim_inner_solve_mixed")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: tol(5)
+      integer(c_int32_t), value :: max_cycles, min_cycles, inner
+      integer(c_int32_t), intent(out) :: cycles
+      real(c_double), intent(out) :: err(5)
+    end function
+    ! FP64 residual measure of the current state
+    integer(c_int32_t) function nim_residual(ctx, err) bind(C, name="nim_residual")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: err(5)
+    end function
     ! check_steady_state + evaluate_D (nse.f90:167-168)
     integer(c_int32_t) function nim_end_step(ctx, dudt, dTdt, dil) bind(C, name="nim_end_step")
       import :: c_int32_t, c_ptr, c_double

@@ -21,6 +21,9 @@ module nim_nse
     real(c_double) :: dudt_epsilon = 1.0d-5                                  ! parameters.f90:151
     logical :: check_steady = .true.
     integer :: max_iter = 200000
+    ! > 0: mixed-precision inner loop with this many FP32 passes per FP64
+    ! residual (tolerances then apply to the FP64 residual measure)
+    integer :: mixed_inner = 0
   end type
 
   type(c_ptr), save :: ctx = c_null_ptr
@@ -125,9 +128,16 @@ contains
       call check(nim_build_momentum_coeffs(ctx), 'nim_build_momentum_coeffs')
       ! the reference's *_old arrays start at zero, so its first pass always
       ! sees the Dirichlet wall values as an increment: at least 2 passes
-      rc = nim_inner_solve(ctx, cfg%tol, int(cfg%max_iter, c_int32_t), merge(2_c_int32_t, 0_c_int32_t, l == 1), &
-                           iters, err)
-      call check(rc, 'nim_inner_solve')
+      if (cfg%mixed_inner > 0) then
+        rc = nim_inner_solve_mixed(ctx, cfg%tol, int(cfg%max_iter / cfg%mixed_inner + 1, c_int32_t), 0_c_int32_t, &
+                                   int(cfg%mixed_inner, c_int32_t), iters, err)
+        call check(rc, 'nim_inner_solve_mixed')
+        iters = (iters - 1) * int(cfg%mixed_inner, c_int32_t)
+      else
+        rc = nim_inner_solve(ctx, cfg%tol, int(cfg%max_iter, c_int32_t), merge(2_c_int32_t, 0_c_int32_t, l == 1), &
+                             iters, err)
+        call check(rc, 'nim_inner_solve')
+      endif
       call check(nim_end_step(ctx, dudt, dTdt, dil), 'nim_end_step')
       print '(A, I6, A, I6, A, 1p, E11.4, A, E11.4, A, E11.4, A, E11.4, A, E11.4, A, E11.4)', 'step #', STEP, &
             '  | # iters. ', iters, '  | err V=', max(err(1), err(2), err(3)), '  | err p=', err(4), &

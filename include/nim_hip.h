@@ -20,6 +20,11 @@
  *    A context is used from one host thread at a time.
  *  - Everything is FP64 (the reference compiles with -fdefault-real-8), except
  *    the FP32 correction passes of nim_inner_solve_mixed.
+ *  - Environment: NIM_GRAPH=0 launches directly instead of through captured
+ *    graphs; NIM_BATCH / NIM_MIXED_BATCH set the passes / cycles between
+ *    host reads of the convergence flag; NIM_SLAB=k (k >= 2, read at context
+ *    creation) runs the sweeps in the skewed k-slab schedule, bitwise
+ *    identical to the default four colour phases.
  */
 #ifndef NIM_HIP_H
 #define NIM_HIP_H

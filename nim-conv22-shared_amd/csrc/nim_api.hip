@@ -21,9 +21,9 @@ __global__ void k_T_coef(Dev d);
 __global__ void k_mom_src(Dev d);
 __global__ void k_ppe_src(Dev d);
 __global__ void k_T_src(Dev d);
-__global__ void k_mom_sweep(Dev d, int color);
-__global__ void k_T_sweep(Dev d, int color);
-__global__ void k_ppe_sweep(Dev d, int color);
+__global__ void k_mom_sweep(Dev d, Sched sc);
+__global__ void k_T_sweep(Dev d, Sched sc);
+__global__ void k_ppe_sweep(Dev d, Sched sc);
 __global__ void k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4, int max_iter, int min_iter);
 __global__ void k_check(Dev d);
 __global__ void k_pressure_coeffs(Dev d);
@@ -38,12 +38,12 @@ __global__ void k_res_ppe(Dev d, View<double> v);
 __global__ void k_res_T_coef(Dev d, View<double> v);
 __global__ void k_res_T(Dev d, View<double> v);
 __global__ void k_d_mom_src(Dev d, View<float> v);
-__global__ void k_d_mom_sweep(Dev d, View<float> v, int color);
+__global__ void k_d_mom_sweep(Dev d, View<float> v, Sched sc);
 __global__ void k_d_ppe_src(Dev d, View<float> v);
-__global__ void k_d_ppe_sweep(Dev d, View<float> v, int color);
+__global__ void k_d_ppe_sweep(Dev d, View<float> v, Sched sc);
 __global__ void k_d_zero_flux(Dev d, View<float> v);
 __global__ void k_d_T_src(Dev d, View<float> v);
-__global__ void k_d_T_sweep(Dev d, View<float> v, int color);
+__global__ void k_d_T_sweep(Dev d, View<float> v, Sched sc);
 __global__ void k_mix_update(Dev d, View<double> x, View<float> dx);
 __global__ void k_cast(const double *in, float *out, long n);
 }  // namespace nim
@@ -82,6 +82,9 @@ struct nim_ctx {
     hipGraphExec_t gexec = nullptr;
     int gbatch = 0;
     int nred = 0;
+    // sweep schedules (64- and 256-cell blocks) and their slab thickness
+    int slab = 0;
+    std::vector<Sched> sched64, sched256;
     // mixed precision: FP32 corrections and constants, residual buffers
     int const_version = 0;        // bumped by every geometry or pressure-coefficient build
     struct {
@@ -96,6 +99,8 @@ struct nim_ctx {
 };
 
 extern "C" const char *nim_last_error(void) { return g_err.c_str(); }
+
+static void init_schedules(nim_ctx *c);
 
 static bool dims_ok(const nim_dims *d)
 {
@@ -185,6 +190,7 @@ extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, n
     d.w_u = prm->w_u; d.w_v = prm->w_v; d.w_w = prm->w_w; d.w_p = prm->w_p; d.w_p2 = prm->w_p2; d.w_T = prm->w_T;
     d.v_limit = prm->v_limit;
     c->nred = (c->N + BS - 1) / BS;
+    init_schedules(c);
     double *gc, *An, *wall;
     int32_t *wpts;
 #define TRY(x)                                                                                          \
@@ -539,29 +545,90 @@ extern "C" int32_t nim_build_momentum_coeffs(nim_ctx *c)
 }
 
 // ---- one outer iteration ---------------------------------------------
-static int color_cells(const nim_ctx *c) { return (c->NT / 2) * ((c->NR + 1) / 2) * c->NZ; }
-static int color_blocks(const nim_ctx *c) { return nblk(color_cells(c)); }
 // momentum kernels: 64 cells per block, one wave per velocity component
 static const dim3 BS3(64, 3);
-static int color_blocks3(const nim_ctx *c) { return (color_cells(c) + 63) / 64; }
 static int cell_blocks3(const nim_ctx *c) { return (c->N + 63) / 64; }
 
-static void launch_stage(nim_ctx *c, int stage)
+// Launches of one relaxation sweep. With slabs of `slab` axial planes
+// (S slabs), launch L runs colour c on slab s for every s + 2c = L. Colour
+// c on slab s then starts after colour c - 1 finished on slabs s - 1, s,
+// s + 1 and before any higher colour touched them, and no stencil (reach 2
+// planes at the walls, so slab >= 2) crosses more than one slab: every cell
+// reads exactly what it reads in the four full-domain colour phases, which
+// are the one-slab case. Colour c + 1 comes back to a slab two launches
+// later, while its data are still in the caches.
+static std::vector<Sched> build_schedule(const nim_ctx *c, int cells_per_block)
+{
+    const int NZ = c->NZ, ks = c->slab, S = (NZ + ks - 1) / ks;
+    const int per_plane = (c->NT / 2) * ((c->NR + 1) / 2);
+    std::vector<Sched> out;
+    for (int L = 0; L < S + 6; L++) {
+        Sched sc{};
+        int blocks = 0;
+        for (int col = 0; col < 4; col++) {
+            int sl = L - 2 * col;
+            if (sl < 0 || sl >= S) continue;
+            int q = sc.n++;
+            sc.color[q] = col;
+            sc.k0[q] = sl * ks;
+            sc.k1[q] = std::min(NZ, (sl + 1) * ks);
+            sc.bstart[q] = blocks;
+            blocks += (per_plane * (sc.k1[q] - sc.k0[q]) + cells_per_block - 1) / cells_per_block;
+        }
+        if (!sc.n) continue;
+        sc.bstart[sc.n] = blocks;
+        out.push_back(sc);
+    }
+    return out;
+}
+
+static void init_schedules(nim_ctx *c)
+{
+    const char *e = getenv("NIM_SLAB");
+    int ks = e ? atoi(e) : 0;
+    c->slab = ks >= 2 && ks < c->NZ ? ks : c->NZ;
+    c->sched64 = build_schedule(c, 64);
+    c->sched256 = build_schedule(c, BS);
+}
+
+static inline int sched_blocks(const Sched &s) { return s.bstart[s.n]; }
+
+// `mark` is called after every launch with the kernel class (timing passes)
+template <class Mark>
+static void launch_stage(nim_ctx *c, int stage, Mark mark)
 {
     const Dev &d = c->dev;
     hipStream_t s = c->stream;
-    int gN = nblk(c->N), gC = color_blocks(c);
+    int gN = nblk(c->N);
     switch (stage) {
-    case 0: hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); break;
+    case 0: hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); mark(0); break;
     case 1:
-        for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_mom_sweep, dim3(color_blocks3(c)), BS3, 0, s, d, q);
+        for (const Sched &sc : c->sched64) {
+            hipLaunchKernelGGL(k_mom_sweep, dim3(sched_blocks(sc)), BS3, 0, s, d, sc);
+            mark(1);
+        }
         break;
-    case 2: hipLaunchKernelGGL(k_ppe_src, dim3(gN), dim3(BS), 0, s, d); break;
-    case 3: for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_ppe_sweep, dim3(gC), dim3(BS), 0, s, d, q); break;
-    case 4: hipLaunchKernelGGL(k_T_coef, dim3(gN), dim3(BS), 0, s, d); break;
-    case 5: hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); break;
-    case 6: for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_T_sweep, dim3(gC), dim3(BS), 0, s, d, q); break;
+    case 2: hipLaunchKernelGGL(k_ppe_src, dim3(gN), dim3(BS), 0, s, d); mark(2); break;
+    case 3:
+        for (const Sched &sc : c->sched256) {
+            hipLaunchKernelGGL(k_ppe_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
+            mark(3);
+        }
+        break;
+    case 4: hipLaunchKernelGGL(k_T_coef, dim3(gN), dim3(BS), 0, s, d); mark(4); break;
+    case 5: hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); mark(6); break;
+    case 6:
+        for (const Sched &sc : c->sched256) {
+            hipLaunchKernelGGL(k_T_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
+            mark(7);
+        }
+        break;
     }
+}
+
+static void launch_stage(nim_ctx *c, int stage)
+{
+    launch_stage(c, stage, [](int) {});
 }
 
 static void launch_iteration(nim_ctx *c)
@@ -692,37 +759,51 @@ static int32_t mixed_prepare(nim_ctx *c)
 
 // FP64 residual of every unknown's equation at the current state, the
 // maximum of |w r / a| per field reduced into the iteration control
-static void launch_residual(nim_ctx *c)
+template <class Mark>
+static void launch_residual(nim_ctx *c, Mark mark)
 {
     const Dev &d = c->dev;
     const View<double> &r = c->mx.v64;
     hipStream_t s = c->stream;
     const int gN = nblk(c->N);
-    hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d);
-    hipLaunchKernelGGL(k_res_mom, dim3(cell_blocks3(c)), BS3, 0, s, d, r);
-    hipLaunchKernelGGL(k_res_ppe_src, dim3(gN), dim3(BS), 0, s, d, r);
-    hipLaunchKernelGGL(k_res_ppe, dim3(gN), dim3(BS), 0, s, d, r);
-    hipLaunchKernelGGL(k_res_T_coef, dim3(gN), dim3(BS), 0, s, d, r);
-    hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d);
-    hipLaunchKernelGGL(k_res_T, dim3(gN), dim3(BS), 0, s, d, r);
-    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, d);
+    hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); mark(0);
+    hipLaunchKernelGGL(k_res_mom, dim3(cell_blocks3(c)), BS3, 0, s, d, r); mark(0);
+    hipLaunchKernelGGL(k_res_ppe_src, dim3(gN), dim3(BS), 0, s, d, r); mark(1);
+    hipLaunchKernelGGL(k_res_ppe, dim3(gN), dim3(BS), 0, s, d, r); mark(1);
+    hipLaunchKernelGGL(k_res_T_coef, dim3(gN), dim3(BS), 0, s, d, r); mark(2);
+    hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); mark(2);
+    hipLaunchKernelGGL(k_res_T, dim3(gN), dim3(BS), 0, s, d, r); mark(2);
+    hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, d); mark(3);
 }
 
 // one FP32 pass of the correction form, stages in the FP64 pass's order
-static void launch_delta(nim_ctx *c)
+template <class Mark>
+static void launch_delta(nim_ctx *c, Mark mark)
 {
     const Dev &d = c->dev;
     const View<float> &v = c->mx.v32;
     hipStream_t s = c->stream;
-    const int gN = nblk(c->N), gC = color_blocks(c);
-    hipLaunchKernelGGL(k_d_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d, v);
-    for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_d_mom_sweep, dim3(color_blocks3(c)), BS3, 0, s, d, v, q);
-    hipLaunchKernelGGL(k_d_ppe_src, dim3(gN), dim3(BS), 0, s, d, v);
-    for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_d_ppe_sweep, dim3(gC), dim3(BS), 0, s, d, v, q);
-    hipLaunchKernelGGL(k_d_zero_flux, dim3(nblk(2 * c->NT * c->NR)), dim3(BS), 0, s, d, v);
-    hipLaunchKernelGGL(k_d_T_src, dim3(gN), dim3(BS), 0, s, d, v);
-    for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_d_T_sweep, dim3(gC), dim3(BS), 0, s, d, v, q);
+    const int gN = nblk(c->N);
+    hipLaunchKernelGGL(k_d_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d, v); mark(4);
+    for (const Sched &sc : c->sched64) {
+        hipLaunchKernelGGL(k_d_mom_sweep, dim3(sched_blocks(sc)), BS3, 0, s, d, v, sc);
+        mark(5);
+    }
+    hipLaunchKernelGGL(k_d_ppe_src, dim3(gN), dim3(BS), 0, s, d, v); mark(6);
+    for (const Sched &sc : c->sched256) {
+        hipLaunchKernelGGL(k_d_ppe_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, v, sc);
+        mark(7);
+    }
+    hipLaunchKernelGGL(k_d_zero_flux, dim3(nblk(2 * c->NT * c->NR)), dim3(BS), 0, s, d, v); mark(8);
+    hipLaunchKernelGGL(k_d_T_src, dim3(gN), dim3(BS), 0, s, d, v); mark(9);
+    for (const Sched &sc : c->sched256) {
+        hipLaunchKernelGGL(k_d_T_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, v, sc);
+        mark(10);
+    }
 }
+
+static void launch_residual(nim_ctx *c) { launch_residual(c, [](int) {}); }
+static void launch_delta(nim_ctx *c) { launch_delta(c, [](int) {}); }
 
 static void launch_update(nim_ctx *c)
 {
@@ -838,54 +919,77 @@ extern "C" int32_t nim_nusselt(nim_ctx *c, double out[5])
     return NIM_OK;
 }
 
+// HIP events after every launch of a timing pass, each tagged with the
+// kernel class of the launch it closes
+struct EventLog {
+    hipStream_t s;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> cls;
+    size_t used = 0;
+    bool ok = true;
+    explicit EventLog(hipStream_t st) : s(st) {}
+    ~EventLog()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    void start()
+    {
+        used = 0;
+        cls.clear();
+        rec(-1);
+    }
+    void rec(int k)
+    {
+        if (used == ev.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) { ok = false; return; }
+            ev.push_back(e);
+        }
+        if (hipEventRecord(ev[used], s) != hipSuccess) ok = false;
+        used++;
+        cls.push_back(k);
+    }
+    // adds the time of every closed launch to ms[class]; counts launches once
+    int32_t collect(double *ms, int32_t *launches, bool count)
+    {
+        if (!ok || used < 2) return fail(NIM_E_HIP, "event recording failed");
+        HIPCHK(hipEventSynchronize(ev[used - 1]));
+        for (size_t q = 1; q < used; q++) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, ev[q - 1], ev[q]));
+            ms[cls[q]] += t;
+            if (launches && count) launches[cls[q]]++;
+        }
+        return NIM_OK;
+    }
+};
+
 extern "C" int32_t nim_kernel_times(nim_ctx *c, int32_t iters, double *ms, int32_t *launches)
 {
     if (!c || !ms || iters < 1) return fail(NIM_E_ARG, "bad argument");
     if (!c->has_geo || !c->has_pcoef) return fail(NIM_E_STATE, "geometry or pressure coefficients missing");
-    const Dev &d = c->dev;
-    hipStream_t s = c->stream;
-    const int gN = nblk(c->N), gC = color_blocks(c);
-    // launch list of one iteration: (class, kernel index)
-    const int cls[17] = {0, 1, 1, 1, 1, 2, 3, 3, 3, 3, 4, 6, 7, 7, 7, 7, 8};
     const double tol[5] = {0, 0, 0, 0, 0};
     int32_t rc = reset_ctl(c, tol, 1 << 30, 1 << 30);
     if (rc) return rc;
-    std::vector<hipEvent_t> ev(18);
-    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     for (int q = 0; q < NIM_KERNEL_CLASSES; q++) {
         ms[q] = 0.;
         if (launches) launches[q] = 0;
     }
+    EventLog log(c->stream);
+    auto mark = [&](int k) { log.rec(k); };
     for (int it = 0; it < iters; it++) {
-        int n = 0;
-        HIPCHK(hipEventRecord(ev[0], s));
-        auto mark = [&]() { (void)hipEventRecord(ev[++n], s); };
-        hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); mark();
-        for (int q = 0; q < 4; q++) {
-            hipLaunchKernelGGL(k_mom_sweep, dim3(color_blocks3(c)), BS3, 0, s, d, q);
-            mark();
-        }
-        hipLaunchKernelGGL(k_ppe_src, dim3(gN), dim3(BS), 0, s, d); mark();
-        for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_ppe_sweep, dim3(gC), dim3(BS), 0, s, d, q); mark(); }
-        hipLaunchKernelGGL(k_T_coef, dim3(gN), dim3(BS), 0, s, d); mark();
-        hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); mark();
-        for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_T_sweep, dim3(gC), dim3(BS), 0, s, d, q); mark(); }
-        hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, d); mark();
+        log.start();
+        for (int st = 0; st <= 6; st++) launch_stage(c, st, mark);
+        hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, c->stream, c->dev);
+        mark(8);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipEventSynchronize(ev[17]));
-        for (int q = 0; q < 17; q++) {
-            float t = 0.f;
-            HIPCHK(hipEventElapsedTime(&t, ev[q], ev[q + 1]));
-            ms[cls[q]] += t;
-            if (launches && it == 0) launches[cls[q]]++;
-        }
+        if ((rc = log.collect(ms, launches, it == 0))) return rc;
     }
     for (int q = 0; q < NIM_KERNEL_CLASSES; q++) ms[q] /= iters;
-    for (auto &e : ev) (void)hipEventDestroy(e);
     // a non-finite increment stops the kernels early, which would time
     // nothing: report it
-    HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (c->h_ctl->status == 4) return fail(NIM_E_NONFINITE, "non-finite increment while timing");
     return NIM_OK;
 }
@@ -898,63 +1002,24 @@ extern "C" int32_t nim_kernel_times_mixed(nim_ctx *c, int32_t cycles, int32_t in
     if (rc) return rc;
     const double tol[5] = {0, 0, 0, 0, 0};
     if ((rc = reset_ctl(c, tol, 1 << 30, 1 << 30))) return rc;
-    const Dev &d = c->dev;
-    const View<double> &r = c->mx.v64;
-    const View<float> &v = c->mx.v32;
-    hipStream_t s = c->stream;
-    const int gN = nblk(c->N), gC = color_blocks(c);
-    std::vector<hipEvent_t> ev;
-    std::vector<int> cls;
-    auto mark = [&](int k) {
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return;
-        (void)hipEventRecord(e, s);
-        ev.push_back(e);
-        cls.push_back(k);
-    };
     for (int q = 0; q < NIM_MIXED_CLASSES; q++) {
         ms[q] = 0.;
         if (launches) launches[q] = 0;
     }
+    EventLog log(c->stream);
+    auto mark = [&](int k) { log.rec(k); };
     for (int it = 0; it < cycles; it++) {
-        ev.clear();
-        cls.clear();
-        mark(-1);
-        hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); mark(0);
-        hipLaunchKernelGGL(k_res_mom, dim3(cell_blocks3(c)), BS3, 0, s, d, r); mark(0);
-        hipLaunchKernelGGL(k_res_ppe_src, dim3(gN), dim3(BS), 0, s, d, r); mark(1);
-        hipLaunchKernelGGL(k_res_ppe, dim3(gN), dim3(BS), 0, s, d, r); mark(1);
-        hipLaunchKernelGGL(k_res_T_coef, dim3(gN), dim3(BS), 0, s, d, r); mark(2);
-        hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); mark(2);
-        hipLaunchKernelGGL(k_res_T, dim3(gN), dim3(BS), 0, s, d, r); mark(2);
-        hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, d); mark(3);
-        for (int p = 0; p < inner; p++) {
-            hipLaunchKernelGGL(k_d_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d, v); mark(4);
-            for (int q = 0; q < 4; q++) {
-                hipLaunchKernelGGL(k_d_mom_sweep, dim3(color_blocks3(c)), BS3, 0, s, d, v, q);
-                mark(5);
-            }
-            hipLaunchKernelGGL(k_d_ppe_src, dim3(gN), dim3(BS), 0, s, d, v); mark(6);
-            for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_d_ppe_sweep, dim3(gC), dim3(BS), 0, s, d, v, q); mark(7); }
-            hipLaunchKernelGGL(k_d_zero_flux, dim3(nblk(2 * c->NT * c->NR)), dim3(BS), 0, s, d, v); mark(8);
-            hipLaunchKernelGGL(k_d_T_src, dim3(gN), dim3(BS), 0, s, d, v); mark(9);
-            for (int q = 0; q < 4; q++) { hipLaunchKernelGGL(k_d_T_sweep, dim3(gC), dim3(BS), 0, s, d, v, q); mark(10); }
-        }
-        launch_update(c); mark(11);
+        log.start();
+        launch_residual(c, mark);
+        for (int p = 0; p < inner; p++) launch_delta(c, mark);
+        launch_update(c);
+        mark(11);
         HIPCHK(hipGetLastError());
-        if (ev.size() != cls.size() || ev.empty()) return fail(NIM_E_HIP, "hipEventCreate failed");
-        HIPCHK(hipEventSynchronize(ev.back()));
-        for (size_t q = 1; q < ev.size(); q++) {
-            float t = 0.f;
-            HIPCHK(hipEventElapsedTime(&t, ev[q - 1], ev[q]));
-            ms[cls[q]] += t;
-            if (launches && it == 0) launches[cls[q]]++;
-        }
-        for (auto e : ev) (void)hipEventDestroy(e);
+        if ((rc = log.collect(ms, launches, it == 0))) return rc;
     }
     for (int q = 0; q < NIM_MIXED_CLASSES; q++) ms[q] /= cycles;
-    HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     if (c->h_ctl->status == 4) return fail(NIM_E_NONFINITE, "non-finite residual while timing");
     return NIM_OK;
 }

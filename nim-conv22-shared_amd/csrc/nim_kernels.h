@@ -132,6 +132,17 @@ __device__ __host__ inline View<double> view64(const Dev &d)
     return v;
 }
 
+// One launch of a relaxation sweep: up to four (colour, slab) items, each
+// the cells of one colour with k in [k0, k1), given consecutive logical
+// blocks starting at bstart. A full-domain colour phase is one item; the
+// skewed slab schedule runs colour c on slab s in launch s + 2c.
+constexpr int MAX_ITEMS = 4;
+struct Sched {
+    int n;
+    int color[MAX_ITEMS], k0[MAX_ITEMS], k1[MAX_ITEMS];
+    int bstart[MAX_ITEMS + 1];
+};
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2. The per-iteration kernels renumber them so that XCD x works through
 // one contiguous range of cells (a bijection whatever the dispatch order):

@@ -40,19 +40,31 @@ __device__ __forceinline__ void err_commit(const Dev &d, int field, double e)
     }
 }
 
-// cell of colour `color` handled by thread `tid`
-__device__ __forceinline__ bool color_cell(const Dev &d, int color, int tid, int &i, int &j, int &k)
+// cell of colour `color` with k in [k0, k1) handled by thread `tid`
+__device__ __forceinline__ bool color_cell(const Dev &d, int color, int k0, int k1, int tid, int &i, int &j, int &k)
 {
     int nth = d.NT >> 1, nrh = (d.NR + 1) >> 1;
     int ih = tid % nth, rest = tid / nth;
     int jj = rest % nrh;
-    k = rest / nrh;
-    if (k >= d.NZ) return false;
+    k = k0 + rest / nrh;
+    if (k >= k1) return false;
     j = ((color + k) & 1) + 2 * jj;
     if (j >= d.NR) return false;
     int r = (color - j - 3 * k) & 3;
     i = 2 * ih + (r >> 1);
     return true;
+}
+
+// cell of this thread in one launch of a sweep schedule: the item that owns
+// the block, then the colour cell within the item
+__device__ __forceinline__ bool sched_cell(const Dev &d, const Sched &s, int lb, int &i, int &j, int &k)
+{
+    int q = 0;
+#pragma unroll
+    for (int t = 1; t < MAX_ITEMS; t++)
+        if (t < s.n && lb >= s.bstart[t]) q = t;
+    int tid = (lb - s.bstart[q]) * (blockDim.x) + threadIdx.x;
+    return color_cell(d, s.color[q], s.k0[q], s.k1[q], tid, i, j, k);
 }
 
 // ---------------------------------------------------------------------
@@ -647,12 +659,12 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
     }
 }
 
-// cell of this thread: one colour of a relaxation phase, or (RES, color < 0)
-// every cell
+// cell of this thread: an item of a sweep schedule, or (RES) every cell
 template <int MODE>
-__device__ __forceinline__ bool sweep_cell(const Dev &d, int color, int tid, int &i, int &j, int &k)
+__device__ __forceinline__ bool sweep_cell(const Dev &d, const Sched &s, int lb, int &i, int &j, int &k)
 {
-    if (MODE != MODE_RES) return color_cell(d, color, tid, i, j, k);
+    if (MODE != MODE_RES) return sched_cell(d, s, lb, i, j, k);
+    int tid = lb * blockDim.x + threadIdx.x;
     if (tid >= d.N) return false;
     i = iof(d, tid % d.NT);
     j = (tid / d.NT) % d.NR;
@@ -662,11 +674,11 @@ __device__ __forceinline__ bool sweep_cell(const Dev &d, int color, int tid, int
 
 // threadIdx.y selects the field (u, v, w are independent within the sweep)
 template <int MODE, typename T>
-__device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, int color)
+__device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, const Sched &sc)
 {
     int i, j, k;
     const int q = threadIdx.y;
-    bool act = sweep_cell<MODE>(d, color, xcd_block() * 64 + threadIdx.x, i, j, k);
+    bool act = sweep_cell<MODE>(d, sc, xcd_block(), i, j, k);
     double e = 0.;
     if (act) {
         const int N = d.N;
@@ -679,29 +691,29 @@ __device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, int co
     if (MODE != MODE_DELTA) err_commit(d, q, e);
 }
 
-__global__ void __launch_bounds__(192) k_mom_sweep(Dev d, int color)
+__global__ void __launch_bounds__(192) k_mom_sweep(Dev d, Sched sc)
 {
     if (d.ctl->done) return;
-    mom_sweep<MODE_GS>(d, view64(d), color);
+    mom_sweep<MODE_GS>(d, view64(d), sc);
 }
 
 __global__ void __launch_bounds__(192) k_res_mom(Dev d, View<double> v)
 {
     if (d.ctl->done) return;
-    mom_sweep<MODE_RES>(d, v, -1);
+    mom_sweep<MODE_RES>(d, v, Sched{});
 }
 
-__global__ void __launch_bounds__(192) k_d_mom_sweep(Dev d, View<float> v, int color)
+__global__ void __launch_bounds__(192) k_d_mom_sweep(Dev d, View<float> v, Sched sc)
 {
     if (d.ctl->done) return;
-    mom_sweep<MODE_DELTA>(d, v, color);
+    mom_sweep<MODE_DELTA>(d, v, sc);
 }
 
 template <int MODE, typename T>
-__device__ __forceinline__ void T_sweep(const Dev &d, const View<T> &v, const T *coef, int color)
+__device__ __forceinline__ void T_sweep(const Dev &d, const View<T> &v, const T *coef, const Sched &sc)
 {
     int i, j, k;
-    bool act = sweep_cell<MODE>(d, color, xcd_block() * blockDim.x + threadIdx.x, i, j, k);
+    bool act = sweep_cell<MODE>(d, sc, xcd_block(), i, j, k);
     double e = 0.;
     if (act) {
         const int N = d.N;
@@ -713,23 +725,23 @@ __device__ __forceinline__ void T_sweep(const Dev &d, const View<T> &v, const T 
     if (MODE != MODE_DELTA) err_commit(d, F_T, e);
 }
 
-__global__ void __launch_bounds__(256) k_T_sweep(Dev d, int color)
+__global__ void __launch_bounds__(256) k_T_sweep(Dev d, Sched sc)
 {
     if (d.ctl->done) return;
     View<double> v = view64(d);
-    T_sweep<MODE_GS>(d, v, v.TF, color);
+    T_sweep<MODE_GS>(d, v, v.TF, sc);
 }
 
 __global__ void __launch_bounds__(256) k_res_T(Dev d, View<double> v)
 {
     if (d.ctl->done) return;
-    T_sweep<MODE_RES>(d, v, v.TF, -1);
+    T_sweep<MODE_RES>(d, v, v.TF, Sched{});
 }
 
-__global__ void __launch_bounds__(256) k_d_T_sweep(Dev d, View<float> v, int color)
+__global__ void __launch_bounds__(256) k_d_T_sweep(Dev d, View<float> v, Sched sc)
 {
     if (d.ctl->done) return;
-    T_sweep<MODE_DELTA>(d, v, v.TF, color);
+    T_sweep<MODE_DELTA>(d, v, v.TF, sc);
 }
 
 // apply_pressure_bc for the wall face at position pos of cell (i,j,k)
@@ -793,10 +805,10 @@ __device__ void pressure_wall(const Dev &d, const View<T> &vw, int b, int pos, i
 // gathered before the first store; the cell's own faces are carried in
 // registers, so face 4 sees the new face 2 and face 6 the new faces 2 and 4.
 template <int MODE, typename T>
-__device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, int color)
+__device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const Sched &sc)
 {
     int i, j, k;
-    bool act = sweep_cell<MODE>(d, color, xcd_block() * blockDim.x + threadIdx.x, i, j, k);
+    bool act = sweep_cell<MODE>(d, sc, xcd_block(), i, j, k);
     double e = 0.;
     if (act) {
         const int N = d.N, NT = d.NT, NR = d.NR, NZ = d.NZ;
@@ -893,22 +905,22 @@ __device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, int c
     if (MODE != MODE_DELTA) err_commit(d, F_P, e);
 }
 
-__global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, int color)
+__global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, Sched sc)
 {
     if (d.ctl->done) return;
-    ppe_sweep<MODE_GS>(d, view64(d), color);
+    ppe_sweep<MODE_GS>(d, view64(d), sc);
 }
 
 __global__ void __launch_bounds__(256) k_res_ppe(Dev d, View<double> v)
 {
     if (d.ctl->done) return;
-    ppe_sweep<MODE_RES>(d, v, -1);
+    ppe_sweep<MODE_RES>(d, v, Sched{});
 }
 
-__global__ void __launch_bounds__(256) k_d_ppe_sweep(Dev d, View<float> v, int color)
+__global__ void __launch_bounds__(256) k_d_ppe_sweep(Dev d, View<float> v, Sched sc)
 {
     if (d.ctl->done) return;
-    ppe_sweep<MODE_DELTA>(d, v, color);
+    ppe_sweep<MODE_DELTA>(d, v, sc);
 }
 
 // ---------------------------------------------------------------------

@@ -180,6 +180,84 @@ def copy_probe(mib):
     return 0
 
 
+def ensemble_bench(args):
+    """The 48-case ensemble: every case one time step per step, the mixed
+    loop and the FP64 loop each as one launch per kernel over all cases,
+    against the same cases solved one after another (mixed loop)."""
+    cases = host.ensemble_cases()
+    t0 = time.perf_counter()
+    built = [(case, host.build_geometry(case), host.initial_state(case, seed=sd)) for case, sd in cases]
+    ctxs = []
+    for case, geo, st in built:
+        c = nim_amd.Context(*case.dims, case)
+        c.upload_geometry(geo)
+        c.build_pressure_coeffs()
+        ctxs.append(c)
+    t_setup = time.perf_counter() - t0
+    ncell = sum(c.ncells for c, _ in cases)
+    tol = np.array([args.tol, args.tol, args.tol, args.tol_p, args.tol])
+    ens = nim_amd.Ensemble(ctxs)
+
+    def run(mode):
+        for c, (case, geo, st) in zip(ctxs, built):
+            c.upload_state(st)
+        passes, t_start = 0, None
+        for s in range(args.warmup + args.steps):
+            if s == args.warmup:
+                for c in ctxs:
+                    c.synchronize()
+                t_start = time.perf_counter()
+            for c in ctxs:
+                c.build_momentum_coeffs()
+            if mode == "sequential":
+                its = []
+                for c in ctxs:
+                    rc, n, err = c.inner_solve_mixed(tol, inner=args.inner, max_cycles=20000)
+                    its.append(n)
+                its = np.array(its)
+            else:
+                inner = args.inner if mode == "mixed" else 0
+                rc, its, err = ens.inner_solve(tol, inner=inner, max_iter=20000 if inner == 0 else 20000,
+                                               min_iter=2 if (s == 0 and inner == 0) else 0)
+            for c in ctxs:
+                c.end_step()
+                c.advance_time()
+                c.nusselt()
+            n_pass = (its - 1) * args.inner if mode != "fp64" else its
+            if s >= args.warmup:
+                passes += sum(int(n) * c.ncells for n, (c, _) in zip(n_pass, cases))
+            print("%s step: passes per case min %d max %d" % (mode, int(n_pass.min()), int(n_pass.max())),
+                  file=sys.stderr, flush=True)
+        for c in ctxs:
+            c.synchronize()
+        wall = time.perf_counter() - t_start
+        return {"steps_per_s": args.steps / wall, "cell_updates_per_s": passes / wall, "s_per_step": wall / args.steps}
+
+    modes = args.ens_modes.split(",")
+    res = {m: run(m) for m in modes}
+    ens.close()
+    for c in ctxs:
+        c.close()
+    sizes = [c.ncells for c, _ in cases]
+    print(json.dumps({
+        "metric": "ensemble of 48 uneven cases: time-steps/s (every case one step) and cell-updates/s",
+        "value": res[modes[0]]["steps_per_s"], "unit": "ensemble steps/s", "higher_is_better": True,
+        "steps": args.steps, "warmup": args.warmup, "n_gpus": 1, "dtype": "f32 sweeps, f64 residual",
+        "data": "synthetic (seeded 1e-3 noise per case)",
+        "config": {"workload": "48 cases, Ra log-uniform [1e3, 1e6], Pr {0.71, 7}, R_i U[0.2, 0.4], n_r {16..48}, "
+                               "n_theta = 2 n_r, n_z {12, 16, 24}, DT 1e-3, tol %g (p %g), %d FP32 passes per FP64 "
+                               "residual" % (args.tol, args.tol_p, args.inner),
+                   "cells_total": ncell, "cells_min": min(sizes), "cells_max": max(sizes)},
+        "batched_mixed": res.get("mixed"), "batched_fp64": res.get("fp64"), "sequential_mixed": res.get("sequential"),
+        "batching_speedup": (res["mixed"]["steps_per_s"] / res["sequential"]["steps_per_s"]
+                             if "mixed" in res and "sequential" in res else None),
+        "mixed_vs_fp64": (res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"]
+                          if "mixed" in res and "fp64" in res else None),
+        "setup_s": t_setup,
+    }))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,8 +272,12 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
-    ap.add_argument("--inner", type=int, default=12, help="FP32 passes per FP64 residual of the mixed loop")
+    ap.add_argument("--inner", type=int, default=None,
+                    help="FP32 passes per FP64 residual of the mixed loop (default 12; 8 for the ensemble)")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--ensemble", action="store_true", help="the 48-case ensemble instead of one case")
+    ap.add_argument("--ens-modes", default="mixed,fp64,sequential",
+                    help="ensemble runs: batched mixed, batched fp64, mixed one case after another")
     # counter passes: only the per-kernel event pass (few dispatches), or a
     # plain copy of known size that calibrates FETCH_SIZE / WRITE_SIZE
     ap.add_argument("--profile-only", action="store_true")
@@ -211,6 +293,10 @@ def main():
         return 2
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.ensemble:
+        args.inner = args.inner or 8
+        return ensemble_bench(args)
+    args.inner = args.inner or 12
 
     case = host.baseline_case(args.case)
     t_setup = time.perf_counter()

@@ -206,6 +206,24 @@ int32_t nim_kernel_times_mixed(nim_ctx *ctx, int32_t cycles, int32_t inner, doub
 void *nim_stream(nim_ctx *ctx);
 int32_t nim_synchronize(nim_ctx *ctx);
 
+/* Ensembles of independent cases (no reference counterpart: the reference
+ * runs one case per process). An ensemble is built over existing contexts,
+ * each with its geometry, pressure coefficients and state uploaded (any mesh
+ * sizes, same device); the contexts stay usable one by one.
+ * nim_ens_inner_solve replaces nim_inner_solve (inner = 0) or
+ * nim_inner_solve_mixed (inner > 0) for all cases at once: every kernel of a
+ * pass is one launch over all cases, so small cases fill the GPU together,
+ * and a case that has converged costs only its blocks' early exit. Each case
+ * gets exactly the result of its own solve. iters[ncase] returns passes
+ * (inner = 0) or cycles (inner > 0) per case, err[5*ncase] the last measure.
+ * Per-step work (nim_build_momentum_coeffs, nim_end_step, nim_advance_time,
+ * nim_nusselt) stays per context. Returns the worst status over the cases. */
+typedef struct nim_ens nim_ens;
+int32_t nim_ens_create(nim_ctx *const *cases, int32_t ncase, nim_ens **out);
+int32_t nim_ens_destroy(nim_ens *ens);
+int32_t nim_ens_inner_solve(nim_ens *ens, const double tol[5], int32_t max_iter, int32_t min_iter, int32_t inner,
+                            int32_t *iters, double *err);
+
 /* Message for the last failure on the calling thread. */
 const char *nim_last_error(void);
 

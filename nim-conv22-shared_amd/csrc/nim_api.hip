@@ -46,6 +46,28 @@ __global__ void k_d_T_src(Dev d, View<float> v);
 __global__ void k_d_T_sweep(Dev d, View<float> v, Sched sc);
 __global__ void k_mix_update(Dev d, View<double> x, View<float> dx);
 __global__ void k_cast(const double *in, float *out, long n);
+__global__ void k_e_mom_src(Ens e);
+__global__ void k_e_mom_sweep(Ens e, int color);
+__global__ void k_e_ppe_src(Ens e);
+__global__ void k_e_ppe_sweep(Ens e, int color);
+__global__ void k_e_T_coef(Ens e);
+__global__ void k_e_T_src(Ens e);
+__global__ void k_e_T_sweep(Ens e, int color);
+__global__ void k_e_check(Ens e);
+__global__ void k_e_res_mom(Ens e);
+__global__ void k_e_res_ppe_src(Ens e);
+__global__ void k_e_res_ppe(Ens e);
+__global__ void k_e_res_T_coef(Ens e);
+__global__ void k_e_res_T(Ens e);
+__global__ void k_e_d_mom_src(Ens e);
+__global__ void k_e_d_mom_sweep(Ens e, int color);
+__global__ void k_e_d_ppe_src(Ens e);
+__global__ void k_e_d_ppe_sweep(Ens e, int color);
+__global__ void k_e_d_zero_flux(Ens e);
+__global__ void k_e_d_T_src(Ens e);
+__global__ void k_e_d_T_sweep(Ens e, int color);
+__global__ void k_e_mix_update(Ens e);
+__global__ void k_e_gather(Ens e, Ctl *out);
 }  // namespace nim
 
 using namespace nim;
@@ -1021,5 +1043,228 @@ extern "C" int32_t nim_kernel_times_mixed(nim_ctx *c, int32_t cycles, int32_t in
     HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->h_ctl->status == 4) return fail(NIM_E_NONFINITE, "non-finite residual while timing");
+    return NIM_OK;
+}
+
+// ---- ensembles -----------------------------------------------------------
+// launch shapes: blocks per case of each kind of launch
+enum { SH_CELL64, SH_CELL256, SH_COL64, SH_COL256, SH_WALL, SH_UPD, NSHAPE };
+
+struct nim_ens {
+    int n = 0;
+    std::vector<nim_ctx *> cs;
+    hipStream_t stream = nullptr;
+    Dev *d_dev = nullptr;
+    View<double> *d_v64 = nullptr;
+    View<float> *d_v32 = nullptr;
+    int *d_bstart = nullptr;         // [NSHAPE][n + 1]
+    int total[NSHAPE] = {};
+    Ctl *d_ctl = nullptr, *h_ctl = nullptr;
+    bool views = false;              // mixed views uploaded
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int gbatch = 0, ginner = -1;
+    std::vector<void *> allocs;
+};
+
+static Ens ens_args(const nim_ens *e, int shape)
+{
+    Ens a;
+    a.dev = e->d_dev;
+    a.v64 = e->d_v64;
+    a.v32 = e->d_v32;
+    a.bstart = e->d_bstart + (size_t)shape * (e->n + 1);
+    a.ncase = e->n;
+    return a;
+}
+
+extern "C" int32_t nim_ens_destroy(nim_ens *e)
+{
+    if (!e) return NIM_OK;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (void *p : e->allocs) (void)hipFree(p);
+    if (e->h_ctl) (void)hipHostFree(e->h_ctl);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_ens_create(nim_ctx *const *cases, int32_t ncase, nim_ens **out)
+{
+    if (!cases || !out || ncase < 1 || ncase > 4096) return fail(NIM_E_ARG, "bad ensemble arguments");
+    *out = nullptr;
+    for (int q = 0; q < ncase; q++) {
+        if (!cases[q]) return fail(NIM_E_ARG, "null case context");
+        if (!cases[q]->has_geo || !cases[q]->has_pcoef)
+            return fail(NIM_E_STATE, "every case needs geometry and pressure coefficients");
+    }
+    nim_ens *e = new nim_ens();
+    e->n = ncase;
+    e->cs.assign(cases, cases + ncase);
+    auto bad = [&](int32_t rc) { nim_ens_destroy(e); return rc; };
+    std::vector<Dev> devs(ncase);
+    std::vector<int> bs((size_t)NSHAPE * (ncase + 1));
+    for (int sh = 0; sh < NSHAPE; sh++) {
+        int acc = 0;
+        for (int q = 0; q < ncase; q++) {
+            const nim_ctx *c = cases[q];
+            bs[(size_t)sh * (ncase + 1) + q] = acc;
+            int cc = (c->NT / 2) * ((c->NR + 1) / 2) * c->NZ;
+            switch (sh) {
+            case SH_CELL64: acc += (c->N + 63) / 64; break;
+            case SH_CELL256: acc += nblk(c->N); break;
+            case SH_COL64: acc += (cc + 63) / 64; break;
+            case SH_COL256: acc += nblk(cc); break;
+            case SH_WALL: acc += nblk(2 * c->NT * c->NR); break;
+            default: acc += nblk(std::max(c->N, c->NF)); break;
+            }
+        }
+        bs[(size_t)sh * (ncase + 1) + ncase] = acc;
+        e->total[sh] = acc;
+    }
+    for (int q = 0; q < ncase; q++) devs[q] = cases[q]->dev;
+    auto alloc = [&](void **p, size_t bytes) -> int32_t {
+        HIPCHK(hipMalloc(p, bytes));
+        e->allocs.push_back(*p);
+        return NIM_OK;
+    };
+    int32_t rc;
+    if ((rc = alloc((void **)&e->d_dev, sizeof(Dev) * ncase)) ||
+        (rc = alloc((void **)&e->d_v64, sizeof(View<double>) * ncase)) ||
+        (rc = alloc((void **)&e->d_v32, sizeof(View<float>) * ncase)) ||
+        (rc = alloc((void **)&e->d_bstart, sizeof(int) * bs.size())) ||
+        (rc = alloc((void **)&e->d_ctl, sizeof(Ctl) * ncase)))
+        return bad(rc);
+    if (hipHostMalloc((void **)&e->h_ctl, sizeof(Ctl) * ncase, 0) != hipSuccess)
+        return bad(fail(NIM_E_HIP, "hipHostMalloc failed"));
+    if (hipStreamCreate(&e->stream) != hipSuccess) return bad(fail(NIM_E_HIP, "hipStreamCreate failed"));
+    if (hipMemcpy(e->d_dev, devs.data(), sizeof(Dev) * ncase, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(e->d_bstart, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bad(fail(NIM_E_HIP, "ensemble table upload failed"));
+    *out = e;
+    return NIM_OK;
+}
+
+// one FP64 pass of every case
+static void ens_pass(nim_ens *e)
+{
+    hipStream_t s = e->stream;
+    hipLaunchKernelGGL(k_e_mom_src, dim3(e->total[SH_CELL64]), BS3, 0, s, ens_args(e, SH_CELL64));
+    for (int q = 0; q < 4; q++)
+        hipLaunchKernelGGL(k_e_mom_sweep, dim3(e->total[SH_COL64]), BS3, 0, s, ens_args(e, SH_COL64), q);
+    hipLaunchKernelGGL(k_e_ppe_src, dim3(e->total[SH_CELL256]), dim3(BS), 0, s, ens_args(e, SH_CELL256));
+    for (int q = 0; q < 4; q++)
+        hipLaunchKernelGGL(k_e_ppe_sweep, dim3(e->total[SH_COL256]), dim3(BS), 0, s, ens_args(e, SH_COL256), q);
+    hipLaunchKernelGGL(k_e_T_coef, dim3(e->total[SH_CELL256]), dim3(BS), 0, s, ens_args(e, SH_CELL256));
+    hipLaunchKernelGGL(k_e_T_src, dim3(e->total[SH_CELL256]), dim3(BS), 0, s, ens_args(e, SH_CELL256));
+    for (int q = 0; q < 4; q++)
+        hipLaunchKernelGGL(k_e_T_sweep, dim3(e->total[SH_COL256]), dim3(BS), 0, s, ens_args(e, SH_COL256), q);
+    hipLaunchKernelGGL(k_e_check, dim3(e->n), dim3(64), 0, s, ens_args(e, SH_CELL256));
+}
+
+// one mixed cycle of every case: FP64 residual, `inner` FP32 passes, update
+static void ens_cycle(nim_ens *e, int inner)
+{
+    hipStream_t s = e->stream;
+    const Ens c64 = ens_args(e, SH_CELL64), c256 = ens_args(e, SH_CELL256);
+    const Ens k64 = ens_args(e, SH_COL64), k256 = ens_args(e, SH_COL256);
+    const dim3 g64(e->total[SH_CELL64]), g256(e->total[SH_CELL256]), gk64(e->total[SH_COL64]), gk256(e->total[SH_COL256]);
+    hipLaunchKernelGGL(k_e_mom_src, g64, BS3, 0, s, c64);
+    hipLaunchKernelGGL(k_e_res_mom, g64, BS3, 0, s, c64);
+    hipLaunchKernelGGL(k_e_res_ppe_src, g256, dim3(BS), 0, s, c256);
+    hipLaunchKernelGGL(k_e_res_ppe, g256, dim3(BS), 0, s, c256);
+    hipLaunchKernelGGL(k_e_res_T_coef, g256, dim3(BS), 0, s, c256);
+    hipLaunchKernelGGL(k_e_T_src, g256, dim3(BS), 0, s, c256);
+    hipLaunchKernelGGL(k_e_res_T, g256, dim3(BS), 0, s, c256);
+    hipLaunchKernelGGL(k_e_check, dim3(e->n), dim3(64), 0, s, c256);
+    for (int p = 0; p < inner; p++) {
+        hipLaunchKernelGGL(k_e_d_mom_src, g64, BS3, 0, s, c64);
+        for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_e_d_mom_sweep, gk64, BS3, 0, s, k64, q);
+        hipLaunchKernelGGL(k_e_d_ppe_src, g256, dim3(BS), 0, s, c256);
+        for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_e_d_ppe_sweep, gk256, dim3(BS), 0, s, k256, q);
+        hipLaunchKernelGGL(k_e_d_zero_flux, dim3(e->total[SH_WALL]), dim3(BS), 0, s, ens_args(e, SH_WALL));
+        hipLaunchKernelGGL(k_e_d_T_src, g256, dim3(BS), 0, s, c256);
+        for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_e_d_T_sweep, gk256, dim3(BS), 0, s, k256, q);
+    }
+    hipLaunchKernelGGL(k_e_mix_update, dim3(e->total[SH_UPD]), dim3(BS), 0, s, ens_args(e, SH_UPD));
+}
+
+static int32_t ens_batch(nim_ens *e, int batch, int inner)
+{
+    auto body = [&]() {
+        for (int b = 0; b < batch; b++) {
+            if (inner > 0) ens_cycle(e, inner);
+            else ens_pass(e);
+        }
+    };
+    if (!use_graph()) {
+        body();
+        HIPCHK(hipGetLastError());
+        return NIM_OK;
+    }
+    if (!e->gexec || e->gbatch != batch || e->ginner != inner) {
+        if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+        if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        body();
+        HIPCHK(hipStreamEndCapture(e->stream, &e->graph));
+        HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+        e->gbatch = batch;
+        e->ginner = inner;
+    }
+    HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_ens_inner_solve(nim_ens *e, const double tol[5], int32_t max_iter, int32_t min_iter,
+                                       int32_t inner, int32_t *iters, double *err)
+{
+    if (!e || !tol) return fail(NIM_E_ARG, "null argument");
+    if (max_iter < 1 || inner < 0 || inner > 1024) return fail(NIM_E_ARG, "bad iteration arguments");
+    int32_t rc;
+    for (nim_ctx *c : e->cs) {
+        if (inner > 0 && (rc = mixed_prepare(c))) return rc;
+        if ((rc = reset_ctl(c, tol, max_iter, min_iter))) return rc;
+    }
+    if (inner > 0 && !e->views) {
+        std::vector<View<double>> v64(e->n);
+        std::vector<View<float>> v32(e->n);
+        for (int q = 0; q < e->n; q++) {
+            v64[q] = e->cs[q]->mx.v64;
+            v32[q] = e->cs[q]->mx.v32;
+        }
+        HIPCHK(hipMemcpy(e->d_v64, v64.data(), sizeof(View<double>) * e->n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->d_v32, v32.data(), sizeof(View<float>) * e->n, hipMemcpyHostToDevice));
+        e->views = true;
+    }
+    // the cases' own streams carry their per-step work (coefficients,
+    // casts, control reset): finish it before the ensemble stream starts
+    for (nim_ctx *c : e->cs) HIPCHK(hipStreamSynchronize(c->stream));
+    int batch = inner > 0 ? std::max(1, (16 + inner) / (inner + 1)) : batch_size();
+    for (;;) {
+        if ((rc = ens_batch(e, batch, inner))) return rc;
+        hipLaunchKernelGGL(k_e_gather, dim3(1), dim3(64), 0, e->stream, ens_args(e, SH_CELL256), e->d_ctl);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(e->h_ctl, e->d_ctl, sizeof(Ctl) * e->n, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        bool all = true;
+        for (int q = 0; q < e->n; q++) all = all && e->h_ctl[q].done;
+        if (all) break;
+    }
+    int32_t worst = NIM_OK;
+    for (int q = 0; q < e->n; q++) {
+        const Ctl &t = e->h_ctl[q];
+        // keep each case's host copy of its control block current
+        *e->cs[q]->h_ctl = t;
+        if (iters) iters[q] = t.iter;
+        if (err)
+            for (int f = 0; f < 5; f++) err[(size_t)q * 5 + f] = t.err[f];
+        if (t.status == 4) worst = NIM_E_NONFINITE;
+        else if (t.status == 3 && worst == NIM_OK) worst = NIM_E_MAXITER;
+    }
+    if (worst == NIM_E_NONFINITE) return fail(worst, "non-finite increment in a case");
+    if (worst == NIM_E_MAXITER) return fail(worst, "iteration cap reached in a case");
     return NIM_OK;
 }

@@ -143,6 +143,16 @@ struct Sched {
     int bstart[MAX_ITEMS + 1];
 };
 
+// An ensemble launch: per-case device descriptors and, for this launch
+// shape, the first block of each case (bstart[ncase] = total blocks)
+struct Ens {
+    const Dev *dev;
+    const View<double> *v64;
+    const View<float> *v32;
+    const int *bstart;
+    int ncase;
+};
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2. The per-iteration kernels renumber them so that XCD x works through
 // one contiguous range of cells (a bijection whatever the dispatch order):

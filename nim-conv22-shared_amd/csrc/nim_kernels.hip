@@ -298,9 +298,9 @@ __device__ __forceinline__ void energy_direction(const Dev &d, int c, int dir, d
 }
 
 template <int MODE>
-__device__ __forceinline__ void T_coef(const Dev &d, const View<double> &v)
+__device__ __forceinline__ void T_coef(const Dev &d, const View<double> &v, int lb)
 {
-    int c = xcd_block() * blockDim.x + threadIdx.x;
+    int c = lb * blockDim.x + threadIdx.x;
     bool act = c < d.N;
     double eT = 0.;
     if (act) {
@@ -379,13 +379,13 @@ __device__ __forceinline__ void T_coef(const Dev &d, const View<double> &v)
 __global__ void __launch_bounds__(256) k_T_coef(Dev d)
 {
     if (d.ctl->done) return;
-    T_coef<MODE_GS>(d, view64(d));
+    T_coef<MODE_GS>(d, view64(d), xcd_block());
 }
 
 __global__ void __launch_bounds__(256) k_res_T_coef(Dev d, View<double> v)
 {
     if (d.ctl->done) return;
-    T_coef<MODE_RES>(d, v);
+    T_coef<MODE_RES>(d, v, xcd_block());
 }
 
 // ---------------------------------------------------------------------
@@ -401,9 +401,9 @@ __global__ void __launch_bounds__(256) k_res_T_coef(Dev d, View<double> v)
 // For corrections (v.old null) the previous-level part of the buoyancy is
 // dropped: every other term is linear in the unknowns.
 template <typename T>
-__device__ __forceinline__ void mom_src(const Dev &d, const View<T> &v)
+__device__ __forceinline__ void mom_src(const Dev &d, const View<T> &v, int lb)
 {
-    int c = xcd_block() * 64 + threadIdx.x;
+    int c = lb * 64 + threadIdx.x;
     const int q = threadIdx.y;
     if (c >= d.N) return;
     const int N = d.N;
@@ -425,13 +425,13 @@ __device__ __forceinline__ void mom_src(const Dev &d, const View<T> &v)
 __global__ void __launch_bounds__(192) k_mom_src(Dev d)
 {
     if (d.ctl->done) return;
-    mom_src(d, view64(d));
+    mom_src(d, view64(d), xcd_block());
 }
 
 __global__ void __launch_bounds__(192) k_d_mom_src(Dev d, View<float> v)
 {
     if (d.ctl->done) return;
-    mom_src(d, v);
+    mom_src(d, v, xcd_block());
 }
 
 // update_pressure_sources (nse.f90:228-235): calculate_R_b_p
@@ -442,9 +442,9 @@ __global__ void __launch_bounds__(192) k_d_mom_src(Dev d, View<float> v)
 // and the corrections use the exact expansion R_b_p(x0 + dx) - R_b_p(x0)
 // (e.g. (a0 + da)^2 - a0^2 = 2 a0 da + da^2), which is small in dx.
 template <int MODE, typename T>
-__device__ __forceinline__ void ppe_src(const Dev &d, const View<T> &v)
+__device__ __forceinline__ void ppe_src(const Dev &d, const View<T> &v, int lb)
 {
-    int c = xcd_block() * blockDim.x + threadIdx.x;
+    int c = lb * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
     const int N = d.N;
     int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
@@ -490,26 +490,26 @@ __device__ __forceinline__ void ppe_src(const Dev &d, const View<T> &v)
 __global__ void __launch_bounds__(256) k_ppe_src(Dev d)
 {
     if (d.ctl->done) return;
-    ppe_src<MODE_GS>(d, view64(d));
+    ppe_src<MODE_GS>(d, view64(d), xcd_block());
 }
 
 __global__ void __launch_bounds__(256) k_res_ppe_src(Dev d, View<double> v)
 {
     if (d.ctl->done) return;
-    ppe_src<MODE_RES>(d, v);
+    ppe_src<MODE_RES>(d, v, xcd_block());
 }
 
 __global__ void __launch_bounds__(256) k_d_ppe_src(Dev d, View<float> v)
 {
     if (d.ctl->done) return;
-    ppe_src<MODE_DELTA>(d, v);
+    ppe_src<MODE_DELTA>(d, v, xcd_block());
 }
 
 // update_temperature_sources (nse.f90:249-256): R_b_T = q = 0, R_D_T
 template <typename T>
-__device__ __forceinline__ void T_src(const Dev &d, const View<T> &v)
+__device__ __forceinline__ void T_src(const Dev &d, const View<T> &v, int lb)
 {
-    int c = xcd_block() * blockDim.x + threadIdx.x;
+    int c = lb * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
     int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     v.src[R_T * d.N + c] = cross_diffusion(d, v.gc, v.fc + (size_t)F_T * d.NF, i, j, k, c, T(d.alpha)) + T(0);
@@ -518,13 +518,13 @@ __device__ __forceinline__ void T_src(const Dev &d, const View<T> &v)
 __global__ void __launch_bounds__(256) k_T_src(Dev d)
 {
     if (d.ctl->done) return;
-    T_src(d, view64(d));
+    T_src(d, view64(d), xcd_block());
 }
 
 __global__ void __launch_bounds__(256) k_d_T_src(Dev d, View<float> v)
 {
     if (d.ctl->done) return;
-    T_src(d, v);
+    T_src(d, v, xcd_block());
 }
 
 // ---------------------------------------------------------------------
@@ -674,11 +674,11 @@ __device__ __forceinline__ bool sweep_cell(const Dev &d, const Sched &s, int lb,
 
 // threadIdx.y selects the field (u, v, w are independent within the sweep)
 template <int MODE, typename T>
-__device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, const Sched &sc)
+__device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, const Sched &sc, int lb)
 {
     int i, j, k;
     const int q = threadIdx.y;
-    bool act = sweep_cell<MODE>(d, sc, xcd_block(), i, j, k);
+    bool act = sweep_cell<MODE>(d, sc, lb, i, j, k);
     double e = 0.;
     if (act) {
         const int N = d.N;
@@ -694,26 +694,26 @@ __device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, const 
 __global__ void __launch_bounds__(192) k_mom_sweep(Dev d, Sched sc)
 {
     if (d.ctl->done) return;
-    mom_sweep<MODE_GS>(d, view64(d), sc);
+    mom_sweep<MODE_GS>(d, view64(d), sc, xcd_block());
 }
 
 __global__ void __launch_bounds__(192) k_res_mom(Dev d, View<double> v)
 {
     if (d.ctl->done) return;
-    mom_sweep<MODE_RES>(d, v, Sched{});
+    mom_sweep<MODE_RES>(d, v, Sched{}, xcd_block());
 }
 
 __global__ void __launch_bounds__(192) k_d_mom_sweep(Dev d, View<float> v, Sched sc)
 {
     if (d.ctl->done) return;
-    mom_sweep<MODE_DELTA>(d, v, sc);
+    mom_sweep<MODE_DELTA>(d, v, sc, xcd_block());
 }
 
 template <int MODE, typename T>
-__device__ __forceinline__ void T_sweep(const Dev &d, const View<T> &v, const T *coef, const Sched &sc)
+__device__ __forceinline__ void T_sweep(const Dev &d, const View<T> &v, const T *coef, const Sched &sc, int lb)
 {
     int i, j, k;
-    bool act = sweep_cell<MODE>(d, sc, xcd_block(), i, j, k);
+    bool act = sweep_cell<MODE>(d, sc, lb, i, j, k);
     double e = 0.;
     if (act) {
         const int N = d.N;
@@ -729,19 +729,19 @@ __global__ void __launch_bounds__(256) k_T_sweep(Dev d, Sched sc)
 {
     if (d.ctl->done) return;
     View<double> v = view64(d);
-    T_sweep<MODE_GS>(d, v, v.TF, sc);
+    T_sweep<MODE_GS>(d, v, v.TF, sc, xcd_block());
 }
 
 __global__ void __launch_bounds__(256) k_res_T(Dev d, View<double> v)
 {
     if (d.ctl->done) return;
-    T_sweep<MODE_RES>(d, v, v.TF, Sched{});
+    T_sweep<MODE_RES>(d, v, v.TF, Sched{}, xcd_block());
 }
 
 __global__ void __launch_bounds__(256) k_d_T_sweep(Dev d, View<float> v, Sched sc)
 {
     if (d.ctl->done) return;
-    T_sweep<MODE_DELTA>(d, v, v.TF, sc);
+    T_sweep<MODE_DELTA>(d, v, v.TF, sc, xcd_block());
 }
 
 // apply_pressure_bc for the wall face at position pos of cell (i,j,k)
@@ -805,10 +805,10 @@ __device__ void pressure_wall(const Dev &d, const View<T> &vw, int b, int pos, i
 // gathered before the first store; the cell's own faces are carried in
 // registers, so face 4 sees the new face 2 and face 6 the new faces 2 and 4.
 template <int MODE, typename T>
-__device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const Sched &sc)
+__device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const Sched &sc, int lb)
 {
     int i, j, k;
-    bool act = sweep_cell<MODE>(d, sc, xcd_block(), i, j, k);
+    bool act = sweep_cell<MODE>(d, sc, lb, i, j, k);
     double e = 0.;
     if (act) {
         const int N = d.N, NT = d.NT, NR = d.NR, NZ = d.NZ;
@@ -908,19 +908,19 @@ __device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const
 __global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, Sched sc)
 {
     if (d.ctl->done) return;
-    ppe_sweep<MODE_GS>(d, view64(d), sc);
+    ppe_sweep<MODE_GS>(d, view64(d), sc, xcd_block());
 }
 
 __global__ void __launch_bounds__(256) k_res_ppe(Dev d, View<double> v)
 {
     if (d.ctl->done) return;
-    ppe_sweep<MODE_RES>(d, v, Sched{});
+    ppe_sweep<MODE_RES>(d, v, Sched{}, xcd_block());
 }
 
 __global__ void __launch_bounds__(256) k_d_ppe_sweep(Dev d, View<float> v, Sched sc)
 {
     if (d.ctl->done) return;
-    ppe_sweep<MODE_DELTA>(d, v, sc);
+    ppe_sweep<MODE_DELTA>(d, v, sc, xcd_block());
 }
 
 // ---------------------------------------------------------------------
@@ -928,10 +928,9 @@ __global__ void __launch_bounds__(256) k_d_ppe_sweep(Dev d, View<float> v, Sched
 // ---------------------------------------------------------------------
 // apply_zero_flux (bc.f90:345-390) on the corrections: the FP64 residual r
 // of den T_w = S(T) closes it, dT_w = (S(dT) + r) / den
-__global__ void __launch_bounds__(256) k_d_zero_flux(Dev d, View<float> v)
+__device__ __forceinline__ void d_zero_flux(const Dev &d, const View<float> &v, int lb)
 {
-    if (d.ctl->done) return;
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    int b = lb * blockDim.x + threadIdx.x;
     const int NT = d.NT, NR = d.NR, NZ = d.NZ, NB = d.NB;
     if (b >= 2 * NT * NR) return;
     float *T = v.fc + (size_t)F_T * d.NF;
@@ -949,12 +948,17 @@ __global__ void __launch_bounds__(256) k_d_zero_flux(Dev d, View<float> v)
     T[fw] = (S + rT[fw]) / W[(W_ZF + 3) * NB + b];
 }
 
-// outer update of the defect correction in FP64: x0 += dx, dx = 0, for the
-// u, v, w, T cells and the u, v, w, p, T faces
-__global__ void __launch_bounds__(256) k_mix_update(Dev d, View<double> x, View<float> dx)
+__global__ void __launch_bounds__(256) k_d_zero_flux(Dev d, View<float> v)
 {
     if (d.ctl->done) return;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    d_zero_flux(d, v, blockIdx.x);
+}
+
+// outer update of the defect correction in FP64: x0 += dx, dx = 0, for the
+// u, v, w, T cells and the u, v, w, p, T faces
+__device__ __forceinline__ void mix_update(const Dev &d, const View<double> &x, const View<float> &dx, int lb)
+{
+    int t = lb * blockDim.x + threadIdx.x;
     if (t < d.N)
         for (int q = 0; q < 4; q++) {
             size_t s = (size_t)q * d.N + t;
@@ -967,6 +971,12 @@ __global__ void __launch_bounds__(256) k_mix_update(Dev d, View<double> x, View<
             x.fc[s] += (double)dx.fc[s];
             dx.fc[s] = 0.f;
         }
+}
+
+__global__ void __launch_bounds__(256) k_mix_update(Dev d, View<double> x, View<float> dx)
+{
+    if (d.ctl->done) return;
+    mix_update(d, x, dx, blockIdx.x);
 }
 
 // FP32 copy of an FP64 block
@@ -998,7 +1008,7 @@ __global__ void __launch_bounds__(256) k_solve_reset(Dev d, double t0, double t1
 // end of one outer iteration: reduce the increments, test convergence
 // (the while condition of integrate_nse, nse.f90:129), clear the slot the
 // next iteration uses
-__global__ void __launch_bounds__(64) k_check(Dev d)
+__device__ __forceinline__ void check(const Dev &d)
 {
     Ctl *ctl = d.ctl;
     if (ctl->done) return;
@@ -1023,6 +1033,8 @@ __global__ void __launch_bounds__(64) k_check(Dev d)
         else if (it >= ctl->max_iter) { ctl->status = 3; ctl->done = 1; }
     }
 }
+
+__global__ void __launch_bounds__(64) k_check(Dev d) { check(d); }
 
 // ---------------------------------------------------------------------
 // once-per-step and init kernels
@@ -1273,6 +1285,96 @@ __global__ void __launch_bounds__(256) k_nusselt(Dev d, double R_i, double R_o, 
         out[3] = a1;
         out[4] = a2;
     }
+}
+
+
+// ---------------------------------------------------------------------
+// ensembles: each kernel of an outer pass launched once over all cases
+// ---------------------------------------------------------------------
+// Block b (after the XCD renumbering) belongs to the case q with
+// bstart[q] <= b < bstart[q + 1] and is block b - bstart[q] of that case's
+// own launch, so every case runs exactly the blocks it runs alone.
+__device__ __forceinline__ int ens_case(const Ens &e, int &lb)
+{
+    int b = xcd_block();
+    int lo = 0, hi = e.ncase - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (e.bstart[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    lb = b - e.bstart[lo];
+    return lo;
+}
+
+// a full-domain colour phase as a one-item schedule
+__device__ __forceinline__ Sched full_phase(const Dev &d, int color)
+{
+    Sched sc{};
+    sc.n = 1;
+    sc.color[0] = color;
+    sc.k0[0] = 0;
+    sc.k1[0] = d.NZ;
+    return sc;
+}
+
+#define ENS_CASE                                                                                        \
+    int lb;                                                                                             \
+    const int q = ens_case(e, lb);                                                                      \
+    if (lb >= e.bstart[q + 1] - e.bstart[q]) return;                                                    \
+    const Dev &d = e.dev[q];                                                                            \
+    if (d.ctl->done) return;
+
+__global__ void __launch_bounds__(192) k_e_mom_src(Ens e) { ENS_CASE mom_src(d, view64(d), lb); }
+__global__ void __launch_bounds__(192) k_e_mom_sweep(Ens e, int color)
+{
+    ENS_CASE mom_sweep<MODE_GS>(d, view64(d), full_phase(d, color), lb);
+}
+__global__ void __launch_bounds__(256) k_e_ppe_src(Ens e) { ENS_CASE ppe_src<MODE_GS>(d, view64(d), lb); }
+__global__ void __launch_bounds__(256) k_e_ppe_sweep(Ens e, int color)
+{
+    ENS_CASE ppe_sweep<MODE_GS>(d, view64(d), full_phase(d, color), lb);
+}
+__global__ void __launch_bounds__(256) k_e_T_coef(Ens e) { ENS_CASE T_coef<MODE_GS>(d, view64(d), lb); }
+__global__ void __launch_bounds__(256) k_e_T_src(Ens e) { ENS_CASE T_src(d, view64(d), lb); }
+__global__ void __launch_bounds__(256) k_e_T_sweep(Ens e, int color)
+{
+    ENS_CASE View<double> v = view64(d);
+    T_sweep<MODE_GS>(d, v, v.TF, full_phase(d, color), lb);
+}
+// one block per case
+__global__ void __launch_bounds__(64) k_e_check(Ens e) { check(e.dev[blockIdx.x]); }
+
+__global__ void __launch_bounds__(192) k_e_res_mom(Ens e) { ENS_CASE mom_sweep<MODE_RES>(d, e.v64[q], Sched{}, lb); }
+__global__ void __launch_bounds__(256) k_e_res_ppe_src(Ens e) { ENS_CASE ppe_src<MODE_RES>(d, e.v64[q], lb); }
+__global__ void __launch_bounds__(256) k_e_res_ppe(Ens e) { ENS_CASE ppe_sweep<MODE_RES>(d, e.v64[q], Sched{}, lb); }
+__global__ void __launch_bounds__(256) k_e_res_T_coef(Ens e) { ENS_CASE T_coef<MODE_RES>(d, e.v64[q], lb); }
+__global__ void __launch_bounds__(256) k_e_res_T(Ens e)
+{
+    ENS_CASE T_sweep<MODE_RES>(d, e.v64[q], e.v64[q].TF, Sched{}, lb);
+}
+__global__ void __launch_bounds__(192) k_e_d_mom_src(Ens e) { ENS_CASE mom_src(d, e.v32[q], lb); }
+__global__ void __launch_bounds__(192) k_e_d_mom_sweep(Ens e, int color)
+{
+    ENS_CASE mom_sweep<MODE_DELTA>(d, e.v32[q], full_phase(d, color), lb);
+}
+__global__ void __launch_bounds__(256) k_e_d_ppe_src(Ens e) { ENS_CASE ppe_src<MODE_DELTA>(d, e.v32[q], lb); }
+__global__ void __launch_bounds__(256) k_e_d_ppe_sweep(Ens e, int color)
+{
+    ENS_CASE ppe_sweep<MODE_DELTA>(d, e.v32[q], full_phase(d, color), lb);
+}
+__global__ void __launch_bounds__(256) k_e_d_zero_flux(Ens e) { ENS_CASE d_zero_flux(d, e.v32[q], lb); }
+__global__ void __launch_bounds__(256) k_e_d_T_src(Ens e) { ENS_CASE T_src(d, e.v32[q], lb); }
+__global__ void __launch_bounds__(256) k_e_d_T_sweep(Ens e, int color)
+{
+    ENS_CASE T_sweep<MODE_DELTA>(d, e.v32[q], e.v32[q].TF, full_phase(d, color), lb);
+}
+__global__ void __launch_bounds__(256) k_e_mix_update(Ens e) { ENS_CASE mix_update(d, e.v64[q], e.v32[q], lb); }
+
+// the iteration control of every case into one array (one host read)
+__global__ void __launch_bounds__(64) k_e_gather(Ens e, Ctl *out)
+{
+    for (int q = threadIdx.x; q < e.ncase; q += blockDim.x) out[q] = *e.dev[q].ctl;
 }
 
 }  // namespace nim

@@ -109,6 +109,12 @@ def lib():
         L.nim_residual.argtypes = [vp, _P]
         L.nim_kernel_times_mixed.restype = i32
         L.nim_kernel_times_mixed.argtypes = [vp, i32, i32, _P, _PI]
+        L.nim_ens_create.restype = i32
+        L.nim_ens_create.argtypes = [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)]
+        L.nim_ens_destroy.restype = i32
+        L.nim_ens_destroy.argtypes = [vp]
+        L.nim_ens_inner_solve.restype = i32
+        L.nim_ens_inner_solve.argtypes = [vp, _P, i32, i32, i32, _PI, _P]
         L.nim_last_error.restype = ctypes.c_char_p
         L.nim_last_error.argtypes = []
         _lib = L
@@ -121,7 +127,8 @@ def exported_symbols():
             "nim_build_pressure_coeffs", "nim_upload_state", "nim_download_state", "nim_build_momentum_coeffs",
             "nim_inner_solve", "nim_end_step", "nim_advance_time", "nim_nusselt", "nim_run_stage",
             "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_kernel_times",
-            "nim_inner_solve_mixed", "nim_residual", "nim_kernel_times_mixed", "nim_last_error"]
+            "nim_inner_solve_mixed", "nim_residual", "nim_kernel_times_mixed", "nim_ens_create", "nim_ens_destroy",
+            "nim_ens_inner_solve", "nim_last_error"]
 
 
 def _check(rc):
@@ -326,3 +333,52 @@ class Context:
         self.advance_time()
         nu = self.nusselt()
         return iters, err, scal, nu
+
+
+class Ensemble:
+    """Independent cases solved together (nim_ens_*): each kernel of a pass
+    is one launch over all cases."""
+
+    def __init__(self, contexts):
+        self.L = lib()
+        self.cases = list(contexts)
+        arr = (ctypes.c_void_p * len(self.cases))(*[c.h.value if isinstance(c.h, ctypes.c_void_p) else c.h
+                                                      for c in self.cases])
+        h = ctypes.c_void_p()
+        _check(self.L.nim_ens_create(arr, len(self.cases), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.nim_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def inner_solve(self, tol, inner=0, max_iter=100000, min_iter=0, raise_on_cap=True):
+        """inner = 0: FP64 loop (iters = passes); inner > 0: mixed loop
+        (iters = cycles). Returns (rc, iters[ncase], err[ncase, 5])."""
+        t = np.ascontiguousarray(tol, dtype=np.float64)
+        n = len(self.cases)
+        it = np.zeros(n, dtype=np.int32)
+        err = np.zeros((n, 5))
+        rc = self.L.nim_ens_inner_solve(self.h, _dp(t), max_iter, min_iter, inner, it.ctypes.data_as(_PI), _dp(err))
+        if rc != NIM_OK and (raise_on_cap or rc != NIM_E_MAXITER):
+            _check(rc)
+        return rc, it, err
+
+    def time_step(self, tol, inner=0, max_iter=100000, min_iter=0):
+        """One solve_nse step of every case."""
+        for c in self.cases:
+            c.build_momentum_coeffs()
+        rc, it, err = self.inner_solve(tol, inner, max_iter, min_iter)
+        out = []
+        for c in self.cases:
+            scal = c.end_step()
+            c.advance_time()
+            out.append((scal, c.nusselt()))
+        return it, err, out

@@ -86,6 +86,24 @@ def baseline_case(name):
     return Case(dims, 0.3, 1.0, 1.0, Ra=Ra, Pr=0.71, DT=DT)
 
 
+def ensemble_cases(n=48, seed=20221, DT=1e-3):
+    """The ensemble of the specification: n independent cases with Ra
+    log-uniform in [1e3, 1e6], Pr in {0.71, 7.0}, hexagon circumradius
+    uniform in [0.2, 0.4], n_r in {16, 24, 32, 48}, n_theta = 2 n_r, n_z in
+    {12, 16, 24}, drawn from a seeded generator. Returns [(case, ic_seed)]."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n):
+        n_r = int(rng.choice([16, 24, 32, 48]))
+        n_z = int(rng.choice([12, 16, 24]))
+        Ra = float(10 ** rng.uniform(3, 6))
+        Pr = float(rng.choice([0.71, 7.0]))
+        R_i = float(rng.uniform(0.2, 0.4))
+        out.append((Case((2 * n_r, n_r, n_z), R_i, 1.0, 1.0, Ra=Ra, Pr=Pr, DT=DT), seed + 1 + q))
+    return out
+
+
 _lib = None
 
 

@@ -224,6 +224,30 @@ int32_t nim_ens_destroy(nim_ens *ens);
 int32_t nim_ens_inner_solve(nim_ens *ens, const double tol[5], int32_t max_iter, int32_t min_iter, int32_t inner,
                             int32_t *iters, double *err);
 
+/* Checkpoints for reversing a run of time steps (no reference counterpart;
+ * the reference keeps one restart state on disk, restart.f90). A pool of
+ * `nslots` device snapshots of a context's whole time-step state (cell
+ * values at the current, previous and time-centred levels, p, dilatation,
+ * all face fields); store/restore copy on the context's stream.
+ * nim_revolve_plan is a host function: the binomial plan that reverses
+ * nsteps steps with nslots snapshots (slot 0 holds the initial state) in the
+ * fewest forward steps (Griewank's optimum r l - C(c + r, c + 1) for
+ * l = C(c + r, c), c = nslots). Actions (codes[q], args[q]):
+ *   NIM_RV_STORE   slot   store the current state
+ *   NIM_RV_RESTORE slot   make the stored state current
+ *   NIM_RV_ADVANCE n      take n forward time steps
+ *   NIM_RV_REVERSE k      the current state is the state before step k:
+ *                         reverse step k (k runs nsteps-1 .. 0, once each)
+ * With codes or args NULL only n_actions and forward_steps are returned. */
+enum { NIM_RV_STORE = 1, NIM_RV_RESTORE = 2, NIM_RV_ADVANCE = 3, NIM_RV_REVERSE = 4 };
+int32_t nim_revolve_plan(int32_t nsteps, int32_t nslots, int32_t *codes, int32_t *args, int64_t capacity,
+                         int64_t *n_actions, int64_t *forward_steps);
+typedef struct nim_ckpt nim_ckpt;
+int32_t nim_ckpt_create(nim_ctx *ctx, int32_t nslots, nim_ckpt **out);
+int32_t nim_ckpt_destroy(nim_ckpt *ckpt);
+int32_t nim_ckpt_store(nim_ckpt *ckpt, int32_t slot);
+int32_t nim_ckpt_restore(nim_ckpt *ckpt, int32_t slot);
+
 /* Message for the last failure on the calling thread. */
 const char *nim_last_error(void);
 

@@ -122,6 +122,11 @@ struct nim_ctx {
 
 extern "C" const char *nim_last_error(void) { return g_err.c_str(); }
 
+// for the other translation units of the library
+int32_t nim_internal_fail(int32_t code, const char *msg) { return fail(code, msg); }
+void *nim_internal_stream(nim_ctx *c) { return (void *)c->stream; }
+const Dev *nim_internal_dev(nim_ctx *c) { return &c->dev; }
+
 static void init_schedules(nim_ctx *c);
 
 static bool dims_ok(const nim_dims *d)

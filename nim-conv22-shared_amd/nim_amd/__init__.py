@@ -109,6 +109,17 @@ def lib():
         L.nim_residual.argtypes = [vp, _P]
         L.nim_kernel_times_mixed.restype = i32
         L.nim_kernel_times_mixed.argtypes = [vp, i32, i32, _P, _PI]
+        L.nim_revolve_plan.restype = i32
+        L.nim_revolve_plan.argtypes = [i32, i32, _PI, _PI, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_int64)]
+        L.nim_ckpt_create.restype = i32
+        L.nim_ckpt_create.argtypes = [vp, i32, ctypes.POINTER(vp)]
+        L.nim_ckpt_destroy.restype = i32
+        L.nim_ckpt_destroy.argtypes = [vp]
+        L.nim_ckpt_store.restype = i32
+        L.nim_ckpt_store.argtypes = [vp, i32]
+        L.nim_ckpt_restore.restype = i32
+        L.nim_ckpt_restore.argtypes = [vp, i32]
         L.nim_ens_create.restype = i32
         L.nim_ens_create.argtypes = [ctypes.POINTER(vp), i32, ctypes.POINTER(vp)]
         L.nim_ens_destroy.restype = i32
@@ -128,7 +139,8 @@ def exported_symbols():
             "nim_inner_solve", "nim_end_step", "nim_advance_time", "nim_nusselt", "nim_run_stage",
             "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_kernel_times",
             "nim_inner_solve_mixed", "nim_residual", "nim_kernel_times_mixed", "nim_ens_create", "nim_ens_destroy",
-            "nim_ens_inner_solve", "nim_last_error"]
+            "nim_ens_inner_solve", "nim_revolve_plan", "nim_ckpt_create", "nim_ckpt_destroy", "nim_ckpt_store",
+            "nim_ckpt_restore", "nim_last_error"]
 
 
 def _check(rc):
@@ -382,3 +394,70 @@ class Ensemble:
             c.advance_time()
             out.append((scal, c.nusselt()))
         return it, err, out
+
+
+RV_STORE, RV_RESTORE, RV_ADVANCE, RV_REVERSE = 1, 2, 3, 4
+
+
+def revolve_plan(nsteps, nslots):
+    """Binomial plan reversing nsteps time steps with nslots snapshots (slot
+    0 = initial state): (list of (action, argument), forward steps)."""
+    L = lib()
+    n, fwd = ctypes.c_int64(), ctypes.c_int64()
+    _check(L.nim_revolve_plan(nsteps, nslots, None, None, 0, ctypes.byref(n), ctypes.byref(fwd)))
+    codes = np.zeros(n.value, dtype=np.int32)
+    args = np.zeros(n.value, dtype=np.int32)
+    _check(L.nim_revolve_plan(nsteps, nslots, codes.ctypes.data_as(_PI), args.ctypes.data_as(_PI), n.value,
+                              ctypes.byref(n), ctypes.byref(fwd)))
+    return list(zip(codes.tolist(), args.tolist())), fwd.value
+
+
+class Checkpoints:
+    """Device snapshots of a context's time-step state (nim_ckpt_*)."""
+
+    def __init__(self, ctx, nslots):
+        self.L = lib()
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(self.L.nim_ckpt_create(ctx.h, nslots, ctypes.byref(h)))
+        self.h = h
+
+    def store(self, slot):
+        _check(self.L.nim_ckpt_store(self.h, slot))
+
+    def restore(self, slot):
+        _check(self.L.nim_ckpt_restore(self.h, slot))
+
+    def close(self):
+        if self.h:
+            self.L.nim_ckpt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def reverse_run(ctx, nsteps, nslots, forward_step, reverse_step):
+    """Visit the states before steps nsteps-1 .. 0 of a run in reverse with
+    nslots device snapshots: forward_step() advances ctx by one time step,
+    reverse_step(k) is called with the state before step k current (e.g. to
+    take the adjoint of step k). Returns the forward steps taken."""
+    plan, fwd = revolve_plan(nsteps, nslots)
+    ck = Checkpoints(ctx, nslots)
+    try:
+        for code, a in plan:
+            if code == RV_STORE:
+                ck.store(a)
+            elif code == RV_RESTORE:
+                ck.restore(a)
+            elif code == RV_ADVANCE:
+                for _ in range(a):
+                    forward_step()
+            else:
+                reverse_step(a)
+    finally:
+        ck.close()
+    return fwd

@@ -106,3 +106,35 @@ def test_driver_three_steps(tmp_path, oracle_colored, inner):
     for k, v in diff.items():
         bound = ORDER_BOUND["T" if "T" in k else "p" if k == "p" else "uvw"]
         assert v < bound, diff
+
+
+NML_RESUME = """&case n_theta=36, n_r=6, n_z=12, R_i=1.0471975511966667, R_o=2., height=2.,
+      blend_in=1.25, blend_out=1.75, Ra=1e5, Pr=0.7, DT=0.01, nsteps={nsteps},
+      tol=1e-6,1e-6,1e-6,5e-3,1e-6, noise=1e-3, seed=20221, check_steady=.false.,
+      restart_every={every}, restart_prefix='{prefix}', restart_in='{rin}', final_out='{out}',
+      mixed_inner={inner} /
+"""
+
+
+@pytest.mark.parametrize("inner", [0, 4])
+def test_restart_resume_is_bitwise(tmp_path, inner):
+    """4 steps in one run against 2 steps, a restart file (restart.f90:8-39,
+    written after the step's solve as solve_nse does) and 2 more steps from
+    it (read_restart + update_time_step, nse.f90:104-108): the final states
+    are bit-identical."""
+    def run(name, nsteps, every=0, rin=""):
+        nml = tmp_path / (name + ".nml")
+        out = tmp_path / (name + ".final")
+        nml.write_text(NML_RESUME.format(nsteps=nsteps, every=every, prefix=tmp_path / "rest", rin=rin, out=out,
+                                         inner=inner))
+        p = subprocess.run([EXE, str(nml)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        return read_restart(out)
+    straight = run("straight", 4)
+    run("first", 2, every=2)
+    assert (tmp_path / "rest2").exists()
+    resumed = run("resumed", 2, rin=tmp_path / "rest2")
+    assert straight["STEP"] == resumed["STEP"] == 5
+    assert straight["TI"] == resumed["TI"]
+    for n in RECORDS:
+        assert np.array_equal(straight[n], resumed[n]), n

@@ -176,3 +176,65 @@ def initial_state(case, amp=1e-3, seed=20221):
     st["meshes_p"] = np.zeros(N)
     st["meshes_dila"] = np.zeros(N)
     return st
+
+
+def find_rms(ctx, exact, mask=None):
+    """find_rms (nse.f90:2801-2839): RMS over cells (or faces) of the
+    difference between an analytical field and the numerical one, for every
+    array named in `exact` ("uxyz", "faces_u", ... as Context.get names
+    them; the reference covers u, v, w, any field works here). `mask`
+    optionally restricts each array to a boolean selection."""
+    import numpy as np
+    out = {}
+    for name, ex in exact.items():
+        num = ctx.get(name)
+        sel = np.ones(num.shape, dtype=bool) if mask is None or name not in mask else mask[name]
+        d = ex[sel] - num[sel]
+        out[name] = float(np.sqrt(np.sum(np.abs(d) ** 2.) / sel.sum()))
+    return out
+
+
+def cell_centroids(geo):
+    """Cell centres as the mean of the six face mid-points (reference
+    numbering)."""
+    import numpy as np
+    mf = geo["meshes_faces"].reshape(-1, 6) - 1
+    mid = geo["faces_mid"].reshape(-1, 3)
+    return mid[mf].mean(axis=1)
+
+
+def cell_nodes(dims):
+    """The 8 corner nodes of every cell (reference numbering, 0-based) in
+    VTK hexahedron order: nodes are i + n_theta (j + (n_r + 1) k) as the
+    O-grid is built (create_hollow), i periodic; the bottom quad is taken r
+    first, then theta, so it turns counter-clockwise seen from +z."""
+    import numpy as np
+    nt, nr, nz = dims
+    i, j, k = np.meshgrid(np.arange(nt), np.arange(nr), np.arange(nz), indexing="ij")
+    i, j, k = (a.transpose(2, 1, 0).ravel() for a in (i, j, k))  # cell n = i + nt (j + nr k)
+    ip = (i + 1) % nt
+    node = lambda a, b, c: a + nt * (b + (nr + 1) * c)
+    return np.stack([node(i, j, k), node(i, j + 1, k), node(ip, j + 1, k), node(ip, j, k),
+                     node(i, j, k + 1), node(i, j + 1, k + 1), node(ip, j + 1, k + 1), node(ip, j, k + 1)], axis=1)
+
+
+def write_vtk(path, case, geo, cell_fields):
+    """Legacy-VTK unstructured grid of the mesh with cell data, e.g.
+    {"T": ctx.get("Txyz"), ...}: the stand-in for the reference's Silo
+    output (postprocess.f90, Silo is not available here)."""
+    import numpy as np
+    xyz = geo["node_xyz"].reshape(-1, 3)
+    cn = cell_nodes(case.dims)
+    n = len(cn)
+    with open(path, "w") as f:
+        f.write("# vtk DataFile Version 3.0\nNIM fields\nASCII\nDATASET UNSTRUCTURED_GRID\n")
+        f.write("POINTS %d double\n" % len(xyz))
+        np.savetxt(f, xyz, fmt="%.17g")
+        f.write("CELLS %d %d\n" % (n, 9 * n))
+        np.savetxt(f, np.hstack([np.full((n, 1), 8), cn]), fmt="%d")
+        f.write("CELL_TYPES %d\n" % n)
+        np.savetxt(f, np.full(n, 12), fmt="%d")
+        f.write("CELL_DATA %d\n" % n)
+        for name, v in cell_fields.items():
+            f.write("SCALARS %s double 1\nLOOKUP_TABLE default\n" % name)
+            np.savetxt(f, np.asarray(v, dtype=np.float64).reshape(n), fmt="%.17g")

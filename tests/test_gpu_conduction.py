@@ -8,9 +8,10 @@ T(r) = ln(r / R_o) / ln(R_i / R_o) and the inner-wall Nusselt number
 (scaled by R_i, as find_Nu2 prints it, nse.f90:2794) is 1 / ln(R_o / R_i).
 This checks the energy sweep, the adiabatic walls and find_Nu2 with the
 mesh and metrics at three resolutions: the errors fall at second order.
-The run starts from the analytical profile and takes DT = 0.25: the slowest
-transient then decays at its physical rate, about 0.89 per step (at DT = 2
-the NIM step damps it only by 0.94 per step, measured with the oracle)."""
+The run starts from the analytical profile. DT shrinks with the mesh
+(0.25, 0.12, 0.06): the NIM step damps the slowest transient at its physical
+rate only while DT stays small against the finest cells' diffusion time;
+twice these values take several times more steps (measured)."""
 import math
 
 import numpy as np
@@ -25,8 +26,11 @@ R_I, R_O = 0.3, 1.0
 TOL = np.array([1e-12, 1e-12, 1e-12, 1e-6, 1e-12])
 
 
+DT = {6: 0.25, 12: 0.12, 24: 0.06}
+
+
 def steady_conduction(n_r):
-    case = host.Case((4 * n_r, n_r, 3), R_I, R_O, 1.0, Ra=1e4, Pr=0.71, DT=0.25, blend=(0.1, 0.2))
+    case = host.Case((4 * n_r, n_r, 3), R_I, R_O, 1.0, Ra=1e4, Pr=0.71, DT=DT[n_r], blend=(0.1, 0.2))
     case.g_beta = 0.0  # no buoyancy; nu keeps its Ra, Pr definition
     geo = host.build_geometry(case)
     cc = host.cell_centroids(geo)
@@ -46,7 +50,7 @@ def steady_conduction(n_r):
     ctx.upload_geometry(geo)
     ctx.build_pressure_coeffs()
     ctx.upload_state(st)
-    for step in range(400):
+    for step in range(900):
         it, err, scal, nu = ctx.time_step(TOL, min_iter=2 if step == 0 else 0)
         if scal["dTdt_max"] < 1e-10:
             break

@@ -143,38 +143,47 @@ __device__ __forceinline__ void cell_faces(const Dev &d, int i, int j, int k, in
 // ---------------------------------------------------------------------
 // NIM exponential-fit constants of one direction (nim.f90:184-287)
 // ---------------------------------------------------------------------
+// Both branches are the reference's expressions rewritten with the
+// reciprocals rk = 1/k, ru = 1/u and t = u/k (= Re/2): the Taylor branch
+// becomes series in t with exact rational coefficients (Horner form, one
+// division instead of about forty), the exponential branch uses
+// p = 1/(e^-Re - 1) and q = 1/(1 - e^Re) (five divisions instead of
+// eleven). The values agree with the reference's to rounding, amplified
+// only by the cancellation the closed forms already have near |Re| =
+// v_limit.
 __device__ void nim_direction(double u, double k, double gp, double gm, double vlim, double A[6])
 {
-    double Re = 2 * u / k;
+    const double rk = 1. / k;
+    const double t = u * rk, Re = 2. * t;
     if (fabs(Re) > vlim) {
-        double e = exp(Re), ie = 1. / e;
-        A[0] = (2 / (k * (ie - 1.)) + 1. / u) * gp;
-        A[1] = (-(2 / (k * (1. - e))) - 1. / u) * gm;
-        A[2] = (u / (k * (ie - 1.))) * gp;
-        A[3] = (u / (k * (1. - e))) * gm;
+        const double ru = 1. / u;
+        const double e = exp(Re), ie = 1. / e;
+        const double p = 1. / (ie - 1.), q = 1. / (1. - e);
+        A[0] = (2. * rk * p + ru) * gp;
+        A[1] = (-(2. * rk * q) - ru) * gm;
+        A[2] = (u * rk * p) * gp;
+        A[3] = (u * rk * q) * gm;
         if (Re > 0) {
-            A[4] = (-k + (u * (ie + 1.) / (-ie + 1.))) / (u * u);
-            A[5] = 1. / (-ie + 1.) - 1. / Re;
+            A[4] = (-k - u * (ie + 1.) * p) * (ru * ru);
+            A[5] = -p - 0.5 * k * ru;
         } else {
-            A[4] = (-k + (u * (1. + e) / (-1. + e))) / (u * u);
-            A[5] = e / (-1. + e) - 1. / Re;
+            A[4] = (-k - u * (1. + e) * q) * (ru * ru);
+            A[5] = -(e * q) - 0.5 * k * ru;
         }
     } else {
-        double u2 = u * u, u3 = u2 * u, u4 = u2 * u2, u5 = u4 * u, u6 = u4 * u2, u7 = u6 * u, u8 = u4 * u4;
-        double u9 = u8 * u, u10 = u8 * u2;
-        double k2 = k * k, k3 = k2 * k, k4 = k2 * k2, k5 = k4 * k, k6 = k4 * k2, k7 = k6 * k, k8 = k4 * k4;
-        double k9 = k8 * k, k10 = k8 * k2, k11 = k10 * k;
-        A[0] = (-(1 / k) - u / (3. * k2) + u3 / (45. * k4) - 2 * u5 / (945. * k6) - u7 / (4725. * k8)
-                - 2 * u9 / (93555. * k10)) * gp;
-        A[1] = (-(1 / k) + u / (3. * k2) - u3 / (45. * k4) + 2 * u5 / (945. * k6) + u7 / (4725. * k8)
-                + 2 * u9 / (93555. * k10)) * gm;
-        A[2] = (-0.5 - u / (2. * k) - u2 / (6. * k2) + u4 / (90. * k4) + u6 / (945. * k6) + u8 / (9450. * k8)
-                - u10 / (93555. * k10)) * gp;
-        A[3] = (-0.5 + u / (2. * k) - u2 / (6. * k2) + u4 / (90. * k4) + u6 / (945. * k6) + u8 / (9450. * k8)
-                - u10 / (93555. * k10)) * gm;
-        A[4] = 1 / (3. * k) - u2 / (45. * k3) + 2 * u4 / (945. * k5) + u6 / (4725. * k7) + 2 * u8 / (93555. * k9)
-               + 1382 * u10 / (6.38512875e8 * k11);
-        A[5] = 0.5 + u / (6. * k) - u3 / (90. * k3) + u5 / (945. * k5) + u7 / (9450. * k7) + u9 / (93555. * k9);
+        constexpr double c3 = 1. / 3., c6 = 1. / 6., c45 = 1. / 45., c90 = 1. / 90., c945 = 1. / 945.;
+        constexpr double c945b = 2. / 945., c4725 = 1. / 4725., c9450 = 1. / 9450., c93555 = 1. / 93555.;
+        constexpr double c93555b = 2. / 93555., c11 = 1382. / 6.38512875e8;
+        const double s2 = t * t;
+        // odd and even series shared by the pairs A0/A1 and A2/A3
+        const double Q = t * (c3 + s2 * (-c45 + s2 * (c945b + s2 * (c4725 + s2 * c93555b))));
+        const double E = s2 * (-c6 + s2 * (c90 + s2 * (c945 + s2 * (c9450 - s2 * c93555))));
+        A[0] = (rk * (-1. - Q)) * gp;
+        A[1] = (rk * (-1. + Q)) * gm;
+        A[2] = (-0.5 - 0.5 * t + E) * gp;
+        A[3] = (-0.5 + 0.5 * t + E) * gm;
+        A[4] = rk * (c3 + s2 * (-c45 + s2 * (c945b + s2 * (c4725 + s2 * (c93555b + s2 * c11)))));
+        A[5] = 0.5 + t * (c6 + s2 * (-c90 + s2 * (c945 + s2 * (c9450 + s2 * c93555))));
     }
 }
 

@@ -217,6 +217,10 @@ extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, n
     d.w_u = prm->w_u; d.w_v = prm->w_v; d.w_w = prm->w_w; d.w_p = prm->w_p; d.w_p2 = prm->w_p2; d.w_T = prm->w_T;
     d.v_limit = prm->v_limit;
     c->nred = (c->N + BS - 1) / BS;
+    {
+        const char *e = getenv("NIM_KFAST");
+        d.kfast = e ? atoi(e) != 0 : 0;
+    }
     init_schedules(c);
     double *gc, *An, *wall;
     int32_t *wpts;
@@ -574,7 +578,13 @@ extern "C" int32_t nim_build_momentum_coeffs(nim_ctx *c)
 // ---- one outer iteration ---------------------------------------------
 // momentum kernels: 64 cells per block, one wave per velocity component
 static const dim3 BS3(64, 3);
-static int cell_blocks3(const nim_ctx *c) { return (c->N + 63) / 64; }
+// blocks of the per-cell kernels (storage order or plane-fastest)
+static int cell_grid(const nim_ctx *c, int bs)
+{
+    if (!c->dev.kfast) return (c->N + bs - 1) / bs;
+    return ((c->NT * c->NR + bs - 1) / bs) * c->NZ;
+}
+static int cell_blocks3(const nim_ctx *c) { return cell_grid(c, 64); }
 
 // Launches of one relaxation sweep. With slabs of `slab` axial planes
 // (S slabs), launch L runs colour c on slab s for every s + 2c = L. Colour
@@ -626,7 +636,7 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark)
 {
     const Dev &d = c->dev;
     hipStream_t s = c->stream;
-    int gN = nblk(c->N);
+    const int gS = cell_grid(c, BS);
     switch (stage) {
     case 0: hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); mark(0); break;
     case 1:
@@ -635,15 +645,15 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark)
             mark(1);
         }
         break;
-    case 2: hipLaunchKernelGGL(k_ppe_src, dim3(gN), dim3(BS), 0, s, d); mark(2); break;
+    case 2: hipLaunchKernelGGL(k_ppe_src, dim3(gS), dim3(BS), 0, s, d); mark(2); break;
     case 3:
         for (const Sched &sc : c->sched256) {
             hipLaunchKernelGGL(k_ppe_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
             mark(3);
         }
         break;
-    case 4: hipLaunchKernelGGL(k_T_coef, dim3(gN), dim3(BS), 0, s, d); mark(4); break;
-    case 5: hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); mark(6); break;
+    case 4: hipLaunchKernelGGL(k_T_coef, dim3(gS), dim3(BS), 0, s, d); mark(4); break;
+    case 5: hipLaunchKernelGGL(k_T_src, dim3(gS), dim3(BS), 0, s, d); mark(6); break;
     case 6:
         for (const Sched &sc : c->sched256) {
             hipLaunchKernelGGL(k_T_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
@@ -795,10 +805,10 @@ static void launch_residual(nim_ctx *c, Mark mark)
     const int gN = nblk(c->N);
     hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); mark(0);
     hipLaunchKernelGGL(k_res_mom, dim3(cell_blocks3(c)), BS3, 0, s, d, r); mark(0);
-    hipLaunchKernelGGL(k_res_ppe_src, dim3(gN), dim3(BS), 0, s, d, r); mark(1);
+    hipLaunchKernelGGL(k_res_ppe_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, r); mark(1);
     hipLaunchKernelGGL(k_res_ppe, dim3(gN), dim3(BS), 0, s, d, r); mark(1);
-    hipLaunchKernelGGL(k_res_T_coef, dim3(gN), dim3(BS), 0, s, d, r); mark(2);
-    hipLaunchKernelGGL(k_T_src, dim3(gN), dim3(BS), 0, s, d); mark(2);
+    hipLaunchKernelGGL(k_res_T_coef, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, r); mark(2);
+    hipLaunchKernelGGL(k_T_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d); mark(2);
     hipLaunchKernelGGL(k_res_T, dim3(gN), dim3(BS), 0, s, d, r); mark(2);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, d); mark(3);
 }
@@ -810,19 +820,18 @@ static void launch_delta(nim_ctx *c, Mark mark)
     const Dev &d = c->dev;
     const View<float> &v = c->mx.v32;
     hipStream_t s = c->stream;
-    const int gN = nblk(c->N);
     hipLaunchKernelGGL(k_d_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d, v); mark(4);
     for (const Sched &sc : c->sched64) {
         hipLaunchKernelGGL(k_d_mom_sweep, dim3(sched_blocks(sc)), BS3, 0, s, d, v, sc);
         mark(5);
     }
-    hipLaunchKernelGGL(k_d_ppe_src, dim3(gN), dim3(BS), 0, s, d, v); mark(6);
+    hipLaunchKernelGGL(k_d_ppe_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, v); mark(6);
     for (const Sched &sc : c->sched256) {
         hipLaunchKernelGGL(k_d_ppe_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, v, sc);
         mark(7);
     }
     hipLaunchKernelGGL(k_d_zero_flux, dim3(nblk(2 * c->NT * c->NR)), dim3(BS), 0, s, d, v); mark(8);
-    hipLaunchKernelGGL(k_d_T_src, dim3(gN), dim3(BS), 0, s, d, v); mark(9);
+    hipLaunchKernelGGL(k_d_T_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, v); mark(9);
     for (const Sched &sc : c->sched256) {
         hipLaunchKernelGGL(k_d_T_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, v, sc);
         mark(10);
@@ -1118,8 +1127,8 @@ extern "C" int32_t nim_ens_create(nim_ctx *const *cases, int32_t ncase, nim_ens 
             bs[(size_t)sh * (ncase + 1) + q] = acc;
             int cc = (c->NT / 2) * ((c->NR + 1) / 2) * c->NZ;
             switch (sh) {
-            case SH_CELL64: acc += (c->N + 63) / 64; break;
-            case SH_CELL256: acc += nblk(c->N); break;
+            case SH_CELL64: acc += cell_grid(c, 64); break;
+            case SH_CELL256: acc += cell_grid(c, BS); break;
             case SH_COL64: acc += (cc + 63) / 64; break;
             case SH_COL256: acc += nblk(cc); break;
             case SH_WALL: acc += nblk(2 * c->NT * c->NR); break;

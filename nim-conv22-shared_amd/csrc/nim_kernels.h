@@ -80,6 +80,7 @@ struct Dev {
     int NT, NR, NZ, N, NF, NB, oT, oZ;
     double nu, Rinv, alpha, rinv, rho, g_beta, DT;
     double w_u, w_v, w_w, w_p, w_p2, w_T, v_limit;
+    int kfast;          // per-cell kernels: blocks numbered plane-fastest (see cell_of_block)
     const double *gc;   // [NGC][N]
     const double *An;   // [3][NF] face area times unit normal
     double *AA;         // [3][15][N] pressure coefficients AA_px, AA_py, AA_pz
@@ -163,6 +164,19 @@ __device__ __forceinline__ int xcd_block()
     int b = blockIdx.x, nb = gridDim.x;
     int q = nb / NXCD, rem = nb % NXCD, x = b % NXCD, r = b / NXCD;
     return x * q + (x < rem ? x : rem) + r;
+}
+
+// Cell of this thread in the per-cell (source and coefficient) kernels, in
+// storage order or (kfast) with the blocks numbered plane-fastest: logical
+// block lb covers cells [t*bs, (t+1)*bs) of plane lb % NZ, t = lb / NZ, so
+// consecutive blocks read the same faces of neighbouring planes. Returns N
+// for threads past the end of a plane.
+__device__ __forceinline__ int cell_of_block(const Dev &d, int lb, int bs)
+{
+    if (!d.kfast) return lb * bs + (int)threadIdx.x;
+    const int ppl = d.NT * d.NR;
+    const int cp = (lb / d.NZ) * bs + (int)threadIdx.x;
+    return cp < ppl ? cp + (lb % d.NZ) * ppl : d.N;
 }
 
 __device__ __forceinline__ int ip1(const Dev &d, int i) { return i + 1 == d.NT ? 0 : i + 1; }

@@ -309,7 +309,7 @@ __device__ __forceinline__ void energy_direction(const Dev &d, int c, int dir, d
 template <int MODE>
 __device__ __forceinline__ void T_coef(const Dev &d, const View<double> &v, int lb)
 {
-    int c = lb * blockDim.x + threadIdx.x;
+    int c = cell_of_block(d, lb, blockDim.x);
     bool act = c < d.N;
     double eT = 0.;
     if (act) {
@@ -412,7 +412,7 @@ __global__ void __launch_bounds__(256) k_res_T_coef(Dev d, View<double> v)
 template <typename T>
 __device__ __forceinline__ void mom_src(const Dev &d, const View<T> &v, int lb)
 {
-    int c = lb * 64 + threadIdx.x;
+    int c = cell_of_block(d, lb, 64);
     const int q = threadIdx.y;
     if (c >= d.N) return;
     const int N = d.N;
@@ -453,7 +453,7 @@ __global__ void __launch_bounds__(192) k_d_mom_src(Dev d, View<float> v)
 template <int MODE, typename T>
 __device__ __forceinline__ void ppe_src(const Dev &d, const View<T> &v, int lb)
 {
-    int c = lb * blockDim.x + threadIdx.x;
+    int c = cell_of_block(d, lb, blockDim.x);
     if (c >= d.N) return;
     const int N = d.N;
     int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
@@ -518,7 +518,7 @@ __global__ void __launch_bounds__(256) k_d_ppe_src(Dev d, View<float> v)
 template <typename T>
 __device__ __forceinline__ void T_src(const Dev &d, const View<T> &v, int lb)
 {
-    int c = lb * blockDim.x + threadIdx.x;
+    int c = cell_of_block(d, lb, blockDim.x);
     if (c >= d.N) return;
     int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     v.src[R_T * d.N + c] = cross_diffusion(d, v.gc, v.fc + (size_t)F_T * d.NF, i, j, k, c, T(d.alpha)) + T(0);

@@ -231,3 +231,28 @@ def test_slab_schedule_is_bitwise_the_colour_order(geo, steps, slab, monkeypatch
         a, b = run(None, mixed), run(slab, mixed)
         for n in names:
             assert np.array_equal(a[n], b[n]), (n, mixed)
+
+
+def test_plane_fastest_cell_order_is_bitwise(geo, steps, monkeypatch):
+    """The per-cell kernels compute every cell from the same inputs whatever
+    order their blocks run in: NIM_KFAST=1 (plane-fastest blocks) ends 300
+    passes of either loop on the same bits."""
+    names = ["faces_" + f for f in "uvwpT"] + [f + "xyz" for f in "uvwT"]
+
+    def run(flag, mixed):
+        monkeypatch.setenv("NIM_KFAST", flag)
+        ctx = G.gpu_context(geo)
+        ctx.upload_state(G.state_from_steps(steps, "s3"))
+        ctx.build_momentum_coeffs()
+        if mixed:
+            ctx.inner_solve_mixed(np.zeros(5), inner=4, max_cycles=76, raise_on_cap=False)
+        else:
+            ctx.inner_solve(np.zeros(5), max_iter=300, raise_on_cap=False)
+        out = {n: ctx.get(n) for n in names}
+        ctx.close()
+        return out
+
+    for mixed in (False, True):
+        a, b = run("0", mixed), run("1", mixed)
+        for n in names:
+            assert np.array_equal(a[n], b[n]), (n, mixed)

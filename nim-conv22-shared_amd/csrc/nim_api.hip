@@ -232,6 +232,7 @@ extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, n
     TRY(dalloc(c, &gc, (size_t)NGC * N));
     TRY(dalloc(c, &An, (size_t)3 * NF));
     TRY(dalloc(c, &d.AA, (size_t)45 * N));
+    TRY(dalloc(c, &d.AQ, (size_t)3 * NAQ * N));
     TRY(dalloc(c, &wall, (size_t)NWS * NB));
     TRY(dalloc(c, &wpts, (size_t)5 * NB));
     TRY(dalloc(c, &d.sc, (size_t)NSC * N));
@@ -762,15 +763,15 @@ static int32_t mixed_prepare(nim_ctx *c)
     auto &m = c->mx;
     const size_t N = c->N, NF = c->NF, NB = c->NB;
     if (!m.alloc) {
-        float *gc, *An, *AA, *wall, *MF, *TF, *sc, *fc, *src, *rc, *rf, *grad;
+        float *gc, *An, *AQ, *wall, *MF, *TF, *sc, *fc, *src, *rc, *rf, *grad;
         int32_t rc_;
-        if ((rc_ = dalloc(c, &gc, NGC * N)) || (rc_ = dalloc(c, &An, 3 * NF)) || (rc_ = dalloc(c, &AA, 45 * N)) ||
+        if ((rc_ = dalloc(c, &gc, NGC * N)) || (rc_ = dalloc(c, &An, 3 * NF)) || (rc_ = dalloc(c, &AQ, 3 * NAQ * N)) ||
             (rc_ = dalloc(c, &wall, NWS * NB)) || (rc_ = dalloc(c, &MF, NCF * N)) || (rc_ = dalloc(c, &TF, NCF * N)) ||
             (rc_ = dalloc(c, &sc, 4 * N)) || (rc_ = dalloc(c, &fc, 5 * NF)) || (rc_ = dalloc(c, &src, NSRC * N)) ||
             (rc_ = dalloc(c, &rc, 4 * N)) || (rc_ = dalloc(c, &rf, 5 * NF)) || (rc_ = dalloc(c, &grad, 9 * N)))
             return rc_;
         View<float> &v = m.v32;
-        v.gc = gc; v.An = An; v.AA = AA; v.wall = wall; v.MF = MF; v.TF = TF;
+        v.gc = gc; v.An = An; v.AQ = AQ; v.wall = wall; v.MF = MF; v.TF = TF;
         v.sc = sc; v.old = nullptr; v.fc = fc; v.src = src; v.rc = rc; v.rf = rf; v.TF32 = nullptr; v.grad = grad;
         m.v64 = view64(c->dev);
         m.v64.rc = rc; m.v64.rf = rf; m.v64.TF32 = TF; m.v64.grad = grad;
@@ -785,7 +786,7 @@ static int32_t mixed_prepare(nim_ctx *c)
         const Dev &d = c->dev;
         cast(d.gc, m.v32.gc, NGC * N);
         cast(d.An, m.v32.An, 3 * NF);
-        cast(d.AA, m.v32.AA, 45 * N);
+        cast(d.AQ, m.v32.AQ, 3 * NAQ * N);
         cast(d.wall, m.v32.wall, NWS * NB);
         m.version = c->const_version;
     }

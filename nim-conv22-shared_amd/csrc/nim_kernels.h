@@ -58,6 +58,15 @@ enum {
 
 // coefficient slots in Dev::MF / Dev::TF: F4(1..8), F1(1..5), F2(1..5), F3(1..5)
 enum { C_F4 = 0, C_F1 = 8, C_F2 = 13, C_F3 = 18, NCF = 23 };
+// pressure coefficients read by the sweep, per direction in Dev::AQ
+// ([3][NAQ][N]): 9 of AA_p? (15 slots, nim.f90:622-697). At u = 0 the
+// coefficients E and F are equal, so each tangential pair of slots holds one
+// value twice and is stored once; AA(12) and AA(14) are not read.
+//   SELF, DIAG, FAR   the normal pair and the diagonal of the face equation
+//   TA0, TB0          first tangential direction, own and partner side
+//   TA1, TB1          second tangential direction, own and partner side
+//   S1, S2            AA(13) and AA(15), the weights of the two cell sources
+enum { Q_SELF = 0, Q_DIAG = 1, Q_FAR = 2, Q_TA0 = 3, Q_TB0 = 4, Q_TA1 = 5, Q_TB1 = 6, Q_S1 = 7, Q_S2 = 8, NAQ = 9 };
 // source slots in Dev::src: R sums for u, v, w; pressure R_D_p + R_b_p; R_D_T
 enum { R_U = 0, R_V = 1, R_W = 2, R_P = 3, R_T = 4, NSRC = 5 };
 // face field slots in Dev::fc
@@ -84,6 +93,7 @@ struct Dev {
     const double *gc;   // [NGC][N]
     const double *An;   // [3][NF] face area times unit normal
     double *AA;         // [3][15][N] pressure coefficients AA_px, AA_py, AA_pz
+    double *AQ;         // [3][NAQ][N] the ones the pressure sweep reads
     double *wall;       // [NWS][NB]
     const int32_t *wpts;// [5][NB] face slots of the 5-point normal stencil
     double *sc;         // [NSC][N]
@@ -109,10 +119,10 @@ enum { MODE_GS = 0, MODE_RES = 1, MODE_DELTA = 2 };
 
 // The arrays a kernel works on: the FP64 state and constants, or the FP32
 // corrections with FP32 copies of the constants. Slot layouts are those of
-// Dev (gc [NGC][N], AA [45][N], wall [NWS][NB], MF/TF [NCF][N], src
+// Dev (gc [NGC][N], AQ [3*NAQ][N], wall [NWS][NB], MF/TF [NCF][N], src
 // [NSRC][N], cells [4][N] at S_CUR, faces [5][NF]).
 template <typename T> struct View {
-    const T *gc, *An, *AA, *wall, *MF, *TF;
+    const T *gc, *An, *AQ, *wall, *MF, *TF;
     T *sc;          // u, v, w, T cell values (corrections in DELTA)
     const T *old;   // previous time level of u, v, w, T; null in DELTA
     T *fc;          // faces of u, v, w, p, T
@@ -127,7 +137,7 @@ template <typename T> struct View {
 __device__ __host__ inline View<double> view64(const Dev &d)
 {
     View<double> v;
-    v.gc = d.gc; v.An = d.An; v.AA = d.AA; v.wall = d.wall; v.MF = d.MF; v.TF = d.TF;
+    v.gc = d.gc; v.An = d.An; v.AQ = d.AQ; v.wall = d.wall; v.MF = d.MF; v.TF = d.TF;
     v.sc = d.sc; v.old = d.sc + (size_t)S_OLD * d.N; v.fc = d.fc; v.src = d.src;
     v.rc = nullptr; v.rf = nullptr; v.TF32 = nullptr; v.grad = nullptr;
     return v;

@@ -565,16 +565,29 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
     cell_faces(d, i, j, k, f);
     // ---- gather
     const T old = C[c], oc = MODE == MODE_DELTA ? T(0) : CO[c];
+    // The diagonals F4(1), F1(1), F2(1), F3(1) are sums of the other
+    // coefficients (nim.f90:292-329, 465-502). In FP64 they are summed
+    // again here in the order the coefficient kernels use, which gives the
+    // stored values bit for bit and saves four loads per cell. The FP32
+    // copies are rounded from the FP64 values, so the FP32 passes load them.
+    constexpr bool sum_diag = sizeof(T) == sizeof(double);
+    constexpr int m0 = sum_diag ? 1 : 0;
     T a[8];
 #pragma unroll
-    for (int m = 0; m < 8; m++) a[m] = coef[(C_F4 + m) * N + c];
+    for (int m = m0; m < 8; m++) a[m] = coef[(C_F4 + m) * N + c];
     T F0 = Fc[f[0]], F1 = Fc[f[1]], F2 = Fc[f[2]], F3 = Fc[f[3]], F4 = Fc[f[4]], F5 = Fc[f[5]];
     T b[5], e[5], h[5];
 #pragma unroll
-    for (int m = 0; m < 5; m++) {
+    for (int m = m0; m < 5; m++) {
         b[m] = coef[(C_F1 + m) * N + c];
         e[m] = coef[(C_F2 + m) * N + c];
         h[m] = coef[(C_F3 + m) * N + c];
+    }
+    if (sum_diag) {
+        a[0] = a[1] + a[2] + a[3] + a[4] + a[5] + a[6] + a[7];
+        b[0] = b[1] + b[2] + T(2) * b[3] + T(2) * b[4];
+        e[0] = e[1] + e[2] + T(2) * e[3] + T(2) * e[4];
+        h[0] = h[1] + h[2] + T(2) * h[3] + T(2) * h[4];
     }
     const T gd0 = gdf[0 * N + c], gd1 = gdf[1 * N + c], gd2 = gdf[2 * N + c];
     const T gd3 = gdf[3 * N + c], gd4 = gdf[4 * N + c], gd5 = gdf[5 * N + c];
@@ -839,14 +852,13 @@ __device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const
 #pragma unroll
         for (int p = 0; p < 6; p++) pf[p] = P[f[p]];
         const T S1 = R[c];
-        const T *__restrict__ Ax = vw.AA, *__restrict__ Ay = vw.AA + (size_t)15 * N, *__restrict__ Az = vw.AA + (size_t)30 * N;
-        T ax[15], ay[15], az[15];
+        const T *__restrict__ Q = vw.AQ;
+        T qx[NAQ], qy[NAQ], qz[NAQ];
 #pragma unroll
-        for (int m = 0; m < 15; m++) {
-            if (m == 11 || m == 13) continue;  // AA(12) and AA(14) are not used by the sweep
-            ax[m] = Ax[m * N + c];
-            ay[m] = Ay[m * N + c];
-            az[m] = Az[m * N + c];
+        for (int m = 0; m < NAQ; m++) {
+            qx[m] = Q[m * N + c];
+            qy[m] = Q[(NAQ + m) * N + c];
+            qz[m] = Q[(2 * NAQ + m) * N + c];
         }
         // x: faces of cell (i, j+1, k)
         const T Sx = R[cidx(d, i, jn, k)];
@@ -885,24 +897,27 @@ __device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const
                 pv = v;
             }
         };
-        // ---- compute (AA(m) is a?[m-1])
+        // ---- compute: AA_p?(m) of nim.f90:622-697 as the Q_ slots
         if (hx) {
-            T B1 = S1 * ax[12], B2 = Sx * ax[14];
-            T S = -ax[0] * pf[0] - ax[2] * x2 - ax[3] * pf[2] - ax[4] * pf[3] - ax[5] * x3 - ax[6] * x4
-                  - ax[7] * pf[4] - ax[8] * pf[5] - ax[9] * x5 - ax[10] * x6 + B1 + B2 + Bx3 + Bx4;
-            face(pf[1], f[1], ax[1], S, q2);
+            T B1 = S1 * qx[Q_S1], B2 = Sx * qx[Q_S2];
+            T S = -qx[Q_SELF] * pf[0] - qx[Q_FAR] * x2 - qx[Q_TA0] * pf[2] - qx[Q_TA0] * pf[3] - qx[Q_TB0] * x3
+                  - qx[Q_TB0] * x4 - qx[Q_TA1] * pf[4] - qx[Q_TA1] * pf[5] - qx[Q_TB1] * x5 - qx[Q_TB1] * x6 + B1 + B2
+                  + Bx3 + Bx4;
+            face(pf[1], f[1], qx[Q_DIAG], S, q2);
         }
         {
-            T B1 = S1 * ay[12], B2 = Sy * ay[14];
-            T S = -ay[0] * pf[0] - ay[1] * pf[1] - ay[2] * y1 - ay[3] * y2 - ay[4] * pf[2] - ay[6] * y4
-                  - ay[7] * pf[4] - ay[8] * pf[5] - ay[9] * y5 - ay[10] * y6 + B1 + B2 + By3 + By4;
-            face(pf[3], f[3], ay[5], S, q4);
+            T B1 = S1 * qy[Q_S1], B2 = Sy * qy[Q_S2];
+            T S = -qy[Q_TA0] * pf[0] - qy[Q_TA0] * pf[1] - qy[Q_TB0] * y1 - qy[Q_TB0] * y2 - qy[Q_SELF] * pf[2]
+                  - qy[Q_FAR] * y4 - qy[Q_TA1] * pf[4] - qy[Q_TA1] * pf[5] - qy[Q_TB1] * y5 - qy[Q_TB1] * y6 + B1 + B2
+                  + By3 + By4;
+            face(pf[3], f[3], qy[Q_DIAG], S, q4);
         }
         if (hz) {
-            T B1 = S1 * az[12], B2 = Sz * az[14];
-            T S = -az[0] * pf[0] - az[1] * pf[1] - az[2] * z1 - az[3] * z2 - az[4] * pf[2] - az[5] * pf[3]
-                  - az[6] * z3 - az[7] * z4 - az[8] * pf[4] - az[10] * z6 + B1 + B2 + Bz3 + Bz4;
-            face(pf[5], f[5], az[9], S, q6);
+            T B1 = S1 * qz[Q_S1], B2 = Sz * qz[Q_S2];
+            T S = -qz[Q_TA0] * pf[0] - qz[Q_TA0] * pf[1] - qz[Q_TB0] * z1 - qz[Q_TB0] * z2 - qz[Q_TA1] * pf[2]
+                  - qz[Q_TA1] * pf[3] - qz[Q_TB1] * z3 - qz[Q_TB1] * z4 - qz[Q_SELF] * pf[4] - qz[Q_FAR] * z6 + B1 + B2
+                  + Bz3 + Bz4;
+            face(pf[5], f[5], qz[Q_DIAG], S, q6);
         }
         // ---- scatter
         if (MODE != MODE_RES) {
@@ -1128,6 +1143,18 @@ __global__ void __launch_bounds__(256) k_pressure_coeffs(Dev d)
         out[12 * N + c] = -s;
         out[13 * N + c] = -s2;
         out[14 * N + c] = s2;
+        // the sweep's copy: tv[t][0] = tv[t][1] and tv[t][2] = tv[t][3]
+        // (E = F), so one of each pair
+        double *q = d.AQ + (size_t)(dir * NAQ) * N;
+        q[Q_SELF * N + c] = A[n0];
+        q[Q_DIAG * N + c] = A[n0 + 1];
+        q[Q_FAR * N + c] = A[n0 + 2];
+        q[Q_TA0 * N + c] = tv[0][0];
+        q[Q_TB0 * N + c] = tv[0][2];
+        q[Q_TA1 * N + c] = tv[1][0];
+        q[Q_TB1 * N + c] = tv[1][2];
+        q[Q_S1 * N + c] = -s;
+        q[Q_S2 * N + c] = s2;
     }
 }
 

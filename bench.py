@@ -38,8 +38,8 @@ HBM_PEAK = 8.0e12
 # touched once, neighbour reuse on chip): the per-cell arrays each kernel
 # reads and writes, counted from its source.
 BYTES_PER_CELL = {
-    "velocity_sources": 36 * 8, "velocity_sweep": 59 * 8, "pressure_sources": 36 * 8, "pressure_sweep": 52 * 8,
-    "energy_coefficients": 47 * 8, "unused": 0, "energy_sources": 13 * 8, "energy_sweep": 39 * 8, "check": 0,
+    "velocity_sources": 36 * 8, "velocity_sweep": 55 * 8, "pressure_sources": 36 * 8, "pressure_sweep": 40 * 8,
+    "energy_coefficients": 47 * 8, "unused": 0, "energy_sources": 13 * 8, "energy_sweep": 35 * 8, "check": 0,
 }
 PATH_BYTES_PER_CELL_UPDATE = 1200  # 150 doubles per cell and outer iteration
 # Algorithmic FP32 bytes per cell of the correction-pass kernels of the
@@ -48,7 +48,7 @@ PATH_BYTES_PER_CELL_UPDATE = 1200  # 150 doubles per cell and outer iteration
 # velocity gradients of the residual state)
 BYTES_PER_CELL_MIXED = {
     "d_velocity_sources": 35 * 4, "d_velocity_sweep": 68 * 4, "d_pressure_sources": 44 * 4,
-    "d_pressure_sweep": 55 * 4, "d_energy_sources": 13 * 4, "d_energy_sweep": 42 * 4,
+    "d_pressure_sweep": 43 * 4, "d_energy_sources": 13 * 4, "d_energy_sweep": 42 * 4,
 }
 FACE_STATE = ("faces_u", "faces_v", "faces_w", "faces_T")
 

@@ -1,0 +1,204 @@
+// Device helpers shared by the time-step kernels and their adjoint: wave
+// reductions, the colour-phase cell enumeration, the finite differences of
+// the face fields and the NIM exponential fits (templated on the scalar, so
+// that the adjoint differentiates the same expressions with dual numbers).
+#pragma once
+#include "nim_kernels.h"
+
+namespace nim {
+
+// ---------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// cell of colour `color` with k in [k0, k1) handled by thread `tid`
+__device__ __forceinline__ bool color_cell(const Dev &d, int color, int k0, int k1, int tid, int &i, int &j, int &k)
+{
+    int nth = d.NT >> 1, nrh = (d.NR + 1) >> 1;
+    int ih = tid % nth, rest = tid / nth;
+    int jj = rest % nrh;
+    k = k0 + rest / nrh;
+    if (k >= k1) return false;
+    j = ((color + k) & 1) + 2 * jj;
+    if (j >= d.NR) return false;
+    int r = (color - j - 3 * k) & 3;
+    i = 2 * ih + (r >> 1);
+    return true;
+}
+
+// ---------------------------------------------------------------------
+// finite differences shared by sources, sweeps and wall updates
+// ---------------------------------------------------------------------
+// derivative across cells of the face value at position fpos, along
+// direction dir (0 theta, 1 r, 2 z): central (f(+1) - f(-1))/4, 2nd-order
+// one-sided at walls (nse.f90:1643-1653 and its repetitions)
+//
+// The three stencil points and their weights are chosen by selects and the
+// points are always loaded, so the loads of all derivatives of a cell issue
+// together instead of one branch (and one memory round trip) at a time.
+// Weights (1, -1, 0) give the central difference exactly (the products by
+// +-1 and 0 are exact); (-3, 4, -1) and (3, -4, 1) the one-sided forms in
+// the reference's operation order.
+template <typename T>
+__device__ __forceinline__ T three_point(T f0, T f1, T f2, bool lo, bool hi)
+{
+    T w0 = lo ? T(-3) : hi ? T(3) : T(1);
+    T w1 = lo ? T(4) : hi ? T(-4) : T(-1);
+    T w2 = lo ? T(-1) : hi ? T(1) : T(0);
+    return (w0 * f0 + w1 * f1 + w2 * f2) / T(4);
+}
+
+template <typename T>
+__device__ __forceinline__ T tder(const Dev &d, const T *F, int fpos, int i, int j, int k, int dir)
+{
+    if (dir == 0) return (F[fidx(d, fpos, ip1(d, i), j, k)] - F[fidx(d, fpos, im1(d, i), j, k)]) / T(4);
+    if (dir == 1) {
+        bool lo = j == 0, hi = j == d.NR - 1;
+        // central: (j+1, j-1); low wall: (0, 1, 2); high wall: (j, j-1, j-2)
+        int j0 = lo ? 0 : hi ? j : j + 1, j1 = lo ? 1 : j - 1, j2 = lo ? 2 : hi ? j - 2 : j1;
+        return three_point(F[fidx(d, fpos, i, j0, k)], F[fidx(d, fpos, i, j1, k)], F[fidx(d, fpos, i, j2, k)], lo, hi);
+    }
+    bool lo = k == 0, hi = k == d.NZ - 1;
+    int k0 = lo ? 0 : hi ? k : k + 1, k1 = lo ? 1 : k - 1, k2 = lo ? 2 : hi ? k - 2 : k1;
+    return three_point(F[fidx(d, fpos, i, j, k0)], F[fidx(d, fpos, i, j, k1)], F[fidx(d, fpos, i, j, k2)], lo, hi);
+}
+
+// cross-diffusion operator of calculate_R_D2 / _R_D_p2 / _R_D_T
+// (nse.f90:1628-2134); g is the per-cell geometry block [NGC][N]
+template <typename T>
+__device__ T cross_diffusion(const Dev &d, const T *g, const T *F, int i, int j, int k, int c, T coef)
+{
+    const int N = d.N;
+    T fxy = T(0.5) * (tder(d, F, 2, i, j, k, 0) - tder(d, F, 1, i, j, k, 0));
+    T fzy = T(0.5) * (tder(d, F, 6, i, j, k, 0) - tder(d, F, 5, i, j, k, 0));
+    T fyx = T(0.5) * (tder(d, F, 4, i, j, k, 1) - tder(d, F, 3, i, j, k, 1));
+    T fzx = T(0.5) * (tder(d, F, 6, i, j, k, 1) - tder(d, F, 5, i, j, k, 1));
+    T fxz = T(0.5) * (tder(d, F, 2, i, j, k, 2) - tder(d, F, 1, i, j, k, 2));
+    T fyz = T(0.5) * (tder(d, F, 4, i, j, k, 2) - tder(d, F, 3, i, j, k, 2));
+    T fx = (F[fidx(d, 2, i, j, k)] - F[fidx(d, 1, i, j, k)]) / T(2);
+    T fy = (F[fidx(d, 4, i, j, k)] - F[fidx(d, 3, i, j, k)]) / T(2);
+    T fz = (F[fidx(d, 6, i, j, k)] - F[fidx(d, 5, i, j, k)]) / T(2);
+    return coef * (g[(G_G + 0) * N + c] * fx + g[(G_G + 1) * N + c] * fy + g[(G_G + 2) * N + c] * fz
+                   - g[(G_GO + 0) * N + c] * fxy - g[(G_GO + 1) * N + c] * fyx - g[(G_GO + 2) * N + c] * fxz
+                   - g[(G_GO + 3) * N + c] * fzx - g[(G_GO + 4) * N + c] * fyz - g[(G_GO + 5) * N + c] * fzy);
+}
+
+// gradient component of a face field by the divergence theorem over the six
+// faces (dphi_dx2..dphi_dz2, nse.f90:2866-2903); An is [3][NF]
+template <typename T>
+__device__ __forceinline__ T dphi(const Dev &d, const T *An, const T phi[6], const int f[6], int comp, T invV)
+{
+    const T *A = An + (size_t)comp * d.NF;
+    T s = phi[1] * A[f[1]] - phi[0] * A[f[0]] + phi[3] * A[f[3]] - phi[2] * A[f[2]] + phi[5] * A[f[5]]
+          - phi[4] * A[f[4]];
+    return invV * s;
+}
+
+__device__ __forceinline__ void cell_faces(const Dev &d, int i, int j, int k, int f[6])
+{
+    for (int p = 0; p < 6; p++) f[p] = fidx(d, p + 1, i, j, k);
+}
+
+// ---------------------------------------------------------------------
+// forward-mode dual numbers (value, derivative)
+// ---------------------------------------------------------------------
+struct Dual {
+    double v, d;
+};
+__device__ __forceinline__ double dval(double x) { return x; }
+__device__ __forceinline__ double dval(Dual x) { return x.v; }
+__device__ __forceinline__ double dexp(double x) { return exp(x); }
+__device__ __forceinline__ Dual dexp(Dual x)
+{
+    double e = exp(x.v);
+    return {e, e * x.d};
+}
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b)
+{
+    double q = a.v / b.v;
+    return {q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+__device__ __forceinline__ Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+__device__ __forceinline__ Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+__device__ __forceinline__ Dual operator/(double a, Dual b)
+{
+    double q = a / b.v;
+    return {q, -q * b.d / b.v};
+}
+
+// ---------------------------------------------------------------------
+// NIM exponential-fit constants of one direction (nim.f90:184-287)
+// ---------------------------------------------------------------------
+// Both branches are the reference's expressions rewritten with the
+// reciprocals rk = 1/k, ru = 1/u and t = u/k (= Re/2): the Taylor branch
+// becomes series in t with exact rational coefficients (Horner form, one
+// division instead of about forty), the exponential branch uses
+// p = 1/(e^-Re - 1) and q = 1/(1 - e^Re) (five divisions instead of
+// eleven). The values agree with the reference's to rounding, amplified
+// only by the cancellation the closed forms already have near |Re| =
+// v_limit.
+// R is double, or Dual for the derivatives the adjoint needs.
+template <typename R>
+__device__ __forceinline__ void nim_direction_t(R u, R k, double gp, double gm, double vlim, R A[6])
+{
+    const R rk = 1. / k;
+    const R t = u * rk, Re = 2. * t;
+    if (fabs(dval(Re)) > vlim) {
+        const R ru = 1. / u;
+        const R e = dexp(Re), ie = 1. / e;
+        const R p = 1. / (ie - 1.), q = 1. / (1. - e);
+        A[0] = (2. * rk * p + ru) * gp;
+        A[1] = (-(2. * rk * q) - ru) * gm;
+        A[2] = (u * rk * p) * gp;
+        A[3] = (u * rk * q) * gm;
+        if (dval(Re) > 0) {
+            A[4] = (-k - u * (ie + 1.) * p) * (ru * ru);
+            A[5] = -p - 0.5 * k * ru;
+        } else {
+            A[4] = (-k - u * (1. + e) * q) * (ru * ru);
+            A[5] = -(e * q) - 0.5 * k * ru;
+        }
+    } else {
+        constexpr double c3 = 1. / 3., c6 = 1. / 6., c45 = 1. / 45., c90 = 1. / 90., c945 = 1. / 945.;
+        constexpr double c945b = 2. / 945., c4725 = 1. / 4725., c9450 = 1. / 9450., c93555 = 1. / 93555.;
+        constexpr double c93555b = 2. / 93555., c11 = 1382. / 6.38512875e8;
+        const R s2 = t * t;
+        // odd and even series shared by the pairs A0/A1 and A2/A3
+        const R Q = t * (c3 + s2 * (-c45 + s2 * (c945b + s2 * (c4725 + s2 * c93555b))));
+        const R E = s2 * (-c6 + s2 * (c90 + s2 * (c945 + s2 * (c9450 - s2 * c93555))));
+        A[0] = (rk * (-1. - Q)) * gp;
+        A[1] = (rk * (-1. + Q)) * gm;
+        A[2] = (-0.5 - 0.5 * t + E) * gp;
+        A[3] = (-0.5 + 0.5 * t + E) * gm;
+        A[4] = rk * (c3 + s2 * (-c45 + s2 * (c945b + s2 * (c4725 + s2 * (c93555b + s2 * c11)))));
+        A[5] = 0.5 + t * (c6 + s2 * (-c90 + s2 * (c945 + s2 * (c9450 + s2 * c93555))));
+    }
+}
+
+
+__device__ void nim_direction(double u, double k, double gp, double gm, double vlim, double A[6])
+{
+    nim_direction_t<double>(u, k, gp, gm, vlim, A);
+}
+
+}  // namespace nim

@@ -248,6 +248,47 @@ int32_t nim_ckpt_destroy(nim_ckpt *ckpt);
 int32_t nim_ckpt_store(nim_ckpt *ckpt, int32_t slot);
 int32_t nim_ckpt_restore(nim_ckpt *ckpt, int32_t slot);
 
+/* Discrete adjoint of the time step (no reference counterpart; the
+ * reference computes no sensitivities). A step solves the NIM equations by
+ * the fixed-point iteration x <- G(x; p) of integrate_nse (nse.f90:117-165);
+ * at its converged x*, the adjoint of the step is the fixed point
+ * lam = G_x^T lam + xbar of the reversed pass, and the step's inputs p
+ * (previous time level, momentum coefficients of the step and so the
+ * time-centred velocity, the Dirichlet wall temperatures, nu and alpha)
+ * receive G_p^T lam. All FP64, on the context's stream.
+ *
+ * A run of steps is reversed from its end: nim_adj_seed_nusselt sets the
+ * adjoint of the final state for J = w0 Nu[0] + w1 Nu[1] + w2 Nu[2]
+ * (the outputs of nim_nusselt); then, for k = last .. 0, with the state
+ * after step k's inner solve current in the context (momentum coefficients
+ * of step k built, previous level = state before step k, no advance yet),
+ * nim_adj_reverse_step iterates the reversed pass until the largest change
+ * per pass is at most tol times the largest adjoint (NIM_E_MAXITER after
+ * max_passes) and carries the adjoint to the state before step k.
+ * nim_adj_gradient returns the accumulated dJ/d(nu, Rinv, alpha, T_hot) as
+ * partial derivatives with respect to the context's constants nu, Rinv =
+ * 1/Re and alpha and the hexagon wall temperature; with nu = sqrt(Pr/Ra),
+ * Rinv = nu and alpha = nu/Pr, dJ/dRa = (g0 + g1 + g2/Pr) (-nu / (2 Ra)).
+ *
+ * For tests: nim_adj_reverse_pass (one reversed pass in place),
+ * nim_adj_reverse_stage (the reverse of nim_run_stage's stage 0..6, or 7
+ * for the momentum coefficient build), nim_adj_clear, and nim_adj_get /
+ * nim_adj_set, which move the adjoint of a context array by the context
+ * array's name in the reference layout (faces_*, *xyz, *xyz_old, uxyzt ..
+ * wxyzt, R_B_u .. R_D_T, M_F1 .. M_F4, MT_F1 .. MT_F4), or the four scalars
+ * "parameters" (this pass) and "gradient" (accumulated). */
+typedef struct nim_adj nim_adj;
+int32_t nim_adj_create(nim_ctx *ctx, nim_adj **out);
+int32_t nim_adj_destroy(nim_adj *adj);
+int32_t nim_adj_seed_nusselt(nim_adj *adj, const double weights[3]);
+int32_t nim_adj_reverse_step(nim_adj *adj, double tol, int32_t max_passes, int32_t *passes, double *delta);
+int32_t nim_adj_gradient(nim_adj *adj, double out[4]);
+int32_t nim_adj_reverse_pass(nim_adj *adj);
+int32_t nim_adj_reverse_stage(nim_adj *adj, int32_t stage);
+int32_t nim_adj_clear(nim_adj *adj);
+int32_t nim_adj_get(nim_adj *adj, const char *name, double *out, int64_t count);
+int32_t nim_adj_set(nim_adj *adj, const char *name, const double *in, int64_t count);
+
 /* Message for the last failure on the calling thread. */
 const char *nim_last_error(void);
 

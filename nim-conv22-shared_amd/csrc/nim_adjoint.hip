@@ -1,0 +1,1158 @@
+// Discrete adjoint of the NIM time step (reverse mode, FP64, gfx950).
+//
+// No counterpart in the reference, which computes no sensitivities. A time
+// step solves the NIM equations by the fixed-point iteration x <- G(x; p) of
+// the outer passes (integrate_nse, nse.f90:117-165); p holds what the step
+// takes from outside: the previous time level, the momentum coefficients of
+// the step (built from the time-centred velocity), the Dirichlet wall values
+// and the diffusivities nu and alpha. At the converged x* the adjoint of the
+// step is the fixed point lam = G_x^T lam + xbar of the transposed pass
+// (reverse accumulation of a converged iteration, Christianson 1994), and
+// the step's inputs receive G_p^T lam.
+//
+// G_x^T lam is one pass in reverse: energy sweep, energy sources, energy
+// coefficients with the adiabatic walls, pressure sweep with its wall
+// updates, pressure sources, momentum sweep, momentum sources; each sweep's
+// colour phases in reverse colour order and, within a cell, its updates in
+// reverse order. Every stage is linearised at the converged state, which a
+// pass no longer moves.
+//
+// Scatter: a forward unit reads neighbour values that other cells of its
+// colour also read, so the reverse adds into their adjoints atomically. The
+// values a unit writes are read by no other cell of its colour (the colour
+// is race-free), so their adjoints are loaded, updated and stored back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "nim_hip.h"
+#include "nim_stencil.h"
+
+using namespace nim;
+
+int32_t nim_internal_fail(int32_t code, const char *msg);
+void *nim_internal_stream(nim_ctx *c);
+const Dev *nim_internal_dev(nim_ctx *c);
+void nim_internal_lengths(nim_ctx *c, double *R_i, double *R_o);
+int32_t nim_internal_move(nim_ctx *c, double *dev, int kind, int mult, const char *name, double *host, int64_t count,
+                          bool to_host);
+
+namespace {
+
+constexpr int BSA = 256;
+// scalar parameter adjoints: partial derivatives with respect to the
+// constants nu (pressure wall, momentum fits), Rinv (momentum cross
+// diffusion), alpha (energy fits and cross diffusion) and the hexagon
+// wall temperature
+enum { P_NU = 0, P_RINV = 1, P_ALPHA = 2, P_THOT = 3, NPAR = 4 };
+
+struct Adj {
+    double *lc;    // [4][N]   cells u, v, w, T of the current iterate
+    double *lf;    // [5][NF]  faces u, v, w, p, T
+    double *lsrc;  // [NSRC][N] sources (consumed within a pass)
+    double *lTF;   // [NCF][N]  energy coefficients (consumed within a pass)
+    double *lMF;   // [NCF][N]  momentum coefficients of the step
+    double *lold;  // [4][N]   previous time level
+    double *ltc;   // [3][N]   time-centred velocity of the step
+    double *par;   // [NPAR]   scalar parameters, this pass
+};
+
+__device__ __forceinline__ void aadd(double *p, double v) { atomicAdd(p, v); }
+
+// sum over the block's waves into *p (every thread of the block calls it)
+__device__ void par_commit(double *p, double v)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0 && v != 0.) atomicAdd(p, v);
+}
+
+// adjoints of the six faces of one unit: the faces the unit writes hold the
+// adjoint of their current value and are stored back; the others collect
+// this unit's contributions and are added atomically at the end
+struct OwnFaces {
+    int f[6];
+    bool wr[6];
+    double l[6];
+    __device__ void load(const double *L)
+    {
+        for (int p = 0; p < 6; p++) l[p] = wr[p] ? L[f[p]] : 0.;
+    }
+    __device__ void add(double *L, int a, double v)
+    {
+        for (int p = 0; p < 6; p++)
+            if (a == f[p]) {
+                l[p] += v;
+                return;
+            }
+        aadd(L + a, v);
+    }
+    __device__ void store(double *L)
+    {
+        for (int p = 0; p < 6; p++) {
+            if (wr[p]) L[f[p]] = l[p];
+            else if (l[p] != 0.) aadd(L + f[p], l[p]);
+        }
+    }
+};
+
+// the points and weights of tder(F, fpos, i, j, k, dir): sum_m w[m] F[a[m]]
+__device__ void tder_pts(const Dev &d, int fpos, int i, int j, int k, int dir, int a[3], double w[3])
+{
+    if (dir == 0) {
+        a[0] = fidx(d, fpos, ip1(d, i), j, k);
+        a[1] = fidx(d, fpos, im1(d, i), j, k);
+        a[2] = a[0];
+        w[0] = 0.25; w[1] = -0.25; w[2] = 0.;
+        return;
+    }
+    const int n = dir == 1 ? d.NR : d.NZ, x = dir == 1 ? j : k;
+    const bool lo = x == 0, hi = x == n - 1;
+    const int x0 = lo ? 0 : hi ? x : x + 1, x1 = lo ? 1 : x - 1, x2 = lo ? 2 : hi ? x - 2 : x1;
+    w[0] = (lo ? -3. : hi ? 3. : 1.) / 4.;
+    w[1] = (lo ? 4. : hi ? -4. : -1.) / 4.;
+    w[2] = (lo ? -1. : hi ? 1. : 0.) / 4.;
+    if (dir == 1) {
+        a[0] = fidx(d, fpos, i, x0, k); a[1] = fidx(d, fpos, i, x1, k); a[2] = fidx(d, fpos, i, x2, k);
+    } else {
+        a[0] = fidx(d, fpos, i, j, x0); a[1] = fidx(d, fpos, i, j, x1); a[2] = fidx(d, fpos, i, j, x2);
+    }
+}
+
+// v * tder(F, fpos, dir) into the adjoint: atomically, or through the
+// unit's own faces
+__device__ void tder_add(const Dev &d, double *L, OwnFaces *o, int fpos, int i, int j, int k, int dir, double v)
+{
+    int a[3];
+    double w[3];
+    tder_pts(d, fpos, i, j, k, dir, a, w);
+    for (int m = 0; m < 3; m++) {
+        if (w[m] == 0.) continue;
+        if (o) o->add(L, a[m], v * w[m]);
+        else aadd(L + a[m], v * w[m]);
+    }
+}
+
+// reverse of out = coef * X(F), X the cross-diffusion operator
+// (cross_diffusion): lo * coef * dX/dF into L; returns X
+__device__ double cross_diffusion_vjp(const Dev &d, const double *F, double *L, int i, int j, int k, int c,
+                                      double coef, double lo)
+{
+    const int N = d.N;
+    const double *g = d.gc;
+    const double X = cross_diffusion(d, g, F, i, j, k, c, 1.0);
+    const double s = lo * coef;
+    if (s == 0.) return X;
+    const double G0 = g[(G_G + 0) * N + c], G1 = g[(G_G + 1) * N + c], G2 = g[(G_G + 2) * N + c];
+    aadd(L + fidx(d, 2, i, j, k), s * G0 / 2.);
+    aadd(L + fidx(d, 1, i, j, k), -s * G0 / 2.);
+    aadd(L + fidx(d, 4, i, j, k), s * G1 / 2.);
+    aadd(L + fidx(d, 3, i, j, k), -s * G1 / 2.);
+    aadd(L + fidx(d, 6, i, j, k), s * G2 / 2.);
+    aadd(L + fidx(d, 5, i, j, k), -s * G2 / 2.);
+    // - GO0 fxy - GO1 fyx - GO2 fxz - GO3 fzx - GO4 fyz - GO5 fzy,
+    // f_ab = (tder(F, hi face, dir) - tder(F, lo face, dir)) / 2
+    const int hiF[6] = {2, 4, 2, 6, 4, 6}, loF[6] = {1, 3, 1, 5, 3, 5}, dir[6] = {0, 1, 2, 1, 2, 0};
+    for (int m = 0; m < 6; m++) {
+        const double v = -s * g[(G_GO + m) * N + c] * 0.5;
+        tder_add(d, L, nullptr, hiF[m], i, j, k, dir[m], v);
+        tder_add(d, L, nullptr, loF[m], i, j, k, dir[m], -v);
+    }
+    return X;
+}
+
+// reverse of out = dphi(An, phi, f, comp, invV): lo * d out / d phi into L
+__device__ void dphi_vjp(const Dev &d, const int f[6], int comp, double invV, double lo, double *L)
+{
+    const double *A = d.An + (size_t)comp * d.NF;
+    const double s = lo * invV;
+    if (s == 0.) return;
+    for (int p = 0; p < 6; p++) aadd(L + f[p], (p & 1 ? s : -s) * A[f[p]]);
+}
+
+// ---------------------------------------------------------------------
+// transport unit (integrate_velocity_field_formulation2_i nse.f90:831-1006,
+// integrate_temperature_field_formulation2_i nse.f90:1009-1089)
+// ---------------------------------------------------------------------
+// Reverse of transport_unit for face field ff and cell slot cs: replaces
+// the adjoints of the unit's cell and owned faces by those of their values
+// before the unit, adds into the faces and cells it read and into the
+// previous level, adds the coefficient adjoints of the cell into lcoef and
+// returns the adjoint of the source B.
+__device__ double transport_vjp(const Dev &d, const Adj &a, int ff, int cs, const double *coef, double *lcoef,
+                                double B, double w, int i, int j, int k, int c)
+{
+    const int N = d.N;
+    const double *Fc = d.fc + (size_t)ff * d.NF;
+    const double *C = d.sc + (size_t)cs * N, *CO = d.sc + (size_t)(S_OLD + cs) * N;
+    double *LF = a.lf + (size_t)ff * d.NF, *LC = a.lc + (size_t)cs * N, *LO = a.lold + (size_t)cs * N;
+    const bool has2 = j < d.NR - 1, has6 = k < d.NZ - 1;
+    const int jn = has2 ? j + 1 : j, kn = has6 ? k + 1 : k, ip = ip1(d, i);
+    const double *gdf = d.gc + (size_t)G_DIFF * N;
+    OwnFaces o;
+    cell_faces(d, i, j, k, o.f);
+    o.wr[0] = false; o.wr[1] = has2; o.wr[2] = false; o.wr[3] = true; o.wr[4] = false; o.wr[5] = has6;
+    o.load(LF);
+    // ---- forward values
+    const double old = C[c], oc = CO[c];
+    double av[8], bv[5], ev[5], hv[5];
+    for (int m = 0; m < 8; m++) av[m] = coef[(C_F4 + m) * N + c];
+    for (int m = 0; m < 5; m++) {
+        bv[m] = coef[(C_F1 + m) * N + c];
+        ev[m] = coef[(C_F2 + m) * N + c];
+        hv[m] = coef[(C_F3 + m) * N + c];
+    }
+    const double F0 = Fc[o.f[0]], F1 = Fc[o.f[1]], F2 = Fc[o.f[2]], F3 = Fc[o.f[3]], F4 = Fc[o.f[4]], F5 = Fc[o.f[5]];
+    const double gd0 = gdf[0 * N + c], gd1 = gdf[1 * N + c], gd2 = gdf[2 * N + c];
+    const double gd3 = gdf[3 * N + c], gd4 = gdf[4 * N + c], gd5 = gdf[5 * N + c];
+    const int cr = cidx(d, i, jn, k), ct = cidx(d, ip, j, k), cz = cidx(d, i, j, kn);
+    const int ar2 = fidx(d, 2, i, jn, k), ar3 = fidx(d, 3, i, jn, k), ar4 = fidx(d, 4, i, jn, k);
+    const int ar5 = fidx(d, 5, i, jn, k), ar6 = fidx(d, 6, i, jn, k);
+    const int at1 = fidx(d, 1, ip, j, k), at2 = fidx(d, 2, ip, j, k), at4 = fidx(d, 4, ip, j, k);
+    const int at5 = fidx(d, 5, ip, j, k), at6 = fidx(d, 6, ip, j, k);
+    const int az1 = fidx(d, 1, i, j, kn), az2 = fidx(d, 2, i, j, kn), az3 = fidx(d, 3, i, j, kn);
+    const int az4 = fidx(d, 4, i, j, kn), az6 = fidx(d, 6, i, j, kn);
+    const double r2 = Fc[ar2], r3 = Fc[ar3], r4 = Fc[ar4], r5 = Fc[ar5], r6 = Fc[ar6], sr = C[cr] + CO[cr];
+    const double t1 = Fc[at1], t2 = Fc[at2], t4 = Fc[at4], t5 = Fc[at5], t6 = Fc[at6], st = C[ct] + CO[ct];
+    const double z1 = Fc[az1], z2 = Fc[az2], z3 = Fc[az3], z4 = Fc[az4], z6 = Fc[az6], sz = C[cz] + CO[cz];
+    const double S0 = av[1] * oc + av[2] * F0 + av[3] * F1 + av[4] * F2 + av[5] * F3 + av[6] * F4 + av[7] * F5 + B;
+    const double nv = (1 - w) * old + (w / av[0]) * S0, ccur = nv + oc;
+    const double S2 = bv[1] * F0 + bv[2] * r2 + bv[3] * ccur + bv[4] * sr - gd0 * (F3 - F2 + r4 - r3) / 4.
+                      - gd1 * (F5 - F4 + r6 - r5) / 4.;
+    const double F1n = has2 ? (1 - w) * F1 + (w / bv[0]) * S2 : F1;
+    const double S4 = ev[1] * F2 + ev[2] * t4 + ev[3] * ccur + ev[4] * st - gd2 * (F1n - F0 + t2 - t1) / 4.
+                      - gd3 * (F5 - F4 + t6 - t5) / 4.;
+    const double F3n = (1 - w) * F3 + (w / ev[0]) * S4;
+    const double S6 = hv[1] * F4 + hv[2] * z6 + hv[3] * ccur + hv[4] * sz - gd4 * (F1n - F0 + z2 - z1) / 4.
+                      - gd5 * (F3n - F2 + z4 - z3) / 4.;
+    // ---- reverse
+    double la[8] = {0, 0, 0, 0, 0, 0, 0, 0}, lb[5] = {0, 0, 0, 0, 0}, le[5] = {0, 0, 0, 0, 0}, lh[5] = {0, 0, 0, 0, 0};
+    double lnv = LC[c], lccur = 0., lsr = 0., lst = 0., lsz = 0.;
+    if (has6) {
+        const double l = o.l[5], g = (w / hv[0]) * l;
+        o.l[5] = (1 - w) * l;
+        o.add(LF, o.f[4], g * hv[1]);
+        aadd(LF + az6, g * hv[2]);
+        lccur += g * hv[3];
+        lsz += g * hv[4];
+        const double q4 = g * gd4 / 4., q5 = g * gd5 / 4.;
+        o.add(LF, o.f[1], -q4); o.add(LF, o.f[0], q4); aadd(LF + az2, -q4); aadd(LF + az1, q4);
+        o.add(LF, o.f[3], -q5); o.add(LF, o.f[2], q5); aadd(LF + az4, -q5); aadd(LF + az3, q5);
+        lh[0] += -(g / hv[0]) * S6; lh[1] += g * F4; lh[2] += g * z6; lh[3] += g * ccur; lh[4] += g * sz;
+    }
+    {
+        const double l = o.l[3], g = (w / ev[0]) * l;
+        o.l[3] = (1 - w) * l;
+        o.add(LF, o.f[2], g * ev[1]);
+        aadd(LF + at4, g * ev[2]);
+        lccur += g * ev[3];
+        lst += g * ev[4];
+        const double q2 = g * gd2 / 4., q3 = g * gd3 / 4.;
+        o.add(LF, o.f[1], -q2); o.add(LF, o.f[0], q2); aadd(LF + at2, -q2); aadd(LF + at1, q2);
+        o.add(LF, o.f[5], -q3); o.add(LF, o.f[4], q3); aadd(LF + at6, -q3); aadd(LF + at5, q3);
+        le[0] += -(g / ev[0]) * S4; le[1] += g * F2; le[2] += g * t4; le[3] += g * ccur; le[4] += g * st;
+    }
+    if (has2) {
+        const double l = o.l[1], g = (w / bv[0]) * l;
+        o.l[1] = (1 - w) * l;
+        o.add(LF, o.f[0], g * bv[1]);
+        aadd(LF + ar2, g * bv[2]);
+        lccur += g * bv[3];
+        lsr += g * bv[4];
+        const double q0 = g * gd0 / 4., q1 = g * gd1 / 4.;
+        o.add(LF, o.f[3], -q0); o.add(LF, o.f[2], q0); aadd(LF + ar4, -q0); aadd(LF + ar3, q0);
+        o.add(LF, o.f[5], -q1); o.add(LF, o.f[4], q1); aadd(LF + ar6, -q1); aadd(LF + ar5, q1);
+        lb[0] += -(g / bv[0]) * S2; lb[1] += g * F0; lb[2] += g * r2; lb[3] += g * ccur; lb[4] += g * sr;
+    }
+    // ccur = nv + oc, partners' s = C + CO
+    lnv += lccur;
+    double loc = lccur;
+    if (lsr != 0.) { aadd(LC + cr, lsr); aadd(LO + cr, lsr); }
+    if (lst != 0.) { aadd(LC + ct, lst); aadd(LO + ct, lst); }
+    if (lsz != 0.) { aadd(LC + cz, lsz); aadd(LO + cz, lsz); }
+    // cell average
+    const double g = (w / av[0]) * lnv;
+    LC[c] = (1 - w) * lnv;
+    loc += g * av[1];
+    aadd(LO + c, loc);
+    o.add(LF, o.f[0], g * av[2]); o.add(LF, o.f[1], g * av[3]); o.add(LF, o.f[2], g * av[4]);
+    o.add(LF, o.f[3], g * av[5]); o.add(LF, o.f[4], g * av[6]); o.add(LF, o.f[5], g * av[7]);
+    la[0] += -(g / av[0]) * S0; la[1] += g * oc; la[2] += g * F0; la[3] += g * F1; la[4] += g * F2;
+    la[5] += g * F3; la[6] += g * F4; la[7] += g * F5;
+    o.store(LF);
+    // the FP64 sweep re-sums its diagonals from the other coefficients
+    // (F4(1) = F4(2) + .. + F4(8), F(1) = F(2) + F(3) + 2 F(4) + 2 F(5)), so
+    // their adjoints go to those
+    for (int m = 1; m < 8; m++) la[m] += la[0];
+    la[0] = 0.;
+    lb[1] += lb[0]; lb[2] += lb[0]; lb[3] += 2. * lb[0]; lb[4] += 2. * lb[0]; lb[0] = 0.;
+    le[1] += le[0]; le[2] += le[0]; le[3] += 2. * le[0]; le[4] += 2. * le[0]; le[0] = 0.;
+    lh[1] += lh[0]; lh[2] += lh[0]; lh[3] += 2. * lh[0]; lh[4] += 2. * lh[0]; lh[0] = 0.;
+    for (int m = 0; m < 8; m++) lcoef[(C_F4 + m) * N + c] += la[m];
+    for (int m = 0; m < 5; m++) {
+        lcoef[(C_F1 + m) * N + c] += lb[m];
+        lcoef[(C_F2 + m) * N + c] += le[m];
+        lcoef[(C_F3 + m) * N + c] += lh[m];
+    }
+    return g;
+}
+
+__device__ __forceinline__ bool phase_cell(const Dev &d, int color, int &i, int &j, int &k)
+{
+    return color_cell(d, color, 0, d.NZ, blockIdx.x * blockDim.x + threadIdx.x, i, j, k);
+}
+
+__device__ __forceinline__ void cell_ijk(const Dev &d, int c, int &i, int &j, int &k)
+{
+    i = iof(d, c % d.NT);
+    j = (c / d.NT) % d.NR;
+    k = c / (d.NT * d.NR);
+}
+
+// momentum sweep, one colour: the three fields of a cell in one thread, so
+// that their coefficient adjoints (one set of M_F per cell) add without races
+__global__ void __launch_bounds__(BSA) k_adj_mom_sweep(Dev d, Adj a, int color)
+{
+    int i, j, k;
+    if (!phase_cell(d, color, i, j, k)) return;
+    const int N = d.N, c = cidx(d, i, j, k);
+    const double w[3] = {d.w_u, d.w_v, d.w_w};
+    for (int q = 0; q < 3; q++) {
+        double lB = transport_vjp(d, a, q, q, d.MF, a.lMF, d.src[(R_U + q) * N + c], w[q], i, j, k, c);
+        a.lsrc[(R_U + q) * N + c] += lB;
+    }
+}
+
+__global__ void __launch_bounds__(BSA) k_adj_T_sweep(Dev d, Adj a, int color)
+{
+    int i, j, k;
+    if (!phase_cell(d, color, i, j, k)) return;
+    const int N = d.N, c = cidx(d, i, j, k);
+    double lB = transport_vjp(d, a, F_T, 3, d.TF, a.lTF, d.src[R_T * N + c], d.w_T, i, j, k, c);
+    a.lsrc[R_T * N + c] += lB;
+}
+
+// ---------------------------------------------------------------------
+// pressure sweep (apply_pressure_bc bc.f90:147-341 for the cell's wall
+// faces, then integrate_pressure_field_i nse.f90:1091-1289)
+// ---------------------------------------------------------------------
+// reverse of pressure_wall for the wall face at position pos; returns the
+// contribution to the adjoint of nu
+__device__ double pressure_wall_vjp(const Dev &d, const Adj &a, OwnFaces &o, double &lS1, int b, int pos, int i,
+                                    int j, int k)
+{
+    const int NB = d.NB;
+    const double *W = d.wall;
+    double *LP = a.lf + (size_t)F_P * d.NF;
+    const double n1 = W[(W_N + 0) * NB + b], n2 = W[(W_N + 1) * NB + b], n3 = W[(W_N + 2) * NB + b];
+    const int fw = fidx(d, pos, i, j, k);
+    const double ap = W[(W_AP + pos - 1) * NB + b], w2 = d.w_p2;
+    const double l = o.l[pos - 1], g = (w2 / ap) * l;
+    o.l[pos - 1] = (1 - w2) * l;
+    // s = - sum over the cell's other faces of AP(q) P(q)
+    for (int q = 1; q <= 6; q++)
+        if (q != pos) o.add(LP, fidx(d, q, i, j, k), -g * W[(W_AP + q - 1) * NB + b]);
+    // B1 = AP(8) R
+    lS1 += g * W[(W_AP + 7) * NB + b];
+    // - B2 - B3, tangential derivatives of p along the wall
+    const double gB2 = W[W_GB2 * NB + b], gB2c = W[W_GB2C * NB + b], gB3 = W[W_GB3 * NB + b];
+    if (pos <= 2) {
+        tder_add(d, LP, &o, pos, i, j, k, 0, -g * gB2c);
+        tder_add(d, LP, &o, pos, i, j, k, 2, -g * gB3);
+    } else {
+        tder_add(d, LP, &o, pos, i, j, k, 1, -g * (j == 0 || j == d.NR - 1 ? gB2 : gB2c));
+        tder_add(d, LP, &o, pos, i, j, k, 0, -g * gB3);
+    }
+    // nu rho d2, d2 = - sum_m A(m) u_n(m) over the 5-point normal stencil
+    const double *U = d.fc, *V = d.fc + d.NF, *Wv = d.fc + 2 * (size_t)d.NF;
+    double *LU = a.lf, *LV = a.lf + d.NF, *LW = a.lf + 2 * (size_t)d.NF;
+    double d2 = 0.;
+    for (int m = 0; m < 5; m++) {
+        const int q = d.wpts[m * NB + b];
+        const double am = W[(W_A + m) * NB + b];
+        d2 -= am * (U[q] * n1 + V[q] * n2 + Wv[q] * n3);
+        const double s = -g * d.nu * d.rho * am;
+        if (s != 0.) {
+            aadd(LU + q, s * n1);
+            aadd(LV + q, s * n2);
+            aadd(LW + q, s * n3);
+        }
+    }
+    // - b_n rho, b_n = - g_beta T_w n3
+    if (n3 != 0.) aadd(a.lf + (size_t)F_T * d.NF + fw, g * d.g_beta * d.rho * n3);
+    return g * d.rho * d2;
+}
+
+__global__ void __launch_bounds__(BSA) k_adj_ppe_sweep(Dev d, Adj a, int color)
+{
+    int i, j, k;
+    double pnu = 0.;
+    if (phase_cell(d, color, i, j, k)) {
+        const int N = d.N, NT = d.NT, NR = d.NR, NZ = d.NZ;
+        const int c = cidx(d, i, j, k);
+        double *LP = a.lf + (size_t)F_P * d.NF, *LS = a.lsrc + (size_t)R_P * N;
+        const double *gdf = d.gc + (size_t)G_DIFF * N;
+        const double w = d.w_p;
+        const bool hx = j < NR - 1, hz = k < NZ - 1;
+        const int jn = hx ? j + 1 : j, kn = hz ? k + 1 : k, ip = ip1(d, i);
+        OwnFaces o;
+        cell_faces(d, i, j, k, o.f);
+        o.wr[0] = j == 0; o.wr[1] = true; o.wr[2] = false; o.wr[3] = true; o.wr[4] = k == 0; o.wr[5] = true;
+        o.load(LP);
+        const double *Q = d.AQ;
+        double qx[NAQ], qy[NAQ], qz[NAQ];
+        for (int m = 0; m < NAQ; m++) {
+            qx[m] = Q[m * N + c];
+            qy[m] = Q[(NAQ + m) * N + c];
+            qz[m] = Q[(2 * NAQ + m) * N + c];
+        }
+        double lS1 = 0.;
+        // z face (face 6)
+        if (hz) {
+            const double l = o.l[5], g = (w / qz[Q_DIAG]) * l;
+            o.l[5] = (1 - w) * l;
+            o.add(LP, o.f[0], -g * qz[Q_TA0]); o.add(LP, o.f[1], -g * qz[Q_TA0]);
+            o.add(LP, fidx(d, 1, i, j, kn), -g * qz[Q_TB0]); o.add(LP, fidx(d, 2, i, j, kn), -g * qz[Q_TB0]);
+            o.add(LP, o.f[2], -g * qz[Q_TA1]); o.add(LP, o.f[3], -g * qz[Q_TA1]);
+            o.add(LP, fidx(d, 3, i, j, kn), -g * qz[Q_TB1]); o.add(LP, fidx(d, 4, i, j, kn), -g * qz[Q_TB1]);
+            o.add(LP, o.f[4], -g * qz[Q_SELF]); o.add(LP, fidx(d, 6, i, j, kn), -g * qz[Q_FAR]);
+            lS1 += g * qz[Q_S1];
+            aadd(LS + cidx(d, i, j, kn), g * qz[Q_S2]);
+            tder_add(d, LP, &o, 6, i, j, k, 1, g * gdf[4 * N + c]);
+            tder_add(d, LP, &o, 6, i, j, k, 0, g * gdf[5 * N + c]);
+        }
+        // y face (face 4)
+        {
+            const double l = o.l[3], g = (w / qy[Q_DIAG]) * l;
+            o.l[3] = (1 - w) * l;
+            o.add(LP, o.f[0], -g * qy[Q_TA0]); o.add(LP, o.f[1], -g * qy[Q_TA0]);
+            o.add(LP, fidx(d, 1, ip, j, k), -g * qy[Q_TB0]); o.add(LP, fidx(d, 2, ip, j, k), -g * qy[Q_TB0]);
+            o.add(LP, o.f[2], -g * qy[Q_SELF]); o.add(LP, fidx(d, 4, ip, j, k), -g * qy[Q_FAR]);
+            o.add(LP, o.f[4], -g * qy[Q_TA1]); o.add(LP, o.f[5], -g * qy[Q_TA1]);
+            o.add(LP, fidx(d, 5, ip, j, k), -g * qy[Q_TB1]); o.add(LP, fidx(d, 6, ip, j, k), -g * qy[Q_TB1]);
+            lS1 += g * qy[Q_S1];
+            aadd(LS + cidx(d, ip, j, k), g * qy[Q_S2]);
+            tder_add(d, LP, &o, 4, i, j, k, 1, g * gdf[2 * N + c]);
+            tder_add(d, LP, &o, 4, i, j, k, 2, g * gdf[3 * N + c]);
+        }
+        // x face (face 2)
+        if (hx) {
+            const double l = o.l[1], g = (w / qx[Q_DIAG]) * l;
+            o.l[1] = (1 - w) * l;
+            o.add(LP, o.f[0], -g * qx[Q_SELF]); o.add(LP, fidx(d, 2, i, jn, k), -g * qx[Q_FAR]);
+            o.add(LP, o.f[2], -g * qx[Q_TA0]); o.add(LP, o.f[3], -g * qx[Q_TA0]);
+            o.add(LP, fidx(d, 3, i, jn, k), -g * qx[Q_TB0]); o.add(LP, fidx(d, 4, i, jn, k), -g * qx[Q_TB0]);
+            o.add(LP, o.f[4], -g * qx[Q_TA1]); o.add(LP, o.f[5], -g * qx[Q_TA1]);
+            o.add(LP, fidx(d, 5, i, jn, k), -g * qx[Q_TB1]); o.add(LP, fidx(d, 6, i, jn, k), -g * qx[Q_TB1]);
+            lS1 += g * qx[Q_S1];
+            aadd(LS + cidx(d, i, jn, k), g * qx[Q_S2]);
+            tder_add(d, LP, &o, 2, i, j, k, 0, g * gdf[0 * N + c]);
+            tder_add(d, LP, &o, 2, i, j, k, 2, g * gdf[1 * N + c]);
+        }
+        // walls, reverse of bottom, top, hexagon, cylinder
+        if (j == NR - 1) pnu += pressure_wall_vjp(d, a, o, lS1, 2 * NT * NR + NT * NZ + i + NT * k, 2, i, j, k);
+        if (j == 0) pnu += pressure_wall_vjp(d, a, o, lS1, 2 * NT * NR + i + NT * k, 1, i, j, k);
+        if (k == NZ - 1) pnu += pressure_wall_vjp(d, a, o, lS1, NT * NR + i + NT * j, 6, i, j, k);
+        if (k == 0) pnu += pressure_wall_vjp(d, a, o, lS1, i + NT * j, 5, i, j, k);
+        o.store(LP);
+        aadd(LS + c, lS1);
+    }
+    par_commit(a.par + P_NU, pnu);
+}
+
+// ---------------------------------------------------------------------
+// sources
+// ---------------------------------------------------------------------
+// update_velocity_sources (mom_src): R = R_D + b + dphi(p) / rho
+__global__ void __launch_bounds__(BSA) k_adj_mom_src(Dev d, Adj a)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double prinv = 0.;
+    if (c < d.N) {
+        const int N = d.N;
+        int i, j, k;
+        cell_ijk(d, c, i, j, k);
+        int f[6];
+        cell_faces(d, i, j, k, f);
+        const double invV = d.gc[G_INVV * N + c];
+        for (int q = 0; q < 3; q++) {
+            const double l = a.lsrc[(R_U + q) * N + c];
+            a.lsrc[(R_U + q) * N + c] = 0.;
+            if (l == 0.) continue;
+            prinv += l * cross_diffusion_vjp(d, d.fc + (size_t)q * d.NF, a.lf + (size_t)q * d.NF, i, j, k, c,
+                                             d.Rinv, l);
+            dphi_vjp(d, f, q, invV, l * d.rinv, a.lf + (size_t)F_P * d.NF);
+            if (q == 2) {
+                const double s = -(d.g_beta * 0.5) * l;
+                a.lc[3 * N + c] += s;
+                a.lold[3 * N + c] += s;
+            }
+        }
+    }
+    par_commit(a.par + P_RINV, prinv);
+}
+
+// update_pressure_sources (ppe_src): R_D_p + R_b_p
+__global__ void __launch_bounds__(BSA) k_adj_ppe_src(Dev d, Adj a)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.N) return;
+    const int N = d.N;
+    const double l = a.lsrc[R_P * N + c];
+    a.lsrc[R_P * N + c] = 0.;
+    if (l == 0.) return;
+    int i, j, k;
+    cell_ijk(d, c, i, j, k);
+    int f[6];
+    cell_faces(d, i, j, k, f);
+    cross_diffusion_vjp(d, d.fc + (size_t)F_P * d.NF, a.lf + (size_t)F_P * d.NF, i, j, k, c, d.rinv, l);
+    const double *U = d.fc, *V = d.fc + d.NF, *W = d.fc + 2 * (size_t)d.NF;
+    double uf[6], vf[6], wf[6];
+    for (int p = 0; p < 6; p++) {
+        uf[p] = U[f[p]];
+        vf[p] = V[f[p]];
+        wf[p] = W[f[p]];
+    }
+    const double invV = d.gc[G_INVV * N + c];
+    const double ux = dphi(d, d.An, uf, f, 0, invV), uy = dphi(d, d.An, uf, f, 1, invV), uz = dphi(d, d.An, uf, f, 2, invV);
+    const double vx = dphi(d, d.An, vf, f, 0, invV), vy = dphi(d, d.An, vf, f, 1, invV), vz = dphi(d, d.An, vf, f, 2, invV);
+    const double wx = dphi(d, d.An, wf, f, 0, invV), wy = dphi(d, d.An, wf, f, 1, invV), wz = dphi(d, d.An, wf, f, 2, invV);
+    // R_b_p = -(ux^2 + vy^2 + wz^2) - 2 (uy vx + vz wy + wx uz) - bzz + (ux + vy + wz) / (2 DT)
+    const double hD = 1. / (2. * d.DT);
+    double *LU = a.lf, *LV = a.lf + d.NF, *LW = a.lf + 2 * (size_t)d.NF, *LT = a.lf + 4 * (size_t)d.NF;
+    dphi_vjp(d, f, 0, invV, l * (-2. * ux + hD), LU);
+    dphi_vjp(d, f, 1, invV, l * (-2. * vx), LU);
+    dphi_vjp(d, f, 2, invV, l * (-2. * wx), LU);
+    dphi_vjp(d, f, 0, invV, l * (-2. * uy), LV);
+    dphi_vjp(d, f, 1, invV, l * (-2. * vy + hD), LV);
+    dphi_vjp(d, f, 2, invV, l * (-2. * wy), LV);
+    dphi_vjp(d, f, 0, invV, l * (-2. * uz), LW);
+    dphi_vjp(d, f, 1, invV, l * (-2. * vz), LW);
+    dphi_vjp(d, f, 2, invV, l * (-2. * wz + hD), LW);
+    // bzz = dphi(-g_beta T faces, z)
+    dphi_vjp(d, f, 2, invV, -l * -d.g_beta, LT);
+}
+
+// update_temperature_sources (T_src): alpha R_D(T)
+__global__ void __launch_bounds__(BSA) k_adj_T_src(Dev d, Adj a)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double palpha = 0.;
+    if (c < d.N) {
+        const int N = d.N;
+        const double l = a.lsrc[R_T * N + c];
+        a.lsrc[R_T * N + c] = 0.;
+        if (l != 0.) {
+            int i, j, k;
+            cell_ijk(d, c, i, j, k);
+            palpha = l * cross_diffusion_vjp(d, d.fc + (size_t)F_T * d.NF, a.lf + (size_t)F_T * d.NF, i, j, k, c,
+                                             d.alpha, l);
+        }
+    }
+    par_commit(a.par + P_ALPHA, palpha);
+}
+
+// ---------------------------------------------------------------------
+// coefficients (nim.f90:160-330 momentum, 334-503 energy) and the adiabatic
+// walls (apply_zero_flux, bc.f90:345-390)
+// ---------------------------------------------------------------------
+// value and partial derivatives with respect to the contravariant velocity
+// and to k of the six exponential-fit constants of one direction
+__device__ void direction_d(const Dev &d, int c, int dir, double ux, double uy, double uz, double kd, double A[6],
+                            double dAu[6], double dAk[6], double &vel, double &kk)
+{
+    const int N = d.N;
+    const double *g = d.gc;
+    vel = ux * g[(G_XI + 3 * dir + 0) * N + c] + uy * g[(G_XI + 3 * dir + 1) * N + c]
+          + uz * g[(G_XI + 3 * dir + 2) * N + c];
+    kk = g[(G_GD + dir) * N + c] * kd;
+    const double gp = g[(G_GF + 2 * dir + 1) * N + c], gm = g[(G_GF + 2 * dir) * N + c];
+    Dual R[6];
+    nim_direction_t<Dual>(Dual{vel, 1.}, Dual{kk, 0.}, gp, gm, d.v_limit, R);
+    for (int m = 0; m < 6; m++) {
+        A[m] = R[m].v;
+        dAu[m] = R[m].d;
+    }
+    nim_direction_t<Dual>(Dual{vel, 0.}, Dual{kk, 1.}, gp, gm, d.v_limit, R);
+    for (int m = 0; m < 6; m++) dAk[m] = R[m].d;
+}
+
+// the velocity of the fits at cell c: time-centred (momentum) or the mean of
+// the current iterate and the previous level (energy)
+__device__ void fit_velocity(const Dev &d, int c, int energy, double u[3])
+{
+    const int N = d.N;
+    for (int q = 0; q < 3; q++)
+        u[q] = energy ? 0.5 * (d.sc[(S_CUR + q) * N + c] + d.sc[(S_OLD + q) * N + c]) : d.sc[(S_TC + q) * N + c];
+}
+
+// adjoint lA of the constants of one direction into the velocity adjoint of
+// cell c (atomic) and the diffusivity; returns the diffusivity part
+__device__ double direction_vjp(const Dev &d, const Adj &a, int c, int dir, int energy, const double dAu[6],
+                                const double dAk[6], const double lA[6])
+{
+    const int N = d.N;
+    double lvel = 0., lk = 0.;
+    for (int m = 0; m < 6; m++) {
+        lvel += lA[m] * dAu[m];
+        lk += lA[m] * dAk[m];
+    }
+    const double *g = d.gc;
+    for (int q = 0; q < 3; q++) {
+        const double lu = lvel * g[(G_XI + 3 * dir + q) * N + c];
+        if (lu == 0.) continue;
+        if (energy) {
+            aadd(a.lc + q * N + c, 0.5 * lu);
+            aadd(a.lold + q * N + c, 0.5 * lu);
+        } else {
+            aadd(a.ltc + q * N + c, lu);
+        }
+    }
+    return lk * g[(G_GD + dir) * N + c];
+}
+
+// reverse of k_coef_A + k_coef_F (energy = 0, adjoints from lMF into the
+// time-centred velocity) or of T_coef (energy = 1, from lTF into the
+// current and previous velocity, then the adiabatic walls); the diffusivity
+// part goes to par[pk]
+__global__ void __launch_bounds__(BSA) k_adj_coef(Dev d, Adj a, int energy, double *par, int pk)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    double pkd = 0.;
+    if (c < d.N) {
+        const int N = d.N;
+        int i, j, k;
+        cell_ijk(d, c, i, j, k);
+        double *LF = energy ? a.lTF : a.lMF;
+        // the forward writes F4 and, where the + neighbour exists, F1..F3:
+        // their adjoints are consumed; the slots it leaves alone keep theirs
+        const bool nb[3] = {j < d.NR - 1, true, k < d.NZ - 1};
+        double lF[NCF];
+        bool any = false;
+        for (int m = 0; m < NCF; m++) {
+            const bool written = m < C_F1 || nb[(m - C_F1) / 5];
+            lF[m] = written ? LF[m * N + c] : 0.;
+            if (written) LF[m * N + c] = 0.;
+            any = any || lF[m] != 0.;
+        }
+        const double kd = energy ? d.alpha : d.nu;
+        if (any) {
+            double u[3];
+            fit_velocity(d, c, energy, u);
+            double A[3][6], dAu[3][6], dAk[3][6], lA[3][6];
+            for (int dir = 0; dir < 3; dir++) {
+                double vel, kk;
+                direction_d(d, c, dir, u[0], u[1], u[2], kd, A[dir], dAu[dir], dAk[dir], vel, kk);
+                for (int m = 0; m < 6; m++) lA[dir][m] = 0.;
+            }
+            // F4: f2 = (1/A14 + 1/A24 + 1/A34 - 1/DT) / 2, f(3+2d) = -A(d)5 / A(d)4,
+            // f(4+2d) = (A(d)5 - 1) / A(d)4, F4 = [f2 + ... + f8, f2, ..., f8]
+            const double lf2 = lF[C_F4 + 0] + lF[C_F4 + 1];
+            for (int dir = 0; dir < 3; dir++) {
+                const double A4 = A[dir][4], A5 = A[dir][5];
+                const double lfa = lF[C_F4 + 0] + lF[C_F4 + 2 + 2 * dir], lfb = lF[C_F4 + 0] + lF[C_F4 + 3 + 2 * dir];
+                lA[dir][4] += lf2 * 0.5 * (-1. / (A4 * A4)) + lfa * (A5 / (A4 * A4)) - lfb * ((A5 - 1.) / (A4 * A4));
+                lA[dir][5] += -lfa / A4 + lfb / A4;
+            }
+            // F1..F3 with the + neighbour in each direction: b = A2 - A0 A5 / A4,
+            // e = B3 - B1 / B4 + B1 B5 / B4, h = A0 / (2 A4), q = B1 / (2 B4),
+            // F = [b + e + 2 h + 2 q, b, e, h, q]
+            const int base[3] = {C_F1, C_F2, C_F3};
+            for (int dir = 0; dir < 3; dir++) {
+                int c2;
+                if (dir == 0) { if (j == d.NR - 1) continue; c2 = c + d.NT; }
+                else if (dir == 1) c2 = cidx(d, ip1(d, i), j, k);
+                else { if (k == d.NZ - 1) continue; c2 = c + d.NT * d.NR; }
+                const int s = base[dir];
+                const double lb = lF[s] + lF[s + 1], le = lF[s] + lF[s + 2];
+                const double lh = 2. * lF[s] + lF[s + 3], lq = 2. * lF[s] + lF[s + 4];
+                if (lb == 0. && le == 0. && lh == 0. && lq == 0.) continue;
+                const double A0 = A[dir][0], A4 = A[dir][4], A5 = A[dir][5];
+                lA[dir][2] += lb;
+                lA[dir][0] += -lb * A5 / A4 + lh / (2. * A4);
+                lA[dir][5] += -lb * A0 / A4;
+                lA[dir][4] += lb * A0 * A5 / (A4 * A4) - lh * A0 / (2. * A4 * A4);
+                double u2[3];
+                fit_velocity(d, c2, energy, u2);
+                double B[6], dBu[6], dBk[6], lB[6] = {0, 0, 0, 0, 0, 0}, vel, kk;
+                direction_d(d, c2, dir, u2[0], u2[1], u2[2], kd, B, dBu, dBk, vel, kk);
+                lB[3] += le;
+                lB[1] += le * (-1. / B[4] + B[5] / B[4]) + lq / (2. * B[4]);
+                lB[4] += le * (B[1] / (B[4] * B[4]) - B[1] * B[5] / (B[4] * B[4])) - lq * B[1] / (2. * B[4] * B[4]);
+                lB[5] += le * B[1] / B[4];
+                pkd += direction_vjp(d, a, c2, dir, energy, dBu, dBk, lB);
+            }
+            for (int dir = 0; dir < 3; dir++) pkd += direction_vjp(d, a, c, dir, energy, dAu[dir], dAk[dir], lA[dir]);
+        }
+        // adiabatic top and bottom walls: T_w = (Z0 T_a - Z1 T_b + Z2 T_c) / Z3
+        if (energy && (k == 0 || k == d.NZ - 1)) {
+            double *LT = a.lf + (size_t)F_T * d.NF;
+            const double *W = d.wall;
+            const int NB = d.NB;
+            int b, fw, fa, fb, fcc;
+            if (k == 0) {
+                b = i + d.NT * j;
+                fw = fidx(d, 5, i, j, 0); fa = fidx(d, 6, i, j, 0); fb = fidx(d, 6, i, j, 1); fcc = fidx(d, 6, i, j, 2);
+            } else {
+                b = d.NT * d.NR + i + d.NT * j;
+                fw = fidx(d, 6, i, j, k); fa = fidx(d, 5, i, j, k); fb = fidx(d, 5, i, j, k - 1); fcc = fidx(d, 5, i, j, k - 2);
+            }
+            const double l = LT[fw] / W[(W_ZF + 3) * NB + b];
+            LT[fw] = 0.;
+            aadd(LT + fa, l * W[(W_ZF + 0) * NB + b]);
+            aadd(LT + fb, -l * W[(W_ZF + 1) * NB + b]);
+            aadd(LT + fcc, l * W[(W_ZF + 2) * NB + b]);
+        }
+    }
+    par_commit(par + pk, pkd);
+}
+
+// ---------------------------------------------------------------------
+// pass and step bookkeeping
+// ---------------------------------------------------------------------
+// Dirichlet slot of a face field: u, v, w on every wall, T on the hexagon
+// and cylinder (the adiabatic top and bottom are unknowns), p on none
+__device__ __forceinline__ bool dirichlet(const Dev &d, int field, int s)
+{
+    if (field == F_P) return false;
+    const int NT = d.NT, NR = d.NR, NZ = d.NZ;
+    if (s < d.oT) {
+        const int j = (s / NT) % (NR + 1);
+        return j == 0 || j == NR;
+    }
+    if (s < d.oZ) return false;
+    if (field == F_T) return false;
+    const int k = (s - d.oZ) / (NT * NR);
+    return k == 0 || k == NZ;
+}
+
+// start of a reverse pass: clear the intermediates and the parameter
+// adjoints of the pass, and the adjoints of the Dirichlet face values
+__global__ void __launch_bounds__(BSA) k_adj_begin(Dev d, Adj a)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    const long N = d.N, NF = d.NF;
+    for (long s = t; s < NSRC * N; s += stride) a.lsrc[s] = 0.;
+    for (long s = t; s < NCF * N; s += stride) { a.lTF[s] = 0.; a.lMF[s] = 0.; }
+    for (long s = t; s < 4 * N; s += stride) a.lold[s] = 0.;
+    for (long s = t; s < 5 * NF; s += stride)
+        if (dirichlet(d, (int)(s / NF), (int)(s % NF))) a.lf[s] = 0.;
+    if (t < NPAR) a.par[t] = 0.;
+}
+
+// lam += seed over cells and faces; delta and magnitude maxima of the
+// change against prev (bit patterns of non-negative doubles, atomic max),
+// then prev = lam
+__global__ void __launch_bounds__(BSA) k_adj_add_seed(double *lam, const double *seed, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    for (long s = t; s < n; s += stride) lam[s] += seed[s];
+}
+
+__global__ void __launch_bounds__(BSA) k_adj_delta(double *lam, double *prev, long n, unsigned long long *out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    double dm = 0., lm = 0.;
+    for (long s = t; s < n; s += stride) {
+        const double x = lam[s];
+        dm = fmax(dm, fabs(x - prev[s]));
+        lm = fmax(lm, fabs(x));
+        prev[s] = x;
+    }
+    dm = wave_max(dm);
+    lm = wave_max(lm);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out, (unsigned long long)__double_as_longlong(dm));
+        atomicMax(out + 1, (unsigned long long)__double_as_longlong(lm));
+    }
+}
+
+// the seed of a reverse step from the adjoint of the state after it:
+// cells cbar plus half the time-centred adjoint (TC' = (x + x') / 2), faces
+// fbar; lam = prev = seed
+__global__ void __launch_bounds__(BSA) k_adj_seed(Dev d, double *seed, double *lam, double *prev, const double *cbar,
+                                                  const double *tcbar, const double *fbar)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    const long N = d.N, n = 4 * N + 5 * (long)d.NF;
+    for (long s = t; s < n; s += stride) {
+        double v;
+        if (s < 4 * N) v = cbar[s] + (s < 3 * N ? 0.5 * tcbar[s] : 0.);
+        else v = fbar[s - 4 * N];
+        seed[s] = v;
+        lam[s] = v;
+        prev[s] = v;
+    }
+}
+
+// after a converged reverse step: the hexagon wall temperatures and the
+// scalar parameters of the last pass into theta, then the adjoint of the
+// state before the step: cells = previous level + half the time-centred
+// adjoint of the state after the step, time-centred velocity from the
+// momentum coefficients (ltc), faces 0 (they only start the next solve)
+__global__ void __launch_bounds__(BSA) k_adj_hex(Dev d, Adj a, double *theta)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.;
+    if (t < d.NT * d.NZ) {
+        const int i = t % d.NT, k = t / d.NT;
+        s = a.lf[(size_t)F_T * d.NF + fidx(d, 1, i, 0, k)];
+    }
+    par_commit(theta + P_THOT, s);
+}
+
+__global__ void __launch_bounds__(BSA) k_adj_carry(Dev d, Adj a, double *cbar, double *tcbar, double *fbar)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    const long N = d.N, NF = d.NF;
+    for (long s = t; s < 4 * N; s += stride) {
+        cbar[s] = a.lold[s] + (s < 3 * N ? 0.5 * tcbar[s] : 0.);
+    }
+    for (long s = t; s < 3 * N; s += stride) tcbar[s] = a.ltc[s];
+    for (long s = t; s < 5 * NF; s += stride) fbar[s] = 0.;
+}
+
+__global__ void k_adj_par_add(double *theta, const double *par)
+{
+    if (threadIdx.x < P_THOT) theta[threadIdx.x] += par[threadIdx.x];
+}
+
+// adjoint of find_Nu2 (nse.f90:2699-2798; k_nusselt): J = wc Nu_cyl + wh Nu_hex
+// into the face adjoint fbar, and its direct part through the hexagon wall
+// temperature into theta
+__global__ void __launch_bounds__(BSA) k_adj_nusselt(Dev d, double *fbar, double *theta, double R_i, double R_o,
+                                                     double wc, double wh)
+{
+    const int NT = d.NT, NR = d.NR, NZ = d.NZ, NB = d.NB, n = NT * NZ;
+    const double *W = d.wall;
+    double a1 = 0., a2 = 0.;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        a2 += W[(W_NU + 4) * NB + 2 * NT * NR + q];
+        a1 += W[(W_NU + 4) * NB + 2 * NT * NR + NT * NZ + q];
+    }
+    __shared__ double sh[BSA / 64][2];
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sh[wv][0] = a1; sh[wv][1] = a2; }
+    __syncthreads();
+    a1 = 0.; a2 = 0.;
+    for (int v = 0; v < (int)(blockDim.x >> 6); v++) { a1 += sh[v][0]; a2 += sh[v][1]; }
+    double *LT = fbar + (size_t)F_T * d.NF;
+    double th = 0.;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int i = q % NT, k = q / NT;
+        // hexagon: Nu_hex = (R_i / a2) sum area dd, dd = -S T_w + A T(0) - B T(1) + C T(2)
+        int b = 2 * NT * NR + q;
+        double s = wh * R_i / a2 * W[(W_NU + 4) * NB + b];
+        th += -s * W[(W_NU + 0) * NB + b];
+        aadd(LT + fidx(d, 2, i, 0, k), s * W[(W_NU + 1) * NB + b]);
+        aadd(LT + fidx(d, 2, i, 1, k), -s * W[(W_NU + 2) * NB + b]);
+        aadd(LT + fidx(d, 2, i, 2, k), s * W[(W_NU + 3) * NB + b]);
+        // cylinder: Nu_cyl = -(R_o / a1) sum area dd (the cold wall value is
+        // not a parameter here)
+        b = 2 * NT * NR + NT * NZ + q;
+        s = -wc * R_o / a1 * W[(W_NU + 4) * NB + b];
+        aadd(LT + fidx(d, 1, i, NR - 1, k), s * W[(W_NU + 1) * NB + b]);
+        aadd(LT + fidx(d, 1, i, NR - 2, k), -s * W[(W_NU + 2) * NB + b]);
+        aadd(LT + fidx(d, 1, i, NR - 3, k), s * W[(W_NU + 3) * NB + b]);
+    }
+    par_commit(theta + P_THOT, th);
+}
+
+}  // namespace
+
+// ---- host side --------------------------------------------------------------
+struct nim_adj {
+    nim_ctx *ctx = nullptr;
+    Adj a{};
+    double *seed = nullptr, *prev = nullptr;  // [4N + 5NF] each, like lam (= lc, lf)
+    double *cbar = nullptr, *tcbar = nullptr, *fbar = nullptr;
+    double *theta = nullptr;                  // [NPAR] accumulated over steps
+    unsigned long long *dmax = nullptr;       // [2]
+    unsigned long long *h_dmax = nullptr;
+    void *block = nullptr;
+};
+
+static double bits_to_double(unsigned long long b)
+{
+    double x;
+    memcpy(&x, &b, sizeof x);
+    return x;
+}
+
+static int grid_for(long n) { return (int)std::min<long>((n + BSA - 1) / BSA, 8192); }
+
+extern "C" int32_t nim_adj_create(nim_ctx *ctx, nim_adj **out)
+{
+    if (!ctx || !out) return nim_internal_fail(NIM_E_ARG, "null argument");
+    *out = nullptr;
+    const Dev *d = nim_internal_dev(ctx);
+    const size_t N = d->N, NF = d->NF, nl = 4 * N + 5 * NF;
+    // lam, seed, prev, lsrc, lTF, lMF, lold, ltc, cbar, tcbar, fbar, par, theta
+    const size_t total = 3 * nl + NSRC * N + 2 * NCF * N + 4 * N + 3 * N + 4 * N + 3 * N + 5 * NF + 2 * NPAR + 8;
+    nim_adj *A = new nim_adj();
+    A->ctx = ctx;
+    if (hipMalloc(&A->block, total * sizeof(double)) != hipSuccess) {
+        delete A;
+        return nim_internal_fail(NIM_E_HIP, "adjoint allocation failed");
+    }
+    if (hipHostMalloc((void **)&A->h_dmax, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipFree(A->block);
+        delete A;
+        return nim_internal_fail(NIM_E_HIP, "adjoint host allocation failed");
+    }
+    double *p = (double *)A->block;
+    auto take = [&](size_t n) { double *q = p; p += n; return q; };
+    A->a.lc = take(nl);
+    A->a.lf = A->a.lc + 4 * N;
+    A->seed = take(nl);
+    A->prev = take(nl);
+    A->a.lsrc = take(NSRC * N);
+    A->a.lTF = take(NCF * N);
+    A->a.lMF = take(NCF * N);
+    A->a.lold = take(4 * N);
+    A->a.ltc = take(3 * N);
+    A->cbar = take(4 * N);
+    A->tcbar = take(3 * N);
+    A->fbar = take(5 * NF);
+    A->a.par = take(NPAR);
+    A->theta = take(NPAR);
+    A->dmax = (unsigned long long *)take(8);
+    hipStream_t s = (hipStream_t)nim_internal_stream(ctx);
+    if (hipMemsetAsync(A->block, 0, total * sizeof(double), s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipHostFree(A->h_dmax);
+        (void)hipFree(A->block);
+        delete A;
+        return nim_internal_fail(NIM_E_HIP, "adjoint clear failed");
+    }
+    *out = A;
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_adj_destroy(nim_adj *A)
+{
+    if (!A) return NIM_OK;
+    (void)hipStreamSynchronize((hipStream_t)nim_internal_stream(A->ctx));
+    (void)hipHostFree(A->h_dmax);
+    (void)hipFree(A->block);
+    delete A;
+    return NIM_OK;
+}
+
+static int32_t adj_check(const char *what)
+{
+    if (hipGetLastError() != hipSuccess) return nim_internal_fail(NIM_E_HIP, what);
+    return NIM_OK;
+}
+
+// one reverse pass at the context's state: lam <- G_x^T lam, parameter
+// adjoints of the pass in lMF, lold, the Dirichlet slots of lf and par
+static void reverse_pass(nim_adj *A, hipStream_t s)
+{
+    const Dev &d = *nim_internal_dev(A->ctx);
+    const int gN = grid_for(d.N);
+    const int per_color = (d.NT / 2) * ((d.NR + 1) / 2) * d.NZ;
+    const int gC = (per_color + BSA - 1) / BSA;
+    hipLaunchKernelGGL(k_adj_begin, dim3(grid_for(std::max<long>(NCF * (long)d.N, 5 * (long)d.NF))), dim3(BSA), 0, s, d,
+                       A->a);
+    for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_T_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
+    hipLaunchKernelGGL(k_adj_T_src, dim3(gN), dim3(BSA), 0, s, d, A->a);
+    hipLaunchKernelGGL(k_adj_coef, dim3(gN), dim3(BSA), 0, s, d, A->a, 1, A->a.par, (int)P_ALPHA);
+    for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_ppe_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
+    hipLaunchKernelGGL(k_adj_ppe_src, dim3(gN), dim3(BSA), 0, s, d, A->a);
+    for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_mom_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
+    hipLaunchKernelGGL(k_adj_mom_src, dim3(gN), dim3(BSA), 0, s, d, A->a);
+}
+
+extern "C" int32_t nim_adj_seed_nusselt(nim_adj *A, const double weights[3])
+{
+    if (!A || !weights) return nim_internal_fail(NIM_E_ARG, "null argument");
+    const Dev &d = *nim_internal_dev(A->ctx);
+    hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
+    double R_i, R_o;
+    nim_internal_lengths(A->ctx, &R_i, &R_o);
+    // out[2] = (out[0] + out[1]) / 2
+    const double wc = weights[0] + 0.5 * weights[2], wh = weights[1] + 0.5 * weights[2];
+    hipLaunchKernelGGL(k_adj_nusselt, dim3(1), dim3(BSA), 0, s, d, A->fbar, A->theta, R_i, R_o, wc, wh);
+    return adj_check("adjoint Nusselt seed launch failed");
+}
+
+extern "C" int32_t nim_adj_reverse_pass(nim_adj *A)
+{
+    if (!A) return nim_internal_fail(NIM_E_ARG, "null argument");
+    hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
+    reverse_pass(A, s);
+    int32_t rc = adj_check("adjoint pass launch failed");
+    if (rc) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) return nim_internal_fail(NIM_E_HIP, "adjoint pass failed");
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_adj_reverse_step(nim_adj *A, double tol, int32_t max_passes, int32_t *passes, double *delta)
+{
+    if (!A || max_passes < 1) return nim_internal_fail(NIM_E_ARG, "bad reverse-step arguments");
+    const Dev &d = *nim_internal_dev(A->ctx);
+    hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
+    const long nl = 4 * (long)d.N + 5 * (long)d.NF;
+    const int gL = grid_for(nl);
+    hipLaunchKernelGGL(k_adj_seed, dim3(gL), dim3(BSA), 0, s, d, A->seed, A->a.lc, A->prev, A->cbar, A->tcbar,
+                       A->fbar);
+    const int check_every = 10;
+    int it = 0;
+    double dm = 0., lm = 0.;
+    bool conv = false;
+    while (it < max_passes) {
+        reverse_pass(A, s);
+        hipLaunchKernelGGL(k_adj_add_seed, dim3(gL), dim3(BSA), 0, s, A->a.lc, A->seed, nl);
+        it++;
+        if (it % check_every == 0 || it == max_passes) {
+            if (hipMemsetAsync(A->dmax, 0, 2 * sizeof(unsigned long long), s) != hipSuccess)
+                return nim_internal_fail(NIM_E_HIP, "adjoint delta clear failed");
+            hipLaunchKernelGGL(k_adj_delta, dim3(gL), dim3(BSA), 0, s, A->a.lc, A->prev, nl, A->dmax);
+            if (hipMemcpyAsync(A->h_dmax, A->dmax, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s) !=
+                    hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                return nim_internal_fail(NIM_E_HIP, "adjoint pass failed");
+            dm = bits_to_double(A->h_dmax[0]) / check_every;
+            lm = bits_to_double(A->h_dmax[1]);
+            if (!std::isfinite(dm) || !std::isfinite(lm))
+                return nim_internal_fail(NIM_E_NONFINITE, "non-finite adjoint");
+            if (dm <= tol * std::max(lm, 1e-300)) {
+                conv = true;
+                break;
+            }
+        }
+    }
+    if (passes) *passes = it;
+    if (delta) *delta = lm > 0. ? dm / lm : dm;
+    // parameter adjoints of the last pass, then the momentum coefficients
+    hipLaunchKernelGGL(k_adj_hex, dim3(grid_for((long)d.NT * d.NZ)), dim3(BSA), 0, s, d, A->a, A->theta);
+    hipLaunchKernelGGL(k_adj_par_add, dim3(1), dim3(64), 0, s, A->theta, A->a.par);
+    if (hipMemsetAsync(A->a.ltc, 0, 3 * (size_t)d.N * sizeof(double), s) != hipSuccess)
+        return nim_internal_fail(NIM_E_HIP, "adjoint clear failed");
+    hipLaunchKernelGGL(k_adj_coef, dim3(grid_for(d.N)), dim3(BSA), 0, s, d, A->a, 0, A->theta, (int)P_NU);
+    hipLaunchKernelGGL(k_adj_carry, dim3(grid_for(nl)), dim3(BSA), 0, s, d, A->a, A->cbar, A->tcbar, A->fbar);
+    int32_t rc = adj_check("adjoint step launch failed");
+    if (rc) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) return nim_internal_fail(NIM_E_HIP, "adjoint step failed");
+    return conv ? NIM_OK : nim_internal_fail(NIM_E_MAXITER, "adjoint fixed point not converged");
+}
+
+extern "C" int32_t nim_adj_gradient(nim_adj *A, double out[4])
+{
+    if (!A || !out) return nim_internal_fail(NIM_E_ARG, "null argument");
+    hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
+    if (hipMemcpyAsync(out, A->theta, NPAR * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return nim_internal_fail(NIM_E_HIP, "adjoint gradient copy failed");
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_adj_clear(nim_adj *A)
+{
+    if (!A) return nim_internal_fail(NIM_E_ARG, "null argument");
+    const Dev &d = *nim_internal_dev(A->ctx);
+    const size_t N = d.N, NF = d.NF, nl = 4 * N + 5 * NF;
+    const size_t total = 3 * nl + NSRC * N + 2 * NCF * N + 4 * N + 3 * N + 4 * N + 3 * N + 5 * NF + 2 * NPAR + 8;
+    hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
+    if (hipMemsetAsync(A->block, 0, total * sizeof(double), s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return nim_internal_fail(NIM_E_HIP, "adjoint clear failed");
+    return NIM_OK;
+}
+
+// the reverse of one forward stage (nim_run_stage numbering: 0 momentum
+// sources, 1 momentum sweep, 2 pressure sources, 3 pressure sweep, 4 energy
+// coefficients and adiabatic walls, 5 energy sources, 6 energy sweep) or,
+// as stage 7, of the momentum coefficient build; scalar parameter adjoints
+// go to the "parameters" buffer
+extern "C" int32_t nim_adj_reverse_stage(nim_adj *A, int32_t stage)
+{
+    if (!A || stage < 0 || stage > 7) return nim_internal_fail(NIM_E_ARG, "bad reverse-stage arguments");
+    const Dev &d = *nim_internal_dev(A->ctx);
+    hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
+    const int gN = grid_for(d.N);
+    const int gC = ((d.NT / 2) * ((d.NR + 1) / 2) * d.NZ + BSA - 1) / BSA;
+    switch (stage) {
+    case 0: hipLaunchKernelGGL(k_adj_mom_src, dim3(gN), dim3(BSA), 0, s, d, A->a); break;
+    case 1:
+        for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_mom_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
+        break;
+    case 2: hipLaunchKernelGGL(k_adj_ppe_src, dim3(gN), dim3(BSA), 0, s, d, A->a); break;
+    case 3:
+        for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_ppe_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
+        break;
+    case 4: hipLaunchKernelGGL(k_adj_coef, dim3(gN), dim3(BSA), 0, s, d, A->a, 1, A->a.par, (int)P_ALPHA); break;
+    case 5: hipLaunchKernelGGL(k_adj_T_src, dim3(gN), dim3(BSA), 0, s, d, A->a); break;
+    case 6:
+        for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_T_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
+        break;
+    default: hipLaunchKernelGGL(k_adj_coef, dim3(gN), dim3(BSA), 0, s, d, A->a, 0, A->a.par, (int)P_NU); break;
+    }
+    int32_t rc = adj_check("adjoint stage launch failed");
+    if (rc) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) return nim_internal_fail(NIM_E_HIP, "adjoint stage failed");
+    return NIM_OK;
+}
+
+// the adjoint of a context array, by the context array's name and in the
+// reference layout (nim_get_array / nim_set_array), or the scalar
+// "parameters" (this pass) and "gradient" (accumulated) in the order
+// nu, Rinv, alpha, T_hot
+static bool adj_lookup(nim_adj *A, const char *name, double *&p, int &kind, int &mult)
+{
+    const Dev &d = *nim_internal_dev(A->ctx);
+    const size_t N = d.N, NF = d.NF;
+    std::string s(name);
+    const char *ff[5] = {"faces_u", "faces_v", "faces_w", "faces_p", "faces_T"};
+    for (int q = 0; q < 5; q++)
+        if (s == ff[q]) { p = A->a.lf + q * NF; kind = 1; mult = 1; return true; }
+    const char *cf[4] = {"u", "v", "w", "T"};
+    for (int q = 0; q < 4; q++) {
+        if (s == std::string(cf[q]) + "xyz") { p = A->a.lc + q * N; kind = 0; mult = 1; return true; }
+        if (s == std::string(cf[q]) + "xyz_old") { p = A->a.lold + q * N; kind = 0; mult = 1; return true; }
+        if (q < 3 && s == std::string(cf[q]) + "xyzt") { p = A->a.ltc + q * N; kind = 0; mult = 1; return true; }
+    }
+    const char *src[5] = {"R_B_u", "R_B_v", "R_B_w", "R_b_p", "R_D_T"};
+    for (int q = 0; q < 5; q++)
+        if (s == src[q]) { p = A->a.lsrc + q * N; kind = 0; mult = 1; return true; }
+    const char *cn[4] = {"F4", "F1", "F2", "F3"};
+    const int co[4] = {C_F4, C_F1, C_F2, C_F3}, cm[4] = {8, 5, 5, 5};
+    for (int q = 0; q < 4; q++) {
+        if (s == std::string("M_") + cn[q]) { p = A->a.lMF + co[q] * N; kind = 0; mult = cm[q]; return true; }
+        if (s == std::string("MT_") + cn[q]) { p = A->a.lTF + co[q] * N; kind = 0; mult = cm[q]; return true; }
+    }
+    if (s == "parameters") { p = A->a.par; kind = -1; mult = NPAR; return true; }
+    if (s == "gradient") { p = A->theta; kind = -1; mult = NPAR; return true; }
+    return false;
+}
+
+static int32_t adj_move(nim_adj *A, const char *name, double *host, int64_t count, bool to_host)
+{
+    if (!A || !name || !host) return nim_internal_fail(NIM_E_ARG, "null argument");
+    double *p;
+    int kind, mult;
+    if (!adj_lookup(A, name, p, kind, mult)) return nim_internal_fail(NIM_E_ARG, "unknown adjoint array");
+    hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
+    if (hipStreamSynchronize(s) != hipSuccess) return nim_internal_fail(NIM_E_HIP, "adjoint copy failed");
+    if (kind < 0) {
+        if (count != mult) return nim_internal_fail(NIM_E_ARG, "wrong element count");
+        hipError_t e = to_host ? hipMemcpy(host, p, mult * sizeof(double), hipMemcpyDeviceToHost)
+                               : hipMemcpy(p, host, mult * sizeof(double), hipMemcpyHostToDevice);
+        return e == hipSuccess ? NIM_OK : nim_internal_fail(NIM_E_HIP, "adjoint copy failed");
+    }
+    return nim_internal_move(A->ctx, p, kind, mult, name, host, count, to_host);
+}
+
+extern "C" int32_t nim_adj_get(nim_adj *A, const char *name, double *out, int64_t count)
+{
+    return adj_move(A, name, out, count, true);
+}
+
+extern "C" int32_t nim_adj_set(nim_adj *A, const char *name, const double *in, int64_t count)
+{
+    return adj_move(A, name, const_cast<double *>(in), count, false);
+}

@@ -126,6 +126,11 @@ extern "C" const char *nim_last_error(void) { return g_err.c_str(); }
 int32_t nim_internal_fail(int32_t code, const char *msg) { return fail(code, msg); }
 void *nim_internal_stream(nim_ctx *c) { return (void *)c->stream; }
 const Dev *nim_internal_dev(nim_ctx *c) { return &c->dev; }
+void nim_internal_lengths(nim_ctx *c, double *R_i, double *R_o)
+{
+    *R_i = c->prm.R_i;
+    *R_o = c->prm.R_o;
+}
 
 static void init_schedules(nim_ctx *c);
 
@@ -502,11 +507,27 @@ static bool lookup(nim_ctx *c, const char *name, ArrRef &a)
     return false;
 }
 
+static int32_t move_ref(nim_ctx *c, const ArrRef &a, const char *name, double *host, int64_t count, bool to_host);
+
 static int32_t move_array(nim_ctx *c, const char *name, double *host, int64_t count, bool to_host)
 {
     if (!c || !name || !host) return fail(NIM_E_ARG, "null argument");
     ArrRef a;
     if (!lookup(c, name, a)) return fail(NIM_E_ARG, std::string("unknown array ") + name);
+    return move_ref(c, a, name, host, count, to_host);
+}
+
+// a device array of the library's layout (kind 0 cells [mult][N], 1 faces,
+// 2 walls [mult][NB]) to or from the reference layout, for the other
+// translation units
+int32_t nim_internal_move(nim_ctx *c, double *dev, int kind, int mult, const char *name, double *host, int64_t count,
+                          bool to_host)
+{
+    return move_ref(c, ArrRef{dev, kind, mult}, name, host, count, to_host);
+}
+
+static int32_t move_ref(nim_ctx *c, const ArrRef &a, const char *name, double *host, int64_t count, bool to_host)
+{
     const int rows = a.kind == 1 ? c->NF : a.kind == 2 ? c->NB : c->N;
     if (count != (int64_t)rows * a.mult) return fail(NIM_E_ARG, std::string("wrong element count for ") + name);
     std::vector<double> buf((size_t)rows * a.mult);

@@ -126,6 +126,23 @@ def lib():
         L.nim_ens_destroy.argtypes = [vp]
         L.nim_ens_inner_solve.restype = i32
         L.nim_ens_inner_solve.argtypes = [vp, _P, i32, i32, i32, _PI, _P]
+        L.nim_adj_create.restype = i32
+        L.nim_adj_create.argtypes = [vp, ctypes.POINTER(vp)]
+        for n in ("nim_adj_destroy", "nim_adj_reverse_pass", "nim_adj_clear"):
+            getattr(L, n).restype = i32
+            getattr(L, n).argtypes = [vp]
+        L.nim_adj_seed_nusselt.restype = i32
+        L.nim_adj_seed_nusselt.argtypes = [vp, _P]
+        L.nim_adj_reverse_step.restype = i32
+        L.nim_adj_reverse_step.argtypes = [vp, ctypes.c_double, i32, ctypes.POINTER(i32), _P]
+        L.nim_adj_gradient.restype = i32
+        L.nim_adj_gradient.argtypes = [vp, _P]
+        L.nim_adj_reverse_stage.restype = i32
+        L.nim_adj_reverse_stage.argtypes = [vp, i32]
+        L.nim_adj_get.restype = i32
+        L.nim_adj_get.argtypes = [vp, ctypes.c_char_p, _P, ctypes.c_int64]
+        L.nim_adj_set.restype = i32
+        L.nim_adj_set.argtypes = [vp, ctypes.c_char_p, _P, ctypes.c_int64]
         L.nim_last_error.restype = ctypes.c_char_p
         L.nim_last_error.argtypes = []
         _lib = L
@@ -140,7 +157,9 @@ def exported_symbols():
             "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_kernel_times",
             "nim_inner_solve_mixed", "nim_residual", "nim_kernel_times_mixed", "nim_ens_create", "nim_ens_destroy",
             "nim_ens_inner_solve", "nim_revolve_plan", "nim_ckpt_create", "nim_ckpt_destroy", "nim_ckpt_store",
-            "nim_ckpt_restore", "nim_last_error"]
+            "nim_ckpt_restore", "nim_adj_create", "nim_adj_destroy", "nim_adj_seed_nusselt", "nim_adj_reverse_step",
+            "nim_adj_gradient", "nim_adj_reverse_pass", "nim_adj_reverse_stage", "nim_adj_clear", "nim_adj_get",
+            "nim_adj_set", "nim_last_error"]
 
 
 def _check(rc):
@@ -461,3 +480,120 @@ def reverse_run(ctx, nsteps, nslots, forward_step, reverse_step):
     finally:
         ck.close()
     return fwd
+
+
+class Adjoint:
+    """Discrete adjoint of the time step of a context (nim_adj_*). Arrays
+    are addressed by the name of the context array they are the adjoint of,
+    in the reference layout."""
+
+    PARAMS = ("nu", "Rinv", "alpha", "T_hot")
+
+    def __init__(self, ctx):
+        self.L = lib()
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(self.L.nim_adj_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.nim_adj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _len(self, name):
+        if name in ("parameters", "gradient"):
+            return 4
+        return self.ctx._len(name)
+
+    def get(self, name):
+        out = np.zeros(self._len(name))
+        _check(self.L.nim_adj_get(self.h, name.encode(), _dp(out), out.size))
+        return out
+
+    def set(self, name, values):
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        _check(self.L.nim_adj_set(self.h, name.encode(), _dp(a), a.size))
+
+    def clear(self):
+        _check(self.L.nim_adj_clear(self.h))
+
+    def reverse_stage(self, stage):
+        _check(self.L.nim_adj_reverse_stage(self.h, int(stage)))
+
+    def reverse_pass(self):
+        _check(self.L.nim_adj_reverse_pass(self.h))
+
+    def seed_nusselt(self, weights):
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        _check(self.L.nim_adj_seed_nusselt(self.h, _dp(w)))
+
+    def reverse_step(self, tol, max_passes=100000):
+        """Adjoint of the step whose converged state is current; returns
+        (passes, relative change per pass at the end)."""
+        n = ctypes.c_int32()
+        dl = np.zeros(1)
+        _check(self.L.nim_adj_reverse_step(self.h, tol, max_passes, ctypes.byref(n), _dp(dl)))
+        return n.value, float(dl[0])
+
+    def gradient(self):
+        out = np.zeros(4)
+        _check(self.L.nim_adj_gradient(self.h, _dp(out)))
+        return dict(zip(self.PARAMS, out.tolist()))
+
+
+def run_steps(ctx, nsteps, tol=None, passes=None):
+    """nsteps time steps from the current state as the adjoint sees them:
+    build_momentum_coeffs, the inner solve (to tol, or exactly `passes`
+    passes), advance_time. Returns nim_nusselt of the final state."""
+    for _ in range(nsteps):
+        _solve(ctx, tol, passes)
+        ctx.advance_time()
+    return ctx.nusselt()
+
+
+def _solve(ctx, tol, passes):
+    ctx.build_momentum_coeffs()
+    if passes is None:
+        ctx.inner_solve(tol)
+    else:
+        rc, it, _ = ctx.inner_solve(np.zeros(5), max_iter=passes, raise_on_cap=False)
+        if it != passes:
+            raise NimError(rc, "inner solve stopped after %d of %d passes" % (it, passes))
+
+
+def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, weights=(0.0, 1.0, 0.0)):
+    """J = weights . nim_nusselt[:3] after nsteps steps of ctx from its
+    current state (as run_steps) and dJ/dT_hot, dJ/dRa by the discrete
+    adjoint, the run reversed with nslots device snapshots (binomial plan).
+    case supplies Ra, Pr and nu. Returns (J, {"T_hot", "Ra"}, info)."""
+    w = np.asarray(weights, dtype=np.float64)
+    adj = Adjoint(ctx)
+    out = {}
+    passes_adj = []
+
+    def forward_step():
+        _solve(ctx, tol, passes)
+        ctx.advance_time()
+
+    def reverse_step(k):
+        _solve(ctx, tol, passes)
+        if k == nsteps - 1:
+            out["J"] = float(np.dot(w, ctx.nusselt()[:3]))
+            adj.seed_nusselt(w)
+        passes_adj.append(adj.reverse_step(adj_tol))
+
+    try:
+        fwd = reverse_run(ctx, nsteps, nslots, forward_step, reverse_step)
+        g = adj.gradient()
+    finally:
+        adj.close()
+    dnu = g["nu"] + g["Rinv"] + g["alpha"] / case.Pr
+    grad = {"T_hot": g["T_hot"], "Ra": dnu * (-case.nu / (2.0 * case.Ra))}
+    return out["J"], grad, dict(partials=g, forward_steps=fwd, adjoint_passes=passes_adj)

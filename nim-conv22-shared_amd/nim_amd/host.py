@@ -32,6 +32,9 @@ class Case:
 
     def __init__(self, dims, R_i, R_o, height, Ra, Pr, DT, blend=None, grading=1.5, hex_order=-1.0, rho=1.0,
                  g_beta=1.0, th_tc=1.0, relax=0.4, v_limit=1e-2, T_hot=1.0, T_cold=0.0):
+        self._args = dict(dims=dims, R_i=R_i, R_o=R_o, height=height, Ra=Ra, Pr=Pr, DT=DT, blend=blend,
+                          grading=grading, hex_order=hex_order, rho=rho, g_beta=g_beta, th_tc=th_tc, relax=relax,
+                          v_limit=v_limit, T_hot=T_hot, T_cold=T_cold)
         self.dims = tuple(int(x) for x in dims)
         self.R_i, self.R_o, self.height = float(R_i), float(R_o), float(height)
         if blend is None:
@@ -51,6 +54,13 @@ class Case:
             setattr(self, n, float(relax))
         self.v_limit = float(v_limit)
         self.T_hot, self.T_cold = float(T_hot), float(T_cold)
+
+    def replace(self, **changes):
+        """The same case with some constructor arguments changed (derived
+        constants recomputed)."""
+        args = dict(self._args)
+        args.update(changes)
+        return Case(**args)
 
     @property
     def ncells(self):

@@ -273,7 +273,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--inner", type=int, default=None,
-                    help="FP32 passes per FP64 residual of the mixed loop (default 12; 8 for the ensemble)")
+                    help="FP32 passes per FP64 residual of the mixed loop (default 24; 8 for the ensemble)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--ensemble", action="store_true", help="the 48-case ensemble instead of one case")
     ap.add_argument("--ens-modes", default="mixed,fp64,sequential",
@@ -296,7 +296,7 @@ def main():
     if args.ensemble:
         args.inner = args.inner or 8
         return ensemble_bench(args)
-    args.inner = args.inner or 12
+    args.inner = args.inner or 24
 
     case = host.baseline_case(args.case)
     t_setup = time.perf_counter()

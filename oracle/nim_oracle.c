@@ -29,12 +29,21 @@
  *
  * Parity status of each function is pinned by tests/test_oracle_reference.py
  * (against the compiled reference's dumps) and tests/test_oracle_math.py.
+ *
+ * Loops whose iterations write disjoint values and read none of them (the
+ * sources, the constants, each colour phase of the colored order, the
+ * error maxima) run under OpenMP when built with it; the natural-order
+ * sweeps and boundary passes stay serial. Results therefore do not depend
+ * on the thread count.
  */
 #include <math.h>
 #include <stddef.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 enum { ORA_NATURAL = 0, ORA_COLORED = 1 };
 
@@ -167,6 +176,25 @@ void ora_get_scalars(ora_ctx *c, double out[6])
 
 void ora_set_time(ora_ctx *c, double TI, int STEP) { c->TI = TI; c->STEP = STEP; }
 
+/* threads the OpenMP loops use (1 without OpenMP) */
+int ora_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void ora_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------ */
 /* index helpers (1-based position/component arguments as in Fortran)  */
 /* ------------------------------------------------------------------ */
@@ -245,6 +273,7 @@ static void nim_direction(double u, double k, double gp, double gm, double vlim,
 static void nim_face_coeffs(ora_ctx *c, const double *A, double *F1, double *F2, double *F3, double *F4)
 {
     const double DT = c->prm.DT;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < c->nmesh; i++) {
         F8(F4, i, 2) = 0.5 * (1. / A3(A, i, 1, 5) + 1. / A3(A, i, 2, 5) + 1. / A3(A, i, 3, 5) - 1. / DT);
         F8(F4, i, 3) = -(A3(A, i, 1, 6) / A3(A, i, 1, 5));
@@ -274,6 +303,7 @@ static void nim_face_coeffs(ora_ctx *c, const double *A, double *F1, double *F2,
 /* calculate_velocity_nim_constants_formulation2 (nim.f90:160-330) */
 void ora_velocity_constants(ora_ctx *c)
 {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < c->nmesh; i++) {
         c->meshes_u[i] = c->uxyzt[i] * XI(c, i, 1, 1) + c->vxyzt[i] * XI(c, i, 1, 2) + c->wxyzt[i] * XI(c, i, 1, 3);
         c->meshes_v[i] = c->uxyzt[i] * XI(c, i, 2, 1) + c->vxyzt[i] * XI(c, i, 2, 2) + c->wxyzt[i] * XI(c, i, 2, 3);
@@ -292,6 +322,7 @@ void ora_velocity_constants(ora_ctx *c)
 /* calculate_temperature_nim_constants_formulation2 (nim.f90:334-503) */
 void ora_temperature_constants(ora_ctx *c)
 {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < c->nmesh; i++) {
         double ut = 0.5 * (c->uxyz[i] + c->uxyz_old[i]);
         double vt = 0.5 * (c->vxyz[i] + c->vxyz_old[i]);
@@ -495,9 +526,12 @@ static double cross_diffusion(const ora_ctx *c, const double *f, int n, double c
 void ora_velocity_sources(ora_ctx *c)
 {
     const double gb = c->prm.g_beta;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < c->nmesh; i++) c->meshes_bz[i] = -(gb * 0.5 * (c->Txyz[i] + c->Txyz_old[i]));
+#pragma omp parallel for schedule(static)
     for (int f = 0; f < c->nface; f++) c->faces_bz[f] = -(gb * c->faces_T[f]);
     const double rinv = 1 / c->prm.rho, Rinv = 1 / c->prm.Re;
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < c->nmesh; n++) {
         double pf[6];
         gather6(c, c->faces_p, n, pf);
@@ -531,6 +565,7 @@ void ora_velocity_sources(ora_ctx *c)
 void ora_pressure_sources(ora_ctx *c)
 {
     const double zero6[6] = {0, 0, 0, 0, 0, 0};
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < c->nmesh; n++) {
         double uf[6], vf[6], wf[6], bzf[6];
         gather6(c, c->faces_u, n, uf);
@@ -553,6 +588,7 @@ void ora_pressure_sources(ora_ctx *c)
  * parameters.f90:179) and calculate_R_D_T (2000-2134) */
 void ora_temperature_sources(ora_ctx *c)
 {
+#pragma omp parallel for schedule(static)
     for (int n = 0; n < c->nmesh; n++) c->R_D_T[n] = cross_diffusion(c, c->faces_T, n, c->prm.alpha);
 }
 
@@ -770,9 +806,11 @@ void ora_velocity_sweep(ora_ctx *c, int order)
     if (order == ORA_NATURAL) {
         for (int i = 0; i < c->nmesh; i++) velocity_unit(c, i);
     } else {
-        for (int col = 0; col < 4; col++)
+        for (int col = 0; col < 4; col++) {
+#pragma omp parallel for schedule(static)
             for (int i = 0; i < c->nmesh; i++)
                 if (cell_color(c, i) == col) velocity_unit(c, i);
+        }
     }
 }
 
@@ -781,9 +819,11 @@ void ora_temperature_sweep(ora_ctx *c, int order)
     if (order == ORA_NATURAL) {
         for (int i = 0; i < c->nmesh; i++) temperature_unit(c, i);
     } else {
-        for (int col = 0; col < 4; col++)
+        for (int col = 0; col < 4; col++) {
+#pragma omp parallel for schedule(static)
             for (int i = 0; i < c->nmesh; i++)
                 if (cell_color(c, i) == col) temperature_unit(c, i);
+        }
     }
 }
 
@@ -805,13 +845,15 @@ void ora_pressure_stage(ora_ctx *c, int order)
         int slot = c->bc_pos[b] == 5 ? 0 : c->bc_pos[b] == 6 ? 1 : c->bc_pos[b] == 1 ? 2 : 3;
         first[c->bc_mesh[b] * 4 + slot] = b;
     }
-    for (int col = 0; col < 4; col++)
+    for (int col = 0; col < 4; col++) {
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < c->nmesh; i++) {
             if (cell_color(c, i) != col) continue;
             for (int s = 0; s < 4; s++)
                 if (first[i * 4 + s] >= 0) pressure_wall(c, first[i * 4 + s]);
             pressure_unit(c, i);
         }
+    }
     free(first);
 }
 
@@ -825,6 +867,7 @@ void ora_pressure_sweep(ora_ctx *c)
 void ora_find_error(ora_ctx *c, double err[5])
 {
     double eu = 0, ev = 0, ew = 0, ep = 0, eT = 0;
+#pragma omp parallel for schedule(static) reduction(max : eu, ev, ew, ep, eT)
     for (int i = 0; i < c->nface; i++) {
         eu = fmax(eu, fabs(c->faces_u[i] - c->faces_u_old[i]));
         ev = fmax(ev, fabs(c->faces_v[i] - c->faces_v_old[i]));
@@ -838,6 +881,7 @@ void ora_find_error(ora_ctx *c, double err[5])
         c->faces_T_old[i] = c->faces_T[i];
     }
     double cu = 0, cv = 0, cw = 0, cT = 0;
+#pragma omp parallel for schedule(static) reduction(max : cu, cv, cw, cT)
     for (int i = 0; i < c->nmesh; i++) {
         cu = fmax(cu, fabs(c->uxyz[i] - c->uxyz_old_con[i]));
         cv = fmax(cv, fabs(c->vxyz[i] - c->vxyz_old_con[i]));

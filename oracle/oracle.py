@@ -82,6 +82,10 @@ def lib():
                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         L.oracle_ic_noise.restype = ctypes.c_double
         L.oracle_ic_noise.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]
+        L.ora_threads.restype = ctypes.c_int
+        L.ora_threads.argtypes = []
+        L.ora_set_threads.restype = None
+        L.ora_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -208,3 +212,12 @@ def load_reference_geometry(orc, geo):
         if src in ONE_BASED:
             v = v.astype(np.int64) - 1
         orc.set(dst, v)
+
+
+def threads():
+    """Threads the oracle's OpenMP loops use."""
+    return int(lib().ora_threads())
+
+
+def set_threads(n):
+    lib().ora_set_threads(int(n))

@@ -285,6 +285,7 @@ def main():
     ap.add_argument("--inner", type=int, default=None,
                     help="FP32 passes per FP64 residual of the mixed loop (default 24; 8 for the ensemble)")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the reversed-pass timing")
     ap.add_argument("--ensemble", action="store_true", help="the 48-case ensemble instead of one case")
     ap.add_argument("--ens-modes", default="mixed,fp64,sequential",
                     help="ensemble runs: batched mixed, batched fp64, mixed one case after another")
@@ -371,6 +372,26 @@ def main():
     achieved = BYTES_PER_CELL[dom] * cells_per_launch / per_launch_s
     iter_ms = sum(v[0] for v in kt.values())
 
+    adjoint = None
+    if not args.no_adjoint:
+        # one reversed pass of the discrete adjoint at the benchmarked state,
+        # from a random adjoint of the iterate (every stage does its full
+        # work), against the forward pass's event time
+        adj = nim_amd.Adjoint(ctx)
+        rng = np.random.default_rng(5)
+        for n in ("uxyz", "vxyz", "wxyz", "Txyz") + FACE_STATE + ("faces_p",):
+            adj.set(n, rng.standard_normal(adj._len(n)))
+        reps = 10
+        adj.reverse_pass()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            adj.reverse_pass()
+        dt_adj = (time.perf_counter() - t0) / reps
+        adj.close()
+        adjoint = {"ms_per_reversed_pass": dt_adj * 1e3, "ms_per_forward_pass": iter_ms,
+                   "reversed_cell_updates_per_s": N / dt_adj,
+                   "note": "wall time of nim_adj_reverse_pass (17 launches and a stream synchronisation each)"}
+
     mixed = None
     if not args.no_mixed:
         wall_m, passes_m, res_m, nu_m = run(True)
@@ -441,6 +462,7 @@ def main():
         "kernel_ms_per_iteration": {k: v[0] for k, v in kt.items()},
         "residual_last_step": [float(x) for x in res64],
         "mixed": mixed,
+        "adjoint": adjoint,
         "setup_s": t_setup,
     }
     if not args.no_cpu:

@@ -28,7 +28,8 @@
  *    maximum change since the start of the solve call.
  *
  * Parity status of each function is pinned by tests/test_oracle_reference.py
- * (against the compiled reference's dumps) and tests/test_oracle_math.py.
+ * (against the compiled reference's dumps) and tests/test_oracle_math.py
+ * (closed forms at high precision, limits, both sweep orders).
  *
  * Loops whose iterations write disjoint values and read none of them (the
  * sources, the constants, each colour phase of the colored order, the
@@ -267,6 +268,12 @@ static void nim_direction(double u, double k, double gp, double gm, double vlim,
                + 2 * u8 / (93555. * k9) - -(1382 * u10 / (6.38512875e8 * k11));
         A[5] = 0.5 + u / (6. * k) - u3 / (90. * k3) + u5 / (945. * k5) - -(u7 / (9450. * k7)) + u9 / (93555. * k9);
     }
+}
+
+/* the six constants of one direction, for tests of the closed forms */
+void ora_nim_direction(double u, double k, double gp, double gm, double vlim, double A[6])
+{
+    nim_direction(u, k, gp, gm, vlim, A);
 }
 
 /* F4, F1..F3 from the exponential-fit constants (nim.f90:292-329) */

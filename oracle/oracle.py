@@ -86,6 +86,8 @@ def lib():
         L.ora_threads.argtypes = []
         L.ora_set_threads.restype = None
         L.ora_set_threads.argtypes = [ctypes.c_int]
+        L.ora_nim_direction.restype = None
+        L.ora_nim_direction.argtypes = [ctypes.c_double] * 5 + [ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -221,3 +223,11 @@ def threads():
 
 def set_threads(n):
     lib().ora_set_threads(int(n))
+
+
+def nim_direction(u, k, gp, gm, v_limit=1e-2):
+    """The six exponential-fit constants of one direction (nim.f90:184-287)."""
+    import numpy as np
+    out = np.zeros(6)
+    lib().ora_nim_direction(u, k, gp, gm, v_limit, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out

@@ -24,7 +24,11 @@
  *    graphs; NIM_BATCH / NIM_MIXED_BATCH set the passes / cycles between
  *    host reads of the convergence flag; NIM_SLAB=k (k >= 2, read at context
  *    creation) runs the sweeps in the skewed k-slab schedule, bitwise
- *    identical to the default four colour phases.
+ *    identical to the default four colour phases. NIM_FUSE_T=0 (read at
+ *    context creation) makes the FP64 loop store the energy coefficients
+ *    from a kernel of their own, as nim_run_stage always does; by default
+ *    the energy sweep builds them in registers (bitwise the same results),
+ *    so after nim_inner_solve the MT_F* arrays are not current.
  */
 #ifndef NIM_HIP_H
 #define NIM_HIP_H

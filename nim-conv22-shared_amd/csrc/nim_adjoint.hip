@@ -37,6 +37,7 @@ int32_t nim_internal_fail(int32_t code, const char *msg);
 void *nim_internal_stream(nim_ctx *c);
 const Dev *nim_internal_dev(nim_ctx *c);
 void nim_internal_lengths(nim_ctx *c, double *R_i, double *R_o);
+void nim_internal_energy_coefs(nim_ctx *c);
 int32_t nim_internal_move(nim_ctx *c, double *dev, int kind, int mult, const char *name, double *host, int64_t count,
                           bool to_host);
 
@@ -999,6 +1000,9 @@ extern "C" int32_t nim_adj_reverse_step(nim_adj *A, double tol, int32_t max_pass
     hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
     const long nl = 4 * (long)d.N + 5 * (long)d.NF;
     const int gL = grid_for(nl);
+    // the reversed energy sweep reads the energy coefficients of the
+    // converged state, which the forward loop may not have stored
+    nim_internal_energy_coefs(A->ctx);
     hipLaunchKernelGGL(k_adj_seed, dim3(gL), dim3(BSA), 0, s, d, A->seed, A->a.lc, A->prev, A->cbar, A->tcbar,
                        A->fbar);
     const int check_every = 10;

@@ -23,6 +23,9 @@ __global__ void k_ppe_src(Dev d);
 __global__ void k_T_src(Dev d);
 __global__ void k_mom_sweep(Dev d, Sched sc);
 __global__ void k_T_sweep(Dev d, Sched sc);
+__global__ void k_T_sweep_fused(Dev d, Sched sc);
+__global__ void k_zero_flux(Dev d);
+__global__ void k_energy_coefs(Dev d);
 __global__ void k_ppe_sweep(Dev d, Sched sc);
 __global__ void k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4, int max_iter, int min_iter);
 __global__ void k_check(Dev d);
@@ -107,6 +110,9 @@ struct nim_ctx {
     // sweep schedules (64- and 256-cell blocks) and their slab thickness
     int slab = 0;
     std::vector<Sched> sched64, sched256;
+    // FP64 loop: energy coefficients built inside the energy sweep instead
+    // of stored by a kernel of their own (default; NIM_FUSE_T=0 turns it off)
+    bool fuse_T = true;
     // mixed precision: FP32 corrections and constants, residual buffers
     int const_version = 0;        // bumped by every geometry or pressure-coefficient build
     struct {
@@ -126,6 +132,13 @@ extern "C" const char *nim_last_error(void) { return g_err.c_str(); }
 int32_t nim_internal_fail(int32_t code, const char *msg) { return fail(code, msg); }
 void *nim_internal_stream(nim_ctx *c) { return (void *)c->stream; }
 const Dev *nim_internal_dev(nim_ctx *c) { return &c->dev; }
+// the energy coefficients at the current state (the FP64 loop does not
+// store them when fuse_T is set)
+void nim_internal_energy_coefs(nim_ctx *c)
+{
+    hipLaunchKernelGGL(k_energy_coefs, dim3((c->N + BS - 1) / BS), dim3(BS), 0, c->stream, c->dev);
+}
+
 void nim_internal_lengths(nim_ctx *c, double *R_i, double *R_o)
 {
     *R_i = c->prm.R_i;
@@ -225,6 +238,8 @@ extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, n
     {
         const char *e = getenv("NIM_KFAST");
         d.kfast = e ? atoi(e) != 0 : 0;
+        const char *f = getenv("NIM_FUSE_T");
+        c->fuse_T = f ? atoi(f) != 0 : true;
     }
     init_schedules(c);
     double *gc, *An, *wall;
@@ -652,9 +667,11 @@ static void init_schedules(nim_ctx *c)
 
 static inline int sched_blocks(const Sched &s) { return s.bstart[s.n]; }
 
-// `mark` is called after every launch with the kernel class (timing passes)
+// `mark` is called after every launch with the kernel class (timing passes).
+// With fused (the FP64 loop when fuse_T is set), stage 4 only updates the
+// adiabatic walls and stage 6 builds the energy coefficients itself.
 template <class Mark>
-static void launch_stage(nim_ctx *c, int stage, Mark mark)
+static void launch_stage(nim_ctx *c, int stage, Mark mark, bool fused = false)
 {
     const Dev &d = c->dev;
     hipStream_t s = c->stream;
@@ -674,11 +691,18 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark)
             mark(3);
         }
         break;
-    case 4: hipLaunchKernelGGL(k_T_coef, dim3(gS), dim3(BS), 0, s, d); mark(4); break;
+    case 4:
+        if (fused)
+            hipLaunchKernelGGL(k_zero_flux, dim3((2 * c->NT * c->NR + BS - 1) / BS), dim3(BS), 0, s, d);
+        else
+            hipLaunchKernelGGL(k_T_coef, dim3(gS), dim3(BS), 0, s, d);
+        mark(4);
+        break;
     case 5: hipLaunchKernelGGL(k_T_src, dim3(gS), dim3(BS), 0, s, d); mark(6); break;
     case 6:
         for (const Sched &sc : c->sched256) {
-            hipLaunchKernelGGL(k_T_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
+            if (fused) hipLaunchKernelGGL(k_T_sweep_fused, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
+            else hipLaunchKernelGGL(k_T_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
             mark(7);
         }
         break;
@@ -692,7 +716,7 @@ static void launch_stage(nim_ctx *c, int stage)
 
 static void launch_iteration(nim_ctx *c)
 {
-    for (int st = 0; st <= 6; st++) launch_stage(c, st);
+    for (int st = 0; st <= 6; st++) launch_stage(c, st, [](int) {}, c->fuse_T);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, c->stream, c->dev);
 }
 
@@ -1037,7 +1061,7 @@ extern "C" int32_t nim_kernel_times(nim_ctx *c, int32_t iters, double *ms, int32
     auto mark = [&](int k) { log.rec(k); };
     for (int it = 0; it < iters; it++) {
         log.start();
-        for (int st = 0; st <= 6; st++) launch_stage(c, st, mark);
+        for (int st = 0; st <= 6; st++) launch_stage(c, st, mark, c->fuse_T);
         hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, c->stream, c->dev);
         mark(8);
         HIPCHK(hipGetLastError());

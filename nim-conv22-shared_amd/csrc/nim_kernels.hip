@@ -156,6 +156,79 @@ __device__ __forceinline__ void energy_direction(const Dev &d, int c, int dir, d
                   d.v_limit, A);
 }
 
+// The energy coefficients of cell c = (i, j, k), handed to put(slot, value)
+// in the slots of Dev::TF (nim.f90:465-502): F4 always, F1..F3 where the +
+// neighbour in that direction exists
+template <class Put>
+__device__ __forceinline__ void energy_coefs(const Dev &d, int c, int i, int j, int k, Put put)
+{
+    double A[3][6];
+#pragma unroll
+    for (int dir = 0; dir < 3; dir++) energy_direction(d, c, dir, A[dir]);
+    double f2 = 0.5 * (1. / A[0][4] + 1. / A[1][4] + 1. / A[2][4] - 1. / d.DT);
+    double f3 = -(A[0][5] / A[0][4]), f4 = (A[0][5] - 1.) / A[0][4];
+    double f5 = -(A[1][5] / A[1][4]), f6 = (A[1][5] - 1.) / A[1][4];
+    double f7 = -(A[2][5] / A[2][4]), f8 = (A[2][5] - 1.) / A[2][4];
+    put(C_F4 + 0, f2 + f3 + f4 + f5 + f6 + f7 + f8);
+    put(C_F4 + 1, f2);
+    put(C_F4 + 2, f3);
+    put(C_F4 + 3, f4);
+    put(C_F4 + 4, f5);
+    put(C_F4 + 5, f6);
+    put(C_F4 + 6, f7);
+    put(C_F4 + 7, f8);
+#pragma unroll
+    for (int dir = 0; dir < 3; dir++) {
+        int c2;
+        if (dir == 0) { if (j == d.NR - 1) continue; c2 = c + d.NT; }
+        else if (dir == 1) c2 = cidx(d, ip1(d, i), j, k);
+        else { if (k == d.NZ - 1) continue; c2 = c + d.NT * d.NR; }
+        double B[6];
+        energy_direction(d, c2, dir, B);
+        double b = A[dir][2] - (A[dir][0] * A[dir][5] / A[dir][4]);
+        double e = B[3] - B[1] / B[4] + B[1] * B[5] / B[4];
+        double h = A[dir][0] / (2. * A[dir][4]);
+        double q = B[1] / (2. * B[4]);
+        const int s = dir == 0 ? C_F1 : dir == 1 ? C_F2 : C_F3;
+        put(s + 0, b + e + 2. * h + 2. * q);
+        put(s + 1, b);
+        put(s + 2, e);
+        put(s + 3, h);
+        put(s + 4, q);
+    }
+}
+
+// apply_zero_flux (bc.f90:345-390) for the adiabatic wall face of cell
+// (i, j, k), k = 0 or NZ - 1: relax it (GS) or write its residual (RES)
+template <int MODE>
+__device__ __forceinline__ void zero_flux_wall(const Dev &d, const View<double> &v, int i, int j, int k, double &eT)
+{
+    double *T = d.fc + (size_t)F_T * d.NF;
+    int b, fw, fa, fb, fcc;
+    if (k == 0) {
+        b = i + d.NT * j;
+        fw = fidx(d, 5, i, j, 0); fa = fidx(d, 6, i, j, 0); fb = fidx(d, 6, i, j, 1); fcc = fidx(d, 6, i, j, 2);
+    } else {
+        b = d.NT * d.NR + i + d.NT * j;
+        fw = fidx(d, 6, i, j, k); fa = fidx(d, 5, i, j, k); fb = fidx(d, 5, i, j, k - 1); fcc = fidx(d, 5, i, j, k - 2);
+    }
+    const double *W = d.wall;
+    const int NB = d.NB;
+    double S = W[(W_ZF + 0) * NB + b] * T[fa] - W[(W_ZF + 1) * NB + b] * T[fb] + W[(W_ZF + 2) * NB + b] * T[fcc];
+    double den = W[(W_ZF + 3) * NB + b];
+    if (MODE == MODE_RES) {
+        // residual of den T_w = S, measured as the change the assignment
+        // would make
+        double r = S - den * T[fw];
+        v.rf[(size_t)F_T * d.NF + fw] = (float)r;
+        eT = fabs(r / den);
+    } else {
+        double nv = S / den;
+        eT = fabs(nv - T[fw]);
+        T[fw] = nv;
+    }
+}
+
 template <int MODE>
 __device__ __forceinline__ void T_coef(const Dev &d, const View<double> &v, int lb)
 {
@@ -164,73 +237,13 @@ __device__ __forceinline__ void T_coef(const Dev &d, const View<double> &v, int 
     double eT = 0.;
     if (act) {
         const int N = d.N;
-        double *F = d.TF;
         int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
-        double A[3][6];
-#pragma unroll
-        for (int dir = 0; dir < 3; dir++) energy_direction(d, c, dir, A[dir]);
-        // F is the FP64 block; RES also keeps an FP32 copy for the corrections
-        auto put = [&](int slot, double x) {
-            F[slot * N + c] = x;
+        // the FP64 block; RES also keeps an FP32 copy for the corrections
+        energy_coefs(d, c, i, j, k, [&](int slot, double x) {
+            d.TF[slot * N + c] = x;
             if (MODE == MODE_RES) v.TF32[slot * N + c] = (float)x;
-        };
-        double f2 = 0.5 * (1. / A[0][4] + 1. / A[1][4] + 1. / A[2][4] - 1. / d.DT);
-        double f3 = -(A[0][5] / A[0][4]), f4 = (A[0][5] - 1.) / A[0][4];
-        double f5 = -(A[1][5] / A[1][4]), f6 = (A[1][5] - 1.) / A[1][4];
-        double f7 = -(A[2][5] / A[2][4]), f8 = (A[2][5] - 1.) / A[2][4];
-        put(C_F4 + 0, f2 + f3 + f4 + f5 + f6 + f7 + f8);
-        put(C_F4 + 1, f2);
-        put(C_F4 + 2, f3);
-        put(C_F4 + 3, f4);
-        put(C_F4 + 4, f5);
-        put(C_F4 + 5, f6);
-        put(C_F4 + 6, f7);
-        put(C_F4 + 7, f8);
-#pragma unroll
-        for (int dir = 0; dir < 3; dir++) {
-            int c2;
-            if (dir == 0) { if (j == d.NR - 1) continue; c2 = c + d.NT; }
-            else if (dir == 1) c2 = cidx(d, ip1(d, i), j, k);
-            else { if (k == d.NZ - 1) continue; c2 = c + d.NT * d.NR; }
-            double B[6];
-            energy_direction(d, c2, dir, B);
-            double b = A[dir][2] - (A[dir][0] * A[dir][5] / A[dir][4]);
-            double e = B[3] - B[1] / B[4] + B[1] * B[5] / B[4];
-            double h = A[dir][0] / (2. * A[dir][4]);
-            double q = B[1] / (2. * B[4]);
-            const int s = dir == 0 ? C_F1 : dir == 1 ? C_F2 : C_F3;
-            put(s + 0, b + e + 2. * h + 2. * q);
-            put(s + 1, b);
-            put(s + 2, e);
-            put(s + 3, h);
-            put(s + 4, q);
-        }
-        if (k == 0 || k == d.NZ - 1) {
-            double *T = d.fc + (size_t)F_T * d.NF;
-            int b, fw, fa, fb, fcc;
-            if (k == 0) {
-                b = i + d.NT * j;
-                fw = fidx(d, 5, i, j, 0); fa = fidx(d, 6, i, j, 0); fb = fidx(d, 6, i, j, 1); fcc = fidx(d, 6, i, j, 2);
-            } else {
-                b = d.NT * d.NR + i + d.NT * j;
-                fw = fidx(d, 6, i, j, k); fa = fidx(d, 5, i, j, k); fb = fidx(d, 5, i, j, k - 1); fcc = fidx(d, 5, i, j, k - 2);
-            }
-            const double *W = d.wall;
-            const int NB = d.NB;
-            double S = W[(W_ZF + 0) * NB + b] * T[fa] - W[(W_ZF + 1) * NB + b] * T[fb] + W[(W_ZF + 2) * NB + b] * T[fcc];
-            double den = W[(W_ZF + 3) * NB + b];
-            if (MODE == MODE_RES) {
-                // residual of den T_w = S, measured as the change the
-                // assignment would make
-                double r = S - den * T[fw];
-                v.rf[(size_t)F_T * d.NF + fw] = (float)r;
-                eT = fabs(r / den);
-            } else {
-                double nv = S / den;
-                eT = fabs(nv - T[fw]);
-                T[fw] = nv;
-            }
-        }
+        });
+        if (k == 0 || k == d.NZ - 1) zero_flux_wall<MODE>(d, v, i, j, k, eT);
     }
     err_commit(d, F_T, eT);
 }
@@ -239,6 +252,32 @@ __global__ void __launch_bounds__(256) k_T_coef(Dev d)
 {
     if (d.ctl->done) return;
     T_coef<MODE_GS>(d, view64(d), xcd_block());
+}
+
+// the adiabatic walls alone (the energy coefficients are then built inside
+// the fused energy sweep)
+__global__ void __launch_bounds__(256) k_zero_flux(Dev d)
+{
+    if (d.ctl->done) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int plane = d.NT * d.NR;
+    double eT = 0.;
+    if (t < 2 * plane) {
+        const int q = t % plane, i = iof(d, q % d.NT), j = q / d.NT, k = t < plane ? 0 : d.NZ - 1;
+        zero_flux_wall<MODE_GS>(d, view64(d), i, j, k, eT);
+    }
+    err_commit(d, F_T, eT);
+}
+
+// the energy coefficients alone at the current state, whatever the loop
+// control says (the adjoint takes them at a converged state)
+__global__ void __launch_bounds__(256) k_energy_coefs(Dev d)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.N) return;
+    const int N = d.N;
+    int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    energy_coefs(d, c, i, j, k, [&](int slot, double x) { d.TF[slot * N + c] = x; });
 }
 
 __global__ void __launch_bounds__(256) k_res_T_coef(Dev d, View<double> v)
@@ -397,10 +436,12 @@ __global__ void __launch_bounds__(256) k_d_T_src(Dev d, View<float> v)
 // carried in registers to the later updates that read them, as the
 // reference's in-place order does. No other cell of the colour writes any
 // location read here, so loading early changes no value.
-// ff is the face field (F_U..F_T), cs the cell slot (0..3).
-template <int MODE, typename T>
+// ff is the face field (F_U..F_T), cs the cell slot (0..3). The
+// coefficients come from the block coef ([NCF][N]) or, with REG, from the
+// caller's registers creg[NCF].
+template <int MODE, typename T, bool REG = false>
 __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, const T *__restrict__ coef, T B, T w,
-                               int i, int j, int k, int c, double &ecell, double &eface)
+                               int i, int j, int k, int c, double &ecell, double &eface, const T *creg = nullptr)
 {
     const int N = d.N;
     T *__restrict__ Fc = vw.fc + (size_t)ff * d.NF;
@@ -424,16 +465,16 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
     constexpr int m0 = sum_diag ? 1 : 0;
     T a[8];
 #pragma unroll
-    for (int m = m0; m < 8; m++) a[m] = coef[(C_F4 + m) * N + c];
+    for (int m = REG ? 0 : m0; m < 8; m++) a[m] = REG ? creg[C_F4 + m] : coef[(C_F4 + m) * N + c];
     T F0 = Fc[f[0]], F1 = Fc[f[1]], F2 = Fc[f[2]], F3 = Fc[f[3]], F4 = Fc[f[4]], F5 = Fc[f[5]];
     T b[5], e[5], h[5];
 #pragma unroll
-    for (int m = m0; m < 5; m++) {
-        b[m] = coef[(C_F1 + m) * N + c];
-        e[m] = coef[(C_F2 + m) * N + c];
-        h[m] = coef[(C_F3 + m) * N + c];
+    for (int m = REG ? 0 : m0; m < 5; m++) {
+        b[m] = REG ? creg[C_F1 + m] : coef[(C_F1 + m) * N + c];
+        e[m] = REG ? creg[C_F2 + m] : coef[(C_F2 + m) * N + c];
+        h[m] = REG ? creg[C_F3 + m] : coef[(C_F3 + m) * N + c];
     }
-    if (sum_diag) {
+    if (sum_diag && !REG) {
         a[0] = a[1] + a[2] + a[3] + a[4] + a[5] + a[6] + a[7];
         b[0] = b[1] + b[2] + T(2) * b[3] + T(2) * b[4];
         e[0] = e[1] + e[2] + T(2) * e[3] + T(2) * e[4];
@@ -602,6 +643,30 @@ __global__ void __launch_bounds__(256) k_T_sweep(Dev d, Sched sc)
     if (d.ctl->done) return;
     View<double> v = view64(d);
     T_sweep<MODE_GS>(d, v, v.TF, sc, xcd_block());
+}
+
+// the energy sweep with its coefficients built in registers from the
+// velocity (T_coef's formulas), for the FP64 loop: the coefficients never
+// go through memory
+__global__ void __launch_bounds__(256) k_T_sweep_fused(Dev d, Sched sc)
+{
+    if (d.ctl->done) return;
+    int i, j, k;
+    bool act = sched_cell(d, sc, xcd_block(), i, j, k);
+    double e = 0.;
+    if (act) {
+        const int N = d.N, c = cidx(d, i, j, k);
+        // slots without a + neighbour stay 0: the unit does not use them
+        double F[NCF];
+#pragma unroll
+        for (int m = 0; m < NCF; m++) F[m] = 0.;
+        energy_coefs(d, c, i, j, k, [&](int slot, double x) { F[slot] = x; });
+        const View<double> v = view64(d);
+        double ec = 0., ef = 0.;
+        transport_unit<MODE_GS, double, true>(d, v, F_T, 3, nullptr, v.src[R_T * N + c], d.w_T, i, j, k, c, ec, ef, F);
+        e = fmax(ec, ef);
+    }
+    err_commit(d, F_T, e);
 }
 
 __global__ void __launch_bounds__(256) k_res_T(Dev d, View<double> v)

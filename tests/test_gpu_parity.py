@@ -158,6 +158,24 @@ def test_fixed_passes_match_oracle_colored(geo, steps):
     ctx.close()
 
 
+def test_fused_energy_sweep_is_bitwise_the_two_kernels(geo, steps, monkeypatch):
+    """NIM_FUSE_T builds the energy coefficients inside the energy sweep
+    (same formulas, same rounding) instead of storing them from a kernel of
+    their own: 500 passes give the same fields bit for bit."""
+    out = []
+    for fuse in ("0", "1"):
+        monkeypatch.setenv("NIM_FUSE_T", fuse)
+        ctx = G.gpu_context(geo)
+        ctx.upload_state(G.state_from_steps(steps, "s3"))
+        ctx.build_momentum_coeffs()
+        rc, it, _ = ctx.inner_solve(np.zeros(5), max_iter=500, raise_on_cap=False)
+        assert it == 500
+        out.append({n: ctx.get(n) for n in ["faces_" + f for f in "uvwpT"] + [f + "xyz" for f in "uvwT"]})
+        ctx.close()
+    for n in out[0]:
+        assert np.array_equal(out[0][n], out[1][n]), n
+
+
 def test_converged_step_vs_reference_order(geo, steps):
     """GPU (4-colour order) against the oracle in the reference's own order,
     one time step from the reference's state after step 3, both converged

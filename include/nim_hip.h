@@ -109,6 +109,19 @@ int32_t nim_ctx_destroy(nim_ctx *ctx);
  * find_Nu2 (nse.f90:2722-2782) from faces_mid. */
 int32_t nim_upload_geometry(nim_ctx *ctx, const nim_geometry *geo);
 
+/* The per-cell metrics of evaluate_all_mesh_constants (geometry.f90:14-32:
+ * create_mesh.f90:254-322, curvilinear.f90:9-160, geometry.f90:76-190,
+ * 395-632) on the current device, from the deformed nodes node_xyz(3,NNODE)
+ * (NNODE = n_theta (n_r+1) (n_z+1), node i + n_theta (j + (n_r+1) k)):
+ * volume(NMESH), xi_x(3,3,NMESH), gij(3,3,NMESH), gij_faces(3,3,6,NMESH),
+ * G1..G3(NMESH), faces_area(NFACE), faces_normal, faces_tangential,
+ * faces_bitangential, faces_mid (3,NFACE), host arrays in the reference
+ * layout as nim_upload_geometry takes them. Context-free. */
+int32_t nim_device_metrics(const nim_dims *dims, const double *node_xyz, double *volume, double *xi_x, double *gij,
+                           double *gij_faces, double *G1, double *G2, double *G3, double *faces_area,
+                           double *faces_normal, double *faces_tangential, double *faces_bitangential,
+                           double *faces_mid);
+
 /* calculate_pressure_nim_constants (nim.f90:556-743), on the device. */
 int32_t nim_build_pressure_coeffs(nim_ctx *ctx);
 

@@ -71,6 +71,55 @@ contains
     end function
   end function
 
+  ! The parts of nimh_build_geometry around the per-cell metrics, for a host
+  ! that evaluates those on the device (nim_device_metrics of libnim_hip.so):
+  ! the deformed nodes (node_xyz(3,NNODE)), the cell/boundary tables, and
+  ! the pressure-wall tables that need the face vectors.
+  integer(c_int32_t) function nimh_nodes(cs, node_xyz) bind(C, name="nimh_nodes") result(rc)
+    type(nim_geo_case), intent(in) :: cs
+    real(c_double), intent(out) :: node_xyz(3, *)
+    real(c_double), allocatable :: nx(:), ny(:), nz(:)
+    integer :: NN
+    NN = cs%n_theta*(cs%n_r+1)*(cs%n_z+1)
+    allocate(nx(NN), ny(NN), nz(NN))
+    call make_nodes(cs, nx, ny, nz)
+    node_xyz(1, 1:NN) = nx; node_xyz(2, 1:NN) = ny; node_xyz(3, 1:NN) = nz
+    rc = 0
+  end function
+
+  integer(c_int32_t) function nimh_mesh_tables(cs, meshes_faces, meshes_neighbors, bc_faces, bc_position, &
+      bc_mesh) bind(C, name="nimh_mesh_tables") result(rc)
+    type(nim_geo_case), intent(in) :: cs
+    integer(c_int32_t), intent(out) :: meshes_faces(6, *), meshes_neighbors(6, *), bc_faces(*), bc_position(*)
+    integer(c_int32_t), intent(out) :: bc_mesh(*)
+    integer :: N, NB
+    N = cs%n_theta*cs%n_r*cs%n_z
+    NB = 2*cs%n_theta*cs%n_r + 2*cs%n_theta*cs%n_z
+    call mesh_tables(cs, meshes_faces(:, 1:N), meshes_neighbors(:, 1:N), bc_faces(1:NB), bc_position(1:NB), &
+                     bc_mesh(1:NB))
+    rc = 0
+  end function
+
+  integer(c_int32_t) function nimh_bc_tables(cs, faces_normal, faces_tangential, faces_bitangential, faces_mid, &
+      meshes_faces, meshes_neighbors, bc_faces, bc_position, bc_mesh, bc_points, bc_a) &
+      bind(C, name="nimh_bc_tables") result(rc)
+    type(nim_geo_case), intent(in) :: cs
+    real(c_double), intent(in) :: faces_normal(3, *), faces_tangential(3, *), faces_bitangential(3, *)
+    real(c_double), intent(in) :: faces_mid(3, *)
+    integer(c_int32_t), intent(in) :: meshes_faces(6, *), meshes_neighbors(6, *), bc_faces(*), bc_position(*)
+    integer(c_int32_t), intent(in) :: bc_mesh(*)
+    integer(c_int32_t), intent(out) :: bc_points(5, *)
+    real(c_double), intent(out) :: bc_a(20, *)
+    integer :: N, NF, NB
+    N = cs%n_theta*cs%n_r*cs%n_z
+    NF = 3*N + cs%n_theta*cs%n_z + cs%n_theta*cs%n_r
+    NB = 2*cs%n_theta*cs%n_r + 2*cs%n_theta*cs%n_z
+    call bc_tables(cs, faces_normal(:, 1:NF), faces_tangential(:, 1:NF), faces_bitangential(:, 1:NF), &
+                   faces_mid(:, 1:NF), meshes_faces(:, 1:N), meshes_neighbors(:, 1:N), bc_faces(1:NB), &
+                   bc_position(1:NB), bc_mesh(1:NB), bc_points(:, 1:NB), bc_a(:, 1:NB))
+    rc = 0
+  end function
+
   ! Initial state: velocities and pressure zero on every face, the Dirichlet
   ! wall temperatures on the hexagon (T_hot) and the cylinder (T_cold)
   ! (boundary_condition_4/5, general.f90:85-117; apply_dirichlet_bc_i returns

@@ -31,7 +31,7 @@ module nim_host_geometry
   use iso_c_binding
   implicit none
   private
-  public :: nim_geo_case, build_geometry, gauss_points
+  public :: nim_geo_case, build_geometry, gauss_points, make_nodes, mesh_tables, bc_tables
 
   integer, parameter :: dp = c_double
 
@@ -345,18 +345,10 @@ contains
 
     allocate(cf_area(6,N), cf_n(3,6,N), cf_t(3,6,N), cf_s(3,6,N), cf_mid(3,6,N))
 
-    !$omp parallel do private(c, i, j, k, p) schedule(static)
+    call mesh_tables(cs, meshes_faces, meshes_neighbors, bc_faces, bc_position, bc_mesh)
+    !$omp parallel do private(c, i, j, k) schedule(static)
     do c = 1, N
       i = mod(c-1, nt); j = mod((c-1)/nt, nr); k = (c-1)/(nt*nr)
-      do p = 1, 6
-        meshes_faces(p, c) = face_number(nt, nr, nz, p, i, j, k)
-      enddo
-      meshes_neighbors(1, c) = merge(c - nt, 0, j > 0)
-      meshes_neighbors(2, c) = merge(c + nt, 0, j < nr-1)
-      meshes_neighbors(3, c) = mod(i-1+nt, nt) + nt*(j + nr*k) + 1
-      meshes_neighbors(4, c) = mod(i+1, nt) + nt*(j + nr*k) + 1
-      meshes_neighbors(5, c) = merge(c - nt*nr, 0, k > 0)
-      meshes_neighbors(6, c) = merge(c + nt*nr, 0, k < nz-1)
       call cell_metrics(c, i, j, k)
     enddo
     !$omp end parallel do
@@ -373,32 +365,8 @@ contains
       enddo
     enddo
 
-    ! boundary-face tables (fill_global_arrays): bottom, top, hexagon, cylinder
-    b = 0
-    do j = 0, nr-1
-      do i = 0, nt-1
-        b = b + 1; bc_position(b) = 5; bc_mesh(b) = i + nt*j + 1
-      enddo
-    enddo
-    do j = 0, nr-1
-      do i = 0, nt-1
-        b = b + 1; bc_position(b) = 6; bc_mesh(b) = i + nt*(j + nr*(nz-1)) + 1
-      enddo
-    enddo
-    do k = 0, nz-1
-      do i = 0, nt-1
-        b = b + 1; bc_position(b) = 1; bc_mesh(b) = i + nt*nr*k + 1
-      enddo
-    enddo
-    do k = 0, nz-1
-      do i = 0, nt-1
-        b = b + 1; bc_position(b) = 2; bc_mesh(b) = i + nt*(nr-1 + nr*k) + 1
-      enddo
-    enddo
-    do b = 1, NB
-      bc_faces(b) = meshes_faces(bc_position(b), bc_mesh(b))
-    enddo
-    call pressure_bc_tables()
+    call bc_tables(cs, faces_normal, faces_tangential, faces_bitangential, faces_mid, meshes_faces, &
+                   meshes_neighbors, bc_faces, bc_position, bc_mesh, bc_points, bc_a)
 
   contains
 
@@ -517,69 +485,128 @@ contains
       enddo
     end subroutine
 
-    ! evaluate_pressure_bc_array (geometry.f90:879-975): 5-point stencil
-    ! normal to each wall and its finite-difference weights
-    subroutine pressure_bc_tables()
-      integer :: bb, M1, M2, M3, M4, fn, q
-      real(dp) :: n(5), t(5), s(5), nv(3), tv(3), sv(3), dd(3), D
-      real(dp), allocatable :: A(:)
-      allocate(A(20))
-      do bb = 1, NB
-        nv = faces_normal(:, bc_faces(bb)); tv = faces_tangential(:, bc_faces(bb)); sv = faces_bitangential(:, bc_faces(bb))
-        M1 = bc_mesh(bb)
-        bc_points(1, bb) = bc_faces(bb)
-        select case (bc_position(bb))
-        case (1, 2)
-          fn = 3 - bc_position(bb)
-          M2 = meshes_neighbors(fn, M1)
-          M3 = meshes_neighbors(4, M1); if (M3 == 0) M3 = meshes_neighbors(3, M1)
-          M4 = meshes_neighbors(6, M1); if (M4 == 0) M4 = meshes_neighbors(5, M1)
-        case (3, 4)
-          fn = 7 - bc_position(bb)
-          M2 = meshes_neighbors(fn, M1)
-          M3 = meshes_neighbors(2, M1); if (M3 == 0) M3 = meshes_neighbors(1, M1)
-          M4 = meshes_neighbors(6, M1); if (M4 == 0) M4 = meshes_neighbors(5, M1)
-        case default
-          fn = 11 - bc_position(bb)
-          M2 = meshes_neighbors(fn, M1)
-          M3 = meshes_neighbors(2, M1); if (M3 == 0) M3 = meshes_neighbors(1, M1)
-          M4 = meshes_neighbors(4, M1); if (M4 == 0) M4 = meshes_neighbors(3, M1)
-        end select
-        bc_points(2, bb) = meshes_faces(fn, M1)
-        bc_points(3, bb) = meshes_faces(fn, M2)
-        bc_points(4, bb) = meshes_faces(fn, M3)
-        bc_points(5, bb) = meshes_faces(fn, M4)
-        do q = 1, 5
-          dd = faces_mid(:, bc_points(q, bb)) - faces_mid(:, bc_points(1, bb))
-          n(q) = dot3(dd, nv); t(q) = dot3(dd, tv); s(q) = dot3(dd, sv)
-        enddo
-        D = (n(3)*s(4)*t(2) - n(3)*s(5)*t(2) - n(2)*s(4)*t(3) + n(2)*s(5)*t(3) - n(3)*s(2)*t(4) + n(2)*s(3)*t(4) &
-             - n(2)*s(5)*t(4) + n(3)*s(5)*t(4) + n(5)*s(3)*t(2) - n(5)*s(4)*t(2) - n(5)*s(2)*t(3) + n(5)*s(4)*t(3) &
-             + n(5)*s(2)*t(4) - n(5)*s(3)*t(4) + n(3)*s(2)*t(5) - n(2)*s(3)*t(5) + n(2)*s(4)*t(5) - n(3)*s(4)*t(5) &
-             + n(4)*(s(5)*t(2) + s(2)*t(3) - s(5)*t(3) - s(2)*t(5) + s(3)*(-t(2) + t(5))))
-        A(2)=2*(-n(5)*s(4)*t(3) + n(4)*s(5)*t(3) + n(5)*s(3)*t(4) - n(3)*s(5)*t(4) - n(4)*s(3)*t(5) + n(3)*s(4)*t(5))/(n(2)*n(2)*D)
-        A(3)=2*( n(5)*s(4)*t(2) - n(4)*s(5)*t(2) - n(5)*s(2)*t(4) + n(2)*s(5)*t(4) + n(4)*s(2)*t(5) - n(2)*s(4)*t(5))/(n(3)*n(3)*D)
-        A(4)=2*( n(5)*s(3)*t(2) - n(3)*s(5)*t(2) - n(5)*s(2)*t(3) + n(2)*s(5)*t(3) + n(3)*s(2)*t(5) - n(2)*s(3)*t(5))/(n(4)*n(4)*D)
-        A(5)=2*( n(4)*s(3)*t(2) - n(3)*s(4)*t(2) - n(4)*s(2)*t(3) + n(2)*s(4)*t(3) + n(3)*s(2)*t(4) - n(2)*s(3)*t(4))/(n(5)*n(5)*D)
-        A(1)=-A(2)-A(3)-A(4)-A(5)
-        A(6)=-A(2)*n(2)-A(3)*n(3)-A(4)*n(4)-A(5)*n(5)
-        A(7)=-A(2)*t(2)-A(3)*t(3)-A(4)*t(4)-A(5)*t(5)
-        A(8)=-A(2)*s(2)-A(3)*s(3)-A(4)*s(4)-A(5)*s(5)
-        A(9)=-A(2)*n(2)*t(2)-A(3)*n(3)*t(3)-A(4)*n(4)*t(4)-A(5)*n(5)*t(5)
-        A(10)=-A(2)*n(2)*s(2)-A(3)*n(3)*s(3)-A(4)*n(4)*s(4)-A(5)*n(5)*s(5)
-        A(11)=-A(2)*s(2)*t(2)-A(3)*s(3)*t(3)-A(4)*s(4)*t(4)-A(5)*s(5)*t(5)
-        A(12)=-A(2)*(t(2)*t(2))/2._dp-A(3)*(t(3)*t(3))/2._dp-A(4)*(t(4)*t(4))/2._dp-A(5)*(t(5)*t(5))/2._dp
-        A(13)=-A(2)*(s(2)*s(2))/2._dp-A(3)*(s(3)*s(3))/2._dp-A(4)*(s(4)*s(4))/2._dp-A(5)*(s(5)*s(5))/2._dp
-        A(14)=-A(2)*n(2)*(t(2)*t(2))/2._dp-A(3)*n(3)*(t(3)*t(3))/2._dp-A(4)*n(4)*(t(4)*t(4))/2._dp-A(5)*n(5)*(t(5)*t(5))/2._dp
-        A(15)=-A(2)*n(2)*(s(2)*s(2))/2._dp-A(3)*n(3)*(s(3)*s(3))/2._dp-A(4)*n(4)*(s(4)*s(4))/2._dp-A(5)*n(5)*(s(5)*s(5))/2._dp
-        A(16)=-A(2)*s(2)*(t(2)*t(2))/2._dp-A(3)*s(3)*(t(3)*t(3))/2._dp-A(4)*s(4)*(t(4)*t(4))/2._dp-A(5)*s(5)*(t(5)*t(5))/2._dp
-        A(17)=-A(2)*t(2)*(s(2)*s(2))/2._dp-A(3)*t(3)*(s(3)*s(3))/2._dp-A(4)*t(4)*(s(4)*s(4))/2._dp-A(5)*t(5)*(s(5)*s(5))/2._dp
-        A(18)=-A(2)*n(2)*t(2)*s(2)-A(3)*n(3)*t(3)*s(3)-A(4)*n(4)*t(4)*s(4)-A(5)*n(5)*t(5)*s(5)
-        A(19)=-A(2)*t(2)**3._dp/6._dp-A(3)*t(3)**3._dp/6._dp-A(4)*t(4)**3._dp/6._dp-A(5)*t(5)**3._dp/6._dp
-        A(20)=-A(2)*s(2)**3._dp/6._dp-A(3)*s(3)**3._dp/6._dp-A(4)*s(4)**3._dp/6._dp-A(5)*s(5)**3._dp/6._dp
-        bc_a(:, bb) = A
-      enddo
-    end subroutine
   end subroutine build_geometry
+
+  ! mesh_tables: the face numbers and neighbours of every cell (create_hollow,
+  ! create_geometry.f90:449-664; neighbour 0 = wall) and the boundary-face
+  ! tables of fill_global_arrays (geometry.f90:978-1032) in the boundary
+  ! group order bottom, top, hexagon, cylinder
+  subroutine mesh_tables(cs, meshes_faces, meshes_neighbors, bc_faces, bc_position, bc_mesh)
+    type(nim_geo_case), intent(in) :: cs
+    integer(c_int32_t), intent(out) :: meshes_faces(:,:), meshes_neighbors(:,:), bc_faces(:), bc_position(:)
+    integer(c_int32_t), intent(out) :: bc_mesh(:)
+    integer :: nt, nr, nz, N, NB, c, i, j, k, p, b
+    nt = cs%n_theta; nr = cs%n_r; nz = cs%n_z
+    N = nt*nr*nz
+    NB = 2*nt*nr + 2*nt*nz
+    do c = 1, N
+      i = mod(c-1, nt); j = mod((c-1)/nt, nr); k = (c-1)/(nt*nr)
+      do p = 1, 6
+        meshes_faces(p, c) = face_number(nt, nr, nz, p, i, j, k)
+      enddo
+      meshes_neighbors(1, c) = merge(c - nt, 0, j > 0)
+      meshes_neighbors(2, c) = merge(c + nt, 0, j < nr-1)
+      meshes_neighbors(3, c) = mod(i-1+nt, nt) + nt*(j + nr*k) + 1
+      meshes_neighbors(4, c) = mod(i+1, nt) + nt*(j + nr*k) + 1
+      meshes_neighbors(5, c) = merge(c - nt*nr, 0, k > 0)
+      meshes_neighbors(6, c) = merge(c + nt*nr, 0, k < nz-1)
+    enddo
+    b = 0
+    do j = 0, nr-1
+      do i = 0, nt-1
+        b = b + 1; bc_position(b) = 5; bc_mesh(b) = i + nt*j + 1
+      enddo
+    enddo
+    do j = 0, nr-1
+      do i = 0, nt-1
+        b = b + 1; bc_position(b) = 6; bc_mesh(b) = i + nt*(j + nr*(nz-1)) + 1
+      enddo
+    enddo
+    do k = 0, nz-1
+      do i = 0, nt-1
+        b = b + 1; bc_position(b) = 1; bc_mesh(b) = i + nt*nr*k + 1
+      enddo
+    enddo
+    do k = 0, nz-1
+      do i = 0, nt-1
+        b = b + 1; bc_position(b) = 2; bc_mesh(b) = i + nt*(nr-1 + nr*k) + 1
+      enddo
+    enddo
+    do b = 1, NB
+      bc_faces(b) = meshes_faces(bc_position(b), bc_mesh(b))
+    enddo
+  end subroutine mesh_tables
+
+  ! evaluate_pressure_bc_array (geometry.f90:879-975): 5-point stencil
+  ! normal to each wall and its finite-difference weights
+  subroutine bc_tables(cs, faces_normal, faces_tangential, faces_bitangential, faces_mid, meshes_faces, &
+                       meshes_neighbors, bc_faces, bc_position, bc_mesh, bc_points, bc_a)
+    type(nim_geo_case), intent(in) :: cs
+    real(dp), intent(in) :: faces_normal(:,:), faces_tangential(:,:), faces_bitangential(:,:), faces_mid(:,:)
+    integer(c_int32_t), intent(in) :: meshes_faces(:,:), meshes_neighbors(:,:), bc_faces(:), bc_position(:)
+    integer(c_int32_t), intent(in) :: bc_mesh(:)
+    integer(c_int32_t), intent(out) :: bc_points(:,:)
+    real(dp), intent(out) :: bc_a(:,:)
+    integer :: bb, M1, M2, M3, M4, fn, q, NB
+    real(dp) :: n(5), t(5), s(5), nv(3), tv(3), sv(3), dd(3), D
+    real(dp), allocatable :: A(:)
+    allocate(A(20))
+    NB = 2*cs%n_theta*cs%n_r + 2*cs%n_theta*cs%n_z
+    do bb = 1, NB
+      nv = faces_normal(:, bc_faces(bb)); tv = faces_tangential(:, bc_faces(bb)); sv = faces_bitangential(:, bc_faces(bb))
+      M1 = bc_mesh(bb)
+      bc_points(1, bb) = bc_faces(bb)
+      select case (bc_position(bb))
+      case (1, 2)
+        fn = 3 - bc_position(bb)
+        M2 = meshes_neighbors(fn, M1)
+        M3 = meshes_neighbors(4, M1); if (M3 == 0) M3 = meshes_neighbors(3, M1)
+        M4 = meshes_neighbors(6, M1); if (M4 == 0) M4 = meshes_neighbors(5, M1)
+      case (3, 4)
+        fn = 7 - bc_position(bb)
+        M2 = meshes_neighbors(fn, M1)
+        M3 = meshes_neighbors(2, M1); if (M3 == 0) M3 = meshes_neighbors(1, M1)
+        M4 = meshes_neighbors(6, M1); if (M4 == 0) M4 = meshes_neighbors(5, M1)
+      case default
+        fn = 11 - bc_position(bb)
+        M2 = meshes_neighbors(fn, M1)
+        M3 = meshes_neighbors(2, M1); if (M3 == 0) M3 = meshes_neighbors(1, M1)
+        M4 = meshes_neighbors(4, M1); if (M4 == 0) M4 = meshes_neighbors(3, M1)
+      end select
+      bc_points(2, bb) = meshes_faces(fn, M1)
+      bc_points(3, bb) = meshes_faces(fn, M2)
+      bc_points(4, bb) = meshes_faces(fn, M3)
+      bc_points(5, bb) = meshes_faces(fn, M4)
+      do q = 1, 5
+        dd = faces_mid(:, bc_points(q, bb)) - faces_mid(:, bc_points(1, bb))
+        n(q) = dot3(dd, nv); t(q) = dot3(dd, tv); s(q) = dot3(dd, sv)
+      enddo
+      D = (n(3)*s(4)*t(2) - n(3)*s(5)*t(2) - n(2)*s(4)*t(3) + n(2)*s(5)*t(3) - n(3)*s(2)*t(4) + n(2)*s(3)*t(4) &
+           - n(2)*s(5)*t(4) + n(3)*s(5)*t(4) + n(5)*s(3)*t(2) - n(5)*s(4)*t(2) - n(5)*s(2)*t(3) + n(5)*s(4)*t(3) &
+           + n(5)*s(2)*t(4) - n(5)*s(3)*t(4) + n(3)*s(2)*t(5) - n(2)*s(3)*t(5) + n(2)*s(4)*t(5) - n(3)*s(4)*t(5) &
+           + n(4)*(s(5)*t(2) + s(2)*t(3) - s(5)*t(3) - s(2)*t(5) + s(3)*(-t(2) + t(5))))
+      A(2)=2*(-n(5)*s(4)*t(3) + n(4)*s(5)*t(3) + n(5)*s(3)*t(4) - n(3)*s(5)*t(4) - n(4)*s(3)*t(5) + n(3)*s(4)*t(5))/(n(2)*n(2)*D)
+      A(3)=2*( n(5)*s(4)*t(2) - n(4)*s(5)*t(2) - n(5)*s(2)*t(4) + n(2)*s(5)*t(4) + n(4)*s(2)*t(5) - n(2)*s(4)*t(5))/(n(3)*n(3)*D)
+      A(4)=2*( n(5)*s(3)*t(2) - n(3)*s(5)*t(2) - n(5)*s(2)*t(3) + n(2)*s(5)*t(3) + n(3)*s(2)*t(5) - n(2)*s(3)*t(5))/(n(4)*n(4)*D)
+      A(5)=2*( n(4)*s(3)*t(2) - n(3)*s(4)*t(2) - n(4)*s(2)*t(3) + n(2)*s(4)*t(3) + n(3)*s(2)*t(4) - n(2)*s(3)*t(4))/(n(5)*n(5)*D)
+      A(1)=-A(2)-A(3)-A(4)-A(5)
+      A(6)=-A(2)*n(2)-A(3)*n(3)-A(4)*n(4)-A(5)*n(5)
+      A(7)=-A(2)*t(2)-A(3)*t(3)-A(4)*t(4)-A(5)*t(5)
+      A(8)=-A(2)*s(2)-A(3)*s(3)-A(4)*s(4)-A(5)*s(5)
+      A(9)=-A(2)*n(2)*t(2)-A(3)*n(3)*t(3)-A(4)*n(4)*t(4)-A(5)*n(5)*t(5)
+      A(10)=-A(2)*n(2)*s(2)-A(3)*n(3)*s(3)-A(4)*n(4)*s(4)-A(5)*n(5)*s(5)
+      A(11)=-A(2)*s(2)*t(2)-A(3)*s(3)*t(3)-A(4)*s(4)*t(4)-A(5)*s(5)*t(5)
+      A(12)=-A(2)*(t(2)*t(2))/2._dp-A(3)*(t(3)*t(3))/2._dp-A(4)*(t(4)*t(4))/2._dp-A(5)*(t(5)*t(5))/2._dp
+      A(13)=-A(2)*(s(2)*s(2))/2._dp-A(3)*(s(3)*s(3))/2._dp-A(4)*(s(4)*s(4))/2._dp-A(5)*(s(5)*s(5))/2._dp
+      A(14)=-A(2)*n(2)*(t(2)*t(2))/2._dp-A(3)*n(3)*(t(3)*t(3))/2._dp-A(4)*n(4)*(t(4)*t(4))/2._dp-A(5)*n(5)*(t(5)*t(5))/2._dp
+      A(15)=-A(2)*n(2)*(s(2)*s(2))/2._dp-A(3)*n(3)*(s(3)*s(3))/2._dp-A(4)*n(4)*(s(4)*s(4))/2._dp-A(5)*n(5)*(s(5)*s(5))/2._dp
+      A(16)=-A(2)*s(2)*(t(2)*t(2))/2._dp-A(3)*s(3)*(t(3)*t(3))/2._dp-A(4)*s(4)*(t(4)*t(4))/2._dp-A(5)*s(5)*(t(5)*t(5))/2._dp
+      A(17)=-A(2)*t(2)*(s(2)*s(2))/2._dp-A(3)*t(3)*(s(3)*s(3))/2._dp-A(4)*t(4)*(s(4)*s(4))/2._dp-A(5)*t(5)*(s(5)*s(5))/2._dp
+      A(18)=-A(2)*n(2)*t(2)*s(2)-A(3)*n(3)*t(3)*s(3)-A(4)*n(4)*t(4)*s(4)-A(5)*n(5)*t(5)*s(5)
+      A(19)=-A(2)*t(2)**3._dp/6._dp-A(3)*t(3)**3._dp/6._dp-A(4)*t(4)**3._dp/6._dp-A(5)*t(5)**3._dp/6._dp
+      A(20)=-A(2)*s(2)**3._dp/6._dp-A(3)*s(3)**3._dp/6._dp-A(4)*s(4)**3._dp/6._dp-A(5)*s(5)**3._dp/6._dp
+      bc_a(:, bb) = A
+    enddo
+  end subroutine bc_tables
 
 end module nim_host_geometry

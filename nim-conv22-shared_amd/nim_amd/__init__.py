@@ -143,6 +143,8 @@ def lib():
         L.nim_adj_get.argtypes = [vp, ctypes.c_char_p, _P, ctypes.c_int64]
         L.nim_adj_set.restype = i32
         L.nim_adj_set.argtypes = [vp, ctypes.c_char_p, _P, ctypes.c_int64]
+        L.nim_device_metrics.restype = i32
+        L.nim_device_metrics.argtypes = [ctypes.POINTER(Dims), _P] + [_P] * 12
         L.nim_last_error.restype = ctypes.c_char_p
         L.nim_last_error.argtypes = []
         _lib = L
@@ -159,7 +161,7 @@ def exported_symbols():
             "nim_ens_inner_solve", "nim_revolve_plan", "nim_ckpt_create", "nim_ckpt_destroy", "nim_ckpt_store",
             "nim_ckpt_restore", "nim_adj_create", "nim_adj_destroy", "nim_adj_seed_nusselt", "nim_adj_reverse_step",
             "nim_adj_gradient", "nim_adj_reverse_pass", "nim_adj_reverse_stage", "nim_adj_clear", "nim_adj_get",
-            "nim_adj_set", "nim_last_error"]
+            "nim_adj_set", "nim_device_metrics", "nim_last_error"]
 
 
 def _check(rc):

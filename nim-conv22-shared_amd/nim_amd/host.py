@@ -130,15 +130,26 @@ def lib():
         L.nimh_initial_state.restype = ctypes.c_int32
         L.nimh_initial_state.argtypes = [ctypes.POINTER(GeoCase), ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_int64] + [P] * 9
+        L.nimh_nodes.restype = ctypes.c_int32
+        L.nimh_nodes.argtypes = [ctypes.POINTER(GeoCase), P]
+        L.nimh_mesh_tables.restype = ctypes.c_int32
+        L.nimh_mesh_tables.argtypes = [ctypes.POINTER(GeoCase)] + [PI] * 5
+        L.nimh_bc_tables.restype = ctypes.c_int32
+        L.nimh_bc_tables.argtypes = [ctypes.POINTER(GeoCase)] + [P] * 4 + [PI] * 6 + [P]
         L.nimh_noise.restype = ctypes.c_double
         L.nimh_noise.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.c_int64]
         _lib = L
     return _lib
 
 
-def build_geometry(case):
+def build_geometry(case, device=False):
     """Every geometry array of the case, reference layout (1-based integer
-    tables)."""
+    tables). With device=True the per-cell metrics (volumes, metric
+    tensors, face vectors and areas, about 400 map evaluations per cell) run
+    on the GPU (nim_device_metrics) and the host computes only the nodes and
+    the tables; the arrays are the same bit for bit."""
+    if device:
+        return _build_geometry_device(case)
     nt, nr, nz = case.dims
     N = nt * nr * nz
     NF = 3 * N + nt * nz + nt * nr
@@ -161,6 +172,41 @@ def build_geometry(case):
     rc = lib().nimh_build_geometry(ctypes.byref(gc), *args)
     if rc != 0:
         raise ValueError("nimh_build_geometry rejected the case")
+    return g
+
+
+def _build_geometry_device(case):
+    from . import lib as hip_lib, _check
+    nt, nr, nz = case.dims
+    N = nt * nr * nz
+    NF = 3 * N + nt * nz + nt * nr
+    NB = 2 * nt * nr + 2 * nt * nz
+    NN = nt * (nr + 1) * (nz + 1)
+    f = lambda n: np.zeros(n)
+    i = lambda n: np.zeros(n, dtype=np.int32)
+    g = dict(volume=f(N), xi_x=f(9 * N), gij=f(9 * N), gij_faces=f(54 * N), G1=f(N), G2=f(N), G3=f(N),
+             faces_area=f(NF), faces_normal=f(3 * NF), faces_tangential=f(3 * NF), faces_bitangential=f(3 * NF),
+             faces_mid=f(3 * NF), meshes_faces=i(6 * N), meshes_neighbors=i(6 * N), bc_faces=i(NB),
+             bc_position=i(NB), bc_mesh=i(NB), bc_points=i(5 * NB), bc_a=f(20 * NB), node_xyz=f(3 * NN))
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32 if a.dtype == np.int32 else ctypes.c_double))
+    gc = case.geo_case()
+    L = lib()
+    if L.nimh_nodes(ctypes.byref(gc), P(g["node_xyz"])) != 0:
+        raise ValueError("nimh_nodes rejected the case")
+    if L.nimh_mesh_tables(ctypes.byref(gc), *[P(g[n]) for n in ("meshes_faces", "meshes_neighbors", "bc_faces",
+                                                                 "bc_position", "bc_mesh")]) != 0:
+        raise ValueError("nimh_mesh_tables rejected the case")
+    from . import Dims
+    H = hip_lib()
+    _check(H.nim_device_metrics(ctypes.byref(Dims(nt, nr, nz)), P(g["node_xyz"]),
+                                *[P(g[n]) for n in ("volume", "xi_x", "gij", "gij_faces", "G1", "G2", "G3",
+                                                    "faces_area", "faces_normal", "faces_tangential",
+                                                    "faces_bitangential", "faces_mid")]))
+    if L.nimh_bc_tables(ctypes.byref(gc), *[P(g[n]) for n in ("faces_normal", "faces_tangential",
+                                                               "faces_bitangential", "faces_mid", "meshes_faces",
+                                                               "meshes_neighbors", "bc_faces", "bc_position",
+                                                               "bc_mesh", "bc_points", "bc_a")]) != 0:
+        raise ValueError("nimh_bc_tables rejected the case")
     return g
 
 

@@ -884,7 +884,10 @@ static double bits_to_double(unsigned long long b)
     return x;
 }
 
+// grids of the grid-stride loops (bookkeeping kernels) and of the one
+// thread per cell kernels
 static int grid_for(long n) { return (int)std::min<long>((n + BSA - 1) / BSA, 8192); }
+static int cell_grid(long n) { return (int)((n + BSA - 1) / BSA); }
 
 extern "C" int32_t nim_adj_create(nim_ctx *ctx, nim_adj **out)
 {
@@ -955,7 +958,7 @@ static int32_t adj_check(const char *what)
 static void reverse_pass(nim_adj *A, hipStream_t s)
 {
     const Dev &d = *nim_internal_dev(A->ctx);
-    const int gN = grid_for(d.N);
+    const int gN = cell_grid(d.N);
     const int per_color = (d.NT / 2) * ((d.NR + 1) / 2) * d.NZ;
     const int gC = (per_color + BSA - 1) / BSA;
     hipLaunchKernelGGL(k_adj_begin, dim3(grid_for(std::max<long>(NCF * (long)d.N, 5 * (long)d.NF))), dim3(BSA), 0, s, d,
@@ -1034,11 +1037,11 @@ extern "C" int32_t nim_adj_reverse_step(nim_adj *A, double tol, int32_t max_pass
     if (passes) *passes = it;
     if (delta) *delta = lm > 0. ? dm / lm : dm;
     // parameter adjoints of the last pass, then the momentum coefficients
-    hipLaunchKernelGGL(k_adj_hex, dim3(grid_for((long)d.NT * d.NZ)), dim3(BSA), 0, s, d, A->a, A->theta);
+    hipLaunchKernelGGL(k_adj_hex, dim3(cell_grid((long)d.NT * d.NZ)), dim3(BSA), 0, s, d, A->a, A->theta);
     hipLaunchKernelGGL(k_adj_par_add, dim3(1), dim3(64), 0, s, A->theta, A->a.par);
     if (hipMemsetAsync(A->a.ltc, 0, 3 * (size_t)d.N * sizeof(double), s) != hipSuccess)
         return nim_internal_fail(NIM_E_HIP, "adjoint clear failed");
-    hipLaunchKernelGGL(k_adj_coef, dim3(grid_for(d.N)), dim3(BSA), 0, s, d, A->a, 0, A->theta, (int)P_NU);
+    hipLaunchKernelGGL(k_adj_coef, dim3(cell_grid(d.N)), dim3(BSA), 0, s, d, A->a, 0, A->theta, (int)P_NU);
     hipLaunchKernelGGL(k_adj_carry, dim3(grid_for(nl)), dim3(BSA), 0, s, d, A->a, A->cbar, A->tcbar, A->fbar);
     int32_t rc = adj_check("adjoint step launch failed");
     if (rc) return rc;
@@ -1078,7 +1081,7 @@ extern "C" int32_t nim_adj_reverse_stage(nim_adj *A, int32_t stage)
     if (!A || stage < 0 || stage > 7) return nim_internal_fail(NIM_E_ARG, "bad reverse-stage arguments");
     const Dev &d = *nim_internal_dev(A->ctx);
     hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
-    const int gN = grid_for(d.N);
+    const int gN = cell_grid(d.N);
     const int gC = ((d.NT / 2) * ((d.NR + 1) / 2) * d.NZ + BSA - 1) / BSA;
     switch (stage) {
     case 0: hipLaunchKernelGGL(k_adj_mom_src, dim3(gN), dim3(BSA), 0, s, d, A->a); break;

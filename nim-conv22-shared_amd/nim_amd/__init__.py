@@ -575,6 +575,8 @@ def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, 
     current state (as run_steps) and dJ/dT_hot, dJ/dRa by the discrete
     adjoint, the run reversed with nslots device snapshots (binomial plan).
     case supplies Ra, Pr and nu. Returns (J, {"T_hot", "Ra"}, info)."""
+    if nsteps < 1 or nslots < 1:
+        raise ValueError("nsteps and nslots must be at least 1")
     w = np.asarray(weights, dtype=np.float64)
     adj = Adjoint(ctx)
     out = {}

@@ -53,3 +53,23 @@ def test_unsupported_dimensions_rejected():
     import pytest
     with pytest.raises(nim_amd.NimError):
         nim_amd.counts(35, 6, 12)  # odd azimuthal count cannot be 4-coloured
+
+
+def test_argument_errors_before_any_device_use():
+    """Entry points validate their arguments before they touch a device, so
+    these run without one and return NIM_E_ARG with a message."""
+    L = nim_amd.lib()
+    out = ctypes.c_void_p()
+    assert L.nim_adj_create(None, ctypes.byref(out)) == nim_amd.NIM_E_ARG
+    assert L.nim_adj_reverse_step(None, 1e-10, 10, None, None) == nim_amd.NIM_E_ARG
+    assert L.nim_adj_destroy(None) == nim_amd.NIM_OK
+    assert b"reverse-step" in L.nim_last_error()
+    a = np.zeros(8)
+    P = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    bad = nim_amd.Dims(35, 6, 12)
+    assert L.nim_device_metrics(ctypes.byref(bad), P, *[P] * 12) == nim_amd.NIM_E_ARG
+    assert L.nim_device_metrics(ctypes.byref(nim_amd.Dims(36, 6, 12)), None, *[P] * 12) == nim_amd.NIM_E_ARG
+    codes = np.zeros(4, dtype=np.int32)
+    n, fwd = ctypes.c_int64(), ctypes.c_int64()
+    PI = codes.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    assert L.nim_revolve_plan(0, 2, PI, PI, 4, ctypes.byref(n), ctypes.byref(fwd)) == nim_amd.NIM_E_ARG

@@ -59,15 +59,26 @@ struct Adj {
     double *lold;  // [4][N]   previous time level
     double *ltc;   // [3][N]   time-centred velocity of the step
     double *par;   // [NPAR]   scalar parameters, this pass
+    double *scr;   // [10][N]  per-cell factors of the pressure sources
 };
 
 __device__ __forceinline__ void aadd(double *p, double v) { atomicAdd(p, v); }
 
 // sum over the block's waves into *p (every thread of the block calls it)
+// (the sum over the block, one atomic per block: every thread of the block
+// calls it, at most once per kernel; one address taking an atomic per wave
+// of a million-thread launch serialises for hundreds of microseconds)
 __device__ void par_commit(double *p, double v)
 {
+    __shared__ double part[BSA / 64];
     v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0 && v != 0.) atomicAdd(p, v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
+        if (t != 0.) atomicAdd(p, t);
+    }
 }
 
 // adjoints of the six faces of one unit: the faces the unit writes hold the
@@ -134,43 +145,6 @@ __device__ void tder_add(const Dev &d, double *L, OwnFaces *o, int fpos, int i, 
         if (o) o->add(L, a[m], v * w[m]);
         else aadd(L + a[m], v * w[m]);
     }
-}
-
-// reverse of out = coef * X(F), X the cross-diffusion operator
-// (cross_diffusion): lo * coef * dX/dF into L; returns X
-__device__ double cross_diffusion_vjp(const Dev &d, const double *F, double *L, int i, int j, int k, int c,
-                                      double coef, double lo)
-{
-    const int N = d.N;
-    const double *g = d.gc;
-    const double X = cross_diffusion(d, g, F, i, j, k, c, 1.0);
-    const double s = lo * coef;
-    if (s == 0.) return X;
-    const double G0 = g[(G_G + 0) * N + c], G1 = g[(G_G + 1) * N + c], G2 = g[(G_G + 2) * N + c];
-    aadd(L + fidx(d, 2, i, j, k), s * G0 / 2.);
-    aadd(L + fidx(d, 1, i, j, k), -s * G0 / 2.);
-    aadd(L + fidx(d, 4, i, j, k), s * G1 / 2.);
-    aadd(L + fidx(d, 3, i, j, k), -s * G1 / 2.);
-    aadd(L + fidx(d, 6, i, j, k), s * G2 / 2.);
-    aadd(L + fidx(d, 5, i, j, k), -s * G2 / 2.);
-    // - GO0 fxy - GO1 fyx - GO2 fxz - GO3 fzx - GO4 fyz - GO5 fzy,
-    // f_ab = (tder(F, hi face, dir) - tder(F, lo face, dir)) / 2
-    const int hiF[6] = {2, 4, 2, 6, 4, 6}, loF[6] = {1, 3, 1, 5, 3, 5}, dir[6] = {0, 1, 2, 1, 2, 0};
-    for (int m = 0; m < 6; m++) {
-        const double v = -s * g[(G_GO + m) * N + c] * 0.5;
-        tder_add(d, L, nullptr, hiF[m], i, j, k, dir[m], v);
-        tder_add(d, L, nullptr, loF[m], i, j, k, dir[m], -v);
-    }
-    return X;
-}
-
-// reverse of out = dphi(An, phi, f, comp, invV): lo * d out / d phi into L
-__device__ void dphi_vjp(const Dev &d, const int f[6], int comp, double invV, double lo, double *L)
-{
-    const double *A = d.An + (size_t)comp * d.NF;
-    const double s = lo * invV;
-    if (s == 0.) return;
-    for (int p = 0; p < 6; p++) aadd(L + f[p], (p & 1 ? s : -s) * A[f[p]]);
 }
 
 // ---------------------------------------------------------------------
@@ -464,51 +438,184 @@ __global__ void __launch_bounds__(BSA) k_adj_ppe_sweep(Dev d, Adj a, int color)
 }
 
 // ---------------------------------------------------------------------
-// sources
+// sources: the transposed stencils gathered per face (no atomics)
 // ---------------------------------------------------------------------
-// update_velocity_sources (mom_src): R = R_D + b + dphi(p) / rho
-__global__ void __launch_bounds__(BSA) k_adj_mom_src(Dev d, Adj a)
+// The cells along a non-periodic direction of n cells whose tder stencil
+// (three_point: central (1, -1, 0) / 4 on (x + 1, x - 1), low wall
+// (-3, 4, -1) / 4 on (0, 1, 2), high wall (3, -4, 1) / 4 on (n - 1, n - 2,
+// n - 3)) reads the value at index x, and the weights they read it with.
+__device__ __forceinline__ int tder_readers(int x, int n, int r[4], double w[4])
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    double prinv = 0.;
-    if (c < d.N) {
-        const int N = d.N;
-        int i, j, k;
-        cell_ijk(d, c, i, j, k);
-        int f[6];
-        cell_faces(d, i, j, k, f);
-        const double invV = d.gc[G_INVV * N + c];
-        for (int q = 0; q < 3; q++) {
-            const double l = a.lsrc[(R_U + q) * N + c];
-            a.lsrc[(R_U + q) * N + c] = 0.;
-            if (l == 0.) continue;
-            prinv += l * cross_diffusion_vjp(d, d.fc + (size_t)q * d.NF, a.lf + (size_t)q * d.NF, i, j, k, c,
-                                             d.Rinv, l);
-            dphi_vjp(d, f, q, invV, l * d.rinv, a.lf + (size_t)F_P * d.NF);
-            if (q == 2) {
-                const double s = -(d.g_beta * 0.5) * l;
-                a.lc[3 * N + c] += s;
-                a.lold[3 * N + c] += s;
+    int m = 0;
+    if (x - 1 >= 1 && x - 1 <= n - 2) { r[m] = x - 1; w[m++] = 0.25; }
+    if (x + 1 >= 1 && x + 1 <= n - 2) { r[m] = x + 1; w[m++] = -0.25; }
+    if (x <= 2) { r[m] = 0; w[m++] = (x == 0 ? -3. : x == 1 ? 4. : -1.) / 4.; }
+    if (x >= n - 3) { r[m] = n - 1; w[m++] = (x == n - 1 ? 3. : x == n - 2 ? -4. : 1.) / 4.; }
+    return m;
+}
+
+// Face slot s of a face field: its family (0 radial, 1 azimuthal, 2 axial)
+// and indices. Radial R(i, x, k) lies between cells (i, x - 1, k) (its
+// face 2) and (i, x, k) (its face 1); azimuthal T(x, j, k) between
+// (x - 1, j, k) (face 4) and (x, j, k) (face 3); axial Z(i, j, x) between
+// (i, j, x - 1) (face 6) and (i, j, x) (face 5).
+struct FaceAt {
+    int fam, i, j, k;  // the coordinate across the face is the family's own (j, i or k)
+};
+
+__device__ __forceinline__ FaceAt face_at(const Dev &d, int s)
+{
+    FaceAt f;
+    const int NT = d.NT, NR = d.NR;
+    if (s < d.oT) {
+        f.fam = 0; f.i = iof(d, s % NT); f.j = (s / NT) % (NR + 1); f.k = s / (NT * (NR + 1));
+    } else if (s < d.oZ) {
+        const int t = s - d.oT;
+        f.fam = 1; f.i = iof(d, t % NT); f.j = (t / NT) % NR; f.k = t / (NT * NR);
+    } else {
+        const int t = s - d.oZ;
+        f.fam = 2; f.i = iof(d, t % NT); f.j = (t / NT) % NR; f.k = t / (NT * NR);
+    }
+    return f;
+}
+
+// The two cells of a face (-1 where a wall leaves one) and the face
+// position in each: lo is the cell below (the face is its 2, 4 or 6), hi
+// the cell above (its 1, 3 or 5)
+__device__ __forceinline__ void face_cells(const Dev &d, const FaceAt &f, int &lo, int &hi)
+{
+    if (f.fam == 0) {
+        lo = f.j > 0 ? cidx(d, f.i, f.j - 1, f.k) : -1;
+        hi = f.j < d.NR ? cidx(d, f.i, f.j, f.k) : -1;
+    } else if (f.fam == 1) {
+        lo = cidx(d, im1(d, f.i), f.j, f.k);
+        hi = cidx(d, f.i, f.j, f.k);
+    } else {
+        lo = f.k > 0 ? cidx(d, f.i, f.j, f.k - 1) : -1;
+        hi = f.k < d.NZ ? cidx(d, f.i, f.j, f.k) : -1;
+    }
+}
+
+// The transpose of the cross-diffusion operator X (cross_diffusion with
+// coef = 1) at face f: sum over the cells whose X reads this face of
+// lam(cell) times the weight it reads it with
+__device__ double cross_diffusion_T(const Dev &d, const double *lam, const FaceAt &f)
+{
+    const int N = d.N, NR = d.NR, NZ = d.NZ;
+    const double *g = d.gc;
+    // normal derivative: G(own family) (F(face 2/4/6) - F(face 1/3/5)) / 2;
+    // cross terms: -GO(m) (tder(face 2/4/6, dir) - tder(face 1/3/5, dir)) / 2
+    // for the two directions along the face
+    int Gslot, m1, dir1, m2, dir2;
+    if (f.fam == 0) { Gslot = G_G + 0; m1 = 0; dir1 = 0; m2 = 2; dir2 = 2; }       // fxy (theta), fxz (z)
+    else if (f.fam == 1) { Gslot = G_G + 1; m1 = 1; dir1 = 1; m2 = 4; dir2 = 2; }  // fyx (r), fyz (z)
+    else { Gslot = G_G + 2; m1 = 5; dir1 = 0; m2 = 3; dir2 = 1; }                  // fzy (theta), fzx (r)
+    int lo, hi;
+    face_cells(d, f, lo, hi);
+    double S = 0.;
+    if (lo >= 0) S += lam[lo] * g[Gslot * N + lo] * 0.5;
+    if (hi >= 0) S += lam[hi] * g[Gslot * N + hi] * -0.5;
+    // the row of cells on each side (along the family's own direction)
+    for (int side = 0; side < 2; side++) {
+        int ci, cj, ck;  // the owning cell of this side
+        if (f.fam == 0) {
+            const int jj = side == 0 ? f.j - 1 : f.j;
+            if (jj < 0 || jj >= NR) continue;
+            ci = f.i; cj = jj; ck = f.k;
+        } else if (f.fam == 1) {
+            ci = side == 0 ? im1(d, f.i) : f.i; cj = f.j; ck = f.k;
+        } else {
+            const int kk = side == 0 ? f.k - 1 : f.k;
+            if (kk < 0 || kk >= NZ) continue;
+            ci = f.i; cj = f.j; ck = kk;
+        }
+        // face 2/4/6 side reads with -GO/2, face 1/3/5 side with +GO/2
+        const double sgn = side == 0 ? -0.5 : 0.5;
+        for (int t = 0; t < 2; t++) {
+            const int m = t == 0 ? m1 : m2, dir = t == 0 ? dir1 : dir2;
+            const double *GO = g + (size_t)(G_GO + m) * N;
+            if (dir == 0) {
+                const int a = cidx(d, im1(d, ci), cj, ck), b = cidx(d, ip1(d, ci), cj, ck);
+                S += lam[a] * GO[a] * (sgn * 0.25) + lam[b] * GO[b] * (sgn * -0.25);
+            } else {
+                int r[4];
+                double w[4];
+                const int nr = dir == 1 ? tder_readers(cj, NR, r, w) : tder_readers(ck, NZ, r, w);
+                for (int q = 0; q < nr; q++) {
+                    const int c2 = dir == 1 ? cidx(d, ci, r[q], ck) : cidx(d, ci, cj, r[q]);
+                    S += lam[c2] * GO[c2] * (sgn * w[q]);
+                }
             }
         }
+    }
+    return S;
+}
+
+// d out / d phi(face) of dphi(phi, comp) = invV (sum of +-phi A_comp) of
+// each cell of the face, times fac(cell, comp): the face is the cell's
+// +side face (2, 4, 6) in the lo cell and its -side face in the hi cell
+template <class Fac>
+__device__ __forceinline__ double dphi_T(const Dev &d, const FaceAt &f, int s, Fac fac)
+{
+    int lo, hi;
+    face_cells(d, f, lo, hi);
+    double r = 0.;
+    for (int comp = 0; comp < 3; comp++) {
+        const double A = d.An[(size_t)comp * d.NF + s];
+        if (lo >= 0) r += fac(lo, comp) * d.gc[G_INVV * d.N + lo] * A;
+        if (hi >= 0) r -= fac(hi, comp) * d.gc[G_INVV * d.N + hi] * A;
+    }
+    return r;
+}
+
+// update_velocity_sources (mom_src): R = Rinv X(F_q) + dphi_q(p) / rho + b
+// (b = -g_beta (T + T_old) / 2 for w); one thread per face slot
+// (grid-stride over the faces, so that the parameter sum takes one atomic
+// per block of a bounded grid)
+__global__ void __launch_bounds__(BSA) k_adj_mom_src_faces(Dev d, Adj a)
+{
+    double prinv = 0.;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < d.NF; s += gridDim.x * blockDim.x) {
+        const int N = d.N, NF = d.NF;
+        const FaceAt f = face_at(d, s);
+        for (int q = 0; q < 3; q++) {
+            const double S = cross_diffusion_T(d, a.lsrc + (size_t)(R_U + q) * N, f);
+            a.lf[(size_t)q * NF + s] += d.Rinv * S;
+            prinv += d.fc[(size_t)q * NF + s] * S;
+        }
+        const double *L = a.lsrc;
+        a.lf[(size_t)F_P * NF + s] +=
+            dphi_T(d, f, s, [&](int c, int comp) { return L[(size_t)(R_U + comp) * N + c] * d.rinv; });
     }
     par_commit(a.par + P_RINV, prinv);
 }
 
-// update_pressure_sources (ppe_src): R_D_p + R_b_p
-__global__ void __launch_bounds__(BSA) k_adj_ppe_src(Dev d, Adj a)
+__global__ void __launch_bounds__(BSA) k_adj_mom_src_cells(Dev d, Adj a)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.N) return;
+    const int N = d.N;
+    const double s = -(d.g_beta * 0.5) * a.lsrc[(R_U + 2) * N + c];
+    a.lc[3 * N + c] += s;
+    a.lold[3 * N + c] += s;
+    for (int q = 0; q < 3; q++) a.lsrc[(R_U + q) * N + c] = 0.;
+}
+
+// update_pressure_sources (ppe_src): R_D_p + R_b_p. The cells' factors
+// first: R_b_p = -(ux^2 + vy^2 + wz^2) - 2 (uy vx + vz wy + wx uz) - bzz
+// + (ux + vy + wz) / (2 DT), bzz = dphi_z(-g_beta T), so with l the source
+// adjoint: d/d(u_x, u_y, u_z) = l (-2 ux + 1/(2DT), -2 vx, -2 wx), and
+// likewise for v and w; the T faces get l g_beta through dphi_z
+__global__ void __launch_bounds__(BSA) k_adj_ppe_factors(Dev d, Adj a)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
     const int N = d.N;
     const double l = a.lsrc[R_P * N + c];
-    a.lsrc[R_P * N + c] = 0.;
-    if (l == 0.) return;
     int i, j, k;
     cell_ijk(d, c, i, j, k);
     int f[6];
     cell_faces(d, i, j, k, f);
-    cross_diffusion_vjp(d, d.fc + (size_t)F_P * d.NF, a.lf + (size_t)F_P * d.NF, i, j, k, c, d.rinv, l);
     const double *U = d.fc, *V = d.fc + d.NF, *W = d.fc + 2 * (size_t)d.NF;
     double uf[6], vf[6], wf[6];
     for (int p = 0; p < 6; p++) {
@@ -520,39 +627,45 @@ __global__ void __launch_bounds__(BSA) k_adj_ppe_src(Dev d, Adj a)
     const double ux = dphi(d, d.An, uf, f, 0, invV), uy = dphi(d, d.An, uf, f, 1, invV), uz = dphi(d, d.An, uf, f, 2, invV);
     const double vx = dphi(d, d.An, vf, f, 0, invV), vy = dphi(d, d.An, vf, f, 1, invV), vz = dphi(d, d.An, vf, f, 2, invV);
     const double wx = dphi(d, d.An, wf, f, 0, invV), wy = dphi(d, d.An, wf, f, 1, invV), wz = dphi(d, d.An, wf, f, 2, invV);
-    // R_b_p = -(ux^2 + vy^2 + wz^2) - 2 (uy vx + vz wy + wx uz) - bzz + (ux + vy + wz) / (2 DT)
     const double hD = 1. / (2. * d.DT);
-    double *LU = a.lf, *LV = a.lf + d.NF, *LW = a.lf + 2 * (size_t)d.NF, *LT = a.lf + 4 * (size_t)d.NF;
-    dphi_vjp(d, f, 0, invV, l * (-2. * ux + hD), LU);
-    dphi_vjp(d, f, 1, invV, l * (-2. * vx), LU);
-    dphi_vjp(d, f, 2, invV, l * (-2. * wx), LU);
-    dphi_vjp(d, f, 0, invV, l * (-2. * uy), LV);
-    dphi_vjp(d, f, 1, invV, l * (-2. * vy + hD), LV);
-    dphi_vjp(d, f, 2, invV, l * (-2. * wy), LV);
-    dphi_vjp(d, f, 0, invV, l * (-2. * uz), LW);
-    dphi_vjp(d, f, 1, invV, l * (-2. * vz), LW);
-    dphi_vjp(d, f, 2, invV, l * (-2. * wz + hD), LW);
-    // bzz = dphi(-g_beta T faces, z)
-    dphi_vjp(d, f, 2, invV, -l * -d.g_beta, LT);
+    double *F = a.scr;
+    F[0 * N + c] = l * (-2. * ux + hD); F[1 * N + c] = l * (-2. * vx); F[2 * N + c] = l * (-2. * wx);
+    F[3 * N + c] = l * (-2. * uy); F[4 * N + c] = l * (-2. * vy + hD); F[5 * N + c] = l * (-2. * wy);
+    F[6 * N + c] = l * (-2. * uz); F[7 * N + c] = l * (-2. * vz); F[8 * N + c] = l * (-2. * wz + hD);
+    F[9 * N + c] = l * d.g_beta;
 }
 
-// update_temperature_sources (T_src): alpha R_D(T)
-__global__ void __launch_bounds__(BSA) k_adj_T_src(Dev d, Adj a)
+__global__ void __launch_bounds__(BSA) k_adj_ppe_src_faces(Dev d, Adj a)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= d.NF) return;
+    const int N = d.N, NF = d.NF;
+    const FaceAt f = face_at(d, s);
+    a.lf[(size_t)F_P * NF + s] += d.rinv * cross_diffusion_T(d, a.lsrc + (size_t)R_P * N, f);
+    const double *F = a.scr;
+    for (int q = 0; q < 3; q++)
+        a.lf[(size_t)q * NF + s] += dphi_T(d, f, s, [&](int c, int comp) { return F[(size_t)(3 * q + comp) * N + c]; });
+    a.lf[(size_t)F_T * NF + s] += dphi_T(d, f, s, [&](int c, int comp) { return comp == 2 ? F[(size_t)9 * N + c] : 0.; });
+}
+
+// update_temperature_sources (T_src): alpha X(T)
+__global__ void __launch_bounds__(BSA) k_adj_T_src_faces(Dev d, Adj a)
+{
     double palpha = 0.;
-    if (c < d.N) {
-        const int N = d.N;
-        const double l = a.lsrc[R_T * N + c];
-        a.lsrc[R_T * N + c] = 0.;
-        if (l != 0.) {
-            int i, j, k;
-            cell_ijk(d, c, i, j, k);
-            palpha = l * cross_diffusion_vjp(d, d.fc + (size_t)F_T * d.NF, a.lf + (size_t)F_T * d.NF, i, j, k, c,
-                                             d.alpha, l);
-        }
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < d.NF; s += gridDim.x * blockDim.x) {
+        const FaceAt f = face_at(d, s);
+        const double S = cross_diffusion_T(d, a.lsrc + (size_t)R_T * d.N, f);
+        a.lf[(size_t)F_T * d.NF + s] += d.alpha * S;
+        palpha += d.fc[(size_t)F_T * d.NF + s] * S;
     }
     par_commit(a.par + P_ALPHA, palpha);
+}
+
+// the consumed source adjoints
+__global__ void __launch_bounds__(BSA) k_adj_clear_src(Dev d, Adj a, int slot)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < d.N) a.lsrc[(size_t)slot * d.N + c] = 0.;
 }
 
 // ---------------------------------------------------------------------
@@ -620,9 +733,8 @@ __device__ double direction_vjp(const Dev &d, const Adj &a, int c, int dir, int 
 // part goes to par[pk]
 __global__ void __launch_bounds__(BSA) k_adj_coef(Dev d, Adj a, int energy, double *par, int pk)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     double pkd = 0.;
-    if (c < d.N) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < d.N; c += gridDim.x * blockDim.x) {
         const int N = d.N;
         int i, j, k;
         cell_ijk(d, c, i, j, k);
@@ -888,6 +1000,9 @@ static double bits_to_double(unsigned long long b)
 // thread per cell kernels
 static int grid_for(long n) { return (int)std::min<long>((n + BSA - 1) / BSA, 8192); }
 static int cell_grid(long n) { return (int)((n + BSA - 1) / BSA); }
+// grid of the grid-stride kernels that sum a scalar parameter: eight blocks
+// per CU, one atomic each
+static int reduce_grid(long n) { return std::min(cell_grid(n), 2048); }
 
 extern "C" int32_t nim_adj_create(nim_ctx *ctx, nim_adj **out)
 {
@@ -895,8 +1010,9 @@ extern "C" int32_t nim_adj_create(nim_ctx *ctx, nim_adj **out)
     *out = nullptr;
     const Dev *d = nim_internal_dev(ctx);
     const size_t N = d->N, NF = d->NF, nl = 4 * N + 5 * NF;
-    // lam, seed, prev, lsrc, lTF, lMF, lold, ltc, cbar, tcbar, fbar, par, theta
-    const size_t total = 3 * nl + NSRC * N + 2 * NCF * N + 4 * N + 3 * N + 4 * N + 3 * N + 5 * NF + 2 * NPAR + 8;
+    // lam, seed, prev, lsrc, lTF, lMF, lold, ltc, scr, cbar, tcbar, fbar, par, theta
+    const size_t total =
+        3 * nl + NSRC * N + 2 * NCF * N + 4 * N + 3 * N + 10 * N + 4 * N + 3 * N + 5 * NF + 2 * NPAR + 8;
     nim_adj *A = new nim_adj();
     A->ctx = ctx;
     if (hipMalloc(&A->block, total * sizeof(double)) != hipSuccess) {
@@ -919,6 +1035,7 @@ extern "C" int32_t nim_adj_create(nim_ctx *ctx, nim_adj **out)
     A->a.lMF = take(NCF * N);
     A->a.lold = take(4 * N);
     A->a.ltc = take(3 * N);
+    A->a.scr = take(10 * N);
     A->cbar = take(4 * N);
     A->tcbar = take(3 * N);
     A->fbar = take(5 * NF);
@@ -953,23 +1070,42 @@ static int32_t adj_check(const char *what)
     return NIM_OK;
 }
 
+// reverse of a source stage (0 momentum, 1 pressure, 2 energy): the face
+// adjoints gathered per face, then the cell terms and the consumed source
+// adjoints cleared
+static void reverse_sources(nim_adj *A, hipStream_t s, int which)
+{
+    const Dev &d = *nim_internal_dev(A->ctx);
+    const int gN = cell_grid(d.N), gF = cell_grid(d.NF), gR = reduce_grid(d.NF);
+    if (which == 0) {
+        hipLaunchKernelGGL(k_adj_mom_src_faces, dim3(gR), dim3(BSA), 0, s, d, A->a);
+        hipLaunchKernelGGL(k_adj_mom_src_cells, dim3(gN), dim3(BSA), 0, s, d, A->a);
+    } else if (which == 1) {
+        hipLaunchKernelGGL(k_adj_ppe_factors, dim3(gN), dim3(BSA), 0, s, d, A->a);
+        hipLaunchKernelGGL(k_adj_ppe_src_faces, dim3(gF), dim3(BSA), 0, s, d, A->a);
+        hipLaunchKernelGGL(k_adj_clear_src, dim3(gN), dim3(BSA), 0, s, d, A->a, (int)R_P);
+    } else {
+        hipLaunchKernelGGL(k_adj_T_src_faces, dim3(gR), dim3(BSA), 0, s, d, A->a);
+        hipLaunchKernelGGL(k_adj_clear_src, dim3(gN), dim3(BSA), 0, s, d, A->a, (int)R_T);
+    }
+}
+
 // one reverse pass at the context's state: lam <- G_x^T lam, parameter
 // adjoints of the pass in lMF, lold, the Dirichlet slots of lf and par
 static void reverse_pass(nim_adj *A, hipStream_t s)
 {
     const Dev &d = *nim_internal_dev(A->ctx);
-    const int gN = cell_grid(d.N);
     const int per_color = (d.NT / 2) * ((d.NR + 1) / 2) * d.NZ;
     const int gC = (per_color + BSA - 1) / BSA;
     hipLaunchKernelGGL(k_adj_begin, dim3(grid_for(std::max<long>(NCF * (long)d.N, 5 * (long)d.NF))), dim3(BSA), 0, s, d,
                        A->a);
     for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_T_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
-    hipLaunchKernelGGL(k_adj_T_src, dim3(gN), dim3(BSA), 0, s, d, A->a);
-    hipLaunchKernelGGL(k_adj_coef, dim3(gN), dim3(BSA), 0, s, d, A->a, 1, A->a.par, (int)P_ALPHA);
+    reverse_sources(A, s, 2);
+    hipLaunchKernelGGL(k_adj_coef, dim3(reduce_grid(d.N)), dim3(BSA), 0, s, d, A->a, 1, A->a.par, (int)P_ALPHA);
     for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_ppe_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
-    hipLaunchKernelGGL(k_adj_ppe_src, dim3(gN), dim3(BSA), 0, s, d, A->a);
+    reverse_sources(A, s, 1);
     for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_mom_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
-    hipLaunchKernelGGL(k_adj_mom_src, dim3(gN), dim3(BSA), 0, s, d, A->a);
+    reverse_sources(A, s, 0);
 }
 
 extern "C" int32_t nim_adj_seed_nusselt(nim_adj *A, const double weights[3])
@@ -1041,7 +1177,7 @@ extern "C" int32_t nim_adj_reverse_step(nim_adj *A, double tol, int32_t max_pass
     hipLaunchKernelGGL(k_adj_par_add, dim3(1), dim3(64), 0, s, A->theta, A->a.par);
     if (hipMemsetAsync(A->a.ltc, 0, 3 * (size_t)d.N * sizeof(double), s) != hipSuccess)
         return nim_internal_fail(NIM_E_HIP, "adjoint clear failed");
-    hipLaunchKernelGGL(k_adj_coef, dim3(cell_grid(d.N)), dim3(BSA), 0, s, d, A->a, 0, A->theta, (int)P_NU);
+    hipLaunchKernelGGL(k_adj_coef, dim3(reduce_grid(d.N)), dim3(BSA), 0, s, d, A->a, 0, A->theta, (int)P_NU);
     hipLaunchKernelGGL(k_adj_carry, dim3(grid_for(nl)), dim3(BSA), 0, s, d, A->a, A->cbar, A->tcbar, A->fbar);
     int32_t rc = adj_check("adjoint step launch failed");
     if (rc) return rc;
@@ -1081,23 +1217,22 @@ extern "C" int32_t nim_adj_reverse_stage(nim_adj *A, int32_t stage)
     if (!A || stage < 0 || stage > 7) return nim_internal_fail(NIM_E_ARG, "bad reverse-stage arguments");
     const Dev &d = *nim_internal_dev(A->ctx);
     hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
-    const int gN = cell_grid(d.N);
     const int gC = ((d.NT / 2) * ((d.NR + 1) / 2) * d.NZ + BSA - 1) / BSA;
     switch (stage) {
-    case 0: hipLaunchKernelGGL(k_adj_mom_src, dim3(gN), dim3(BSA), 0, s, d, A->a); break;
+    case 0: reverse_sources(A, s, 0); break;
     case 1:
         for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_mom_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
         break;
-    case 2: hipLaunchKernelGGL(k_adj_ppe_src, dim3(gN), dim3(BSA), 0, s, d, A->a); break;
+    case 2: reverse_sources(A, s, 1); break;
     case 3:
         for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_ppe_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
         break;
-    case 4: hipLaunchKernelGGL(k_adj_coef, dim3(gN), dim3(BSA), 0, s, d, A->a, 1, A->a.par, (int)P_ALPHA); break;
-    case 5: hipLaunchKernelGGL(k_adj_T_src, dim3(gN), dim3(BSA), 0, s, d, A->a); break;
+    case 4: hipLaunchKernelGGL(k_adj_coef, dim3(reduce_grid(d.N)), dim3(BSA), 0, s, d, A->a, 1, A->a.par, (int)P_ALPHA); break;
+    case 5: reverse_sources(A, s, 2); break;
     case 6:
         for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_T_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
         break;
-    default: hipLaunchKernelGGL(k_adj_coef, dim3(gN), dim3(BSA), 0, s, d, A->a, 0, A->a.par, (int)P_NU); break;
+    default: hipLaunchKernelGGL(k_adj_coef, dim3(reduce_grid(d.N)), dim3(BSA), 0, s, d, A->a, 0, A->a.par, (int)P_NU); break;
     }
     int32_t rc = adj_check("adjoint stage launch failed");
     if (rc) return rc;

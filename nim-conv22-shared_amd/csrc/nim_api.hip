@@ -109,6 +109,7 @@ struct nim_ctx {
     int nred = 0;
     // sweep schedules (64- and 256-cell blocks) and their slab thickness
     int slab = 0;
+    bool rslab_sweeps = true;  // sweeps: one radial slab per XCD (NIM_RSLAB=0: storage order)
     std::vector<Sched> sched64, sched256;
     // FP64 loop: energy coefficients built inside the energy sweep instead
     // of stored by a kernel of their own (default; NIM_FUSE_T=0 turns it off)
@@ -238,6 +239,10 @@ extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, n
     {
         const char *e = getenv("NIM_KFAST");
         d.kfast = e ? atoi(e) != 0 : 0;
+        const char *r = getenv("NIM_RSLAB");
+        c->rslab_sweeps = r ? atoi(r) != 0 : true;
+        d.rslab = !d.kfast && c->rslab_sweeps && c->NR % 8 == 0 &&
+                  ((size_t)c->NT * (c->NR / 8)) % BS == 0;
         const char *f = getenv("NIM_FUSE_T");
         c->fuse_T = f ? atoi(f) != 0 : true;
     }
@@ -651,6 +656,12 @@ static std::vector<Sched> build_schedule(const nim_ctx *c, int cells_per_block)
         }
         if (!sc.n) continue;
         sc.bstart[sc.n] = blocks;
+        // one item over all planes: one radial slab per XCD when the
+        // colour's rows split evenly and in whole blocks
+        const int nth = c->NT / 2, nrh = (c->NR + 1) / 2;
+        if (c->rslab_sweeps && sc.n == 1 && sc.k0[0] == 0 && sc.k1[0] == NZ && nrh % 8 == 0 &&
+            (nth * (nrh / 8)) % cells_per_block == 0)
+            sc.rsb = nth * (nrh / 8) / cells_per_block;
         out.push_back(sc);
     }
     return out;

@@ -90,6 +90,7 @@ struct Dev {
     double nu, Rinv, alpha, rinv, rho, g_beta, DT;
     double w_u, w_v, w_w, w_p, w_p2, w_T, v_limit;
     int kfast;          // per-cell kernels: blocks numbered plane-fastest (see cell_of_block)
+    int rslab;          // per-cell kernels: one radial slab per XCD (see cell_of_block)
     const double *gc;   // [NGC][N]
     const double *An;   // [3][NF] face area times unit normal
     double *AA;         // [3][15][N] pressure coefficients AA_px, AA_py, AA_pz
@@ -152,6 +153,7 @@ struct Sched {
     int n;
     int color[MAX_ITEMS], k0[MAX_ITEMS], k1[MAX_ITEMS];
     int bstart[MAX_ITEMS + 1];
+    int rsb;  // one item over all planes, one radial slab per XCD: blocks per slab and plane (0: storage order)
 };
 
 // An ensemble launch: per-case device descriptors and, for this launch
@@ -176,13 +178,25 @@ __device__ __forceinline__ int xcd_block()
     return x * q + (x < rem ? x : rem) + r;
 }
 
-// Cell of this thread in the per-cell (source and coefficient) kernels, in
-// storage order or (kfast) with the blocks numbered plane-fastest: logical
-// block lb covers cells [t*bs, (t+1)*bs) of plane lb % NZ, t = lb / NZ, so
-// consecutive blocks read the same faces of neighbouring planes. Returns N
-// for threads past the end of a plane.
+// Cell of this thread in the per-cell (source and coefficient) kernels:
+//  * storage order: logical block lb covers cells [lb*bs, (lb+1)*bs), so
+//    XCD x works through NZ/8 contiguous planes and the planes on either
+//    side of its range are read by a second XCD as well;
+//  * rslab: XCD x (logical blocks [x*q, (x+1)*q)) works through radial slab
+//    x (rows [x*NR/8, (x+1)*NR/8)) plane after plane, so the planes above and
+//    below a cell come through its own L2 and only the rows at the slab edges
+//    are read twice. Set only when NR % 8 == 0, NT*NR/8 is a multiple of
+//    every block size and the grid is N/bs blocks;
+//  * kfast: blocks numbered plane-fastest: logical block lb covers cells
+//    [t*bs, (t+1)*bs) of plane lb % NZ, t = lb / NZ. Returns N for threads
+//    past the end of a plane.
 __device__ __forceinline__ int cell_of_block(const Dev &d, int lb, int bs)
 {
+    if (d.rslab) {
+        const int sp = d.NT * (d.NR / NXCD), bps = sp / bs, q = d.NZ * bps;
+        const int x = lb / q, r = lb - x * q, k = r / bps;
+        return k * (d.NT * d.NR) + x * sp + (r - k * bps) * bs + (int)threadIdx.x;
+    }
     if (!d.kfast) return lb * bs + (int)threadIdx.x;
     const int ppl = d.NT * d.NR;
     const int cp = (lb / d.NZ) * bs + (int)threadIdx.x;

@@ -29,6 +29,15 @@ __device__ __forceinline__ void err_commit(const Dev &d, int field, double e)
 // the block, then the colour cell within the item
 __device__ __forceinline__ bool sched_cell(const Dev &d, const Sched &s, int lb, int &i, int &j, int &k)
 {
+    if (s.rsb) {
+        // XCD x takes rows-of-this-colour [x*nrh/8, (x+1)*nrh/8) of every
+        // plane (a radial slab), plane after plane
+        const int nth = d.NT >> 1, nrh = (d.NR + 1) >> 1, q = s.bstart[1] / NXCD;
+        const int x = lb / q, r = lb - x * q, kk = r / s.rsb;
+        const int tid = kk * (nth * nrh) + x * (nth * (nrh / NXCD)) + (r - kk * s.rsb) * (int)blockDim.x +
+                        (int)threadIdx.x;
+        return color_cell(d, s.color[0], s.k0[0], s.k1[0], tid, i, j, k);
+    }
     int q = 0;
 #pragma unroll
     for (int t = 1; t < MAX_ITEMS; t++)

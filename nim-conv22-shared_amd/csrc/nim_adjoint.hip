@@ -88,21 +88,28 @@ struct OwnFaces {
     int f[6];
     bool wr[6];
     double l[6];
+    // (fully unrolled: the arrays stay in registers)
     __device__ void load(const double *L)
     {
+#pragma unroll
         for (int p = 0; p < 6; p++) l[p] = wr[p] ? L[f[p]] : 0.;
     }
+    // (the six faces are distinct: at most one matches; predicated, so that
+    // l keeps static indices)
     __device__ void add(double *L, int a, double v)
     {
-        for (int p = 0; p < 6; p++)
-            if (a == f[p]) {
-                l[p] += v;
-                return;
-            }
-        aadd(L + a, v);
+        bool hit = false;
+#pragma unroll
+        for (int p = 0; p < 6; p++) {
+            const bool h = a == f[p];
+            l[p] += h ? v : 0.;
+            hit = hit || h;
+        }
+        if (!hit) aadd(L + a, v);
     }
     __device__ void store(double *L)
     {
+#pragma unroll
         for (int p = 0; p < 6; p++) {
             if (wr[p]) L[f[p]] = l[p];
             else if (l[p] != 0.) aadd(L + f[p], l[p]);
@@ -111,7 +118,7 @@ struct OwnFaces {
 };
 
 // the points and weights of tder(F, fpos, i, j, k, dir): sum_m w[m] F[a[m]]
-__device__ void tder_pts(const Dev &d, int fpos, int i, int j, int k, int dir, int a[3], double w[3])
+__device__ __forceinline__ void tder_pts(const Dev &d, int fpos, int i, int j, int k, int dir, int a[3], double w[3])
 {
     if (dir == 0) {
         a[0] = fidx(d, fpos, ip1(d, i), j, k);
@@ -133,18 +140,17 @@ __device__ void tder_pts(const Dev &d, int fpos, int i, int j, int k, int dir, i
     }
 }
 
-// v * tder(F, fpos, dir) into the adjoint: atomically, or through the
-// unit's own faces
-__device__ void tder_add(const Dev &d, double *L, OwnFaces *o, int fpos, int i, int j, int k, int dir, double v)
+// v * tder(F, fpos, dir) into the adjoint, through the unit's own faces
+// (the others atomically)
+__device__ __forceinline__ void tder_add(const Dev &d, double *L, OwnFaces &o, int fpos, int i, int j, int k, int dir,
+                                         double v)
 {
     int a[3];
     double w[3];
     tder_pts(d, fpos, i, j, k, dir, a, w);
-    for (int m = 0; m < 3; m++) {
-        if (w[m] == 0.) continue;
-        if (o) o->add(L, a[m], v * w[m]);
-        else aadd(L + a[m], v * w[m]);
-    }
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+        if (w[m] != 0.) o.add(L, a[m], v * w[m]);
 }
 
 // ---------------------------------------------------------------------
@@ -154,9 +160,10 @@ __device__ void tder_add(const Dev &d, double *L, OwnFaces *o, int fpos, int i, 
 // Reverse of transport_unit for face field ff and cell slot cs: replaces
 // the adjoints of the unit's cell and owned faces by those of their values
 // before the unit, adds into the faces and cells it read and into the
-// previous level, adds the coefficient adjoints of the cell into lcoef and
-// returns the adjoint of the source B.
-__device__ double transport_vjp(const Dev &d, const Adj &a, int ff, int cs, const double *coef, double *lcoef,
+// previous level, hands the coefficient adjoints of the cell to sink(slot,
+// value) and returns the adjoint of the source B.
+template <class Sink>
+__device__ __forceinline__ double transport_vjp(const Dev &d, const Adj &a, int ff, int cs, const double *coef, Sink sink,
                                 double B, double w, int i, int j, int k, int c)
 {
     const int N = d.N;
@@ -208,37 +215,37 @@ __device__ double transport_vjp(const Dev &d, const Adj &a, int ff, int cs, cons
     if (has6) {
         const double l = o.l[5], g = (w / hv[0]) * l;
         o.l[5] = (1 - w) * l;
-        o.add(LF, o.f[4], g * hv[1]);
+        o.l[4] += g * hv[1];
         aadd(LF + az6, g * hv[2]);
         lccur += g * hv[3];
         lsz += g * hv[4];
         const double q4 = g * gd4 / 4., q5 = g * gd5 / 4.;
-        o.add(LF, o.f[1], -q4); o.add(LF, o.f[0], q4); aadd(LF + az2, -q4); aadd(LF + az1, q4);
-        o.add(LF, o.f[3], -q5); o.add(LF, o.f[2], q5); aadd(LF + az4, -q5); aadd(LF + az3, q5);
+        o.l[1] += -q4; o.l[0] += q4; aadd(LF + az2, -q4); aadd(LF + az1, q4);
+        o.l[3] += -q5; o.l[2] += q5; aadd(LF + az4, -q5); aadd(LF + az3, q5);
         lh[0] += -(g / hv[0]) * S6; lh[1] += g * F4; lh[2] += g * z6; lh[3] += g * ccur; lh[4] += g * sz;
     }
     {
         const double l = o.l[3], g = (w / ev[0]) * l;
         o.l[3] = (1 - w) * l;
-        o.add(LF, o.f[2], g * ev[1]);
+        o.l[2] += g * ev[1];
         aadd(LF + at4, g * ev[2]);
         lccur += g * ev[3];
         lst += g * ev[4];
         const double q2 = g * gd2 / 4., q3 = g * gd3 / 4.;
-        o.add(LF, o.f[1], -q2); o.add(LF, o.f[0], q2); aadd(LF + at2, -q2); aadd(LF + at1, q2);
-        o.add(LF, o.f[5], -q3); o.add(LF, o.f[4], q3); aadd(LF + at6, -q3); aadd(LF + at5, q3);
+        o.l[1] += -q2; o.l[0] += q2; aadd(LF + at2, -q2); aadd(LF + at1, q2);
+        o.l[5] += -q3; o.l[4] += q3; aadd(LF + at6, -q3); aadd(LF + at5, q3);
         le[0] += -(g / ev[0]) * S4; le[1] += g * F2; le[2] += g * t4; le[3] += g * ccur; le[4] += g * st;
     }
     if (has2) {
         const double l = o.l[1], g = (w / bv[0]) * l;
         o.l[1] = (1 - w) * l;
-        o.add(LF, o.f[0], g * bv[1]);
+        o.l[0] += g * bv[1];
         aadd(LF + ar2, g * bv[2]);
         lccur += g * bv[3];
         lsr += g * bv[4];
         const double q0 = g * gd0 / 4., q1 = g * gd1 / 4.;
-        o.add(LF, o.f[3], -q0); o.add(LF, o.f[2], q0); aadd(LF + ar4, -q0); aadd(LF + ar3, q0);
-        o.add(LF, o.f[5], -q1); o.add(LF, o.f[4], q1); aadd(LF + ar6, -q1); aadd(LF + ar5, q1);
+        o.l[3] += -q0; o.l[2] += q0; aadd(LF + ar4, -q0); aadd(LF + ar3, q0);
+        o.l[5] += -q1; o.l[4] += q1; aadd(LF + ar6, -q1); aadd(LF + ar5, q1);
         lb[0] += -(g / bv[0]) * S2; lb[1] += g * F0; lb[2] += g * r2; lb[3] += g * ccur; lb[4] += g * sr;
     }
     // ccur = nv + oc, partners' s = C + CO
@@ -252,8 +259,8 @@ __device__ double transport_vjp(const Dev &d, const Adj &a, int ff, int cs, cons
     LC[c] = (1 - w) * lnv;
     loc += g * av[1];
     aadd(LO + c, loc);
-    o.add(LF, o.f[0], g * av[2]); o.add(LF, o.f[1], g * av[3]); o.add(LF, o.f[2], g * av[4]);
-    o.add(LF, o.f[3], g * av[5]); o.add(LF, o.f[4], g * av[6]); o.add(LF, o.f[5], g * av[7]);
+    o.l[0] += g * av[2]; o.l[1] += g * av[3]; o.l[2] += g * av[4];
+    o.l[3] += g * av[5]; o.l[4] += g * av[6]; o.l[5] += g * av[7];
     la[0] += -(g / av[0]) * S0; la[1] += g * oc; la[2] += g * F0; la[3] += g * F1; la[4] += g * F2;
     la[5] += g * F3; la[6] += g * F4; la[7] += g * F5;
     o.store(LF);
@@ -265,11 +272,11 @@ __device__ double transport_vjp(const Dev &d, const Adj &a, int ff, int cs, cons
     lb[1] += lb[0]; lb[2] += lb[0]; lb[3] += 2. * lb[0]; lb[4] += 2. * lb[0]; lb[0] = 0.;
     le[1] += le[0]; le[2] += le[0]; le[3] += 2. * le[0]; le[4] += 2. * le[0]; le[0] = 0.;
     lh[1] += lh[0]; lh[2] += lh[0]; lh[3] += 2. * lh[0]; lh[4] += 2. * lh[0]; lh[0] = 0.;
-    for (int m = 0; m < 8; m++) lcoef[(C_F4 + m) * N + c] += la[m];
+    for (int m = 0; m < 8; m++) sink(C_F4 + m, la[m]);
     for (int m = 0; m < 5; m++) {
-        lcoef[(C_F1 + m) * N + c] += lb[m];
-        lcoef[(C_F2 + m) * N + c] += le[m];
-        lcoef[(C_F3 + m) * N + c] += lh[m];
+        sink(C_F1 + m, lb[m]);
+        sink(C_F2 + m, le[m]);
+        sink(C_F3 + m, lh[m]);
     }
     return g;
 }
@@ -286,18 +293,26 @@ __device__ __forceinline__ void cell_ijk(const Dev &d, int c, int &i, int &j, in
     k = c / (d.NT * d.NR);
 }
 
-// momentum sweep, one colour: the three fields of a cell in one thread, so
-// that their coefficient adjoints (one set of M_F per cell) add without races
-__global__ void __launch_bounds__(BSA) k_adj_mom_sweep(Dev d, Adj a, int color)
+// momentum sweep, one colour: one wave per velocity component
+// (threadIdx.y) over 64 cells; the three components' adjoints of the
+// coefficients M_F they share meet in LDS, and wave q adds the slots
+// m = q mod 3 of the three into lMF
+__global__ void __launch_bounds__(192) k_adj_mom_sweep(Dev d, Adj a, int color)
 {
+    __shared__ double part[3][NCF][64];
     int i, j, k;
-    if (!phase_cell(d, color, i, j, k)) return;
-    const int N = d.N, c = cidx(d, i, j, k);
-    const double w[3] = {d.w_u, d.w_v, d.w_w};
-    for (int q = 0; q < 3; q++) {
-        double lB = transport_vjp(d, a, q, q, d.MF, a.lMF, d.src[(R_U + q) * N + c], w[q], i, j, k, c);
+    const bool act = phase_cell(d, color, i, j, k);
+    const int q = threadIdx.y, t = threadIdx.x, N = d.N;
+    const int c = act ? cidx(d, i, j, k) : 0;
+    if (act) {
+        const double w = q == 0 ? d.w_u : q == 1 ? d.w_v : d.w_w;
+        const double lB = transport_vjp(d, a, q, q, d.MF, [&](int m, double v) { part[q][m][t] = v; },
+                                        d.src[(R_U + q) * N + c], w, i, j, k, c);
         a.lsrc[(R_U + q) * N + c] += lB;
     }
+    __syncthreads();
+    if (act)
+        for (int m = q; m < NCF; m += 3) a.lMF[(size_t)m * N + c] += part[0][m][t] + part[1][m][t] + part[2][m][t];
 }
 
 __global__ void __launch_bounds__(BSA) k_adj_T_sweep(Dev d, Adj a, int color)
@@ -305,7 +320,9 @@ __global__ void __launch_bounds__(BSA) k_adj_T_sweep(Dev d, Adj a, int color)
     int i, j, k;
     if (!phase_cell(d, color, i, j, k)) return;
     const int N = d.N, c = cidx(d, i, j, k);
-    double lB = transport_vjp(d, a, F_T, 3, d.TF, a.lTF, d.src[R_T * N + c], d.w_T, i, j, k, c);
+    double *LT = a.lTF;
+    double lB = transport_vjp(d, a, F_T, 3, d.TF, [&](int m, double v) { LT[(size_t)m * N + c] += v; },
+                              d.src[R_T * N + c], d.w_T, i, j, k, c);
     a.lsrc[R_T * N + c] += lB;
 }
 
@@ -315,7 +332,7 @@ __global__ void __launch_bounds__(BSA) k_adj_T_sweep(Dev d, Adj a, int color)
 // ---------------------------------------------------------------------
 // reverse of pressure_wall for the wall face at position pos; returns the
 // contribution to the adjoint of nu
-__device__ double pressure_wall_vjp(const Dev &d, const Adj &a, OwnFaces &o, double &lS1, int b, int pos, int i,
+__device__ __forceinline__ double pressure_wall_vjp(const Dev &d, const Adj &a, OwnFaces &o, double &lS1, int b, int pos, int i,
                                     int j, int k)
 {
     const int NB = d.NB;
@@ -334,11 +351,11 @@ __device__ double pressure_wall_vjp(const Dev &d, const Adj &a, OwnFaces &o, dou
     // - B2 - B3, tangential derivatives of p along the wall
     const double gB2 = W[W_GB2 * NB + b], gB2c = W[W_GB2C * NB + b], gB3 = W[W_GB3 * NB + b];
     if (pos <= 2) {
-        tder_add(d, LP, &o, pos, i, j, k, 0, -g * gB2c);
-        tder_add(d, LP, &o, pos, i, j, k, 2, -g * gB3);
+        tder_add(d, LP, o, pos, i, j, k, 0, -g * gB2c);
+        tder_add(d, LP, o, pos, i, j, k, 2, -g * gB3);
     } else {
-        tder_add(d, LP, &o, pos, i, j, k, 1, -g * (j == 0 || j == d.NR - 1 ? gB2 : gB2c));
-        tder_add(d, LP, &o, pos, i, j, k, 0, -g * gB3);
+        tder_add(d, LP, o, pos, i, j, k, 1, -g * (j == 0 || j == d.NR - 1 ? gB2 : gB2c));
+        tder_add(d, LP, o, pos, i, j, k, 0, -g * gB3);
     }
     // nu rho d2, d2 = - sum_m A(m) u_n(m) over the 5-point normal stencil
     const double *U = d.fc, *V = d.fc + d.NF, *Wv = d.fc + 2 * (size_t)d.NF;
@@ -395,8 +412,8 @@ __global__ void __launch_bounds__(BSA) k_adj_ppe_sweep(Dev d, Adj a, int color)
             o.add(LP, o.f[4], -g * qz[Q_SELF]); o.add(LP, fidx(d, 6, i, j, kn), -g * qz[Q_FAR]);
             lS1 += g * qz[Q_S1];
             aadd(LS + cidx(d, i, j, kn), g * qz[Q_S2]);
-            tder_add(d, LP, &o, 6, i, j, k, 1, g * gdf[4 * N + c]);
-            tder_add(d, LP, &o, 6, i, j, k, 0, g * gdf[5 * N + c]);
+            tder_add(d, LP, o, 6, i, j, k, 1, g * gdf[4 * N + c]);
+            tder_add(d, LP, o, 6, i, j, k, 0, g * gdf[5 * N + c]);
         }
         // y face (face 4)
         {
@@ -409,8 +426,8 @@ __global__ void __launch_bounds__(BSA) k_adj_ppe_sweep(Dev d, Adj a, int color)
             o.add(LP, fidx(d, 5, ip, j, k), -g * qy[Q_TB1]); o.add(LP, fidx(d, 6, ip, j, k), -g * qy[Q_TB1]);
             lS1 += g * qy[Q_S1];
             aadd(LS + cidx(d, ip, j, k), g * qy[Q_S2]);
-            tder_add(d, LP, &o, 4, i, j, k, 1, g * gdf[2 * N + c]);
-            tder_add(d, LP, &o, 4, i, j, k, 2, g * gdf[3 * N + c]);
+            tder_add(d, LP, o, 4, i, j, k, 1, g * gdf[2 * N + c]);
+            tder_add(d, LP, o, 4, i, j, k, 2, g * gdf[3 * N + c]);
         }
         // x face (face 2)
         if (hx) {
@@ -423,8 +440,8 @@ __global__ void __launch_bounds__(BSA) k_adj_ppe_sweep(Dev d, Adj a, int color)
             o.add(LP, fidx(d, 5, i, jn, k), -g * qx[Q_TB1]); o.add(LP, fidx(d, 6, i, jn, k), -g * qx[Q_TB1]);
             lS1 += g * qx[Q_S1];
             aadd(LS + cidx(d, i, jn, k), g * qx[Q_S2]);
-            tder_add(d, LP, &o, 2, i, j, k, 0, g * gdf[0 * N + c]);
-            tder_add(d, LP, &o, 2, i, j, k, 2, g * gdf[1 * N + c]);
+            tder_add(d, LP, o, 2, i, j, k, 0, g * gdf[0 * N + c]);
+            tder_add(d, LP, o, 2, i, j, k, 2, g * gdf[1 * N + c]);
         }
         // walls, reverse of bottom, top, hexagon, cylinder
         if (j == NR - 1) pnu += pressure_wall_vjp(d, a, o, lS1, 2 * NT * NR + NT * NZ + i + NT * k, 2, i, j, k);
@@ -443,15 +460,19 @@ __global__ void __launch_bounds__(BSA) k_adj_ppe_sweep(Dev d, Adj a, int color)
 // The cells along a non-periodic direction of n cells whose tder stencil
 // (three_point: central (1, -1, 0) / 4 on (x + 1, x - 1), low wall
 // (-3, 4, -1) / 4 on (0, 1, 2), high wall (3, -4, 1) / 4 on (n - 1, n - 2,
-// n - 3)) reads the value at index x, and the weights they read it with.
-__device__ __forceinline__ int tder_readers(int x, int n, int r[4], double w[4])
+// n - 3)) reads the value at index x, and the weights they read it with: four
+// candidates x - 1, x + 1 (central readers), 0 and n - 1 (wall readers),
+// weight 0 (at a valid index) where a candidate does not read x
+__device__ __forceinline__ void row_readers(int x, int n, int r[4], double w[4])
 {
-    int m = 0;
-    if (x - 1 >= 1 && x - 1 <= n - 2) { r[m] = x - 1; w[m++] = 0.25; }
-    if (x + 1 >= 1 && x + 1 <= n - 2) { r[m] = x + 1; w[m++] = -0.25; }
-    if (x <= 2) { r[m] = 0; w[m++] = (x == 0 ? -3. : x == 1 ? 4. : -1.) / 4.; }
-    if (x >= n - 3) { r[m] = n - 1; w[m++] = (x == n - 1 ? 3. : x == n - 2 ? -4. : 1.) / 4.; }
-    return m;
+    r[0] = x - 1 < 0 ? 0 : x - 1;
+    w[0] = (x - 1 >= 1 && x - 1 <= n - 2) ? 0.25 : 0.;
+    r[1] = x + 1 > n - 1 ? n - 1 : x + 1;
+    w[1] = (x + 1 >= 1 && x + 1 <= n - 2) ? -0.25 : 0.;
+    r[2] = 0;
+    w[2] = x <= 2 ? (x == 0 ? -3. : x == 1 ? 4. : -1.) / 4. : 0.;
+    r[3] = n - 1;
+    w[3] = x >= n - 3 ? (x == n - 1 ? 3. : x == n - 2 ? -4. : 1.) / 4. : 0.;
 }
 
 // Face slot s of a face field: its family (0 radial, 1 azimuthal, 2 axial)
@@ -497,73 +518,111 @@ __device__ __forceinline__ void face_cells(const Dev &d, const FaceAt &f, int &l
 }
 
 // The transpose of the cross-diffusion operator X (cross_diffusion with
-// coef = 1) at face f: sum over the cells whose X reads this face of
-// lam(cell) times the weight it reads it with
-__device__ double cross_diffusion_T(const Dev &d, const double *lam, const FaceAt &f)
+// coef = 1) at face f, as the cells whose X reads the face and the weight
+// each reads it with: X^T lam at the face is the sum of lam(cell) times
+// weight. A fixed 18 entries, unused ones with weight 0 at a valid cell, so
+// that every load issues unconditionally:
+//  * the normal derivative G(own family) (F(face 2/4/6) - F(face 1/3/5)) / 2
+//    of the cell below the face (+G/2) and above it (-G/2);
+//  * the cross terms -GO(m) (tder(face 2/4/6, dir) - tder(face 1/3/5, dir)) / 2
+//    of the two directions along the face, in the rows of both cells: the
+//    cells whose tder stencil reads the face (theta: i -+ 1 at +-1/4, two of
+//    the four entries; r and z: row_readers), times -GO/2 in the row below
+//    and +GO/2 in the row above.
+struct Readers {
+    int c[18];
+    double w[18];
+};
+
+__device__ __forceinline__ void face_readers(const Dev &d, const FaceAt &f, Readers &R)
 {
     const int N = d.N, NR = d.NR, NZ = d.NZ;
     const double *g = d.gc;
-    // normal derivative: G(own family) (F(face 2/4/6) - F(face 1/3/5)) / 2;
-    // cross terms: -GO(m) (tder(face 2/4/6, dir) - tder(face 1/3/5, dir)) / 2
-    // for the two directions along the face
+    // own-family G slot; the two directions along the face, their GO slots
+    // (fxy, fxz / fyx, fyz / fzy, fzx) and directions (0 theta, 1 r, 2 z)
     int Gslot, m1, dir1, m2, dir2;
-    if (f.fam == 0) { Gslot = G_G + 0; m1 = 0; dir1 = 0; m2 = 2; dir2 = 2; }       // fxy (theta), fxz (z)
-    else if (f.fam == 1) { Gslot = G_G + 1; m1 = 1; dir1 = 1; m2 = 4; dir2 = 2; }  // fyx (r), fyz (z)
-    else { Gslot = G_G + 2; m1 = 5; dir1 = 0; m2 = 3; dir2 = 1; }                  // fzy (theta), fzx (r)
-    int lo, hi;
-    face_cells(d, f, lo, hi);
-    double S = 0.;
-    if (lo >= 0) S += lam[lo] * g[Gslot * N + lo] * 0.5;
-    if (hi >= 0) S += lam[hi] * g[Gslot * N + hi] * -0.5;
-    // the row of cells on each side (along the family's own direction)
+    if (f.fam == 0) { Gslot = G_G + 0; m1 = 0; dir1 = 0; m2 = 2; dir2 = 2; }
+    else if (f.fam == 1) { Gslot = G_G + 1; m1 = 1; dir1 = 1; m2 = 4; dir2 = 2; }
+    else { Gslot = G_G + 2; m1 = 5; dir1 = 0; m2 = 3; dir2 = 1; }
+    // the cell below (side 0) and above (side 1), clamped into the grid where
+    // a wall leaves one out
+    bool ok[2] = {true, true};
+    int ci[2] = {f.i, f.i}, cj[2] = {f.j, f.j}, ck[2] = {f.k, f.k};
+    if (f.fam == 0) {
+        ok[0] = f.j > 0; ok[1] = f.j < NR;
+        cj[0] = f.j > 0 ? f.j - 1 : 0; cj[1] = f.j < NR ? f.j : NR - 1;
+    } else if (f.fam == 1) {
+        ci[0] = im1(d, f.i);
+    } else {
+        ok[0] = f.k > 0; ok[1] = f.k < NZ;
+        ck[0] = f.k > 0 ? f.k - 1 : 0; ck[1] = f.k < NZ ? f.k : NZ - 1;
+    }
+    int n = 0;
+#pragma unroll
     for (int side = 0; side < 2; side++) {
-        int ci, cj, ck;  // the owning cell of this side
-        if (f.fam == 0) {
-            const int jj = side == 0 ? f.j - 1 : f.j;
-            if (jj < 0 || jj >= NR) continue;
-            ci = f.i; cj = jj; ck = f.k;
-        } else if (f.fam == 1) {
-            ci = side == 0 ? im1(d, f.i) : f.i; cj = f.j; ck = f.k;
-        } else {
-            const int kk = side == 0 ? f.k - 1 : f.k;
-            if (kk < 0 || kk >= NZ) continue;
-            ci = f.i; cj = f.j; ck = kk;
-        }
-        // face 2/4/6 side reads with -GO/2, face 1/3/5 side with +GO/2
+        const int c = cidx(d, ci[side], cj[side], ck[side]);
+        const double G = g[(size_t)Gslot * N + c];
+        R.c[n] = c;
+        R.w[n++] = ok[side] ? G * (side == 0 ? 0.5 : -0.5) : 0.;
+    }
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
         const double sgn = side == 0 ? -0.5 : 0.5;
+#pragma unroll
         for (int t = 0; t < 2; t++) {
             const int m = t == 0 ? m1 : m2, dir = t == 0 ? dir1 : dir2;
             const double *GO = g + (size_t)(G_GO + m) * N;
+            int r[4], cc[4];
+            double w[4];
             if (dir == 0) {
-                const int a = cidx(d, im1(d, ci), cj, ck), b = cidx(d, ip1(d, ci), cj, ck);
-                S += lam[a] * GO[a] * (sgn * 0.25) + lam[b] * GO[b] * (sgn * -0.25);
+                cc[0] = cidx(d, im1(d, ci[side]), cj[side], ck[side]);
+                cc[1] = cidx(d, ip1(d, ci[side]), cj[side], ck[side]);
+                cc[2] = cc[0];
+                cc[3] = cc[1];
+                w[0] = 0.25; w[1] = -0.25; w[2] = 0.; w[3] = 0.;
             } else {
-                int r[4];
-                double w[4];
-                const int nr = dir == 1 ? tder_readers(cj, NR, r, w) : tder_readers(ck, NZ, r, w);
-                for (int q = 0; q < nr; q++) {
-                    const int c2 = dir == 1 ? cidx(d, ci, r[q], ck) : cidx(d, ci, cj, r[q]);
-                    S += lam[c2] * GO[c2] * (sgn * w[q]);
-                }
+                if (dir == 1) row_readers(cj[side], NR, r, w);
+                else row_readers(ck[side], NZ, r, w);
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    cc[q] = dir == 1 ? cidx(d, ci[side], r[q], ck[side]) : cidx(d, ci[side], cj[side], r[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double go = GO[cc[q]];
+                R.c[n] = cc[q];
+                R.w[n++] = ok[side] ? go * (sgn * w[q]) : 0.;
             }
         }
     }
+}
+
+__device__ __forceinline__ double readers_sum(const Readers &R, const double *lam)
+{
+    double S = 0.;
+#pragma unroll
+    for (int n = 0; n < 18; n++) S += lam[R.c[n]] * R.w[n];
     return S;
 }
 
 // d out / d phi(face) of dphi(phi, comp) = invV (sum of +-phi A_comp) of
 // each cell of the face, times fac(cell, comp): the face is the cell's
-// +side face (2, 4, 6) in the lo cell and its -side face in the hi cell
+// +side face (2, 4, 6) in the lo cell and its -side face in the hi cell.
+// A missing cell (wall) is replaced by the other one with weight 0.
 template <class Fac>
 __device__ __forceinline__ double dphi_T(const Dev &d, const FaceAt &f, int s, Fac fac)
 {
     int lo, hi;
     face_cells(d, f, lo, hi);
+    const bool lok = lo >= 0, hok = hi >= 0;
+    const int l = lok ? lo : hi, h = hok ? hi : lo;
+    const double vl = d.gc[G_INVV * d.N + l], vh = d.gc[G_INVV * d.N + h];
     double r = 0.;
+#pragma unroll
     for (int comp = 0; comp < 3; comp++) {
         const double A = d.An[(size_t)comp * d.NF + s];
-        if (lo >= 0) r += fac(lo, comp) * d.gc[G_INVV * d.N + lo] * A;
-        if (hi >= 0) r -= fac(hi, comp) * d.gc[G_INVV * d.N + hi] * A;
+        const double fl = fac(l, comp), fh = fac(h, comp);
+        r += (lok ? fl * vl * A : 0.) - (hok ? fh * vh * A : 0.);
     }
     return r;
 }
@@ -578,10 +637,15 @@ __global__ void __launch_bounds__(BSA) k_adj_mom_src_faces(Dev d, Adj a)
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < d.NF; s += gridDim.x * blockDim.x) {
         const int N = d.N, NF = d.NF;
         const FaceAt f = face_at(d, s);
+        Readers R;
+        face_readers(d, f, R);
+        double S[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) S[q] = readers_sum(R, a.lsrc + (size_t)(R_U + q) * N);
+#pragma unroll
         for (int q = 0; q < 3; q++) {
-            const double S = cross_diffusion_T(d, a.lsrc + (size_t)(R_U + q) * N, f);
-            a.lf[(size_t)q * NF + s] += d.Rinv * S;
-            prinv += d.fc[(size_t)q * NF + s] * S;
+            a.lf[(size_t)q * NF + s] += d.Rinv * S[q];
+            prinv += d.fc[(size_t)q * NF + s] * S[q];
         }
         const double *L = a.lsrc;
         a.lf[(size_t)F_P * NF + s] +=
@@ -641,7 +705,9 @@ __global__ void __launch_bounds__(BSA) k_adj_ppe_src_faces(Dev d, Adj a)
     if (s >= d.NF) return;
     const int N = d.N, NF = d.NF;
     const FaceAt f = face_at(d, s);
-    a.lf[(size_t)F_P * NF + s] += d.rinv * cross_diffusion_T(d, a.lsrc + (size_t)R_P * N, f);
+    Readers R;
+    face_readers(d, f, R);
+    a.lf[(size_t)F_P * NF + s] += d.rinv * readers_sum(R, a.lsrc + (size_t)R_P * N);
     const double *F = a.scr;
     for (int q = 0; q < 3; q++)
         a.lf[(size_t)q * NF + s] += dphi_T(d, f, s, [&](int c, int comp) { return F[(size_t)(3 * q + comp) * N + c]; });
@@ -654,7 +720,9 @@ __global__ void __launch_bounds__(BSA) k_adj_T_src_faces(Dev d, Adj a)
     double palpha = 0.;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < d.NF; s += gridDim.x * blockDim.x) {
         const FaceAt f = face_at(d, s);
-        const double S = cross_diffusion_T(d, a.lsrc + (size_t)R_T * d.N, f);
+        Readers R;
+        face_readers(d, f, R);
+        const double S = readers_sum(R, a.lsrc + (size_t)R_T * d.N);
         a.lf[(size_t)F_T * d.NF + s] += d.alpha * S;
         palpha += d.fc[(size_t)F_T * d.NF + s] * S;
     }
@@ -702,35 +770,15 @@ __device__ void fit_velocity(const Dev &d, int c, int energy, double u[3])
         u[q] = energy ? 0.5 * (d.sc[(S_CUR + q) * N + c] + d.sc[(S_OLD + q) * N + c]) : d.sc[(S_TC + q) * N + c];
 }
 
-// adjoint lA of the constants of one direction into the velocity adjoint of
-// cell c (atomic) and the diffusivity; returns the diffusivity part
-__device__ double direction_vjp(const Dev &d, const Adj &a, int c, int dir, int energy, const double dAu[6],
-                                const double dAk[6], const double lA[6])
-{
-    const int N = d.N;
-    double lvel = 0., lk = 0.;
-    for (int m = 0; m < 6; m++) {
-        lvel += lA[m] * dAu[m];
-        lk += lA[m] * dAk[m];
-    }
-    const double *g = d.gc;
-    for (int q = 0; q < 3; q++) {
-        const double lu = lvel * g[(G_XI + 3 * dir + q) * N + c];
-        if (lu == 0.) continue;
-        if (energy) {
-            aadd(a.lc + q * N + c, 0.5 * lu);
-            aadd(a.lold + q * N + c, 0.5 * lu);
-        } else {
-            aadd(a.ltc + q * N + c, lu);
-        }
-    }
-    return lk * g[(G_GD + dir) * N + c];
-}
-
 // reverse of k_coef_A + k_coef_F (energy = 0, adjoints from lMF into the
 // time-centred velocity) or of T_coef (energy = 1, from lTF into the
 // current and previous velocity, then the adiabatic walls); the diffusivity
-// part goes to par[pk]
+// part goes to par[pk]. Gathered per cell: the constants A of a cell's
+// direction d enter its own F4 and F(d) and, as B, the F(d) of its -
+// neighbour in d (nim.f90:304-327). The cell's thread takes the adjoints of
+// all three, evaluates A and its derivatives once per direction and writes
+// only its own cell. The coefficient adjoints it consumes are cleared by
+// k_adj_coef_clear afterwards: the + neighbour reads them here too.
 __global__ void __launch_bounds__(BSA) k_adj_coef(Dev d, Adj a, int energy, double *par, int pk)
 {
     double pkd = 0.;
@@ -738,66 +786,75 @@ __global__ void __launch_bounds__(BSA) k_adj_coef(Dev d, Adj a, int energy, doub
         const int N = d.N;
         int i, j, k;
         cell_ijk(d, c, i, j, k);
-        double *LF = energy ? a.lTF : a.lMF;
-        // the forward writes F4 and, where the + neighbour exists, F1..F3:
-        // their adjoints are consumed; the slots it leaves alone keep theirs
-        const bool nb[3] = {j < d.NR - 1, true, k < d.NZ - 1};
-        double lF[NCF];
-        bool any = false;
-        for (int m = 0; m < NCF; m++) {
-            const bool written = m < C_F1 || nb[(m - C_F1) / 5];
-            lF[m] = written ? LF[m * N + c] : 0.;
-            if (written) LF[m * N + c] = 0.;
-            any = any || lF[m] != 0.;
-        }
+        const double *LF = energy ? a.lTF : a.lMF;
         const double kd = energy ? d.alpha : d.nu;
-        if (any) {
-            double u[3];
-            fit_velocity(d, c, energy, u);
-            double A[3][6], dAu[3][6], dAk[3][6], lA[3][6];
-            for (int dir = 0; dir < 3; dir++) {
-                double vel, kk;
-                direction_d(d, c, dir, u[0], u[1], u[2], kd, A[dir], dAu[dir], dAk[dir], vel, kk);
-                for (int m = 0; m < 6; m++) lA[dir][m] = 0.;
+        // the forward wrote F(d) of this cell where its + neighbour exists,
+        // and F(d) of the - neighbour where that exists
+        const bool hasp[3] = {j < d.NR - 1, true, k < d.NZ - 1};
+        const bool hasm[3] = {j > 0, true, k > 0};
+        const int cm[3] = {c - d.NT, cidx(d, im1(d, i), j, k), c - d.NT * d.NR};
+        const int base[3] = {C_F1, C_F2, C_F3};
+        double l4[8];
+        for (int m = 0; m < 8; m++) l4[m] = LF[(C_F4 + m) * N + c];
+        const double lf2 = l4[0] + l4[1];
+        double lu[3] = {0., 0., 0.}, u[3];
+        bool fitted = false;
+        for (int dir = 0; dir < 3; dir++) {
+            const int s = base[dir];
+            const double lfa = l4[0] + l4[2 + 2 * dir], lfb = l4[0] + l4[3 + 2 * dir];
+            double lb = 0., lh = 0., le = 0., lq = 0.;
+            if (hasp[dir]) {
+                const double f0 = LF[s * N + c];
+                lb = f0 + LF[(s + 1) * N + c];
+                lh = 2. * f0 + LF[(s + 3) * N + c];
             }
+            if (hasm[dir]) {
+                const int m = cm[dir];
+                const double f0 = LF[s * N + m];
+                le = f0 + LF[(s + 2) * N + m];
+                lq = 2. * f0 + LF[(s + 4) * N + m];
+            }
+            if (lf2 == 0. && lfa == 0. && lfb == 0. && lb == 0. && lh == 0. && le == 0. && lq == 0.) continue;
+            if (!fitted) {
+                fit_velocity(d, c, energy, u);
+                fitted = true;
+            }
+            double A[6], dAu[6], dAk[6], vel, kk;
+            direction_d(d, c, dir, u[0], u[1], u[2], kd, A, dAu, dAk, vel, kk);
+            const double A0 = A[0], A1 = A[1], A4 = A[4], A5 = A[5], A44 = A4 * A4;
+            double lA[6] = {0., 0., 0., 0., 0., 0.};
             // F4: f2 = (1/A14 + 1/A24 + 1/A34 - 1/DT) / 2, f(3+2d) = -A(d)5 / A(d)4,
             // f(4+2d) = (A(d)5 - 1) / A(d)4, F4 = [f2 + ... + f8, f2, ..., f8]
-            const double lf2 = lF[C_F4 + 0] + lF[C_F4 + 1];
-            for (int dir = 0; dir < 3; dir++) {
-                const double A4 = A[dir][4], A5 = A[dir][5];
-                const double lfa = lF[C_F4 + 0] + lF[C_F4 + 2 + 2 * dir], lfb = lF[C_F4 + 0] + lF[C_F4 + 3 + 2 * dir];
-                lA[dir][4] += lf2 * 0.5 * (-1. / (A4 * A4)) + lfa * (A5 / (A4 * A4)) - lfb * ((A5 - 1.) / (A4 * A4));
-                lA[dir][5] += -lfa / A4 + lfb / A4;
+            lA[4] += lf2 * 0.5 * (-1. / A44) + lfa * (A5 / A44) - lfb * ((A5 - 1.) / A44);
+            lA[5] += -lfa / A4 + lfb / A4;
+            // F(d) = [b + e + 2 h + 2 q, b, e, h, q] with b = A2 - A0 A5 / A4,
+            // h = A0 / (2 A4) of this cell, e = B3 - B1 / B4 + B1 B5 / B4,
+            // q = B1 / (2 B4) of its + neighbour: here b, h of this cell's
+            // F(d) and e, q of the - neighbour's
+            lA[2] += lb;
+            lA[0] += -lb * A5 / A4 + lh / (2. * A4);
+            lA[5] += -lb * A0 / A4;
+            lA[4] += lb * A0 * A5 / A44 - lh * A0 / (2. * A44);
+            lA[3] += le;
+            lA[1] += le * (-1. / A4 + A5 / A4) + lq / (2. * A4);
+            lA[4] += le * (A1 / A44 - A1 * A5 / A44) - lq * A1 / (2. * A44);
+            lA[5] += le * A1 / A4;
+            double lvel = 0., lk = 0.;
+            for (int m = 0; m < 6; m++) {
+                lvel += lA[m] * dAu[m];
+                lk += lA[m] * dAk[m];
             }
-            // F1..F3 with the + neighbour in each direction: b = A2 - A0 A5 / A4,
-            // e = B3 - B1 / B4 + B1 B5 / B4, h = A0 / (2 A4), q = B1 / (2 B4),
-            // F = [b + e + 2 h + 2 q, b, e, h, q]
-            const int base[3] = {C_F1, C_F2, C_F3};
-            for (int dir = 0; dir < 3; dir++) {
-                int c2;
-                if (dir == 0) { if (j == d.NR - 1) continue; c2 = c + d.NT; }
-                else if (dir == 1) c2 = cidx(d, ip1(d, i), j, k);
-                else { if (k == d.NZ - 1) continue; c2 = c + d.NT * d.NR; }
-                const int s = base[dir];
-                const double lb = lF[s] + lF[s + 1], le = lF[s] + lF[s + 2];
-                const double lh = 2. * lF[s] + lF[s + 3], lq = 2. * lF[s] + lF[s + 4];
-                if (lb == 0. && le == 0. && lh == 0. && lq == 0.) continue;
-                const double A0 = A[dir][0], A4 = A[dir][4], A5 = A[dir][5];
-                lA[dir][2] += lb;
-                lA[dir][0] += -lb * A5 / A4 + lh / (2. * A4);
-                lA[dir][5] += -lb * A0 / A4;
-                lA[dir][4] += lb * A0 * A5 / (A4 * A4) - lh * A0 / (2. * A4 * A4);
-                double u2[3];
-                fit_velocity(d, c2, energy, u2);
-                double B[6], dBu[6], dBk[6], lB[6] = {0, 0, 0, 0, 0, 0}, vel, kk;
-                direction_d(d, c2, dir, u2[0], u2[1], u2[2], kd, B, dBu, dBk, vel, kk);
-                lB[3] += le;
-                lB[1] += le * (-1. / B[4] + B[5] / B[4]) + lq / (2. * B[4]);
-                lB[4] += le * (B[1] / (B[4] * B[4]) - B[1] * B[5] / (B[4] * B[4])) - lq * B[1] / (2. * B[4] * B[4]);
-                lB[5] += le * B[1] / B[4];
-                pkd += direction_vjp(d, a, c2, dir, energy, dBu, dBk, lB);
+            for (int q = 0; q < 3; q++) lu[q] += lvel * d.gc[(G_XI + 3 * dir + q) * N + c];
+            pkd += lk * d.gc[(G_GD + dir) * N + c];
+        }
+        for (int q = 0; q < 3; q++) {
+            if (lu[q] == 0.) continue;
+            if (energy) {
+                a.lc[q * N + c] += 0.5 * lu[q];
+                a.lold[q * N + c] += 0.5 * lu[q];
+            } else {
+                a.ltc[q * N + c] += lu[q];
             }
-            for (int dir = 0; dir < 3; dir++) pkd += direction_vjp(d, a, c, dir, energy, dAu[dir], dAk[dir], lA[dir]);
         }
         // adiabatic top and bottom walls: T_w = (Z0 T_a - Z1 T_b + Z2 T_c) / Z3
         if (energy && (k == 0 || k == d.NZ - 1)) {
@@ -820,6 +877,19 @@ __global__ void __launch_bounds__(BSA) k_adj_coef(Dev d, Adj a, int energy, doub
         }
     }
     par_commit(par + pk, pkd);
+}
+
+// the coefficient adjoints k_adj_coef consumed: F4 and, where the forward
+// wrote them (the + neighbour exists), F1..F3; the others keep theirs
+__global__ void __launch_bounds__(BSA) k_adj_coef_clear(Dev d, Adj a, int energy)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= d.N) return;
+    const int N = d.N, j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
+    double *LF = energy ? a.lTF : a.lMF;
+    const bool nb[3] = {j < d.NR - 1, true, k < d.NZ - 1};
+    for (int m = 0; m < NCF; m++)
+        if (m < C_F1 || nb[(m - C_F1) / 5]) LF[(size_t)m * N + c] = 0.;
 }
 
 // ---------------------------------------------------------------------
@@ -1070,6 +1140,15 @@ static int32_t adj_check(const char *what)
     return NIM_OK;
 }
 
+// reverse of a coefficient build (energy 0 momentum, 1 energy with the
+// adiabatic walls), then the consumed coefficient adjoints cleared
+static void reverse_coefs(nim_adj *A, hipStream_t s, int energy, double *par, int pk)
+{
+    const Dev &d = *nim_internal_dev(A->ctx);
+    hipLaunchKernelGGL(k_adj_coef, dim3(reduce_grid(d.N)), dim3(BSA), 0, s, d, A->a, energy, par, pk);
+    hipLaunchKernelGGL(k_adj_coef_clear, dim3(cell_grid(d.N)), dim3(BSA), 0, s, d, A->a, energy);
+}
+
 // reverse of a source stage (0 momentum, 1 pressure, 2 energy): the face
 // adjoints gathered per face, then the cell terms and the consumed source
 // adjoints cleared
@@ -1101,10 +1180,11 @@ static void reverse_pass(nim_adj *A, hipStream_t s)
                        A->a);
     for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_T_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
     reverse_sources(A, s, 2);
-    hipLaunchKernelGGL(k_adj_coef, dim3(reduce_grid(d.N)), dim3(BSA), 0, s, d, A->a, 1, A->a.par, (int)P_ALPHA);
+    reverse_coefs(A, s, 1, A->a.par, P_ALPHA);
     for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_ppe_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
     reverse_sources(A, s, 1);
-    for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_mom_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
+    for (int col = 3; col >= 0; col--)
+        hipLaunchKernelGGL(k_adj_mom_sweep, dim3((per_color + 63) / 64), dim3(64, 3), 0, s, d, A->a, col);
     reverse_sources(A, s, 0);
 }
 
@@ -1177,7 +1257,7 @@ extern "C" int32_t nim_adj_reverse_step(nim_adj *A, double tol, int32_t max_pass
     hipLaunchKernelGGL(k_adj_par_add, dim3(1), dim3(64), 0, s, A->theta, A->a.par);
     if (hipMemsetAsync(A->a.ltc, 0, 3 * (size_t)d.N * sizeof(double), s) != hipSuccess)
         return nim_internal_fail(NIM_E_HIP, "adjoint clear failed");
-    hipLaunchKernelGGL(k_adj_coef, dim3(reduce_grid(d.N)), dim3(BSA), 0, s, d, A->a, 0, A->theta, (int)P_NU);
+    reverse_coefs(A, s, 0, A->theta, P_NU);
     hipLaunchKernelGGL(k_adj_carry, dim3(grid_for(nl)), dim3(BSA), 0, s, d, A->a, A->cbar, A->tcbar, A->fbar);
     int32_t rc = adj_check("adjoint step launch failed");
     if (rc) return rc;
@@ -1217,22 +1297,24 @@ extern "C" int32_t nim_adj_reverse_stage(nim_adj *A, int32_t stage)
     if (!A || stage < 0 || stage > 7) return nim_internal_fail(NIM_E_ARG, "bad reverse-stage arguments");
     const Dev &d = *nim_internal_dev(A->ctx);
     hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
-    const int gC = ((d.NT / 2) * ((d.NR + 1) / 2) * d.NZ + BSA - 1) / BSA;
+    const int per_color = (d.NT / 2) * ((d.NR + 1) / 2) * d.NZ;
+    const int gC = (per_color + BSA - 1) / BSA;
     switch (stage) {
     case 0: reverse_sources(A, s, 0); break;
     case 1:
-        for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_mom_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
+        for (int col = 3; col >= 0; col--)
+            hipLaunchKernelGGL(k_adj_mom_sweep, dim3((per_color + 63) / 64), dim3(64, 3), 0, s, d, A->a, col);
         break;
     case 2: reverse_sources(A, s, 1); break;
     case 3:
         for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_ppe_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
         break;
-    case 4: hipLaunchKernelGGL(k_adj_coef, dim3(reduce_grid(d.N)), dim3(BSA), 0, s, d, A->a, 1, A->a.par, (int)P_ALPHA); break;
+    case 4: reverse_coefs(A, s, 1, A->a.par, P_ALPHA); break;
     case 5: reverse_sources(A, s, 2); break;
     case 6:
         for (int col = 3; col >= 0; col--) hipLaunchKernelGGL(k_adj_T_sweep, dim3(gC), dim3(BSA), 0, s, d, A->a, col);
         break;
-    default: hipLaunchKernelGGL(k_adj_coef, dim3(reduce_grid(d.N)), dim3(BSA), 0, s, d, A->a, 0, A->a.par, (int)P_NU); break;
+    default: reverse_coefs(A, s, 0, A->a.par, P_NU); break;
     }
     int32_t rc = adj_check("adjoint stage launch failed");
     if (rc) return rc;

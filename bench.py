@@ -274,6 +274,23 @@ def ensemble_bench(args):
     return 0
 
 
+def time_reversed_pass(ctx, reps):
+    """Wall time (ms) of one reversed pass of the discrete adjoint at the
+    context's state, from a random adjoint of the iterate (every stage does
+    its full work), after one untimed pass."""
+    adj = nim_amd.Adjoint(ctx)
+    rng = np.random.default_rng(5)
+    for n in ("uxyz", "vxyz", "wxyz", "Txyz") + FACE_STATE + ("faces_p",):
+        adj.set(n, rng.standard_normal(adj._len(n)))
+    adj.reverse_pass()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        adj.reverse_pass()
+    dt = (time.perf_counter() - t0) / reps
+    adj.close()
+    return dt * 1e3
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -298,6 +315,8 @@ def main():
     # counter passes: only the per-kernel event pass (few dispatches), or a
     # plain copy of known size that calibrates FETCH_SIZE / WRITE_SIZE
     ap.add_argument("--profile-only", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only time reversed passes of the discrete adjoint at the initial state")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--pmc-summary", metavar="DIR", help="print the HBM traffic summary of counter runs under DIR")
     args = ap.parse_args()
@@ -319,6 +338,12 @@ def main():
     t_setup = time.perf_counter()
     ctx, geo, st = setup(case)
     t_setup = time.perf_counter() - t_setup
+    if args.adjoint_only:
+        ctx.build_momentum_coeffs()
+        ms = time_reversed_pass(ctx, reps=20)
+        ctx.close()
+        print(json.dumps({"case": args.case, "cells": case.ncells, "reps": 20, "ms_per_reversed_pass": ms}))
+        return 0
     if args.profile_only:
         ctx.build_momentum_coeffs()
         kt = ctx.kernel_times(args.profile_iters)
@@ -383,20 +408,10 @@ def main():
         # one reversed pass of the discrete adjoint at the benchmarked state,
         # from a random adjoint of the iterate (every stage does its full
         # work), against the forward pass's event time
-        adj = nim_amd.Adjoint(ctx)
-        rng = np.random.default_rng(5)
-        for n in ("uxyz", "vxyz", "wxyz", "Txyz") + FACE_STATE + ("faces_p",):
-            adj.set(n, rng.standard_normal(adj._len(n)))
-        reps = 10
-        adj.reverse_pass()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            adj.reverse_pass()
-        dt_adj = (time.perf_counter() - t0) / reps
-        adj.close()
+        dt_adj = time_reversed_pass(ctx, reps=10) * 1e-3
         adjoint = {"ms_per_reversed_pass": dt_adj * 1e3, "ms_per_forward_pass": iter_ms,
                    "reversed_cell_updates_per_s": N / dt_adj,
-                   "note": "wall time of nim_adj_reverse_pass (17 launches and a stream synchronisation each)"}
+                   "note": "wall time of nim_adj_reverse_pass (22 launches and a stream synchronisation each)"}
 
     mixed = None
     if not args.no_mixed:

@@ -28,7 +28,11 @@
  *    context creation) makes the FP64 loop store the energy coefficients
  *    from a kernel of their own, as nim_run_stage always does; by default
  *    the energy sweep builds them in registers (bitwise the same results),
- *    so after nim_inner_solve the MT_F* arrays are not current.
+ *    so after nim_inner_solve the MT_F* arrays are not current. Also read
+ *    at context creation, each bitwise the same results: NIM_RSLAB=0 gives
+ *    each XCD a contiguous range of planes instead of a radial slab;
+ *    NIM_KFAST=1 numbers the per-cell kernels' blocks plane-fastest;
+ *    NIM_TILE=1 computes the energy sources from LDS-staged tiles.
  */
 #ifndef NIM_HIP_H
 #define NIM_HIP_H

@@ -21,6 +21,7 @@ __global__ void k_T_coef(Dev d);
 __global__ void k_mom_src(Dev d);
 __global__ void k_ppe_src(Dev d);
 __global__ void k_T_src(Dev d);
+template <int TI> __global__ void k_T_src_tile(Dev d);
 __global__ void k_mom_sweep(Dev d, Sched sc);
 __global__ void k_T_sweep(Dev d, Sched sc);
 __global__ void k_T_sweep_fused(Dev d, Sched sc);
@@ -110,6 +111,7 @@ struct nim_ctx {
     // sweep schedules (64- and 256-cell blocks) and their slab thickness
     int slab = 0;
     bool rslab_sweeps = true;  // sweeps: one radial slab per XCD (NIM_RSLAB=0: storage order)
+    int tile = 0;              // energy sources from LDS tiles of tile x 256/tile cells (NIM_TILE=1; 0: per cell)
     std::vector<Sched> sched64, sched256;
     // FP64 loop: energy coefficients built inside the energy sweep instead
     // of stored by a kernel of their own (default; NIM_FUSE_T=0 turns it off)
@@ -243,6 +245,9 @@ extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, n
         c->rslab_sweeps = r ? atoi(r) != 0 : true;
         d.rslab = !d.kfast && c->rslab_sweeps && c->NR % 8 == 0 &&
                   ((size_t)c->NT * (c->NR / 8)) % BS == 0;
+        const char *tl = getenv("NIM_TILE");
+        const bool tiles = tl ? atoi(tl) != 0 : false;
+        c->tile = !tiles ? 0 : c->NT % 64 == 0 ? 64 : c->NT % 32 == 0 ? 32 : 0;
         const char *f = getenv("NIM_FUSE_T");
         c->fuse_T = f ? atoi(f) != 0 : true;
     }
@@ -628,6 +633,20 @@ static int cell_grid(const nim_ctx *c, int bs)
 }
 static int cell_blocks3(const nim_ctx *c) { return cell_grid(c, 64); }
 
+// energy sources (update_temperature_sources): from LDS tiles where the
+// azimuthal count allows, else one thread per cell
+static void launch_T_src(const nim_ctx *c, hipStream_t s)
+{
+    const Dev &d = c->dev;
+    if (c->tile) {
+        const int TJ = 256 / c->tile, blocks = (c->NT / c->tile) * ((c->NR + TJ - 1) / TJ) * c->NZ;
+        if (c->tile == 64) hipLaunchKernelGGL(k_T_src_tile<64>, dim3(blocks), dim3(256), 0, s, d);
+        else hipLaunchKernelGGL(k_T_src_tile<32>, dim3(blocks), dim3(256), 0, s, d);
+    } else {
+        hipLaunchKernelGGL(k_T_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d);
+    }
+}
+
 // Launches of one relaxation sweep. With slabs of `slab` axial planes
 // (S slabs), launch L runs colour c on slab s for every s + 2c = L. Colour
 // c on slab s then starts after colour c - 1 finished on slabs s - 1, s,
@@ -709,7 +728,7 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark, bool fused = false)
             hipLaunchKernelGGL(k_T_coef, dim3(gS), dim3(BS), 0, s, d);
         mark(4);
         break;
-    case 5: hipLaunchKernelGGL(k_T_src, dim3(gS), dim3(BS), 0, s, d); mark(6); break;
+    case 5: launch_T_src(c, s); mark(6); break;
     case 6:
         for (const Sched &sc : c->sched256) {
             if (fused) hipLaunchKernelGGL(k_T_sweep_fused, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
@@ -865,7 +884,7 @@ static void launch_residual(nim_ctx *c, Mark mark)
     hipLaunchKernelGGL(k_res_ppe_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, r); mark(1);
     hipLaunchKernelGGL(k_res_ppe, dim3(gN), dim3(BS), 0, s, d, r); mark(1);
     hipLaunchKernelGGL(k_res_T_coef, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, r); mark(2);
-    hipLaunchKernelGGL(k_T_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d); mark(2);
+    launch_T_src(c, s); mark(2);
     hipLaunchKernelGGL(k_res_T, dim3(gN), dim3(BS), 0, s, d, r); mark(2);
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, s, d); mark(3);
 }

@@ -428,6 +428,25 @@ __global__ void __launch_bounds__(256) k_T_src(Dev d)
     T_src(d, view64(d), xcd_block());
 }
 
+// the same from an LDS-staged tile of 256 cells (NT a multiple of TI)
+template <int TI>
+__global__ void __launch_bounds__(256) k_T_src_tile(Dev d)
+{
+    if (d.ctl->done) return;
+    __shared__ FaceTile<TI> s;
+    int i0, j0, k;
+    tile_of_block<TI>(d, xcd_block(), i0, j0, k);
+    const double *F = d.fc + (size_t)F_T * d.NF;
+    load_tile<TI>(d, F, s, i0, j0, k);
+    __syncthreads();
+    const int li = threadIdx.x % TI, jr = threadIdx.x / TI, i = i0 + li, j = j0 + jr;
+    if (j >= d.NR) return;
+    const int c = cidx(d, i, j, k);
+    d.src[R_T * d.N + c] = cross_diffusion_tile<TI>(d, s, F, i, j, k, c, li, jr, j0, d.alpha) + 0.;
+}
+template __global__ void k_T_src_tile<64>(Dev);
+template __global__ void k_T_src_tile<32>(Dev);
+
 __global__ void __launch_bounds__(256) k_d_T_src(Dev d, View<float> v)
 {
     if (d.ctl->done) return;

@@ -74,12 +74,22 @@ __device__ __forceinline__ T tder(const Dev &d, const T *F, int fpos, int i, int
     return three_point(F[fidx(d, fpos, i, j, k0)], F[fidx(d, fpos, i, j, k1)], F[fidx(d, fpos, i, j, k2)], lo, hi);
 }
 
+// the cross-diffusion combination of the nine derivatives of one cell
+template <typename T>
+__device__ __forceinline__ T cross_combine(const Dev &d, const T *g, int c, T coef, T fx, T fy, T fz, T fxy, T fyx,
+                                           T fxz, T fzx, T fyz, T fzy)
+{
+    const int N = d.N;
+    return coef * (g[(G_G + 0) * N + c] * fx + g[(G_G + 1) * N + c] * fy + g[(G_G + 2) * N + c] * fz
+                   - g[(G_GO + 0) * N + c] * fxy - g[(G_GO + 1) * N + c] * fyx - g[(G_GO + 2) * N + c] * fxz
+                   - g[(G_GO + 3) * N + c] * fzx - g[(G_GO + 4) * N + c] * fyz - g[(G_GO + 5) * N + c] * fzy);
+}
+
 // cross-diffusion operator of calculate_R_D2 / _R_D_p2 / _R_D_T
 // (nse.f90:1628-2134); g is the per-cell geometry block [NGC][N]
 template <typename T>
 __device__ T cross_diffusion(const Dev &d, const T *g, const T *F, int i, int j, int k, int c, T coef)
 {
-    const int N = d.N;
     T fxy = T(0.5) * (tder(d, F, 2, i, j, k, 0) - tder(d, F, 1, i, j, k, 0));
     T fzy = T(0.5) * (tder(d, F, 6, i, j, k, 0) - tder(d, F, 5, i, j, k, 0));
     T fyx = T(0.5) * (tder(d, F, 4, i, j, k, 1) - tder(d, F, 3, i, j, k, 1));
@@ -89,9 +99,89 @@ __device__ T cross_diffusion(const Dev &d, const T *g, const T *F, int i, int j,
     T fx = (F[fidx(d, 2, i, j, k)] - F[fidx(d, 1, i, j, k)]) / T(2);
     T fy = (F[fidx(d, 4, i, j, k)] - F[fidx(d, 3, i, j, k)]) / T(2);
     T fz = (F[fidx(d, 6, i, j, k)] - F[fidx(d, 5, i, j, k)]) / T(2);
-    return coef * (g[(G_G + 0) * N + c] * fx + g[(G_G + 1) * N + c] * fy + g[(G_G + 2) * N + c] * fz
-                   - g[(G_GO + 0) * N + c] * fxy - g[(G_GO + 1) * N + c] * fyx - g[(G_GO + 2) * N + c] * fxz
-                   - g[(G_GO + 3) * N + c] * fzx - g[(G_GO + 4) * N + c] * fyz - g[(G_GO + 5) * N + c] * fzy);
+    return cross_combine(d, g, c, coef, fx, fy, fz, fxy, fyx, fxz, fzx, fyz, fzy);
+}
+
+// ---------------------------------------------------------------------
+// LDS-staged tiles of one plane for the source kernels
+// ---------------------------------------------------------------------
+// A block of 256 threads takes a tile of TI x TJ cells (TI consecutive
+// azimuthal indices, TJ = 256 / TI radial rows) of one plane k and stages in
+// LDS, with one coalesced pass, the faces of that plane its cells' stencils
+// read: radial faces of rows j0 .. j0+TJ, azimuthal faces of rows j0-2 ..
+// j0+TJ+1 (the one-sided wall closures reach two rows) and axial faces of
+// planes k and k+1, each with the theta neighbours. The derivatives along
+// theta and r and the cell's own face differences then come from LDS; the
+// ones along z (other planes) from global memory. The arithmetic is that of
+// cross_diffusion, term by term, so the results are the same bits.
+template <int TI>
+struct FaceTile {
+    static constexpr int TJ = 256 / TI;
+    double R[TJ + 1][TI + 2];     // R(i0-1+li, j0+jj, k)
+    double Tf[TJ + 4][TI + 1];    // T(i0+li, j0-2+jj, k)
+    double Z[2][TJ + 4][TI + 2];  // Z(i0-1+li, j0-2+jj, k+kk)
+};
+
+__device__ __forceinline__ int wrap_i(const Dev &d, int i) { return i < 0 ? i + d.NT : i >= d.NT ? i - d.NT : i; }
+
+template <int TI>
+__device__ __forceinline__ void load_tile(const Dev &d, const double *F, FaceTile<TI> &s, int i0, int j0, int k)
+{
+    constexpr int TJ = FaceTile<TI>::TJ;
+    const int NR = d.NR;
+    for (int e = threadIdx.x; e < (TJ + 1) * (TI + 2); e += blockDim.x) {
+        const int jj = e / (TI + 2), li = e - jj * (TI + 2), j = j0 + jj;
+        if (j <= NR) s.R[jj][li] = F[fidx(d, 1, wrap_i(d, i0 - 1 + li), j, k)];
+    }
+    for (int e = threadIdx.x; e < (TJ + 4) * (TI + 1); e += blockDim.x) {
+        const int jj = e / (TI + 1), li = e - jj * (TI + 1), j = j0 - 2 + jj;
+        if (j >= 0 && j < NR) s.Tf[jj][li] = F[fidx(d, 3, wrap_i(d, i0 + li), j, k)];
+    }
+    for (int e = threadIdx.x; e < 2 * (TJ + 4) * (TI + 2); e += blockDim.x) {
+        const int kk = e / ((TJ + 4) * (TI + 2)), r = e - kk * ((TJ + 4) * (TI + 2));
+        const int jj = r / (TI + 2), li = r - jj * (TI + 2), j = j0 - 2 + jj;
+        if (j >= 0 && j < NR) s.Z[kk][jj][li] = F[fidx(d, 5, wrap_i(d, i0 - 1 + li), j, k + kk)];
+    }
+}
+
+// cross_diffusion of cell (i, j, k) = (i0 + li, j0 + jr, k) from a staged tile
+template <int TI>
+__device__ __forceinline__ double cross_diffusion_tile(const Dev &d, const FaceTile<TI> &s, const double *F, int i,
+                                                       int j, int k, int c, int li, int jr, int j0, double coef)
+{
+    // rows of the r-derivative's three points (tder, dir 1), as tile rows of Tf and Z
+    const bool lo = j == 0, hi = j == d.NR - 1;
+    const int ja = (lo ? 0 : hi ? j : j + 1) - j0 + 2, jb = (lo ? 1 : j - 1) - j0 + 2;
+    const int jc = (lo ? 2 : hi ? j - 2 : j - 1) - j0 + 2, jm = jr + 2;
+    const double tR2 = (s.R[jr + 1][li + 2] - s.R[jr + 1][li]) / 4.;
+    const double tR1 = (s.R[jr][li + 2] - s.R[jr][li]) / 4.;
+    const double tZ6 = (s.Z[1][jm][li + 2] - s.Z[1][jm][li]) / 4.;
+    const double tZ5 = (s.Z[0][jm][li + 2] - s.Z[0][jm][li]) / 4.;
+    const double fxy = 0.5 * (tR2 - tR1);
+    const double fzy = 0.5 * (tZ6 - tZ5);
+    const double fyx = 0.5 * (three_point(s.Tf[ja][li + 1], s.Tf[jb][li + 1], s.Tf[jc][li + 1], lo, hi) -
+                              three_point(s.Tf[ja][li], s.Tf[jb][li], s.Tf[jc][li], lo, hi));
+    const double fzx = 0.5 * (three_point(s.Z[1][ja][li + 1], s.Z[1][jb][li + 1], s.Z[1][jc][li + 1], lo, hi) -
+                              three_point(s.Z[0][ja][li + 1], s.Z[0][jb][li + 1], s.Z[0][jc][li + 1], lo, hi));
+    const double fxz = 0.5 * (tder(d, F, 2, i, j, k, 2) - tder(d, F, 1, i, j, k, 2));
+    const double fyz = 0.5 * (tder(d, F, 4, i, j, k, 2) - tder(d, F, 3, i, j, k, 2));
+    const double fx = (s.R[jr + 1][li + 1] - s.R[jr][li + 1]) / 2.;
+    const double fy = (s.Tf[jm][li + 1] - s.Tf[jm][li]) / 2.;
+    const double fz = (s.Z[1][jm][li + 1] - s.Z[0][jm][li + 1]) / 2.;
+    return cross_combine(d, d.gc, c, coef, fx, fy, fz, fxy, fyx, fxz, fzx, fyz, fzy);
+}
+
+// tile of block lb (a bijection onto the NT/TI x ceil(NR/TJ) x NZ tiles);
+// the grid is tile_blocks of them
+template <int TI>
+__device__ __forceinline__ void tile_of_block(const Dev &d, int lb, int &i0, int &j0, int &k)
+{
+    constexpr int TJ = FaceTile<TI>::TJ;
+    const int nti = d.NT / TI, ntj = (d.NR + TJ - 1) / TJ;
+    k = lb / (nti * ntj);
+    const int r = lb - k * nti * ntj, tj = r / nti;
+    i0 = (r - tj * nti) * TI;
+    j0 = tj * TJ;
 }
 
 // gradient component of a face field by the divergence theorem over the six

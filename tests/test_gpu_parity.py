@@ -276,31 +276,29 @@ def test_plane_fastest_cell_order_is_bitwise(geo, steps, monkeypatch):
             assert np.array_equal(a[n], b[n]), (n, mixed)
 
 
-def test_radial_slab_order_is_bitwise(monkeypatch):
-    """One radial slab per XCD (the default wherever n_r splits into 8 slabs
-    of whole blocks) changes which thread takes which cell, not what a cell
-    computes: 200 passes of either loop end on the same bits as storage
-    order. 128 x 64 x 4 qualifies for the per-cell kernels (128 * 64 / 8 =
-    1024 cells per slab and plane) and for the sweeps (64 * 32 / 8 = 256
-    cells of a colour per slab and plane)."""
+def _bitwise_under(monkeypatch, var, dims, passes=200):
+    """Run `passes` passes of the FP64 loop and of the mixed loop with the
+    environment switch `var` at 0 and at 1 from the same seeded state of a
+    case of dimensions `dims` (n_theta, n_r, n_z); every field must end on
+    the same bits."""
     from nim_amd import host
     import nim_amd
-    case = host.Case((128, 64, 4), 0.25, 1.0, 0.8, Ra=1e4, Pr=0.71, DT=1e-3)
+    case = host.Case(dims, 0.25, 1.0, 0.8, Ra=1e4, Pr=0.71, DT=1e-3)
     geo = host.build_geometry(case, device=True)
     st = host.initial_state(case, amp=1e-2, seed=7)
     names = ["faces_" + f for f in "uvwpT"] + [f + "xyz" for f in "uvwT"]
 
     def run(flag, mixed):
-        monkeypatch.setenv("NIM_RSLAB", flag)
+        monkeypatch.setenv(var, flag)
         ctx = nim_amd.Context(*case.dims, case)
         ctx.upload_geometry(geo)
         ctx.build_pressure_coeffs()
         ctx.upload_state(st)
         ctx.build_momentum_coeffs()
         if mixed:
-            ctx.inner_solve_mixed(np.zeros(5), inner=4, max_cycles=50, raise_on_cap=False)
+            ctx.inner_solve_mixed(np.zeros(5), inner=4, max_cycles=passes // 4, raise_on_cap=False)
         else:
-            ctx.inner_solve(np.zeros(5), max_iter=200, raise_on_cap=False)
+            ctx.inner_solve(np.zeros(5), max_iter=passes, raise_on_cap=False)
         out = {n: ctx.get(n) for n in names}
         ctx.close()
         return out
@@ -308,4 +306,22 @@ def test_radial_slab_order_is_bitwise(monkeypatch):
     for mixed in (False, True):
         a, b = run("0", mixed), run("1", mixed)
         for n in names:
-            assert np.array_equal(a[n], b[n]), (n, mixed)
+            assert np.array_equal(a[n], b[n]), (var, dims, n, mixed)
+
+
+def test_radial_slab_order_is_bitwise(monkeypatch):
+    """One radial slab per XCD (the default wherever n_r splits into 8 slabs
+    of whole blocks) changes which thread takes which cell, not what a cell
+    computes. 128 x 64 x 4 qualifies for the per-cell kernels (128 * 64 / 8 =
+    1024 cells per slab and plane) and for the sweeps (64 * 32 / 8 = 256
+    cells of a colour per slab and plane)."""
+    _bitwise_under(monkeypatch, "NIM_RSLAB", (128, 64, 4))
+
+
+@pytest.mark.parametrize("dims", [(128, 64, 4), (96, 13, 5)])
+def test_tiled_sources_are_bitwise(monkeypatch, dims):
+    """The energy sources from LDS-staged tiles (NIM_TILE=1, where n_theta is
+    a multiple of 32) compute each term as the per-cell kernel does: 64-wide
+    tiles at n_theta = 128, 32-wide tiles with a partial last row of tiles at
+    n_theta = 96, n_r = 13."""
+    _bitwise_under(monkeypatch, "NIM_TILE", dims)

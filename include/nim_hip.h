@@ -52,8 +52,9 @@ enum {
     NIM_E_STATE = 5       /* call out of order (e.g. no geometry uploaded) */
 };
 
-/* O-grid size: n_theta (azimuthal, periodic, must be even), n_r (radial,
- * >= 3) and n_z (axial, >= 3); NMESH_i, NMESH_j, NMESH_k of
+/* O-grid size: n_theta (azimuthal, periodic, even, >= 4), n_r (radial,
+ * >= 3) and n_z (axial, >= 4: the four-point closures of the two adiabatic
+ * walls must not reach each other's wall face); NMESH_i, NMESH_j, NMESH_k of
  * parameters.f90:87-89. */
 typedef struct {
     int32_t n_theta, n_r, n_z;

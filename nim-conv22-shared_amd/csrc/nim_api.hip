@@ -152,13 +152,15 @@ static void init_schedules(nim_ctx *c);
 
 static bool dims_ok(const nim_dims *d)
 {
-    return d && d->n_theta >= 4 && d->n_theta % 2 == 0 && d->n_r >= 3 && d->n_z >= 3 &&
+    // n_z >= 4: the four-point closures of the two adiabatic walls
+    // (apply_zero_flux) would otherwise read each other's wall face
+    return d && d->n_theta >= 4 && d->n_theta % 2 == 0 && d->n_r >= 3 && d->n_z >= 4 &&
            (int64_t)d->n_theta * d->n_r * d->n_z < (int64_t)1 << 30;
 }
 
 extern "C" int32_t nim_counts(const nim_dims *d, int64_t *nmesh, int64_t *nface, int64_t *nbc)
 {
-    if (!dims_ok(d)) return fail(NIM_E_ARG, "unsupported dimensions (n_theta even >= 4, n_r >= 3, n_z >= 3)");
+    if (!dims_ok(d)) return fail(NIM_E_ARG, "unsupported dimensions (n_theta even >= 4, n_r >= 3, n_z >= 4)");
     int64_t NT = d->n_theta, NR = d->n_r, NZ = d->n_z;
     if (nmesh) *nmesh = NT * NR * NZ;
     if (nface) *nface = 3 * NT * NR * NZ + NT * NZ + NT * NR;  // parameters.f90:117

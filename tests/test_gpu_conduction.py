@@ -30,7 +30,7 @@ DT = {6: 0.25, 12: 0.12, 24: 0.06}
 
 
 def steady_conduction(n_r):
-    case = host.Case((4 * n_r, n_r, 3), R_I, R_O, 1.0, Ra=1e4, Pr=0.71, DT=DT[n_r], blend=(0.1, 0.2))
+    case = host.Case((4 * n_r, n_r, 4), R_I, R_O, 1.0, Ra=1e4, Pr=0.71, DT=DT[n_r], blend=(0.1, 0.2))
     case.g_beta = 0.0  # no buoyancy; nu keeps its Ra, Pr definition
     geo = host.build_geometry(case)
     cc = host.cell_centroids(geo)

@@ -306,7 +306,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--inner", type=int, default=None,
-                    help="FP32 passes per FP64 residual of the mixed loop (default 48; 8 for the ensemble)")
+                    help="FP32 passes per FP64 residual of the mixed loop (default 96; 8 for the ensemble)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the reversed-pass timing")
     ap.add_argument("--ensemble", action="store_true", help="the 48-case ensemble instead of one case")
@@ -332,7 +332,7 @@ def main():
     if args.ensemble:
         args.inner = args.inner or 8
         return ensemble_bench(args)
-    args.inner = args.inner or 48
+    args.inner = args.inner or 96
 
     case = host.baseline_case(args.case)
     t_setup = time.perf_counter()

@@ -112,7 +112,8 @@ struct nim_ctx {
     int slab = 0;
     bool rslab_sweeps = true;  // sweeps: one radial slab per XCD (NIM_RSLAB=0: storage order)
     int tile = 0;              // energy sources from LDS tiles of tile x 256/tile cells (NIM_TILE=1; 0: per cell)
-    std::vector<Sched> sched64, sched256;
+    std::vector<Sched> sched64, sched256;  // FP64 sweeps
+    std::vector<Sched> dsched64, dsched256;  // FP32 correction sweeps (storage order: the radial slabs slow them)
     // FP64 loop: energy coefficients built inside the energy sweep instead
     // of stored by a kernel of their own (default; NIM_FUSE_T=0 turns it off)
     bool fuse_T = true;
@@ -657,7 +658,7 @@ static void launch_T_src(const nim_ctx *c, hipStream_t s)
 // reads exactly what it reads in the four full-domain colour phases, which
 // are the one-slab case. Colour c + 1 comes back to a slab two launches
 // later, while its data are still in the caches.
-static std::vector<Sched> build_schedule(const nim_ctx *c, int cells_per_block)
+static std::vector<Sched> build_schedule(const nim_ctx *c, int cells_per_block, bool rslab)
 {
     const int NZ = c->NZ, ks = c->slab, S = (NZ + ks - 1) / ks;
     const int per_plane = (c->NT / 2) * ((c->NR + 1) / 2);
@@ -680,7 +681,7 @@ static std::vector<Sched> build_schedule(const nim_ctx *c, int cells_per_block)
         // one item over all planes: one radial slab per XCD when the
         // colour's rows split evenly and in whole blocks
         const int nth = c->NT / 2, nrh = (c->NR + 1) / 2;
-        if (c->rslab_sweeps && sc.n == 1 && sc.k0[0] == 0 && sc.k1[0] == NZ && nrh % 8 == 0 &&
+        if (rslab && sc.n == 1 && sc.k0[0] == 0 && sc.k1[0] == NZ && nrh % 8 == 0 &&
             (nth * (nrh / 8)) % cells_per_block == 0)
             sc.rsb = nth * (nrh / 8) / cells_per_block;
         out.push_back(sc);
@@ -693,8 +694,10 @@ static void init_schedules(nim_ctx *c)
     const char *e = getenv("NIM_SLAB");
     int ks = e ? atoi(e) : 0;
     c->slab = ks >= 2 && ks < c->NZ ? ks : c->NZ;
-    c->sched64 = build_schedule(c, 64);
-    c->sched256 = build_schedule(c, BS);
+    c->sched64 = build_schedule(c, 64, c->rslab_sweeps);
+    c->sched256 = build_schedule(c, BS, c->rslab_sweeps);
+    c->dsched64 = build_schedule(c, 64, false);
+    c->dsched256 = build_schedule(c, BS, false);
 }
 
 static inline int sched_blocks(const Sched &s) { return s.bstart[s.n]; }
@@ -899,18 +902,18 @@ static void launch_delta(nim_ctx *c, Mark mark)
     const View<float> &v = c->mx.v32;
     hipStream_t s = c->stream;
     hipLaunchKernelGGL(k_d_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d, v); mark(4);
-    for (const Sched &sc : c->sched64) {
+    for (const Sched &sc : c->dsched64) {
         hipLaunchKernelGGL(k_d_mom_sweep, dim3(sched_blocks(sc)), BS3, 0, s, d, v, sc);
         mark(5);
     }
     hipLaunchKernelGGL(k_d_ppe_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, v); mark(6);
-    for (const Sched &sc : c->sched256) {
+    for (const Sched &sc : c->dsched256) {
         hipLaunchKernelGGL(k_d_ppe_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, v, sc);
         mark(7);
     }
     hipLaunchKernelGGL(k_d_zero_flux, dim3(nblk(2 * c->NT * c->NR)), dim3(BS), 0, s, d, v); mark(8);
     hipLaunchKernelGGL(k_d_T_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, v); mark(9);
-    for (const Sched &sc : c->sched256) {
+    for (const Sched &sc : c->dsched256) {
         hipLaunchKernelGGL(k_d_T_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, v, sc);
         mark(10);
     }

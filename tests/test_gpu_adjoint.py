@@ -227,3 +227,36 @@ def test_autograd_harness(adjoint_run):
     assert abs(out.item() - J) <= 1e-12 * abs(J)
     assert abs(gT.item() - 3.0 * grad["T_hot"]) <= 1e-9 * abs(3.0 * grad["T_hot"])
     assert abs(gR.item() - 3.0 * grad["Ra"]) <= 1e-9 * abs(3.0 * grad["Ra"])
+
+
+def test_reversed_stages_beyond_one_grid():
+    """The reversed sources and coefficient builds loop over faces or cells
+    in grid-stride fashion on a grid of at most 2048 blocks of 256 threads,
+    so that each block adds its parameter sum with one atomic. Here (128 x
+    48 x 32 cells, 600,064 face slots > 2048 * 256) every thread takes more
+    than one item. The state is the perturbed initial one, not converged:
+    each stage's transpose holds at any state. Random velocities spread the
+    cell Peclet numbers Re = 2 |u| h / nu over every value up to their
+    maximum, and near the switch between the fits' Taylor and closed forms
+    (|Re| = 1e-2) the closed forms' cancellation (about 5e-14 / Re^2)
+    swamps a finite difference: 3e-7 and 1e-7 of the coefficient stages'
+    dot products at perturbations of 0.2 and 1e-2. At 1e-4 (Re below 2e-3)
+    every fit takes its Taylor branch; the closed forms are covered by the
+    converged case above."""
+    case = host.Case((128, 48, 32), 0.3, 1.0, 1.0, Ra=1e4, Pr=0.71, DT=2e-3)
+    ctx = nim_amd.autograd.make_context(case, amp=1e-4, seed=11)
+    try:
+        ctx.build_momentum_coeffs()
+        ctx.run_stage(4)
+        x0 = {n: ctx.get(n) for n in ALL}
+        assert x0["faces_u"].size > 2048 * 256
+        for stage in (0, 2, 4, 5, 7):
+            rng = np.random.default_rng(300 + stage)
+            v = {n: rng.standard_normal(x0[n].size) * scale(x0[n]) for n in ALL}
+            lam = {n: rng.standard_normal(x0[n].size) for n in ALL}
+            eps, order = FD_STEP.get(stage, (1e-6, 2))
+            lhs, rhs = dot_test(ctx, x0, stage, lam, v, eps, order)
+            print(STAGE_NAMES[stage], lhs, rhs, (lhs - rhs) / max(abs(lhs), abs(rhs)))
+            assert abs(lhs - rhs) <= 1e-7 * max(abs(lhs), abs(rhs)), (STAGE_NAMES[stage], lhs, rhs)
+    finally:
+        ctx.close()

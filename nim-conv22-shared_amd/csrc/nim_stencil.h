@@ -44,12 +44,18 @@ __device__ __forceinline__ bool color_cell(const Dev &d, int color, int k0, int 
 // direction dir (0 theta, 1 r, 2 z): central (f(+1) - f(-1))/4, 2nd-order
 // one-sided at walls (nse.f90:1643-1653 and its repetitions)
 //
-// The three stencil points and their weights are chosen by selects and the
-// points are always loaded, so the loads of all derivatives of a cell issue
-// together instead of one branch (and one memory round trip) at a time.
-// Weights (1, -1, 0) give the central difference exactly (the products by
-// +-1 and 0 are exact); (-3, 4, -1) and (3, -4, 1) the one-sided forms in
-// the reference's operation order.
+// The stencil points and their weights are chosen by selects, so the loads
+// of all derivatives of a cell issue together instead of one branch (and
+// one memory round trip) at a time. Weights (1, -1, 0) give the central
+// difference exactly (the products by +-1 and 0 are exact); (-3, 4, -1) and
+// (3, -4, 1) the one-sided forms in the reference's operation order. The
+// third point is loaded only at a wall: a wave's cells share j and k (half
+// an azimuthal row), so that condition is uniform over the wave, and the
+// central rows (all but 2 of n_r and of n_z) issue two loads, not three.
+// Its weight is then 0 and it enters as +0, which leaves every sum's bits
+// as the product 0 * f(-1) did (the FP64 loop's fields are bitwise those of
+// three loads; in FP32 the compiler's contractions into fused multiply-adds
+// shift with the code and move the last bit).
 template <typename T>
 __device__ __forceinline__ T three_point(T f0, T f1, T f2, bool lo, bool hi)
 {
@@ -66,12 +72,16 @@ __device__ __forceinline__ T tder(const Dev &d, const T *F, int fpos, int i, int
     if (dir == 1) {
         bool lo = j == 0, hi = j == d.NR - 1;
         // central: (j+1, j-1); low wall: (0, 1, 2); high wall: (j, j-1, j-2)
-        int j0 = lo ? 0 : hi ? j : j + 1, j1 = lo ? 1 : j - 1, j2 = lo ? 2 : hi ? j - 2 : j1;
-        return three_point(F[fidx(d, fpos, i, j0, k)], F[fidx(d, fpos, i, j1, k)], F[fidx(d, fpos, i, j2, k)], lo, hi);
+        int j0 = lo ? 0 : hi ? j : j + 1, j1 = lo ? 1 : j - 1, j2 = lo ? 2 : j - 2;
+        const T f0 = F[fidx(d, fpos, i, j0, k)], f1 = F[fidx(d, fpos, i, j1, k)];
+        const T f2 = (lo || hi) ? F[fidx(d, fpos, i, j2, k)] : T(0);
+        return three_point(f0, f1, f2, lo, hi);
     }
     bool lo = k == 0, hi = k == d.NZ - 1;
-    int k0 = lo ? 0 : hi ? k : k + 1, k1 = lo ? 1 : k - 1, k2 = lo ? 2 : hi ? k - 2 : k1;
-    return three_point(F[fidx(d, fpos, i, j, k0)], F[fidx(d, fpos, i, j, k1)], F[fidx(d, fpos, i, j, k2)], lo, hi);
+    int k0 = lo ? 0 : hi ? k : k + 1, k1 = lo ? 1 : k - 1, k2 = lo ? 2 : k - 2;
+    const T f0 = F[fidx(d, fpos, i, j, k0)], f1 = F[fidx(d, fpos, i, j, k1)];
+    const T f2 = (lo || hi) ? F[fidx(d, fpos, i, j, k2)] : T(0);
+    return three_point(f0, f1, f2, lo, hi);
 }
 
 // the cross-diffusion combination of the nine derivatives of one cell

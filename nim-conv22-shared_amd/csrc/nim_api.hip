@@ -712,7 +712,7 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark, bool fused = false)
     hipStream_t s = c->stream;
     const int gS = cell_grid(c, BS);
     switch (stage) {
-    case 0: hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); mark(0); break;
+    case 0: hipLaunchKernelGGL(k_mom_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d); mark(0); break;
     case 1:
         for (const Sched &sc : c->sched64) {
             hipLaunchKernelGGL(k_mom_sweep, dim3(sched_blocks(sc)), BS3, 0, s, d, sc);
@@ -884,7 +884,7 @@ static void launch_residual(nim_ctx *c, Mark mark)
     const View<double> &r = c->mx.v64;
     hipStream_t s = c->stream;
     const int gN = nblk(c->N);
-    hipLaunchKernelGGL(k_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d); mark(0);
+    hipLaunchKernelGGL(k_mom_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d); mark(0);
     hipLaunchKernelGGL(k_res_mom, dim3(cell_blocks3(c)), BS3, 0, s, d, r); mark(0);
     hipLaunchKernelGGL(k_res_ppe_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, r); mark(1);
     hipLaunchKernelGGL(k_res_ppe, dim3(gN), dim3(BS), 0, s, d, r); mark(1);
@@ -901,7 +901,7 @@ static void launch_delta(nim_ctx *c, Mark mark)
     const Dev &d = c->dev;
     const View<float> &v = c->mx.v32;
     hipStream_t s = c->stream;
-    hipLaunchKernelGGL(k_d_mom_src, dim3(cell_blocks3(c)), BS3, 0, s, d, v); mark(4);
+    hipLaunchKernelGGL(k_d_mom_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, v); mark(4);
     for (const Sched &sc : c->dsched64) {
         hipLaunchKernelGGL(k_d_mom_sweep, dim3(sched_blocks(sc)), BS3, 0, s, d, v, sc);
         mark(5);
@@ -1246,7 +1246,7 @@ extern "C" int32_t nim_ens_create(nim_ctx *const *cases, int32_t ncase, nim_ens 
 static void ens_pass(nim_ens *e)
 {
     hipStream_t s = e->stream;
-    hipLaunchKernelGGL(k_e_mom_src, dim3(e->total[SH_CELL64]), BS3, 0, s, ens_args(e, SH_CELL64));
+    hipLaunchKernelGGL(k_e_mom_src, dim3(e->total[SH_CELL256]), dim3(BS), 0, s, ens_args(e, SH_CELL256));
     for (int q = 0; q < 4; q++)
         hipLaunchKernelGGL(k_e_mom_sweep, dim3(e->total[SH_COL64]), BS3, 0, s, ens_args(e, SH_COL64), q);
     hipLaunchKernelGGL(k_e_ppe_src, dim3(e->total[SH_CELL256]), dim3(BS), 0, s, ens_args(e, SH_CELL256));
@@ -1266,7 +1266,7 @@ static void ens_cycle(nim_ens *e, int inner)
     const Ens c64 = ens_args(e, SH_CELL64), c256 = ens_args(e, SH_CELL256);
     const Ens k64 = ens_args(e, SH_COL64), k256 = ens_args(e, SH_COL256);
     const dim3 g64(e->total[SH_CELL64]), g256(e->total[SH_CELL256]), gk64(e->total[SH_COL64]), gk256(e->total[SH_COL256]);
-    hipLaunchKernelGGL(k_e_mom_src, g64, BS3, 0, s, c64);
+    hipLaunchKernelGGL(k_e_mom_src, g256, dim3(BS), 0, s, c256);
     hipLaunchKernelGGL(k_e_res_mom, g64, BS3, 0, s, c64);
     hipLaunchKernelGGL(k_e_res_ppe_src, g256, dim3(BS), 0, s, c256);
     hipLaunchKernelGGL(k_e_res_ppe, g256, dim3(BS), 0, s, c256);
@@ -1275,7 +1275,7 @@ static void ens_cycle(nim_ens *e, int inner)
     hipLaunchKernelGGL(k_e_res_T, g256, dim3(BS), 0, s, c256);
     hipLaunchKernelGGL(k_e_check, dim3(e->n), dim3(64), 0, s, c256);
     for (int p = 0; p < inner; p++) {
-        hipLaunchKernelGGL(k_e_d_mom_src, g64, BS3, 0, s, c64);
+        hipLaunchKernelGGL(k_e_d_mom_src, g256, dim3(BS), 0, s, c256);
         for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_e_d_mom_sweep, gk64, BS3, 0, s, k64, q);
         hipLaunchKernelGGL(k_e_d_ppe_src, g256, dim3(BS), 0, s, c256);
         for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_e_d_ppe_sweep, gk256, dim3(BS), 0, s, k256, q);

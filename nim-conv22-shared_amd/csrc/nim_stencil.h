@@ -84,19 +84,54 @@ __device__ __forceinline__ T tder(const Dev &d, const T *F, int fpos, int i, int
     return three_point(f0, f1, f2, lo, hi);
 }
 
-// the cross-diffusion combination of the nine derivatives of one cell
+// the cross-diffusion combination of the nine derivatives of one cell,
+// with the cell's G (3) and GO (6) metrics given or loaded
+template <typename T>
+__device__ __forceinline__ T cross_combine(T coef, const T G[3], const T GO[6], T fx, T fy, T fz, T fxy, T fyx, T fxz,
+                                           T fzx, T fyz, T fzy)
+{
+    return coef * (G[0] * fx + G[1] * fy + G[2] * fz - GO[0] * fxy - GO[1] * fyx - GO[2] * fxz - GO[3] * fzx
+                   - GO[4] * fyz - GO[5] * fzy);
+}
+
+template <typename T>
+__device__ __forceinline__ void cross_metrics(const Dev &d, const T *g, int c, T G[3], T GO[6])
+{
+    const int N = d.N;
+#pragma unroll
+    for (int m = 0; m < 3; m++) G[m] = g[(G_G + m) * N + c];
+#pragma unroll
+    for (int m = 0; m < 6; m++) GO[m] = g[(G_GO + m) * N + c];
+}
+
 template <typename T>
 __device__ __forceinline__ T cross_combine(const Dev &d, const T *g, int c, T coef, T fx, T fy, T fz, T fxy, T fyx,
                                            T fxz, T fzx, T fyz, T fzy)
 {
-    const int N = d.N;
-    return coef * (g[(G_G + 0) * N + c] * fx + g[(G_G + 1) * N + c] * fy + g[(G_G + 2) * N + c] * fz
-                   - g[(G_GO + 0) * N + c] * fxy - g[(G_GO + 1) * N + c] * fyx - g[(G_GO + 2) * N + c] * fxz
-                   - g[(G_GO + 3) * N + c] * fzx - g[(G_GO + 4) * N + c] * fyz - g[(G_GO + 5) * N + c] * fzy);
+    T G[3], GO[6];
+    cross_metrics(d, g, c, G, GO);
+    return cross_combine(coef, G, GO, fx, fy, fz, fxy, fyx, fxz, fzx, fyz, fzy);
 }
 
 // cross-diffusion operator of calculate_R_D2 / _R_D_p2 / _R_D_T
-// (nse.f90:1628-2134); g is the per-cell geometry block [NGC][N]
+// (nse.f90:1628-2134) with the cell's metrics G, GO already loaded
+template <typename T>
+__device__ __forceinline__ T cross_diffusion(const Dev &d, const T G[3], const T GO[6], const T *F, int i, int j,
+                                             int k, T coef)
+{
+    T fxy = T(0.5) * (tder(d, F, 2, i, j, k, 0) - tder(d, F, 1, i, j, k, 0));
+    T fzy = T(0.5) * (tder(d, F, 6, i, j, k, 0) - tder(d, F, 5, i, j, k, 0));
+    T fyx = T(0.5) * (tder(d, F, 4, i, j, k, 1) - tder(d, F, 3, i, j, k, 1));
+    T fzx = T(0.5) * (tder(d, F, 6, i, j, k, 1) - tder(d, F, 5, i, j, k, 1));
+    T fxz = T(0.5) * (tder(d, F, 2, i, j, k, 2) - tder(d, F, 1, i, j, k, 2));
+    T fyz = T(0.5) * (tder(d, F, 4, i, j, k, 2) - tder(d, F, 3, i, j, k, 2));
+    T fx = (F[fidx(d, 2, i, j, k)] - F[fidx(d, 1, i, j, k)]) / T(2);
+    T fy = (F[fidx(d, 4, i, j, k)] - F[fidx(d, 3, i, j, k)]) / T(2);
+    T fz = (F[fidx(d, 6, i, j, k)] - F[fidx(d, 5, i, j, k)]) / T(2);
+    return cross_combine(coef, G, GO, fx, fy, fz, fxy, fyx, fxz, fzx, fyz, fzy);
+}
+
+// the same, loading the metrics; g is the per-cell geometry block [NGC][N]
 template <typename T>
 __device__ T cross_diffusion(const Dev &d, const T *g, const T *F, int i, int j, int k, int c, T coef)
 {

@@ -618,7 +618,8 @@ __device__ __forceinline__ bool sweep_cell(const Dev &d, const Sched &s, int lb,
     return true;
 }
 
-// threadIdx.y selects the field (u, v, w are independent within the sweep)
+// residual pass of the mixed loop: threadIdx.y selects the field (u, v, w
+// are independent within the sweep)
 template <int MODE, typename T>
 __device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, const Sched &sc, int lb)
 {
@@ -637,10 +638,47 @@ __device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, const 
     if (MODE != MODE_DELTA) err_commit(d, q, e);
 }
 
-__global__ void __launch_bounds__(192) k_mom_sweep(Dev d, Sched sc)
+// The same with one thread per cell for the three components: the cell's
+// momentum coefficients M_F (shared by u, v and w; in FP64 with the
+// diagonals re-summed) are loaded once and handed to each unit in registers.
+template <int MODE, typename T>
+__device__ __forceinline__ void mom_sweep_cell(const Dev &d, const View<T> &v, const Sched &sc, int lb)
+{
+    int i, j, k;
+    bool act = sweep_cell<MODE>(d, sc, lb, i, j, k);
+    double e[3] = {0., 0., 0.};
+    if (act) {
+        const int N = d.N;
+        const int c = cidx(d, i, j, k);
+        const T *__restrict__ coef = v.MF;
+        T creg[NCF];
+#pragma unroll
+        for (int m = 0; m < NCF; m++) creg[m] = coef[m * N + c];
+        if (sizeof(T) == sizeof(double)) {
+            // the FP64 sweep re-sums the diagonals as transport_unit does
+            T *a = creg + C_F4;
+            a[0] = a[1] + a[2] + a[3] + a[4] + a[5] + a[6] + a[7];
+#pragma unroll
+            for (int b0 = C_F1; b0 <= C_F3; b0 += 5)
+                creg[b0] = creg[b0 + 1] + creg[b0 + 2] + T(2) * creg[b0 + 3] + T(2) * creg[b0 + 4];
+        }
+        const T w[3] = {T(d.w_u), T(d.w_v), T(d.w_w)};
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            double ec = 0., ef = 0.;
+            transport_unit<MODE, T, true>(d, v, q, q, coef, v.src[q * N + c], w[q], i, j, k, c, ec, ef, creg);
+            e[q] = fmax(ec, ef);
+        }
+    }
+    if (MODE != MODE_DELTA)
+#pragma unroll
+        for (int q = 0; q < 3; q++) err_commit(d, q, e[q]);
+}
+
+__global__ void __launch_bounds__(256) k_mom_sweep(Dev d, Sched sc)
 {
     if (d.ctl->done) return;
-    mom_sweep<MODE_GS>(d, view64(d), sc, xcd_block());
+    mom_sweep_cell<MODE_GS>(d, view64(d), sc, xcd_block());
 }
 
 __global__ void __launch_bounds__(192) k_res_mom(Dev d, View<double> v)
@@ -649,10 +687,10 @@ __global__ void __launch_bounds__(192) k_res_mom(Dev d, View<double> v)
     mom_sweep<MODE_RES>(d, v, Sched{}, xcd_block());
 }
 
-__global__ void __launch_bounds__(192) k_d_mom_sweep(Dev d, View<float> v, Sched sc)
+__global__ void __launch_bounds__(256) k_d_mom_sweep(Dev d, View<float> v, Sched sc)
 {
     if (d.ctl->done) return;
-    mom_sweep<MODE_DELTA>(d, v, sc, xcd_block());
+    mom_sweep_cell<MODE_DELTA>(d, v, sc, xcd_block());
 }
 
 template <int MODE, typename T>
@@ -1310,9 +1348,9 @@ __device__ __forceinline__ Sched full_phase(const Dev &d, int color)
     if (d.ctl->done) return;
 
 __global__ void __launch_bounds__(256) k_e_mom_src(Ens e) { ENS_CASE mom_src_cell(d, view64(d), lb); }
-__global__ void __launch_bounds__(192) k_e_mom_sweep(Ens e, int color)
+__global__ void __launch_bounds__(256) k_e_mom_sweep(Ens e, int color)
 {
-    ENS_CASE mom_sweep<MODE_GS>(d, view64(d), full_phase(d, color), lb);
+    ENS_CASE mom_sweep_cell<MODE_GS>(d, view64(d), full_phase(d, color), lb);
 }
 __global__ void __launch_bounds__(256) k_e_ppe_src(Ens e) { ENS_CASE ppe_src<MODE_GS>(d, view64(d), lb); }
 __global__ void __launch_bounds__(256) k_e_ppe_sweep(Ens e, int color)
@@ -1338,9 +1376,9 @@ __global__ void __launch_bounds__(256) k_e_res_T(Ens e)
     ENS_CASE T_sweep<MODE_RES>(d, e.v64[q], e.v64[q].TF, Sched{}, lb);
 }
 __global__ void __launch_bounds__(256) k_e_d_mom_src(Ens e) { ENS_CASE mom_src_cell(d, e.v32[q], lb); }
-__global__ void __launch_bounds__(192) k_e_d_mom_sweep(Ens e, int color)
+__global__ void __launch_bounds__(256) k_e_d_mom_sweep(Ens e, int color)
 {
-    ENS_CASE mom_sweep<MODE_DELTA>(d, e.v32[q], full_phase(d, color), lb);
+    ENS_CASE mom_sweep_cell<MODE_DELTA>(d, e.v32[q], full_phase(d, color), lb);
 }
 __global__ void __launch_bounds__(256) k_e_d_ppe_src(Ens e) { ENS_CASE ppe_src<MODE_DELTA>(d, e.v32[q], lb); }
 __global__ void __launch_bounds__(256) k_e_d_ppe_sweep(Ens e, int color)

@@ -113,7 +113,7 @@ struct nim_ctx {
     bool rslab_sweeps = true;  // sweeps: one radial slab per XCD (NIM_RSLAB=0: storage order)
     int tile = 0;              // energy sources from LDS tiles of tile x 256/tile cells (NIM_TILE=1; 0: per cell)
     std::vector<Sched> sched256;   // FP64 sweeps
-    std::vector<Sched> dsched256;  // FP32 correction sweeps (storage order: the radial slabs slow them)
+    std::vector<Sched> dsched64, dsched256;  // FP32 correction sweeps (storage order: the radial slabs slow them)
     // FP64 loop: energy coefficients built inside the energy sweep instead
     // of stored by a kernel of their own (default; NIM_FUSE_T=0 turns it off)
     bool fuse_T = true;
@@ -695,6 +695,7 @@ static void init_schedules(nim_ctx *c)
     int ks = e ? atoi(e) : 0;
     c->slab = ks >= 2 && ks < c->NZ ? ks : c->NZ;
     c->sched256 = build_schedule(c, BS, c->rslab_sweeps);
+    c->dsched64 = build_schedule(c, 64, false);
     c->dsched256 = build_schedule(c, BS, false);
 }
 
@@ -900,8 +901,8 @@ static void launch_delta(nim_ctx *c, Mark mark)
     const View<float> &v = c->mx.v32;
     hipStream_t s = c->stream;
     hipLaunchKernelGGL(k_d_mom_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, v); mark(4);
-    for (const Sched &sc : c->dsched256) {
-        hipLaunchKernelGGL(k_d_mom_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, v, sc);
+    for (const Sched &sc : c->dsched64) {
+        hipLaunchKernelGGL(k_d_mom_sweep, dim3(sched_blocks(sc)), BS3, 0, s, d, v, sc);
         mark(5);
     }
     hipLaunchKernelGGL(k_d_ppe_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, v); mark(6);
@@ -1141,7 +1142,7 @@ extern "C" int32_t nim_kernel_times_mixed(nim_ctx *c, int32_t cycles, int32_t in
 
 // ---- ensembles -----------------------------------------------------------
 // launch shapes: blocks per case of each kind of launch
-enum { SH_CELL64, SH_CELL256, SH_COL256, SH_WALL, SH_UPD, NSHAPE };
+enum { SH_CELL64, SH_CELL256, SH_COL64, SH_COL256, SH_WALL, SH_UPD, NSHAPE };
 
 struct nim_ens {
     int n = 0;
@@ -1208,6 +1209,7 @@ extern "C" int32_t nim_ens_create(nim_ctx *const *cases, int32_t ncase, nim_ens 
             switch (sh) {
             case SH_CELL64: acc += cell_grid(c, 64); break;
             case SH_CELL256: acc += cell_grid(c, BS); break;
+            case SH_COL64: acc += (cc + 63) / 64; break;
             case SH_COL256: acc += nblk(cc); break;
             case SH_WALL: acc += nblk(2 * c->NT * c->NR); break;
             default: acc += nblk(std::max(c->N, c->NF)); break;
@@ -1261,8 +1263,8 @@ static void ens_cycle(nim_ens *e, int inner)
 {
     hipStream_t s = e->stream;
     const Ens c64 = ens_args(e, SH_CELL64), c256 = ens_args(e, SH_CELL256);
-    const Ens k256 = ens_args(e, SH_COL256);
-    const dim3 g64(e->total[SH_CELL64]), g256(e->total[SH_CELL256]), gk256(e->total[SH_COL256]);
+    const Ens k64 = ens_args(e, SH_COL64), k256 = ens_args(e, SH_COL256);
+    const dim3 g64(e->total[SH_CELL64]), g256(e->total[SH_CELL256]), gk64(e->total[SH_COL64]), gk256(e->total[SH_COL256]);
     hipLaunchKernelGGL(k_e_mom_src, g256, dim3(BS), 0, s, c256);
     hipLaunchKernelGGL(k_e_res_mom, g64, BS3, 0, s, c64);
     hipLaunchKernelGGL(k_e_res_ppe_src, g256, dim3(BS), 0, s, c256);
@@ -1273,7 +1275,7 @@ static void ens_cycle(nim_ens *e, int inner)
     hipLaunchKernelGGL(k_e_check, dim3(e->n), dim3(64), 0, s, c256);
     for (int p = 0; p < inner; p++) {
         hipLaunchKernelGGL(k_e_d_mom_src, g256, dim3(BS), 0, s, c256);
-        for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_e_d_mom_sweep, gk256, dim3(BS), 0, s, k256, q);
+        for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_e_d_mom_sweep, gk64, BS3, 0, s, k64, q);
         hipLaunchKernelGGL(k_e_d_ppe_src, g256, dim3(BS), 0, s, c256);
         for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_e_d_ppe_sweep, gk256, dim3(BS), 0, s, k256, q);
         hipLaunchKernelGGL(k_e_d_zero_flux, dim3(e->total[SH_WALL]), dim3(BS), 0, s, ens_args(e, SH_WALL));

@@ -490,11 +490,13 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
     // ---- gather
     const T old = C[c], oc = MODE == MODE_DELTA ? T(0) : CO[c];
     // The diagonals F4(1), F1(1), F2(1), F3(1) are sums of the other
-    // coefficients (nim.f90:292-329, 465-502). In FP64 they are summed
-    // again here in the order the coefficient kernels use, which gives the
-    // stored values bit for bit and saves four loads per cell. The FP32
-    // copies are rounded from the FP64 values, so the FP32 passes load them.
-    constexpr bool sum_diag = sizeof(T) == sizeof(double);
+    // coefficients (nim.f90:292-329, 465-502). They are summed again here in
+    // the order the coefficient kernels use, which saves four loads per cell:
+    // in FP64 that gives the stored values bit for bit; the FP32 correction
+    // passes sum the FP32 copies and may differ from the rounded FP64
+    // diagonal in the last bit, a rounding of the correction's smoother only
+    // (the FP64 residual fixes what the corrections converge to).
+    constexpr bool sum_diag = true;
     constexpr int m0 = sum_diag ? 1 : 0;
     T a[8];
 #pragma unroll
@@ -618,8 +620,8 @@ __device__ __forceinline__ bool sweep_cell(const Dev &d, const Sched &s, int lb,
     return true;
 }
 
-// residual pass of the mixed loop: threadIdx.y selects the field (u, v, w
-// are independent within the sweep)
+// residual pass and FP32 corrections of the mixed loop: threadIdx.y selects
+// the field (u, v, w are independent within the sweep)
 template <int MODE, typename T>
 __device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, const Sched &sc, int lb)
 {
@@ -687,10 +689,12 @@ __global__ void __launch_bounds__(192) k_res_mom(Dev d, View<double> v)
     mom_sweep<MODE_RES>(d, v, Sched{}, xcd_block());
 }
 
-__global__ void __launch_bounds__(256) k_d_mom_sweep(Dev d, View<float> v, Sched sc)
+// FP32 corrections: one wave per component (the FP32 unit is light on
+// registers, and this runs 2% faster than one thread per cell)
+__global__ void __launch_bounds__(192) k_d_mom_sweep(Dev d, View<float> v, Sched sc)
 {
     if (d.ctl->done) return;
-    mom_sweep_cell<MODE_DELTA>(d, v, sc, xcd_block());
+    mom_sweep<MODE_DELTA>(d, v, sc, xcd_block());
 }
 
 template <int MODE, typename T>
@@ -1376,9 +1380,9 @@ __global__ void __launch_bounds__(256) k_e_res_T(Ens e)
     ENS_CASE T_sweep<MODE_RES>(d, e.v64[q], e.v64[q].TF, Sched{}, lb);
 }
 __global__ void __launch_bounds__(256) k_e_d_mom_src(Ens e) { ENS_CASE mom_src_cell(d, e.v32[q], lb); }
-__global__ void __launch_bounds__(256) k_e_d_mom_sweep(Ens e, int color)
+__global__ void __launch_bounds__(192) k_e_d_mom_sweep(Ens e, int color)
 {
-    ENS_CASE mom_sweep_cell<MODE_DELTA>(d, e.v32[q], full_phase(d, color), lb);
+    ENS_CASE mom_sweep<MODE_DELTA>(d, e.v32[q], full_phase(d, color), lb);
 }
 __global__ void __launch_bounds__(256) k_e_d_ppe_src(Ens e) { ENS_CASE ppe_src<MODE_DELTA>(d, e.v32[q], lb); }
 __global__ void __launch_bounds__(256) k_e_d_ppe_sweep(Ens e, int color)

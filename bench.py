@@ -324,7 +324,7 @@ def main():
         print(pmc_summary(args.pmc_summary, args.case, args.copy_probe or 1024))
         return 0
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs one case per device and has no multi-GPU "
+        print("bench.py measures one GPU only: the library runs one case per device and has no multi-GPU "
               "path (one case lives on one device)", file=sys.stderr)
         return 2
     if args.copy_probe:

@@ -1,12 +1,12 @@
 /* Seeded initial-condition noise shared by the oracle and the reference driver.
  *
  * TEST INFRASTRUCTURE: part of the CPU oracle. The product computes the same
- * sequence in nim-conv22-shared_amd/csrc/nim_host.cpp; a CPU test checks that
- * both agree value for value.
+ * sequence in nim-conv22-shared_amd/fortran/nim_host_noise.c; a CPU test
+ * checks that both agree value for value.
  *
  * The reference has no random initial condition (general.f90:16-31 returns 0
- * for every field); the perturbation is an addition of this build so that the
- * solver is exercised away from the trivial state. The generator is the
+ * for every field); the benchmark cases add this seeded perturbation so that
+ * the solver is exercised away from the trivial state. The generator is the
  * splitmix64 finaliser applied to a key made of (seed, field, cell index), so
  * every cell's value is independent of the traversal order and of the mesh
  * decomposition.

@@ -9,7 +9,9 @@
 !       restart_every=0, restart_prefix='restart/rest.step', restart_in='',
 !       final_out='', mixed_inner=0 /
 ! mixed_inner > 0 selects the mixed-precision inner loop (FP64 residuals,
-! that many FP32 correction passes per residual).
+! that many FP32 correction passes per residual). device_metrics=.false.
+! computes the per-cell metrics on the host instead of the device (the same
+! bits either way).
 program nim3d_hip
   use iso_c_binding
   use nim_host_geometry
@@ -22,11 +24,11 @@ program nim3d_hip
   real(c_double) :: Ra = 1.0d5, Pr = 0.7d0, DT = 0.01d0, noise = 1.0d-3, tol(5) = (/1d-6, 1d-6, 1d-6, 5d-3, 1d-6/)
   real(c_double) :: T_hot = 1.0d0, T_cold = 0.0d0
   integer(c_int64_t) :: seed = 20221
-  logical :: check_steady = .true.
+  logical :: check_steady = .true., device_metrics = .true.
   character(len=256) :: nml = 'nim3d.nml', restart_prefix = 'restart/rest.step', restart_in = '', final_out = ''
   namelist /case/ n_theta, n_r, n_z, R_i, R_o, height, blend_in, blend_out, grading, hex_order, Ra, Pr, DT, &
                   nsteps, tol, noise, seed, T_hot, T_cold, restart_every, restart_prefix, restart_in, &
-                  check_steady, final_out, mixed_inner
+                  check_steady, final_out, mixed_inner, device_metrics
 
   if (command_argument_count() >= 1) call get_command_argument(1, nml)
   open(newunit=u, file=trim(nml), status='old', iostat=ios)
@@ -38,6 +40,7 @@ program nim3d_hip
                          3.14159265359d0)
   cfg%Ra = Ra; cfg%Pr = Pr; cfg%DT = DT; cfg%tol = tol; cfg%noise = noise; cfg%seed = seed
   cfg%T_hot = T_hot; cfg%T_cold = T_cold; cfg%check_steady = check_steady; cfg%mixed_inner = mixed_inner
+  cfg%device_metrics = device_metrics
   call nim_setup(cfg)
   if (len_trim(restart_in) > 0) call nim_read_restart(trim(restart_in))
   done = nim_solve(cfg, nsteps, restart_every, trim(restart_prefix))

@@ -50,6 +50,17 @@ module nim_hip_api
       type(c_ptr), value :: ctx
       type(nim_geometry), intent(in) :: geo
     end function
+    ! the per-cell metrics of evaluate_all_mesh_constants (geometry.f90:395-632)
+    ! on the device, bit for bit those of build_geometry, from the mesh nodes
+    integer(c_int32_t) function nim_device_metrics(dims, node_xyz, volume, xi_x, gij, gij_faces, G1, G2, G3, &
+        faces_area, faces_normal, faces_tangential, faces_bitangential, faces_mid) bind(C, name="nim_device_metrics")
+      import :: c_int32_t, c_double, nim_dims
+      type(nim_dims), intent(in) :: dims
+      real(c_double), intent(in) :: node_xyz(*)
+      real(c_double), intent(out) :: volume(*), xi_x(*), gij(*), gij_faces(*), G1(*), G2(*), G3(*)
+      real(c_double), intent(out) :: faces_area(*), faces_normal(*), faces_tangential(*), faces_bitangential(*)
+      real(c_double), intent(out) :: faces_mid(*)
+    end function
     ! calculate_pressure_nim_constants (nim.f90:556-743)
     integer(c_int32_t) function nim_build_pressure_coeffs(ctx) bind(C, name="nim_build_pressure_coeffs")
       import :: c_int32_t, c_ptr

@@ -1,5 +1,6 @@
 """The Fortran drop-in driver nim3d_hip end to end on the reference's own
-case: mesh and metrics from the Fortran host, 3 time steps through the C-ABI
+case: mesh from the Fortran host, metrics on the device (or the host), 3 time
+steps through the C-ABI
 from the seeded initial state, restart file in the reference's record layout
 (restart.f90:8-39), compared with the reference's own state after step 3
 (tests/golden/ref_steps.npz, written by the compiled reference).
@@ -138,3 +139,19 @@ def test_restart_resume_is_bitwise(tmp_path, inner):
     assert straight["TI"] == resumed["TI"]
     for n in RECORDS:
         assert np.array_equal(straight[n], resumed[n]), n
+
+
+def test_device_and_host_metrics_run_the_same_bits(tmp_path):
+    """device_metrics=.true. (the default: nim_device_metrics) and .false.
+    (the Fortran host's build_geometry) give the same final state bit for
+    bit: the two metric paths produce the same arrays."""
+    def run(flag):
+        nml = tmp_path / ("m%s.nml" % flag)
+        out = tmp_path / ("m%s.final" % flag)
+        nml.write_text(NML.format(out=out, inner=0).replace(" /\n", ", device_metrics=.%s. /\n" % flag))
+        p = subprocess.run([EXE, str(nml)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+        assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+        return read_restart(out)
+    a, b = run("true"), run("false")
+    for n in RECORDS:
+        assert np.array_equal(a[n], b[n]), n

@@ -72,6 +72,21 @@ enum { R_U = 0, R_V = 1, R_W = 2, R_P = 3, R_T = 4, NSRC = 5 };
 // face field slots in Dev::fc
 enum { F_U = 0, F_V = 1, F_W = 2, F_P = 3, F_T = 4 };
 
+// Load of data that a sweep reads once per pass (coefficients, metric
+// differences, FP32 residuals), marked non-temporal when NT is set
+// (global_load ... nt): the caches then favour the faces and cells that the
+// other colour phases and the next kernels read again. The value is the same
+// either way. NT is a template argument: a run-time choice between the two
+// loads of one address can be merged into a plain load before it is folded.
+template <bool NT = true, typename T>
+__device__ __forceinline__ T ld_pass(const T *p)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
 constexpr int ERR_COPIES = 64;  // spread of the per-iteration max-increment atomics
 constexpr int ERR_STRIDE = 16;  // one 128-byte cache line per copy
 __device__ __host__ inline size_t err_slot(int slot, int field, int copy)

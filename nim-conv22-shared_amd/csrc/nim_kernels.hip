@@ -472,7 +472,7 @@ __global__ void __launch_bounds__(256) k_d_T_src(Dev d, View<float> v)
 // ff is the face field (F_U..F_T), cs the cell slot (0..3). The
 // coefficients come from the block coef ([NCF][N]) or, with REG, from the
 // caller's registers creg[NCF].
-template <int MODE, typename T, bool REG = false>
+template <int MODE, typename T, bool REG = false, bool NTG = !REG>
 __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, const T *__restrict__ coef, T B, T w,
                                int i, int j, int k, int c, double &ecell, double &eface, const T *creg = nullptr)
 {
@@ -498,16 +498,21 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
     // (the FP64 residual fixes what the corrections converge to).
     constexpr bool sum_diag = true;
     constexpr int m0 = sum_diag ? 1 : 0;
+    // Non-temporal loads of the coefficients and (NTG) the metric
+    // differences, except in the FP64 momentum sweep (REG, not NTG), where
+    // they made that kernel 4% slower. In the FP32 momentum sweep the three
+    // waves of a block share these lines: the sweep itself slows by 6% but
+    // the source kernels after it gain more (one FP32 pass 5% faster in all).
     T a[8];
 #pragma unroll
-    for (int m = REG ? 0 : m0; m < 8; m++) a[m] = REG ? creg[C_F4 + m] : coef[(C_F4 + m) * N + c];
+    for (int m = REG ? 0 : m0; m < 8; m++) a[m] = REG ? creg[C_F4 + m] : ld_pass(&coef[(C_F4 + m) * N + c]);
     T F0 = Fc[f[0]], F1 = Fc[f[1]], F2 = Fc[f[2]], F3 = Fc[f[3]], F4 = Fc[f[4]], F5 = Fc[f[5]];
     T b[5], e[5], h[5];
 #pragma unroll
     for (int m = REG ? 0 : m0; m < 5; m++) {
-        b[m] = REG ? creg[C_F1 + m] : coef[(C_F1 + m) * N + c];
-        e[m] = REG ? creg[C_F2 + m] : coef[(C_F2 + m) * N + c];
-        h[m] = REG ? creg[C_F3 + m] : coef[(C_F3 + m) * N + c];
+        b[m] = REG ? creg[C_F1 + m] : ld_pass(&coef[(C_F1 + m) * N + c]);
+        e[m] = REG ? creg[C_F2 + m] : ld_pass(&coef[(C_F2 + m) * N + c]);
+        h[m] = REG ? creg[C_F3 + m] : ld_pass(&coef[(C_F3 + m) * N + c]);
     }
     if (sum_diag && !REG) {
         a[0] = a[1] + a[2] + a[3] + a[4] + a[5] + a[6] + a[7];
@@ -515,8 +520,9 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
         e[0] = e[1] + e[2] + T(2) * e[3] + T(2) * e[4];
         h[0] = h[1] + h[2] + T(2) * h[3] + T(2) * h[4];
     }
-    const T gd0 = gdf[0 * N + c], gd1 = gdf[1 * N + c], gd2 = gdf[2 * N + c];
-    const T gd3 = gdf[3 * N + c], gd4 = gdf[4 * N + c], gd5 = gdf[5 * N + c];
+    const T gd0 = ld_pass<NTG>(&gdf[0 * N + c]), gd1 = ld_pass<NTG>(&gdf[1 * N + c]);
+    const T gd2 = ld_pass<NTG>(&gdf[2 * N + c]), gd3 = ld_pass<NTG>(&gdf[3 * N + c]);
+    const T gd4 = ld_pass<NTG>(&gdf[4 * N + c]), gd5 = ld_pass<NTG>(&gdf[5 * N + c]);
     // face 2 partner: cell (i, j+1, k)
     const int cr = cidx(d, i, jn, k);
     const T r2 = Fc[fidx(d, 2, i, jn, k)], r3 = Fc[fidx(d, 3, i, jn, k)], r4 = Fc[fidx(d, 4, i, jn, k)];
@@ -536,10 +542,10 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
     float *rC = MODE == MODE_GS ? nullptr : vw.rc + (size_t)cs * N;
     T q0 = T(0), q2 = T(0), q4 = T(0), q6 = T(0);
     if (MODE == MODE_DELTA) {
-        q0 = rC[c];
-        q2 = rF[f[1]];
-        q4 = rF[f[3]];
-        q6 = rF[f[5]];
+        q0 = ld_pass(&rC[c]);
+        q2 = ld_pass(&rF[f[1]]);
+        q4 = ld_pass(&rF[f[3]]);
+        q6 = ld_pass(&rF[f[5]]);
     }
     // ---- cell average
     const T S0 = a[1] * oc + a[2] * F0 + a[3] * F1 + a[4] * F2 + a[5] * F3 + a[6] * F4 + a[7] * F5 + B;
@@ -738,7 +744,7 @@ __global__ void __launch_bounds__(256) k_T_sweep_fused(Dev d, Sched sc)
         energy_coefs(d, c, i, j, k, [&](int slot, double x) { F[slot] = x; });
         const View<double> v = view64(d);
         double ec = 0., ef = 0.;
-        transport_unit<MODE_GS, double, true>(d, v, F_T, 3, nullptr, v.src[R_T * N + c], d.w_T, i, j, k, c, ec, ef, F);
+        transport_unit<MODE_GS, double, true, true>(d, v, F_T, 3, nullptr, v.src[R_T * N + c], d.w_T, i, j, k, c, ec, ef, F);
         e = fmax(ec, ef);
     }
     err_commit(d, F_T, e);
@@ -846,34 +852,34 @@ __device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const
         T qx[NAQ], qy[NAQ], qz[NAQ];
 #pragma unroll
         for (int m = 0; m < NAQ; m++) {
-            qx[m] = Q[m * N + c];
-            qy[m] = Q[(NAQ + m) * N + c];
-            qz[m] = Q[(2 * NAQ + m) * N + c];
+            qx[m] = ld_pass(&Q[m * N + c]);
+            qy[m] = ld_pass(&Q[(NAQ + m) * N + c]);
+            qz[m] = ld_pass(&Q[(2 * NAQ + m) * N + c]);
         }
         // x: faces of cell (i, j+1, k)
         const T Sx = R[cidx(d, i, jn, k)];
         const T x2 = P[fidx(d, 2, i, jn, k)], x3 = P[fidx(d, 3, i, jn, k)], x4 = P[fidx(d, 4, i, jn, k)];
         const T x5 = P[fidx(d, 5, i, jn, k)], x6 = P[fidx(d, 6, i, jn, k)];
-        const T Bx3 = gdf[0 * N + c] * tder(d, (const T *)P, 2, i, j, k, 0);
-        const T Bx4 = gdf[1 * N + c] * tder(d, (const T *)P, 2, i, j, k, 2);
+        const T Bx3 = ld_pass(&gdf[0 * N + c]) * tder(d, (const T *)P, 2, i, j, k, 0);
+        const T Bx4 = ld_pass(&gdf[1 * N + c]) * tder(d, (const T *)P, 2, i, j, k, 2);
         // y: faces of cell (i+1, j, k)
         const T Sy = R[cidx(d, ip, j, k)];
         const T y1 = P[fidx(d, 1, ip, j, k)], y2 = P[fidx(d, 2, ip, j, k)], y4 = P[fidx(d, 4, ip, j, k)];
         const T y5 = P[fidx(d, 5, ip, j, k)], y6 = P[fidx(d, 6, ip, j, k)];
-        const T By3 = gdf[2 * N + c] * tder(d, (const T *)P, 4, i, j, k, 1);
-        const T By4 = gdf[3 * N + c] * tder(d, (const T *)P, 4, i, j, k, 2);
+        const T By3 = ld_pass(&gdf[2 * N + c]) * tder(d, (const T *)P, 4, i, j, k, 1);
+        const T By4 = ld_pass(&gdf[3 * N + c]) * tder(d, (const T *)P, 4, i, j, k, 2);
         // z: faces of cell (i, j, k+1)
         const T Sz = R[cidx(d, i, j, kn)];
         const T z1 = P[fidx(d, 1, i, j, kn)], z2 = P[fidx(d, 2, i, j, kn)], z3 = P[fidx(d, 3, i, j, kn)];
         const T z4 = P[fidx(d, 4, i, j, kn)], z6 = P[fidx(d, 6, i, j, kn)];
-        const T Bz3 = gdf[4 * N + c] * tder(d, (const T *)P, 6, i, j, k, 1);
-        const T Bz4 = gdf[5 * N + c] * tder(d, (const T *)P, 6, i, j, k, 0);
+        const T Bz3 = ld_pass(&gdf[4 * N + c]) * tder(d, (const T *)P, 6, i, j, k, 1);
+        const T Bz4 = ld_pass(&gdf[5 * N + c]) * tder(d, (const T *)P, 6, i, j, k, 0);
         float *rF = MODE == MODE_GS ? nullptr : vw.rf + (size_t)F_P * d.NF;
         T q2 = T(0), q4 = T(0), q6 = T(0);
         if (MODE == MODE_DELTA) {
-            q2 = rF[f[1]];
-            q4 = rF[f[3]];
-            q6 = rF[f[5]];
+            q2 = ld_pass(&rF[f[1]]);
+            q4 = ld_pass(&rF[f[3]]);
+            q6 = ld_pass(&rF[f[5]]);
         }
         // one face equation: relax in place, or write its residual
         auto face = [&](T &pv, int slot, T diag, T S, T q) {

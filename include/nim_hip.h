@@ -32,7 +32,9 @@
  *    at context creation, each bitwise the same results: NIM_RSLAB=0 gives
  *    each XCD a contiguous range of planes instead of a radial slab;
  *    NIM_KFAST=1 numbers the per-cell kernels' blocks plane-fastest;
- *    NIM_TILE=1 computes the energy sources from LDS-staged tiles.
+ *    NIM_TILE=1 computes the energy sources from LDS-staged tiles;
+ *    NIM_NT=0/1 turns off/on the sweeps' non-temporal loads of what a pass
+ *    reads once (default: on from 2^19 cells).
  */
 #ifndef NIM_HIP_H
 #define NIM_HIP_H

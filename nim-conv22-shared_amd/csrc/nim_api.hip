@@ -24,10 +24,10 @@ __global__ void k_T_src(Dev d);
 template <int TI> __global__ void k_T_src_tile(Dev d);
 __global__ void k_mom_sweep(Dev d, Sched sc);
 __global__ void k_T_sweep(Dev d, Sched sc);
-__global__ void k_T_sweep_fused(Dev d, Sched sc);
+template <bool NTL> __global__ void k_T_sweep_fused(Dev d, Sched sc);
 __global__ void k_zero_flux(Dev d);
 __global__ void k_energy_coefs(Dev d);
-__global__ void k_ppe_sweep(Dev d, Sched sc);
+template <bool NTL> __global__ void k_ppe_sweep(Dev d, Sched sc);
 __global__ void k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4, int max_iter, int min_iter);
 __global__ void k_check(Dev d);
 __global__ void k_pressure_coeffs(Dev d);
@@ -42,12 +42,12 @@ __global__ void k_res_ppe(Dev d, View<double> v);
 __global__ void k_res_T_coef(Dev d, View<double> v);
 __global__ void k_res_T(Dev d, View<double> v);
 __global__ void k_d_mom_src(Dev d, View<float> v);
-__global__ void k_d_mom_sweep(Dev d, View<float> v, Sched sc);
+template <bool NTL> __global__ void k_d_mom_sweep(Dev d, View<float> v, Sched sc);
 __global__ void k_d_ppe_src(Dev d, View<float> v);
-__global__ void k_d_ppe_sweep(Dev d, View<float> v, Sched sc);
+template <bool NTL> __global__ void k_d_ppe_sweep(Dev d, View<float> v, Sched sc);
 __global__ void k_d_zero_flux(Dev d, View<float> v);
 __global__ void k_d_T_src(Dev d, View<float> v);
-__global__ void k_d_T_sweep(Dev d, View<float> v, Sched sc);
+template <bool NTL> __global__ void k_d_T_sweep(Dev d, View<float> v, Sched sc);
 __global__ void k_mix_update(Dev d, View<double> x, View<float> dx);
 __global__ void k_cast(const double *in, float *out, long n);
 __global__ void k_e_mom_src(Ens e);
@@ -92,6 +92,7 @@ static int32_t fail(int32_t code, const std::string &msg)
     } while (0)
 
 static constexpr int BS = 256;
+static constexpr int NT_MIN_CELLS = 1 << 19;  // see nim_ctx::nt
 
 struct nim_ctx {
     nim_dims dims;
@@ -112,6 +113,12 @@ struct nim_ctx {
     int slab = 0;
     bool rslab_sweeps = true;  // sweeps: one radial slab per XCD (NIM_RSLAB=0: storage order)
     int tile = 0;              // energy sources from LDS tiles of tile x 256/tile cells (NIM_TILE=1; 0: per cell)
+    // sweeps load what a pass reads once (coefficients, metric differences,
+    // FP32 residuals) non-temporally: from NT_MIN_CELLS cells on, where a
+    // pass's arrays overflow the caches (1M cells: the FP32 pass 5% faster);
+    // below, the arrays stay cached from pass to pass and plain loads are
+    // faster (up to 12% for the reference-sized cases). NIM_NT=0/1 forces it.
+    bool nt = false;
     std::vector<Sched> sched256;   // FP64 sweeps
     std::vector<Sched> dsched64, dsched256;  // FP32 correction sweeps (storage order: the radial slabs slow them)
     // FP64 loop: energy coefficients built inside the energy sweep instead
@@ -253,6 +260,8 @@ extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, n
         c->tile = !tiles ? 0 : c->NT % 64 == 0 ? 64 : c->NT % 32 == 0 ? 32 : 0;
         const char *f = getenv("NIM_FUSE_T");
         c->fuse_T = f ? atoi(f) != 0 : true;
+        const char *n = getenv("NIM_NT");
+        c->nt = n ? atoi(n) != 0 : c->N >= NT_MIN_CELLS;
     }
     init_schedules(c);
     double *gc, *An, *wall;
@@ -721,7 +730,8 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark, bool fused = false)
     case 2: hipLaunchKernelGGL(k_ppe_src, dim3(gS), dim3(BS), 0, s, d); mark(2); break;
     case 3:
         for (const Sched &sc : c->sched256) {
-            hipLaunchKernelGGL(k_ppe_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
+            hipLaunchKernelGGL(c->nt ? k_ppe_sweep<true> : k_ppe_sweep<false>, dim3(sched_blocks(sc)), dim3(BS), 0,
+                               s, d, sc);
             mark(3);
         }
         break;
@@ -735,7 +745,9 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark, bool fused = false)
     case 5: launch_T_src(c, s); mark(6); break;
     case 6:
         for (const Sched &sc : c->sched256) {
-            if (fused) hipLaunchKernelGGL(k_T_sweep_fused, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
+            if (fused)
+                hipLaunchKernelGGL(c->nt ? k_T_sweep_fused<true> : k_T_sweep_fused<false>, dim3(sched_blocks(sc)),
+                                   dim3(BS), 0, s, d, sc);
             else hipLaunchKernelGGL(k_T_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
             mark(7);
         }
@@ -902,18 +914,21 @@ static void launch_delta(nim_ctx *c, Mark mark)
     hipStream_t s = c->stream;
     hipLaunchKernelGGL(k_d_mom_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, v); mark(4);
     for (const Sched &sc : c->dsched64) {
-        hipLaunchKernelGGL(k_d_mom_sweep, dim3(sched_blocks(sc)), BS3, 0, s, d, v, sc);
+        hipLaunchKernelGGL(c->nt ? k_d_mom_sweep<true> : k_d_mom_sweep<false>, dim3(sched_blocks(sc)), BS3, 0, s, d,
+                           v, sc);
         mark(5);
     }
     hipLaunchKernelGGL(k_d_ppe_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, v); mark(6);
     for (const Sched &sc : c->dsched256) {
-        hipLaunchKernelGGL(k_d_ppe_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, v, sc);
+        hipLaunchKernelGGL(c->nt ? k_d_ppe_sweep<true> : k_d_ppe_sweep<false>, dim3(sched_blocks(sc)), dim3(BS), 0,
+                           s, d, v, sc);
         mark(7);
     }
     hipLaunchKernelGGL(k_d_zero_flux, dim3(nblk(2 * c->NT * c->NR)), dim3(BS), 0, s, d, v); mark(8);
     hipLaunchKernelGGL(k_d_T_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d, v); mark(9);
     for (const Sched &sc : c->dsched256) {
-        hipLaunchKernelGGL(k_d_T_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, v, sc);
+        hipLaunchKernelGGL(c->nt ? k_d_T_sweep<true> : k_d_T_sweep<false>, dim3(sched_blocks(sc)), dim3(BS), 0, s,
+                           d, v, sc);
         mark(10);
     }
 }

@@ -472,7 +472,7 @@ __global__ void __launch_bounds__(256) k_d_T_src(Dev d, View<float> v)
 // ff is the face field (F_U..F_T), cs the cell slot (0..3). The
 // coefficients come from the block coef ([NCF][N]) or, with REG, from the
 // caller's registers creg[NCF].
-template <int MODE, typename T, bool REG = false, bool NTG = !REG>
+template <int MODE, typename T, bool REG = false, bool NTL = false, bool NTG = NTL && !REG>
 __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, const T *__restrict__ coef, T B, T w,
                                int i, int j, int k, int c, double &ecell, double &eface, const T *creg = nullptr)
 {
@@ -498,21 +498,21 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
     // (the FP64 residual fixes what the corrections converge to).
     constexpr bool sum_diag = true;
     constexpr int m0 = sum_diag ? 1 : 0;
-    // Non-temporal loads of the coefficients and (NTG) the metric
-    // differences, except in the FP64 momentum sweep (REG, not NTG), where
-    // they made that kernel 4% slower. In the FP32 momentum sweep the three
-    // waves of a block share these lines: the sweep itself slows by 6% but
-    // the source kernels after it gain more (one FP32 pass 5% faster in all).
+    // NTL: non-temporal loads of the coefficients, residuals and (NTG) the
+    // metric differences. The FP64 momentum sweep (REG) keeps NTG off: the
+    // non-temporal loads made it 4% slower. In the FP32 momentum sweep the
+    // three waves of a block share these lines: the sweep itself slows by 6%
+    // but the source kernels after it gain more (one FP32 pass 5% faster).
     T a[8];
 #pragma unroll
-    for (int m = REG ? 0 : m0; m < 8; m++) a[m] = REG ? creg[C_F4 + m] : ld_pass(&coef[(C_F4 + m) * N + c]);
+    for (int m = REG ? 0 : m0; m < 8; m++) a[m] = REG ? creg[C_F4 + m] : ld_pass<NTL>(&coef[(C_F4 + m) * N + c]);
     T F0 = Fc[f[0]], F1 = Fc[f[1]], F2 = Fc[f[2]], F3 = Fc[f[3]], F4 = Fc[f[4]], F5 = Fc[f[5]];
     T b[5], e[5], h[5];
 #pragma unroll
     for (int m = REG ? 0 : m0; m < 5; m++) {
-        b[m] = REG ? creg[C_F1 + m] : ld_pass(&coef[(C_F1 + m) * N + c]);
-        e[m] = REG ? creg[C_F2 + m] : ld_pass(&coef[(C_F2 + m) * N + c]);
-        h[m] = REG ? creg[C_F3 + m] : ld_pass(&coef[(C_F3 + m) * N + c]);
+        b[m] = REG ? creg[C_F1 + m] : ld_pass<NTL>(&coef[(C_F1 + m) * N + c]);
+        e[m] = REG ? creg[C_F2 + m] : ld_pass<NTL>(&coef[(C_F2 + m) * N + c]);
+        h[m] = REG ? creg[C_F3 + m] : ld_pass<NTL>(&coef[(C_F3 + m) * N + c]);
     }
     if (sum_diag && !REG) {
         a[0] = a[1] + a[2] + a[3] + a[4] + a[5] + a[6] + a[7];
@@ -542,10 +542,10 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
     float *rC = MODE == MODE_GS ? nullptr : vw.rc + (size_t)cs * N;
     T q0 = T(0), q2 = T(0), q4 = T(0), q6 = T(0);
     if (MODE == MODE_DELTA) {
-        q0 = ld_pass(&rC[c]);
-        q2 = ld_pass(&rF[f[1]]);
-        q4 = ld_pass(&rF[f[3]]);
-        q6 = ld_pass(&rF[f[5]]);
+        q0 = ld_pass<NTL>(&rC[c]);
+        q2 = ld_pass<NTL>(&rF[f[1]]);
+        q4 = ld_pass<NTL>(&rF[f[3]]);
+        q6 = ld_pass<NTL>(&rF[f[5]]);
     }
     // ---- cell average
     const T S0 = a[1] * oc + a[2] * F0 + a[3] * F1 + a[4] * F2 + a[5] * F3 + a[6] * F4 + a[7] * F5 + B;
@@ -628,7 +628,7 @@ __device__ __forceinline__ bool sweep_cell(const Dev &d, const Sched &s, int lb,
 
 // residual pass and FP32 corrections of the mixed loop: threadIdx.y selects
 // the field (u, v, w are independent within the sweep)
-template <int MODE, typename T>
+template <int MODE, typename T, bool NTL = false>
 __device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, const Sched &sc, int lb)
 {
     int i, j, k;
@@ -640,7 +640,7 @@ __device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, const 
         int c = cidx(d, i, j, k);
         const T w = T(q == 0 ? d.w_u : q == 1 ? d.w_v : d.w_w);
         double ec = 0., ef = 0.;
-        transport_unit<MODE>(d, v, q, q, v.MF, v.src[q * N + c], w, i, j, k, c, ec, ef);
+        transport_unit<MODE, T, false, NTL>(d, v, q, q, v.MF, v.src[q * N + c], w, i, j, k, c, ec, ef);
         e = fmax(ec, ef);
     }
     if (MODE != MODE_DELTA) err_commit(d, q, e);
@@ -697,13 +697,16 @@ __global__ void __launch_bounds__(192) k_res_mom(Dev d, View<double> v)
 
 // FP32 corrections: one wave per component (the FP32 unit is light on
 // registers, and this runs 2% faster than one thread per cell)
+template <bool NTL>
 __global__ void __launch_bounds__(192) k_d_mom_sweep(Dev d, View<float> v, Sched sc)
 {
     if (d.ctl->done) return;
-    mom_sweep<MODE_DELTA>(d, v, sc, xcd_block());
+    mom_sweep<MODE_DELTA, float, NTL>(d, v, sc, xcd_block());
 }
+template __global__ void k_d_mom_sweep<false>(Dev, View<float>, Sched);
+template __global__ void k_d_mom_sweep<true>(Dev, View<float>, Sched);
 
-template <int MODE, typename T>
+template <int MODE, typename T, bool NTL = false>
 __device__ __forceinline__ void T_sweep(const Dev &d, const View<T> &v, const T *coef, const Sched &sc, int lb)
 {
     int i, j, k;
@@ -713,7 +716,7 @@ __device__ __forceinline__ void T_sweep(const Dev &d, const View<T> &v, const T 
         const int N = d.N;
         int c = cidx(d, i, j, k);
         double ec = 0., ef = 0.;
-        transport_unit<MODE>(d, v, F_T, 3, coef, v.src[R_T * N + c], T(d.w_T), i, j, k, c, ec, ef);
+        transport_unit<MODE, T, false, NTL>(d, v, F_T, 3, coef, v.src[R_T * N + c], T(d.w_T), i, j, k, c, ec, ef);
         e = fmax(ec, ef);
     }
     if (MODE != MODE_DELTA) err_commit(d, F_T, e);
@@ -729,6 +732,7 @@ __global__ void __launch_bounds__(256) k_T_sweep(Dev d, Sched sc)
 // the energy sweep with its coefficients built in registers from the
 // velocity (T_coef's formulas), for the FP64 loop: the coefficients never
 // go through memory
+template <bool NTL>
 __global__ void __launch_bounds__(256) k_T_sweep_fused(Dev d, Sched sc)
 {
     if (d.ctl->done) return;
@@ -744,11 +748,13 @@ __global__ void __launch_bounds__(256) k_T_sweep_fused(Dev d, Sched sc)
         energy_coefs(d, c, i, j, k, [&](int slot, double x) { F[slot] = x; });
         const View<double> v = view64(d);
         double ec = 0., ef = 0.;
-        transport_unit<MODE_GS, double, true, true>(d, v, F_T, 3, nullptr, v.src[R_T * N + c], d.w_T, i, j, k, c, ec, ef, F);
+        transport_unit<MODE_GS, double, true, NTL, NTL>(d, v, F_T, 3, nullptr, v.src[R_T * N + c], d.w_T, i, j, k, c, ec, ef, F);
         e = fmax(ec, ef);
     }
     err_commit(d, F_T, e);
 }
+template __global__ void k_T_sweep_fused<false>(Dev, Sched);
+template __global__ void k_T_sweep_fused<true>(Dev, Sched);
 
 __global__ void __launch_bounds__(256) k_res_T(Dev d, View<double> v)
 {
@@ -756,11 +762,14 @@ __global__ void __launch_bounds__(256) k_res_T(Dev d, View<double> v)
     T_sweep<MODE_RES>(d, v, v.TF, Sched{}, xcd_block());
 }
 
+template <bool NTL>
 __global__ void __launch_bounds__(256) k_d_T_sweep(Dev d, View<float> v, Sched sc)
 {
     if (d.ctl->done) return;
-    T_sweep<MODE_DELTA>(d, v, v.TF, sc, xcd_block());
+    T_sweep<MODE_DELTA, float, NTL>(d, v, v.TF, sc, xcd_block());
 }
+template __global__ void k_d_T_sweep<false>(Dev, View<float>, Sched);
+template __global__ void k_d_T_sweep<true>(Dev, View<float>, Sched);
 
 // apply_pressure_bc for the wall face at position pos of cell (i,j,k)
 // (bc.f90:147-341). Every term is linear in the unknowns (p, u, v, w, T and
@@ -822,7 +831,7 @@ __device__ void pressure_wall(const Dev &d, const View<T> &vw, int b, int pos, i
 // After the walls, every input of the three interior face updates is
 // gathered before the first store; the cell's own faces are carried in
 // registers, so face 4 sees the new face 2 and face 6 the new faces 2 and 4.
-template <int MODE, typename T>
+template <int MODE, typename T, bool NTL = false>
 __device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const Sched &sc, int lb)
 {
     int i, j, k;
@@ -852,34 +861,34 @@ __device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const
         T qx[NAQ], qy[NAQ], qz[NAQ];
 #pragma unroll
         for (int m = 0; m < NAQ; m++) {
-            qx[m] = ld_pass(&Q[m * N + c]);
-            qy[m] = ld_pass(&Q[(NAQ + m) * N + c]);
-            qz[m] = ld_pass(&Q[(2 * NAQ + m) * N + c]);
+            qx[m] = ld_pass<NTL>(&Q[m * N + c]);
+            qy[m] = ld_pass<NTL>(&Q[(NAQ + m) * N + c]);
+            qz[m] = ld_pass<NTL>(&Q[(2 * NAQ + m) * N + c]);
         }
         // x: faces of cell (i, j+1, k)
         const T Sx = R[cidx(d, i, jn, k)];
         const T x2 = P[fidx(d, 2, i, jn, k)], x3 = P[fidx(d, 3, i, jn, k)], x4 = P[fidx(d, 4, i, jn, k)];
         const T x5 = P[fidx(d, 5, i, jn, k)], x6 = P[fidx(d, 6, i, jn, k)];
-        const T Bx3 = ld_pass(&gdf[0 * N + c]) * tder(d, (const T *)P, 2, i, j, k, 0);
-        const T Bx4 = ld_pass(&gdf[1 * N + c]) * tder(d, (const T *)P, 2, i, j, k, 2);
+        const T Bx3 = ld_pass<NTL>(&gdf[0 * N + c]) * tder(d, (const T *)P, 2, i, j, k, 0);
+        const T Bx4 = ld_pass<NTL>(&gdf[1 * N + c]) * tder(d, (const T *)P, 2, i, j, k, 2);
         // y: faces of cell (i+1, j, k)
         const T Sy = R[cidx(d, ip, j, k)];
         const T y1 = P[fidx(d, 1, ip, j, k)], y2 = P[fidx(d, 2, ip, j, k)], y4 = P[fidx(d, 4, ip, j, k)];
         const T y5 = P[fidx(d, 5, ip, j, k)], y6 = P[fidx(d, 6, ip, j, k)];
-        const T By3 = ld_pass(&gdf[2 * N + c]) * tder(d, (const T *)P, 4, i, j, k, 1);
-        const T By4 = ld_pass(&gdf[3 * N + c]) * tder(d, (const T *)P, 4, i, j, k, 2);
+        const T By3 = ld_pass<NTL>(&gdf[2 * N + c]) * tder(d, (const T *)P, 4, i, j, k, 1);
+        const T By4 = ld_pass<NTL>(&gdf[3 * N + c]) * tder(d, (const T *)P, 4, i, j, k, 2);
         // z: faces of cell (i, j, k+1)
         const T Sz = R[cidx(d, i, j, kn)];
         const T z1 = P[fidx(d, 1, i, j, kn)], z2 = P[fidx(d, 2, i, j, kn)], z3 = P[fidx(d, 3, i, j, kn)];
         const T z4 = P[fidx(d, 4, i, j, kn)], z6 = P[fidx(d, 6, i, j, kn)];
-        const T Bz3 = ld_pass(&gdf[4 * N + c]) * tder(d, (const T *)P, 6, i, j, k, 1);
-        const T Bz4 = ld_pass(&gdf[5 * N + c]) * tder(d, (const T *)P, 6, i, j, k, 0);
+        const T Bz3 = ld_pass<NTL>(&gdf[4 * N + c]) * tder(d, (const T *)P, 6, i, j, k, 1);
+        const T Bz4 = ld_pass<NTL>(&gdf[5 * N + c]) * tder(d, (const T *)P, 6, i, j, k, 0);
         float *rF = MODE == MODE_GS ? nullptr : vw.rf + (size_t)F_P * d.NF;
         T q2 = T(0), q4 = T(0), q6 = T(0);
         if (MODE == MODE_DELTA) {
-            q2 = ld_pass(&rF[f[1]]);
-            q4 = ld_pass(&rF[f[3]]);
-            q6 = ld_pass(&rF[f[5]]);
+            q2 = ld_pass<NTL>(&rF[f[1]]);
+            q4 = ld_pass<NTL>(&rF[f[3]]);
+            q6 = ld_pass<NTL>(&rF[f[5]]);
         }
         // one face equation: relax in place, or write its residual
         auto face = [&](T &pv, int slot, T diag, T S, T q) {
@@ -925,11 +934,14 @@ __device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const
     if (MODE != MODE_DELTA) err_commit(d, F_P, e);
 }
 
+template <bool NTL>
 __global__ void __launch_bounds__(256) k_ppe_sweep(Dev d, Sched sc)
 {
     if (d.ctl->done) return;
-    ppe_sweep<MODE_GS>(d, view64(d), sc, xcd_block());
+    ppe_sweep<MODE_GS, double, NTL>(d, view64(d), sc, xcd_block());
 }
+template __global__ void k_ppe_sweep<false>(Dev, Sched);
+template __global__ void k_ppe_sweep<true>(Dev, Sched);
 
 __global__ void __launch_bounds__(256) k_res_ppe(Dev d, View<double> v)
 {
@@ -937,11 +949,14 @@ __global__ void __launch_bounds__(256) k_res_ppe(Dev d, View<double> v)
     ppe_sweep<MODE_RES>(d, v, Sched{}, xcd_block());
 }
 
+template <bool NTL>
 __global__ void __launch_bounds__(256) k_d_ppe_sweep(Dev d, View<float> v, Sched sc)
 {
     if (d.ctl->done) return;
-    ppe_sweep<MODE_DELTA>(d, v, sc, xcd_block());
+    ppe_sweep<MODE_DELTA, float, NTL>(d, v, sc, xcd_block());
 }
+template __global__ void k_d_ppe_sweep<false>(Dev, View<float>, Sched);
+template __global__ void k_d_ppe_sweep<true>(Dev, View<float>, Sched);
 
 // ---------------------------------------------------------------------
 // mixed precision: FP32 corrections between FP64 residuals
@@ -1365,14 +1380,14 @@ __global__ void __launch_bounds__(256) k_e_mom_sweep(Ens e, int color)
 __global__ void __launch_bounds__(256) k_e_ppe_src(Ens e) { ENS_CASE ppe_src<MODE_GS>(d, view64(d), lb); }
 __global__ void __launch_bounds__(256) k_e_ppe_sweep(Ens e, int color)
 {
-    ENS_CASE ppe_sweep<MODE_GS>(d, view64(d), full_phase(d, color), lb);
+    ENS_CASE ppe_sweep<MODE_GS, double, true>(d, view64(d), full_phase(d, color), lb);
 }
 __global__ void __launch_bounds__(256) k_e_T_coef(Ens e) { ENS_CASE T_coef<MODE_GS>(d, view64(d), lb); }
 __global__ void __launch_bounds__(256) k_e_T_src(Ens e) { ENS_CASE T_src(d, view64(d), lb); }
 __global__ void __launch_bounds__(256) k_e_T_sweep(Ens e, int color)
 {
     ENS_CASE View<double> v = view64(d);
-    T_sweep<MODE_GS>(d, v, v.TF, full_phase(d, color), lb);
+    T_sweep<MODE_GS, double, true>(d, v, v.TF, full_phase(d, color), lb);
 }
 // one block per case
 __global__ void __launch_bounds__(64) k_e_check(Ens e) { check(e.dev[blockIdx.x]); }

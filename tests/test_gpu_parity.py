@@ -325,3 +325,11 @@ def test_tiled_sources_are_bitwise(monkeypatch, dims):
     tiles at n_theta = 128, 32-wide tiles with a partial last row of tiles at
     n_theta = 96, n_r = 13."""
     _bitwise_under(monkeypatch, "NIM_TILE", dims)
+
+
+@pytest.mark.gpu
+def test_non_temporal_sweep_loads_are_bitwise(monkeypatch):
+    """The sweeps with non-temporal loads of what a pass reads once
+    (NIM_NT=1, the default from 2^19 cells on) load the same values as with
+    plain loads, in the FP64 loop and in the FP32 passes of the mixed loop."""
+    _bitwise_under(monkeypatch, "NIM_NT", (128, 64, 4))

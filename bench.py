@@ -133,8 +133,9 @@ def _pmc_rows(d):
     for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
         for r in csv.DictReader(open(path)):
             n = r["Kernel_Name"]
-            # the plain kernel name: no namespace, template arguments or parameters
-            n = "copy_probe" if "neg_kernel" in n else n.split("(")[0].split("<")[0].replace("nim::", "")
+            # the plain kernel name: no return type (templates), namespace,
+            # template arguments or parameters
+            n = "copy_probe" if "neg_kernel" in n else n.split("(")[0].split("<")[0].split(" ")[-1].replace("nim::", "")
             rows.append((n, float(r["Counter_Value"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
     return rows
 

@@ -22,7 +22,7 @@ __global__ void k_mom_src(Dev d);
 __global__ void k_ppe_src(Dev d);
 __global__ void k_T_src(Dev d);
 template <int TI> __global__ void k_T_src_tile(Dev d);
-__global__ void k_mom_sweep(Dev d, Sched sc);
+template <bool NTL> __global__ void k_mom_sweep(Dev d, Sched sc);
 __global__ void k_T_sweep(Dev d, Sched sc);
 template <bool NTL> __global__ void k_T_sweep_fused(Dev d, Sched sc);
 __global__ void k_zero_flux(Dev d);
@@ -723,7 +723,8 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark, bool fused = false)
     case 0: hipLaunchKernelGGL(k_mom_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d); mark(0); break;
     case 1:
         for (const Sched &sc : c->sched256) {
-            hipLaunchKernelGGL(k_mom_sweep, dim3(sched_blocks(sc)), dim3(BS), 0, s, d, sc);
+            hipLaunchKernelGGL(c->nt ? k_mom_sweep<true> : k_mom_sweep<false>, dim3(sched_blocks(sc)), dim3(BS), 0,
+                               s, d, sc);
             mark(1);
         }
         break;

@@ -471,10 +471,12 @@ __global__ void __launch_bounds__(256) k_d_T_src(Dev d, View<float> v)
 // location read here, so loading early changes no value.
 // ff is the face field (F_U..F_T), cs the cell slot (0..3). The
 // coefficients come from the block coef ([NCF][N]) or, with REG, from the
-// caller's registers creg[NCF].
+// caller's registers creg[NCF]; the metric differences from the block or,
+// when given, from the caller's registers greg[6].
 template <int MODE, typename T, bool REG = false, bool NTL = false, bool NTG = NTL && !REG>
 __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, const T *__restrict__ coef, T B, T w,
-                               int i, int j, int k, int c, double &ecell, double &eface, const T *creg = nullptr)
+                               int i, int j, int k, int c, double &ecell, double &eface, const T *creg = nullptr,
+                               const T *greg = nullptr)
 {
     const int N = d.N;
     T *__restrict__ Fc = vw.fc + (size_t)ff * d.NF;
@@ -499,10 +501,9 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
     constexpr bool sum_diag = true;
     constexpr int m0 = sum_diag ? 1 : 0;
     // NTL: non-temporal loads of the coefficients, residuals and (NTG) the
-    // metric differences. The FP64 momentum sweep (REG) keeps NTG off: the
-    // non-temporal loads made it 4% slower. In the FP32 momentum sweep the
-    // three waves of a block share these lines: the sweep itself slows by 6%
-    // but the source kernels after it gain more (one FP32 pass 5% faster).
+    // metric differences. In the FP32 momentum sweep the three waves of a
+    // block share these lines: the sweep itself slows by 6% but the source
+    // kernels after it gain more (one FP32 pass 5% faster).
     T a[8];
 #pragma unroll
     for (int m = REG ? 0 : m0; m < 8; m++) a[m] = REG ? creg[C_F4 + m] : ld_pass<NTL>(&coef[(C_F4 + m) * N + c]);
@@ -520,9 +521,10 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
         e[0] = e[1] + e[2] + T(2) * e[3] + T(2) * e[4];
         h[0] = h[1] + h[2] + T(2) * h[3] + T(2) * h[4];
     }
-    const T gd0 = ld_pass<NTG>(&gdf[0 * N + c]), gd1 = ld_pass<NTG>(&gdf[1 * N + c]);
-    const T gd2 = ld_pass<NTG>(&gdf[2 * N + c]), gd3 = ld_pass<NTG>(&gdf[3 * N + c]);
-    const T gd4 = ld_pass<NTG>(&gdf[4 * N + c]), gd5 = ld_pass<NTG>(&gdf[5 * N + c]);
+    T gd[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) gd[m] = greg ? greg[m] : ld_pass<NTG>(&gdf[m * N + c]);
+    const T gd0 = gd[0], gd1 = gd[1], gd2 = gd[2], gd3 = gd[3], gd4 = gd[4], gd5 = gd[5];
     // face 2 partner: cell (i, j+1, k)
     const int cr = cidx(d, i, jn, k);
     const T r2 = Fc[fidx(d, 2, i, jn, k)], r3 = Fc[fidx(d, 3, i, jn, k)], r4 = Fc[fidx(d, 4, i, jn, k)];
@@ -648,8 +650,10 @@ __device__ __forceinline__ void mom_sweep(const Dev &d, const View<T> &v, const 
 
 // The same with one thread per cell for the three components: the cell's
 // momentum coefficients M_F (shared by u, v and w; in FP64 with the
-// diagonals re-summed) are loaded once and handed to each unit in registers.
-template <int MODE, typename T>
+// diagonals re-summed) and its metric differences are loaded once and handed
+// to each unit in registers (the units' stores would otherwise make each
+// reload them).
+template <int MODE, typename T, bool NTL = false>
 __device__ __forceinline__ void mom_sweep_cell(const Dev &d, const View<T> &v, const Sched &sc, int lb)
 {
     int i, j, k;
@@ -661,7 +665,7 @@ __device__ __forceinline__ void mom_sweep_cell(const Dev &d, const View<T> &v, c
         const T *__restrict__ coef = v.MF;
         T creg[NCF];
 #pragma unroll
-        for (int m = 0; m < NCF; m++) creg[m] = coef[m * N + c];
+        for (int m = 0; m < NCF; m++) creg[m] = ld_pass<NTL>(&coef[m * N + c]);
         if (sizeof(T) == sizeof(double)) {
             // the FP64 sweep re-sums the diagonals as transport_unit does
             T *a = creg + C_F4;
@@ -670,11 +674,14 @@ __device__ __forceinline__ void mom_sweep_cell(const Dev &d, const View<T> &v, c
             for (int b0 = C_F1; b0 <= C_F3; b0 += 5)
                 creg[b0] = creg[b0 + 1] + creg[b0 + 2] + T(2) * creg[b0 + 3] + T(2) * creg[b0 + 4];
         }
+        T greg[6];
+#pragma unroll
+        for (int m = 0; m < 6; m++) greg[m] = v.gc[(G_DIFF + m) * N + c];
         const T w[3] = {T(d.w_u), T(d.w_v), T(d.w_w)};
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             double ec = 0., ef = 0.;
-            transport_unit<MODE, T, true>(d, v, q, q, coef, v.src[q * N + c], w[q], i, j, k, c, ec, ef, creg);
+            transport_unit<MODE, T, true>(d, v, q, q, coef, v.src[q * N + c], w[q], i, j, k, c, ec, ef, creg, greg);
             e[q] = fmax(ec, ef);
         }
     }
@@ -683,11 +690,14 @@ __device__ __forceinline__ void mom_sweep_cell(const Dev &d, const View<T> &v, c
         for (int q = 0; q < 3; q++) err_commit(d, q, e[q]);
 }
 
+template <bool NTL>
 __global__ void __launch_bounds__(256) k_mom_sweep(Dev d, Sched sc)
 {
     if (d.ctl->done) return;
-    mom_sweep_cell<MODE_GS>(d, view64(d), sc, xcd_block());
+    mom_sweep_cell<MODE_GS, double, NTL>(d, view64(d), sc, xcd_block());
 }
+template __global__ void k_mom_sweep<false>(Dev, Sched);
+template __global__ void k_mom_sweep<true>(Dev, Sched);
 
 __global__ void __launch_bounds__(192) k_res_mom(Dev d, View<double> v)
 {

@@ -117,7 +117,7 @@ struct nim_ctx {
     // FP32 residuals) non-temporally: from NT_MIN_CELLS cells on, where a
     // pass's arrays overflow the caches (1M cells: the FP32 pass 5% faster);
     // below, the arrays stay cached from pass to pass and plain loads are
-    // faster (up to 12% for the reference-sized cases). NIM_NT=0/1 forces it.
+    // faster (the mixed loop 16% at 110,592 cells). NIM_NT=0/1 forces it.
     bool nt = false;
     std::vector<Sched> sched256;   // FP64 sweeps
     std::vector<Sched> dsched64, dsched256;  // FP32 correction sweeps (storage order: the radial slabs slow them)

@@ -158,17 +158,28 @@ void nim_internal_lengths(nim_ctx *c, double *R_i, double *R_o)
 
 static void init_schedules(nim_ctx *c);
 
+// The device code addresses per-cell slot groups as slot * N + c and the
+// face fields as field * NF + f in 32-bit integers. The largest products are
+// the 45 pressure-coefficient slots (45 N) and the five face fields (5 NF),
+// so both must stay within INT32_MAX: at most 47,721,858 cells.
+static constexpr int64_t MAX_SLOTS_PER_CELL = 45;
+
 static bool dims_ok(const nim_dims *d)
 {
     // n_z >= 4: the four-point closures of the two adiabatic walls
     // (apply_zero_flux) would otherwise read each other's wall face
-    return d && d->n_theta >= 4 && d->n_theta % 2 == 0 && d->n_r >= 3 && d->n_z >= 4 &&
-           (int64_t)d->n_theta * d->n_r * d->n_z < (int64_t)1 << 30;
+    if (!d || d->n_theta < 4 || d->n_theta % 2 != 0 || d->n_r < 3 || d->n_z < 4) return false;
+    const int64_t NT = d->n_theta, NR = d->n_r, NZ = d->n_z;
+    if (NT * NR > INT32_MAX) return false;  // keeps the next product within int64
+    const int64_t N = NT * NR * NZ, NF = 3 * N + NT * NZ + NT * NR;
+    return MAX_SLOTS_PER_CELL * N <= INT32_MAX && 5 * NF <= INT32_MAX;
 }
 
 extern "C" int32_t nim_counts(const nim_dims *d, int64_t *nmesh, int64_t *nface, int64_t *nbc)
 {
-    if (!dims_ok(d)) return fail(NIM_E_ARG, "unsupported dimensions (n_theta even >= 4, n_r >= 3, n_z >= 4)");
+    if (!dims_ok(d))
+        return fail(NIM_E_ARG, "unsupported dimensions (n_theta even >= 4, n_r >= 3, n_z >= 4, "
+                               "at most 47,721,858 cells)");
     int64_t NT = d->n_theta, NR = d->n_r, NZ = d->n_z;
     if (nmesh) *nmesh = NT * NR * NZ;
     if (nface) *nface = 3 * NT * NR * NZ + NT * NZ + NT * NR;  // parameters.f90:117

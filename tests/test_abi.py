@@ -55,6 +55,25 @@ def test_unsupported_dimensions_rejected():
         nim_amd.counts(35, 6, 12)  # odd azimuthal count cannot be 4-coloured
 
 
+def test_cases_beyond_32_bit_device_indexing_rejected():
+    """The device code indexes slot groups as slot * N + c in 32-bit ints
+    (up to 45 N for the pressure coefficients, 5 NF for the face fields):
+    nim_counts and nim_ctx_create refuse larger cases with NIM_E_ARG before
+    any device use."""
+    import pytest
+    # 47,185,920 cells: the largest 45 N within INT32_MAX here
+    assert nim_amd.counts(1024, 512, 90)[0] == 47185920
+    for dims in ((1024, 512, 134), (1024, 512, 92), (65536, 65536, 65536), (2**30, 2**30, 2**30)):
+        with pytest.raises(nim_amd.NimError):
+            nim_amd.counts(*dims)
+        L = nim_amd.lib()
+        h = ctypes.c_void_p()
+        prm = nim_amd.Params()
+        assert L.nim_ctx_create(ctypes.byref(nim_amd.Dims(*dims)), ctypes.byref(prm), ctypes.byref(h)) == \
+            nim_amd.NIM_E_ARG
+        assert b"47,721,858" in L.nim_last_error()
+
+
 def test_argument_errors_before_any_device_use():
     """Entry points validate their arguments before they touch a device, so
     these run without one and return NIM_E_ARG with a message."""

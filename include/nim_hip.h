@@ -174,6 +174,21 @@ int32_t nim_residual(nim_ctx *ctx, double err[5]);
  * which integrate_nse runs after its loop. */
 int32_t nim_end_step(nim_ctx *ctx, double *dudt_max, double *dTdt_max, double *dilatation);
 
+/* Allocation-free form of nim_end_step. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It runs
+ * the same kernels, with their per-block partial results in a fixed device
+ * array of the library (3 MiB per device, held until the process ends)
+ * instead of the context's buffer, and calls no hipMalloc, hipFree or other
+ * allocating routine. out is caller-owned device memory of the context's
+ * device for 3 doubles {dudt_max, dTdt_max, dilatation}; the work is only
+ * enqueued on the context's stream (read out after nim_synchronize). Also
+ * writes meshes_dila, as nim_end_step does. At most 16,777,216 cells
+ * (NIM_E_ARG beyond). All calls on one device share the array: calls that
+ * overlap in time on one device (other threads, other contexts' streams)
+ * race. */
+int32_t nim_end_step_noalloc(nim_ctx *ctx, double *out);
+
 /* update_time_step (nse.f90:1292-1313). */
 int32_t nim_advance_time(nim_ctx *ctx);
 

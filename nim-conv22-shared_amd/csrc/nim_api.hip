@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -94,11 +95,44 @@ static int32_t fail(int32_t code, const std::string &msg)
 static constexpr int BS = 256;
 static constexpr int NT_MIN_CELLS = 1 << 19;  // see nim_ctx::nt
 
+// Scratch of nim_end_step_noalloc: the per-block partial results of the
+// end-of-step reductions (6 doubles per block of 256 cells), in a
+// file-scope device array instead of the context's allocation, so that the
+// entry point allocates nothing. Each device holds its own copy for the
+// life of the process; every call on a device shares it.
+static constexpr int NOALLOC_MAX_CELLS = 1 << 24;
+static constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)6 * (NOALLOC_MAX_CELLS / 256);
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+// Device address of `scratch` on the current device, resolved once per
+// device (hipGetSymbolAddress with that device current) and cached. The
+// array's name is only a host-side stand-in; host code uses this address.
+static double *noalloc_scratch_current_device()
+{
+    static std::mutex mu;
+    static std::vector<double *> ptrs;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((size_t)dev >= ptrs.size()) ptrs.resize(dev + 1, nullptr);
+    if (!ptrs[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        ptrs[dev] = (double *)p;
+    }
+    return ptrs[dev];
+}
+
 struct nim_ctx {
     nim_dims dims;
     nim_params prm;
     int NT, NR, NZ, N, NF, NB, oT, oZ;
     hipStream_t stream = nullptr;
+    int device = 0;             // the device current at creation
+    double *scratch = nullptr;  // this device's address of `scratch` (nim_end_step_noalloc)
     Dev dev;
     std::vector<int32_t> slot_ref, ref_slot;
     bool has_geo = false, has_pcoef = false;
@@ -309,6 +343,13 @@ extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, n
         nim_ctx_destroy(c);
         return fail(NIM_E_HIP, "hipStreamCreate failed");
     }
+    // set-up of the allocation-free entry point on this device (null if
+    // unavailable: nim_end_step_noalloc then reports NIM_E_HIP)
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        nim_ctx_destroy(c);
+        return fail(NIM_E_HIP, "hipGetDevice failed");
+    }
+    c->scratch = noalloc_scratch_current_device();
 #undef TRY
     *out = c;
     return NIM_OK;
@@ -1040,6 +1081,31 @@ extern "C" int32_t nim_end_step(nim_ctx *c, double *dudt, double *dTdt, double *
     if (dudt) *dudt = c->h_out[0];
     if (dTdt) *dTdt = c->h_out[1];
     if (dil) *dil = c->h_out[2];
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_end_step_noalloc(nim_ctx *c, double *out)
+{
+    if (!c || !out) return fail(NIM_E_ARG, "null argument");
+    if (c->N > NOALLOC_MAX_CELLS)
+        return fail(NIM_E_ARG, "nim_end_step_noalloc: at most 16,777,216 cells (fixed scratch size)");
+    if (!c->scratch) return fail(NIM_E_HIP, "nim_end_step_noalloc: scratch address unavailable on this device");
+    // `out` must be device-accessible: a kernel writes it
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, out) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(NIM_E_ARG, "nim_end_step_noalloc: out is not device memory");
+    }
+    if ((at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged) || at.device != c->device)
+        return fail(NIM_E_ARG, "nim_end_step_noalloc: out is not memory of the context's device");
+    // the same kernels as nim_end_step, with the partials in `scratch`;
+    // every block writes its partials before the final kernel reads them,
+    // so nothing needs clearing
+    Dev d = c->dev;
+    d.red = c->scratch;
+    hipLaunchKernelGGL(k_end_step_partial, dim3(c->nred), dim3(BS), 0, c->stream, d);
+    hipLaunchKernelGGL(k_end_step_final, dim3(1), dim3(BS), 0, c->stream, d, c->nred, out);
+    HIPCHK(hipGetLastError());
     return NIM_OK;
 }
 

@@ -91,6 +91,8 @@ def lib():
         L.nim_inner_solve.argtypes = [vp, _P, i32, i32, _PI, _P]
         L.nim_end_step.restype = i32
         L.nim_end_step.argtypes = [vp, _P, _P, _P]
+        L.nim_end_step_noalloc.restype = i32
+        L.nim_end_step_noalloc.argtypes = [vp, vp]
         L.nim_nusselt.restype = i32
         L.nim_nusselt.argtypes = [vp, _P]
         L.nim_run_stage.restype = i32
@@ -155,7 +157,7 @@ def exported_symbols():
     """Names of the C-ABI entry points declared in include/nim_hip.h."""
     return ["nim_counts", "nim_face_map", "nim_ctx_create", "nim_ctx_destroy", "nim_upload_geometry",
             "nim_build_pressure_coeffs", "nim_upload_state", "nim_download_state", "nim_build_momentum_coeffs",
-            "nim_inner_solve", "nim_end_step", "nim_advance_time", "nim_nusselt", "nim_run_stage",
+            "nim_inner_solve", "nim_end_step", "nim_end_step_noalloc", "nim_advance_time", "nim_nusselt", "nim_run_stage",
             "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_kernel_times",
             "nim_inner_solve_mixed", "nim_residual", "nim_kernel_times_mixed", "nim_ens_create", "nim_ens_destroy",
             "nim_ens_inner_solve", "nim_revolve_plan", "nim_ckpt_create", "nim_ckpt_destroy", "nim_ckpt_store",
@@ -292,6 +294,13 @@ class Context:
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _check(self.L.nim_end_step(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return dict(dudt_max=a.value, dTdt_max=b.value, Dilatation=c.value)
+
+    def end_step_noalloc(self, out_ptr):
+        """nim_end_step_noalloc: enqueue the end-of-step reductions with the
+        library's fixed scratch, writing {dudt_max, dTdt_max, Dilatation} to
+        the device address out_ptr (3 doubles on the context's device).
+        Synchronize before reading them."""
+        _check(self.L.nim_end_step_noalloc(self.h, ctypes.c_void_p(int(out_ptr))))
 
     def advance_time(self):
         _check(self.L.nim_advance_time(self.h))

@@ -304,6 +304,9 @@ def main():
     # (parameters.f90:146-150); its max increment relaxes with a slow
     # constant mode (p is defined up to a constant)
     ap.add_argument("--tol-p", type=float, default=5e-5)
+    ap.add_argument("--p-measure", choices=("maxabs", "halfrange"), default="maxabs",
+                    help="what tol-p bounds: max |dp| (find_error) or half the range of dp (blind to p's constant)")
+    ap.add_argument("--max-iter", type=int, default=200000, help="cap on passes (FP64) or cycles (mixed) per step")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
@@ -340,6 +343,7 @@ def main():
     t_setup = time.perf_counter()
     ctx, geo, st = setup(case)
     t_setup = time.perf_counter() - t_setup
+    ctx.set_p_measure(nim_amd.P_HALFRANGE if args.p_measure == "halfrange" else nim_amd.P_MAXABS)
     if args.adjoint_only:
         ctx.build_momentum_coeffs()
         ms = time_reversed_pass(ctx, reps=20)
@@ -370,10 +374,10 @@ def main():
                 t0 = time.perf_counter()
             ctx.build_momentum_coeffs()
             if mixed:
-                rc, n, err = ctx.inner_solve_mixed(tol, inner=args.inner, max_cycles=20000)
+                rc, n, err = ctx.inner_solve_mixed(tol, inner=args.inner, max_cycles=args.max_iter)
                 n = (n - 1) * args.inner
             else:
-                rc, n, err = ctx.inner_solve(tol, max_iter=20000, min_iter=2 if first else 0)
+                rc, n, err = ctx.inner_solve(tol, max_iter=args.max_iter, min_iter=2 if first else 0)
             print("%s step: %d passes, err %s" % ("mixed" if mixed else "fp64", n, " ".join("%.2e" % e for e in err)),
                   file=sys.stderr, flush=True)
             if last:
@@ -458,8 +462,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded 1e-3 noise on cell averages)",
         "config": {"workload": "%s: %d x %d x %d (theta x r x z) = %d cells, Ra=%g, Pr=%g, DT=%g, FP64, "
-                               "tol %g (p %g)" % (args.case, case.dims[0], case.dims[1], case.dims[2], N, case.Ra,
-                                                  case.Pr, case.DT, args.tol, args.tol_p),
+                               "tol %g (p %g on %s)" % (args.case, case.dims[0], case.dims[1], case.dims[2], N, case.Ra,
+                                                        case.Pr, case.DT, args.tol, args.tol_p,
+                                                        "max |dp|" if args.p_measure == "maxabs"
+                                                        else "half the range of dp"),
                    "cells": N, "precision": "fp64", "sweep_order": "4-colour"},
         "steps_per_s": args.steps / wall,
         "inner_iterations": total_iters,

@@ -151,6 +151,20 @@ int32_t nim_build_momentum_coeffs(nim_ctx *ctx);
 int32_t nim_inner_solve(nim_ctx *ctx, const double tol[5], int32_t max_iter, int32_t min_iter,
                         int32_t *iters, double err[5]);
 
+/* The measure that tol[3] (pressure) bounds in nim_inner_solve,
+ * nim_inner_solve_mixed, nim_residual and ensembles of this context, and
+ * that err[3] returns:
+ *   NIM_P_MAXABS     max |dp| over the faces, find_error's measure
+ *                    (nse.f90:260-325); the default
+ *   NIM_P_HALFRANGE  (max dp - min dp) / 2 over the faces.
+ * p is determined only up to a constant (the pressure equation has
+ * Neumann-type walls only), and the relaxation moves p's constant level by a
+ * nearly uniform amount every pass, which max |dp| counts and the half range
+ * does not. The iteration is the same; only the stopping test differs. Read
+ * at every solve call; the oracle has the same measure. */
+enum { NIM_P_MAXABS = 0, NIM_P_HALFRANGE = 1 };
+int32_t nim_set_p_measure(nim_ctx *ctx, int32_t mode);
+
 /* Mixed-precision form of the same inner loop (integrate_nse,
  * nse.f90:117-176); the reference has no counterpart. Each cycle computes in
  * FP64 the residual r = S(x) - a x of every unknown's equation (the equation

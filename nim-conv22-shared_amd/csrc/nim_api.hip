@@ -29,7 +29,8 @@ template <bool NTL> __global__ void k_T_sweep_fused(Dev d, Sched sc);
 __global__ void k_zero_flux(Dev d);
 __global__ void k_energy_coefs(Dev d);
 template <bool NTL> __global__ void k_ppe_sweep(Dev d, Sched sc);
-__global__ void k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4, int max_iter, int min_iter);
+__global__ void k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4, int max_iter, int min_iter,
+                              int p_measure);
 __global__ void k_check(Dev d);
 __global__ void k_pressure_coeffs(Dev d);
 __global__ void k_wall_pressure_coeffs(Dev d);
@@ -131,6 +132,7 @@ struct nim_ctx {
     nim_params prm;
     int NT, NR, NZ, N, NF, NB, oT, oZ;
     hipStream_t stream = nullptr;
+    int p_measure = P_MAXABS;   // pressure convergence measure (nim_set_p_measure)
     int device = 0;             // the device current at creation
     double *scratch = nullptr;  // this device's address of `scratch` (nim_end_step_noalloc)
     Dev dev;
@@ -331,7 +333,7 @@ extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, n
     TRY(dalloc(c, &d.src, (size_t)NSRC * N));
     TRY(dalloc(c, &d.red, (size_t)6 * c->nred + 64));
     TRY(dalloc(c, &d.ctl, (size_t)1));
-    TRY(dalloc(c, &d.errs, (size_t)2 * 5 * ERR_COPIES * ERR_STRIDE));
+    TRY(dalloc(c, &d.errs, (size_t)2 * NERR * ERR_COPIES * ERR_STRIDE));
     TRY(dalloc(c, &c->d_out, (size_t)16));
     d.gc = gc; d.An = An; d.wall = wall; d.wpts = wpts;
     if (hipHostMalloc((void **)&c->h_ctl, sizeof(Ctl), 0) != hipSuccess ||
@@ -822,8 +824,16 @@ static void launch_iteration(nim_ctx *c)
 static int32_t reset_ctl(nim_ctx *c, const double tol[5], int max_iter, int min_iter)
 {
     hipLaunchKernelGGL(k_solve_reset, dim3(1), dim3(256), 0, c->stream, c->dev, tol[0], tol[1], tol[2], tol[3],
-                       tol[4], max_iter, min_iter);
+                       tol[4], max_iter, min_iter, c->p_measure);
     HIPCHK(hipGetLastError());
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_set_p_measure(nim_ctx *c, int32_t mode)
+{
+    if (!c) return fail(NIM_E_ARG, "null context");
+    if (mode != NIM_P_MAXABS && mode != NIM_P_HALFRANGE) return fail(NIM_E_ARG, "unknown pressure measure");
+    c->p_measure = mode;
     return NIM_OK;
 }
 

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cfloat>
+
 namespace nim {
 
 // per-cell geometry slots in Dev::gc
@@ -89,13 +91,21 @@ __device__ __forceinline__ T ld_pass(const T *p)
 
 constexpr int ERR_COPIES = 64;  // spread of the per-iteration max-increment atomics
 constexpr int ERR_STRIDE = 16;  // one 128-byte cache line per copy
+// reduced per pass: max |increment| of u, v, w, p, T (fields 0-4), and the
+// largest pressure increment and the largest negated one (5, 6; ordered
+// keys of signed doubles), whose half sum is half the range of the
+// pressure increments
+enum { E_PHI = 5, E_PLO = 6, NERR = 7 };
 __device__ __host__ inline size_t err_slot(int slot, int field, int copy)
 {
-    return ((size_t)(slot * 5 + field) * ERR_COPIES + copy) * ERR_STRIDE;
+    return ((size_t)(slot * NERR + field) * ERR_COPIES + copy) * ERR_STRIDE;
 }
 
+// pressure convergence measure (Ctl::p_measure)
+enum { P_MAXABS = 0, P_HALFRANGE = 1 };
+
 struct Ctl {
-    int32_t done, iter, status, max_iter, min_iter, pad[3];
+    int32_t done, iter, status, max_iter, min_iter, p_measure, pad[2];
     double tol[5];
     double err[5];
 };
@@ -119,7 +129,7 @@ struct Dev {
     double *src;        // [NSRC][N]
     double *red;        // reduction scratch
     Ctl *ctl;
-    unsigned long long *errs;  // [2][5][ERR_COPIES] at err_slot()
+    unsigned long long *errs;  // [2][NERR][ERR_COPIES] at err_slot()
 };
 
 // What a sweep routine does with the equation of one unknown, a phi = S(x):

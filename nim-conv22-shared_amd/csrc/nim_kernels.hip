@@ -17,11 +17,47 @@ namespace nim {
 
 __device__ __forceinline__ void err_commit(const Dev &d, int field, double e)
 {
+    e = e <= DBL_MAX ? e : INFINITY;  // fmax drops NaN: carry it as infinity
     e = wave_max(e);
     if ((threadIdx.x & 63) == 0) {
         int slot = d.ctl->iter & 1;
         unsigned long long *p = d.errs + err_slot(slot, field, blockIdx.x & (ERR_COPIES - 1));
         atomicMax(p, (unsigned long long)__double_as_longlong(e));
+    }
+}
+
+// key of a signed double that orders like the double under unsigned
+// comparison (positive: sign bit set; negative: all bits flipped); key 0 is
+// below every key of a number and marks "no value"
+__device__ __host__ __forceinline__ unsigned long long order_key(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    return (b >> 63) ? ~b : b | 0x8000000000000000ull;
+}
+__device__ __forceinline__ double order_value(unsigned long long k)
+{
+    return __longlong_as_double((long long)((k >> 63) ? k & 0x7fffffffffffffffull : ~k));
+}
+
+// the pressure increments of a pass: max |dp|, max dp and max -dp
+struct PErr {
+    double a = 0., hi = -INFINITY, lo = -INFINITY;
+    __device__ void add(double dp)
+    {
+        a = err_max(a, dp);
+        hi = fmax(hi, dp);
+        lo = fmax(lo, -dp);
+    }
+};
+
+__device__ __forceinline__ void err_commit_p(const Dev &d, const PErr &e)
+{
+    err_commit(d, F_P, e.a);
+    const double hi = wave_max(e.hi), lo = wave_max(e.lo);
+    if ((threadIdx.x & 63) == 0 && hi > -INFINITY) {
+        const int slot = d.ctl->iter & 1, cp = blockIdx.x & (ERR_COPIES - 1);
+        atomicMax(d.errs + err_slot(slot, E_PHI, cp), order_key(hi));
+        atomicMax(d.errs + err_slot(slot, E_PLO, cp), order_key(lo));
     }
 }
 
@@ -555,10 +591,10 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
     if (MODE == MODE_RES) {
         T r = S0 - a[0] * old;
         rC[c] = (float)r;
-        ecell = fmax(ecell, fabs((double)(w * r / a[0])));
+        ecell = err_max(ecell, ((double)(w * r / a[0])));
     } else {
         nv = (1 - w) * old + (w / a[0]) * (MODE == MODE_DELTA ? S0 + q0 : S0);
-        ecell = fmax(ecell, fabs((double)(nv - old)));
+        ecell = err_max(ecell, ((double)(nv - old)));
     }
     const T ccur = nv + oc;
     // ---- face 2
@@ -569,10 +605,10 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
         if (MODE == MODE_RES) {
             T r = S - b[0] * F1;
             rF[f[1]] = (float)r;
-            eface = fmax(eface, fabs((double)(w * r / b[0])));
+            eface = err_max(eface, ((double)(w * r / b[0])));
         } else {
             T v = (1 - w) * F1 + (w / b[0]) * (MODE == MODE_DELTA ? S + q2 : S);
-            eface = fmax(eface, fabs((double)(v - F1)));
+            eface = err_max(eface, ((double)(v - F1)));
             F1 = v;
         }
     }
@@ -584,10 +620,10 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
         if (MODE == MODE_RES) {
             T r = S - e[0] * F3;
             rF[f[3]] = (float)r;
-            eface = fmax(eface, fabs((double)(w * r / e[0])));
+            eface = err_max(eface, ((double)(w * r / e[0])));
         } else {
             T v = (1 - w) * F3 + (w / e[0]) * (MODE == MODE_DELTA ? S + q4 : S);
-            eface = fmax(eface, fabs((double)(v - F3)));
+            eface = err_max(eface, ((double)(v - F3)));
             F3 = v;
         }
     }
@@ -599,10 +635,10 @@ __device__ void transport_unit(const Dev &d, const View<T> &vw, int ff, int cs, 
         if (MODE == MODE_RES) {
             T r = S - h[0] * F5;
             rF[f[5]] = (float)r;
-            eface = fmax(eface, fabs((double)(w * r / h[0])));
+            eface = err_max(eface, ((double)(w * r / h[0])));
         } else {
             T v = (1 - w) * F5 + (w / h[0]) * (MODE == MODE_DELTA ? S + q6 : S);
-            eface = fmax(eface, fabs((double)(v - F5)));
+            eface = err_max(eface, ((double)(v - F5)));
             F5 = v;
         }
     }
@@ -786,7 +822,7 @@ template __global__ void k_d_T_sweep<true>(Dev, View<float>, Sched);
 // the pressure source), so the correction form is the same formula on the
 // corrections plus the residual.
 template <int MODE, typename T>
-__device__ void pressure_wall(const Dev &d, const View<T> &vw, int b, int pos, int i, int j, int k, int c, double &e)
+__device__ void pressure_wall(const Dev &d, const View<T> &vw, int b, int pos, int i, int j, int k, int c, PErr &e)
 {
     const int NB = d.NB;
     const T *W = vw.wall;
@@ -828,11 +864,11 @@ __device__ void pressure_wall(const Dev &d, const View<T> &vw, int b, int pos, i
     if (MODE == MODE_RES) {
         T r = S - ap * o;
         rF[fw] = (float)r;
-        e = fmax(e, fabs((double)(w2 * r / ap)));
+        e.add((double)(w2 * r / ap));
     } else {
         T v = (1 - w2) * o + w2 / ap * (MODE == MODE_DELTA ? S + rF[fw] : S);
         P[fw] = v;
-        e = fmax(e, fabs((double)(v - o)));
+        if (MODE == MODE_GS) e.add((double)(v - o));
     }
 }
 
@@ -846,7 +882,7 @@ __device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const
 {
     int i, j, k;
     bool act = sweep_cell<MODE>(d, sc, lb, i, j, k);
-    double e = 0.;
+    PErr e;
     if (act) {
         const int N = d.N, NT = d.NT, NR = d.NR, NZ = d.NZ;
         int c = cidx(d, i, j, k);
@@ -905,10 +941,10 @@ __device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const
             if (MODE == MODE_RES) {
                 T r = S - diag * pv;
                 rF[slot] = (float)r;
-                e = fmax(e, fabs((double)(w * r / diag)));
+                e.add((double)(w * r / diag));
             } else {
                 T v = (1 - w) * pv + w / diag * (MODE == MODE_DELTA ? S + q : S);
-                e = fmax(e, fabs((double)(v - pv)));
+                if (MODE == MODE_GS) e.add((double)(v - pv));
                 pv = v;
             }
         };
@@ -941,7 +977,7 @@ __device__ __forceinline__ void ppe_sweep(const Dev &d, const View<T> &vw, const
             if (hz) P[f[5]] = pf[5];
         }
     }
-    if (MODE != MODE_DELTA) err_commit(d, F_P, e);
+    if (MODE != MODE_DELTA) err_commit_p(d, e);
 }
 
 template <bool NTL>
@@ -1034,10 +1070,11 @@ __global__ void __launch_bounds__(256) k_cast(const double *in, float *out, long
 // ---------------------------------------------------------------------
 // iteration control
 // ---------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4, int max_iter, int min_iter)
+__global__ void __launch_bounds__(256) k_solve_reset(Dev d, double t0, double t1, double t2, double t3, double t4,
+                                                    int max_iter, int min_iter, int p_measure)
 {
     int l = threadIdx.x;
-    for (int q = l; q < 2 * 5 * ERR_COPIES; q += blockDim.x) d.errs[(size_t)q * ERR_STRIDE] = 0ull;
+    for (int q = l; q < 2 * NERR * ERR_COPIES; q += blockDim.x) d.errs[(size_t)q * ERR_STRIDE] = 0ull;
     if (l == 0) {
         Ctl *c = d.ctl;
         c->done = 0;
@@ -1045,6 +1082,7 @@ __global__ void __launch_bounds__(256) k_solve_reset(Dev d, double t0, double t1
         c->status = 0;
         c->max_iter = max_iter;
         c->min_iter = min_iter;
+        c->p_measure = p_measure;
         c->tol[0] = t0; c->tol[1] = t1; c->tol[2] = t2; c->tol[3] = t3; c->tol[4] = t4;
         for (int q = 0; q < 5; q++) c->err[q] = 1.;
     }
@@ -1064,13 +1102,25 @@ __device__ __forceinline__ void check(const Dev &d)
         e[q] = wave_max(__longlong_as_double((long long)d.errs[err_slot(s, q, l)]));
         d.errs[err_slot(s ^ 1, q, l)] = 0ull;
     }
+    // largest pressure increment and largest negated one (keys: unsigned max)
+    unsigned long long khi = d.errs[err_slot(s, E_PHI, l)], klo = d.errs[err_slot(s, E_PLO, l)];
+    d.errs[err_slot(s ^ 1, E_PHI, l)] = 0ull;
+    d.errs[err_slot(s ^ 1, E_PLO, l)] = 0ull;
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long a = __shfl_xor(khi, o), b = __shfl_xor(klo, o);
+        khi = a > khi ? a : khi;
+        klo = b > klo ? b : klo;
+    }
     if (l == 0) {
         int it = ctl->iter + 1;
         ctl->iter = it;
         bool fin = true, conv = it >= ctl->min_iter;
+        for (int q = 0; q < 5; q++) fin = fin && isfinite(e[q]);
+        // half the range of the increments: blind to a change of p's
+        // constant level (no pressure value this pass: the largest magnitude)
+        if (ctl->p_measure == P_HALFRANGE && fin && khi && klo) e[3] = (order_value(khi) + order_value(klo)) / 2;
         for (int q = 0; q < 5; q++) {
             ctl->err[q] = e[q];
-            fin = fin && isfinite(e[q]);
             conv = conv && !(e[q] > ctl->tol[q]);
         }
         if (!fin) { ctl->status = 4; ctl->done = 1; }

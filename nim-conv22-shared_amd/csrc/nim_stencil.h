@@ -10,6 +10,14 @@ namespace nim {
 // ---------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------
+// running maximum of |x| for a convergence measure: a non-finite x (NaN
+// included, which fmax would drop) counts as infinity, so it is reported
+__device__ __forceinline__ double err_max(double e, double x)
+{
+    const double a = fabs(x);
+    return fmax(e, a <= DBL_MAX ? a : INFINITY);
+}
+
 __device__ __forceinline__ double wave_max(double v)
 {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));

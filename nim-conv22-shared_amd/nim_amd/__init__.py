@@ -14,6 +14,7 @@ PKG = os.path.dirname(HERE)
 LIB_PATH = os.path.join(PKG, "libnim_hip.so")
 
 NIM_OK, NIM_E_ARG, NIM_E_HIP, NIM_E_MAXITER, NIM_E_NONFINITE, NIM_E_STATE = range(6)
+P_MAXABS, P_HALFRANGE = 0, 1  # pressure convergence measures (nim_set_p_measure)
 
 STAGES = {
     "velocity_sources": 0, "velocity_sweep": 1, "pressure_sources": 2, "pressure_stage": 3,
@@ -89,6 +90,8 @@ def lib():
         L.nim_download_state.argtypes = [vp, ctypes.POINTER(State)]
         L.nim_inner_solve.restype = i32
         L.nim_inner_solve.argtypes = [vp, _P, i32, i32, _PI, _P]
+        L.nim_set_p_measure.restype = i32
+        L.nim_set_p_measure.argtypes = [vp, i32]
         L.nim_end_step.restype = i32
         L.nim_end_step.argtypes = [vp, _P, _P, _P]
         L.nim_end_step_noalloc.restype = i32
@@ -157,7 +160,7 @@ def exported_symbols():
     """Names of the C-ABI entry points declared in include/nim_hip.h."""
     return ["nim_counts", "nim_face_map", "nim_ctx_create", "nim_ctx_destroy", "nim_upload_geometry",
             "nim_build_pressure_coeffs", "nim_upload_state", "nim_download_state", "nim_build_momentum_coeffs",
-            "nim_inner_solve", "nim_end_step", "nim_end_step_noalloc", "nim_advance_time", "nim_nusselt", "nim_run_stage",
+            "nim_inner_solve", "nim_set_p_measure", "nim_end_step", "nim_end_step_noalloc", "nim_advance_time", "nim_nusselt", "nim_run_stage",
             "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_kernel_times",
             "nim_inner_solve_mixed", "nim_residual", "nim_kernel_times_mixed", "nim_ens_create", "nim_ens_destroy",
             "nim_ens_inner_solve", "nim_revolve_plan", "nim_ckpt_create", "nim_ckpt_destroy", "nim_ckpt_store",
@@ -289,6 +292,12 @@ class Context:
         if rc != NIM_OK and (raise_on_cap or rc != NIM_E_MAXITER):
             _check(rc)
         return rc, it.value, err
+
+    def set_p_measure(self, mode):
+        """P_MAXABS: tol[3] bounds max |dp| (find_error's measure); P_HALFRANGE:
+        (max dp - min dp) / 2, which a change of p's constant level does not
+        move."""
+        _check(self.L.nim_set_p_measure(self.h, int(mode)))
 
     def end_step(self):
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

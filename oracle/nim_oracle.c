@@ -82,6 +82,10 @@ typedef struct {
     double *R_B_u, *R_B_v, *R_B_w, *R_C_u, *R_C_v, *R_C_w, *R_D_u, *R_D_v, *R_D_w;
     double *R_B_p, *R_D_p, *R_D_T;
     double Dilatation, dudt_max, dTdt_max;
+    /* p's convergence measure: 0 max |dp| (find_error, nse.f90:260-325);
+     * 1 half the range of dp, (max dp - min dp) / 2, which does not see a
+     * change of p's constant level (p is fixed only up to a constant) */
+    int p_measure;
 } ora_ctx;
 
 /* ------------------------------------------------------------------ */
@@ -870,16 +874,22 @@ void ora_pressure_sweep(ora_ctx *c)
     for (int i = 0; i < c->nmesh; i++) pressure_unit(c, i);
 }
 
-/* find_error (nse.f90:260-325) */
+void ora_set_p_measure(ora_ctx *c, int mode) { c->p_measure = mode; }
+
+/* find_error (nse.f90:260-325); with p_measure 1 the pressure increment is
+ * taken as half its range over the faces instead of its largest magnitude */
 void ora_find_error(ora_ctx *c, double err[5])
 {
-    double eu = 0, ev = 0, ew = 0, ep = 0, eT = 0;
-#pragma omp parallel for schedule(static) reduction(max : eu, ev, ew, ep, eT)
+    double eu = 0, ev = 0, ew = 0, ep = 0, eT = 0, pmax = -INFINITY, pmin = INFINITY;
+#pragma omp parallel for schedule(static) reduction(max : eu, ev, ew, ep, eT, pmax) reduction(min : pmin)
     for (int i = 0; i < c->nface; i++) {
+        const double dp = c->faces_p[i] - c->faces_p_old[i];
         eu = fmax(eu, fabs(c->faces_u[i] - c->faces_u_old[i]));
         ev = fmax(ev, fabs(c->faces_v[i] - c->faces_v_old[i]));
         ew = fmax(ew, fabs(c->faces_w[i] - c->faces_w_old[i]));
-        ep = fmax(ep, fabs(c->faces_p[i] - c->faces_p_old[i]));
+        ep = fmax(ep, fabs(dp));
+        pmax = fmax(pmax, dp);
+        pmin = fmin(pmin, dp);
         eT = fmax(eT, fabs(c->faces_T[i] - c->faces_T_old[i]));
         c->faces_u_old[i] = c->faces_u[i];
         c->faces_v_old[i] = c->faces_v[i];
@@ -900,6 +910,7 @@ void ora_find_error(ora_ctx *c, double err[5])
         c->Txyz_old_con[i] = c->Txyz[i];
     }
     err[0] = fmax(eu, cu); err[1] = fmax(ev, cv); err[2] = fmax(ew, cw); err[3] = ep; err[4] = fmax(eT, cT);
+    if (c->p_measure == 1) err[3] = isfinite(ep) ? (pmax - pmin) / 2 : ep;
 }
 
 static void snapshot_old(ora_ctx *c)

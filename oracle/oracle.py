@@ -70,6 +70,8 @@ def lib():
         for n in ("ora_velocity_sweep", "ora_temperature_sweep", "ora_pressure_stage"):
             getattr(L, n).argtypes = [vp, ctypes.c_int]
         L.ora_find_error.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+        L.ora_set_p_measure.restype = None
+        L.ora_set_p_measure.argtypes = [vp, ctypes.c_int]
         L.ora_iteration.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         L.ora_inner_solve.restype = ctypes.c_int
         L.ora_inner_solve.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_int,
@@ -148,6 +150,11 @@ class Oracle:
     def check_steady_state(self): self.L.ora_check_steady_state(self.h)
     def evaluate_D(self): self.L.ora_evaluate_D(self.h)
     def update_time_step(self): self.L.ora_update_time_step(self.h)
+
+    def set_p_measure(self, mode):
+        """0: max |dp| as find_error; 1: (max dp - min dp) / 2, blind to a
+        change of p's constant level."""
+        self.L.ora_set_p_measure(self.h, int(mode))
 
     def find_error(self):
         e = np.zeros(5)

@@ -276,6 +276,53 @@ def ensemble_bench(args):
     return 0
 
 
+def adjoint_bench(args):
+    """The specification's adjoint run: J = the hexagon Nusselt number
+    averaged over every step of the run, its gradient with respect to the
+    temperature of every hexagon wall face and to Ra by the discrete
+    adjoint, the run reversed with binomial checkpointing (C4: 400 steps,
+    20 snapshots; C5: 200 steps) and each reversed fixed point solved by
+    GMRES. The forward steps converge to the bench tolerances."""
+    case = host.baseline_case(args.adjoint_case)
+    nsteps = args.adj_steps or (400 if args.adjoint_case == "C4" else 200)
+    t0 = time.perf_counter()
+    ctx, geo, st = setup(case)
+    ctx.set_p_measure(nim_amd.P_HALFRANGE if args.p_measure == "halfrange" else nim_amd.P_MAXABS)
+    t_setup = time.perf_counter() - t0
+    tol = np.array([args.tol, args.tol, args.tol, args.tol_p, args.tol])
+    plan, fwd_plan = nim_amd.revolve_plan(nsteps, args.adj_slots)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    inner = (args.inner or 96) if args.mixed_forward else 0
+    J, grad, info = nim_amd.nusselt_gradient(ctx, case, nsteps, args.adj_slots, args.adj_tol, tol=tol, average=True,
+                                             solver="gmres", mixed_inner=inner)
+    wall = time.perf_counter() - t0
+    ctx.close()
+    gf = grad["T_faces"]
+    rev = [n for n, _ in info["adjoint_passes"]]
+    nt, nr, nz = case.dims
+    print(json.dumps({
+        "metric": "checkpointed discrete adjoint of the time-averaged hexagon Nusselt number: wall time",
+        "value": wall, "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
+        "data": "synthetic (seeded 1e-3 noise on cell averages)",
+        "config": {"workload": "%s: %d x %d x %d (theta x r x z) = %d cells, Ra=%g, Pr=%g, DT=%g, %d steps, %d "
+                               "snapshots, forward %s to tol %g (p %g on %s), reversed fixed points by GMRES to %g"
+                               % (args.adjoint_case, nt, nr, nz, case.ncells, case.Ra, case.Pr, case.DT, nsteps,
+                                  args.adj_slots, "mixed (%d FP32 passes per FP64 residual)" % inner if inner
+                                  else "FP64", args.tol, args.tol_p,
+                                  "max |dp|" if args.p_measure == "maxabs" else "half the range of dp", args.adj_tol),
+                   "hexagon_faces": nt * nz},
+        "J": J, "dJ_dRa": grad["Ra"], "dJ_dT_hot": grad["T_hot"],
+        "dJ_dT_faces": {"count": int(gf.size), "sum": float(gf.sum()), "min": float(gf.min()), "max": float(gf.max()),
+                        "first8": [float(x) for x in gf[:8]]},
+        "forward_steps": info["forward_steps"], "forward_steps_planned": fwd_plan, "forward_s": info["forward_s"],
+        "forward_passes": info["forward_passes"], "adjoint_s": info["adjoint_s"],
+        "reversed_passes_total": int(sum(rev)), "reversed_passes_per_step": float(np.mean(rev)),
+        "reversed_passes_max": int(max(rev)), "setup_s": t_setup,
+    }))
+    return 0
+
+
 def time_reversed_pass(ctx, reps):
     """Wall time (ms) of one reversed pass of the discrete adjoint at the
     context's state, from a random adjoint of the iterate (every stage does
@@ -322,6 +369,13 @@ def main():
     ap.add_argument("--profile-only", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only time reversed passes of the discrete adjoint at the initial state")
+    ap.add_argument("--adjoint-case", choices=("C4", "C5"), default=None,
+                    help="the specification's checkpointed adjoint run instead of the throughput bench")
+    ap.add_argument("--adj-steps", type=int, default=None, help="time steps of the adjoint run (C4: 400, C5: 200)")
+    ap.add_argument("--adj-slots", type=int, default=20, help="snapshots of the adjoint run")
+    ap.add_argument("--adj-tol", type=float, default=1e-10, help="relative residual of each reversed fixed point")
+    ap.add_argument("--mixed-forward", action="store_true",
+                    help="adjoint run: forward steps with the mixed-precision loop")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--pmc-summary", metavar="DIR", help="print the HBM traffic summary of counter runs under DIR")
     args = ap.parse_args()
@@ -337,6 +391,8 @@ def main():
     if args.ensemble:
         args.inner = args.inner or 8
         return ensemble_bench(args)
+    if args.adjoint_case:
+        return adjoint_bench(args)
     args.inner = args.inner or 96
 
     case = host.baseline_case(args.case)

@@ -270,12 +270,24 @@ int32_t nim_synchronize(nim_ctx *ctx);
  * gets exactly the result of its own solve. iters[ncase] returns passes
  * (inner = 0) or cycles (inner > 0) per case, err[5*ncase] the last measure.
  * Per-step work (nim_build_momentum_coeffs, nim_end_step, nim_advance_time,
- * nim_nusselt) stays per context. Returns the worst status over the cases. */
+ * nim_nusselt) stays per context, or runs batched with the solve in
+ * nim_ens_time_step. Returns the worst status over the cases. */
 typedef struct nim_ens nim_ens;
 int32_t nim_ens_create(nim_ctx *const *cases, int32_t ncase, nim_ens **out);
 int32_t nim_ens_destroy(nim_ens *ens);
 int32_t nim_ens_inner_solve(nim_ens *ens, const double tol[5], int32_t max_iter, int32_t min_iter, int32_t inner,
                             int32_t *iters, double *err);
+/* One whole time step of every case (solve_nse's loop body, nse.f90:49-73,
+ * without the I/O): momentum coefficients, the inner solve as
+ * nim_ens_inner_solve, the end-of-step reductions, the time shift and the
+ * Nusselt numbers. The per-step work is one launch per kernel over all
+ * cases, captured in two graphs around the solve, on the ensemble's stream;
+ * out[8 q .. 8 q + 7] returns case q's {dudt_max, dTdt_max, dilatation,
+ * Nu_H, Nu_C, Average, Area1, Area2} (nim_end_step, nim_nusselt) in one
+ * read. Each case gets exactly the results of the same calls on its own
+ * context. */
+int32_t nim_ens_time_step(nim_ens *ens, const double tol[5], int32_t max_iter, int32_t min_iter, int32_t inner,
+                          int32_t *iters, double *err, double *out);
 
 /* Checkpoints for reversing a run of time steps (no reference counterpart;
  * the reference keeps one restart state on disk, restart.f90). A pool of
@@ -336,6 +348,19 @@ int32_t nim_adj_destroy(nim_adj *adj);
 int32_t nim_adj_seed_nusselt(nim_adj *adj, const double weights[3]);
 int32_t nim_adj_reverse_step(nim_adj *adj, double tol, int32_t max_passes, int32_t *passes, double *delta);
 int32_t nim_adj_gradient(nim_adj *adj, double out[4]);
+/* nim_adj_reverse_step with the reversed fixed point (I - G_x^T) lam = seed
+ * solved by restarted GMRES(restart) with the reversed pass as the
+ * operator, to a relative residual tol (true residual at each restart);
+ * passes returns the reversed passes used, relres the final relative
+ * residual. NIM_E_MAXITER after max_passes. Allocates its Krylov basis
+ * ((restart + 4) vectors of 4 NMESH + 5 NFACE doubles) on first use. */
+int32_t nim_adj_reverse_step_gmres(nim_adj *adj, double tol, int32_t max_passes, int32_t restart,
+                                   int32_t *passes, double *relres);
+/* The hexagon wall temperature gradient face by face, accumulated like
+ * the scalar one (whose value is their sum): dJ/dT of each hexagon face,
+ * count = n_theta * n_z values in the reference's boundary order (BCs group
+ * 4, face i_theta + n_theta k_z; faces_T at the wall, bc.f90:48-57). */
+int32_t nim_adj_gradient_faces(nim_adj *adj, double *out, int64_t count);
 int32_t nim_adj_reverse_pass(nim_adj *adj);
 int32_t nim_adj_reverse_stage(nim_adj *adj, int32_t stage);
 int32_t nim_adj_clear(nim_adj *adj);

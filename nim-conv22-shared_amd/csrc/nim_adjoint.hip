@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "nim_hip.h"
 #include "nim_stencil.h"
@@ -975,13 +976,14 @@ __global__ void __launch_bounds__(BSA) k_adj_seed(Dev d, double *seed, double *l
 // state before the step: cells = previous level + half the time-centred
 // adjoint of the state after the step, time-centred velocity from the
 // momentum coefficients (ltc), faces 0 (they only start the next solve)
-__global__ void __launch_bounds__(BSA) k_adj_hex(Dev d, Adj a, double *theta)
+__global__ void __launch_bounds__(BSA) k_adj_hex(Dev d, Adj a, double *theta, double *thf)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0.;
     if (t < d.NT * d.NZ) {
         const int i = t % d.NT, k = t / d.NT;
         s = a.lf[(size_t)F_T * d.NF + fidx(d, 1, i, 0, k)];
+        thf[t] += s;  // hexagon face t = i + n_theta k, the reference's boundary order
     }
     par_commit(theta + P_THOT, s);
 }
@@ -1005,8 +1007,8 @@ __global__ void k_adj_par_add(double *theta, const double *par)
 // adjoint of find_Nu2 (nse.f90:2699-2798; k_nusselt): J = wc Nu_cyl + wh Nu_hex
 // into the face adjoint fbar, and its direct part through the hexagon wall
 // temperature into theta
-__global__ void __launch_bounds__(BSA) k_adj_nusselt(Dev d, double *fbar, double *theta, double R_i, double R_o,
-                                                     double wc, double wh)
+__global__ void __launch_bounds__(BSA) k_adj_nusselt(Dev d, double *fbar, double *theta, double *thf, double R_i,
+                                                     double R_o, double wc, double wh)
 {
     const int NT = d.NT, NR = d.NR, NZ = d.NZ, NB = d.NB, n = NT * NZ;
     const double *W = d.wall;
@@ -1031,6 +1033,7 @@ __global__ void __launch_bounds__(BSA) k_adj_nusselt(Dev d, double *fbar, double
         int b = 2 * NT * NR + q;
         double s = wh * R_i / a2 * W[(W_NU + 4) * NB + b];
         th += -s * W[(W_NU + 0) * NB + b];
+        thf[q] += -s * W[(W_NU + 0) * NB + b];
         aadd(LT + fidx(d, 2, i, 0, k), s * W[(W_NU + 1) * NB + b]);
         aadd(LT + fidx(d, 2, i, 1, k), -s * W[(W_NU + 2) * NB + b]);
         aadd(LT + fidx(d, 2, i, 2, k), s * W[(W_NU + 3) * NB + b]);
@@ -1045,6 +1048,89 @@ __global__ void __launch_bounds__(BSA) k_adj_nusselt(Dev d, double *fbar, double
     par_commit(theta + P_THOT, th);
 }
 
+
+// ---------------------------------------------------------------------
+// restarted GMRES on (I - G_x^T) lam = seed (the reversed pass as operator)
+// ---------------------------------------------------------------------
+// Dirichlet slot of the adjoint vector [4N cells | 5NF faces]
+__device__ __forceinline__ bool lam_fixed(const Dev &d, long s)
+{
+    const long c4 = 4 * (long)d.N;
+    if (s < c4) return false;
+    const long f = s - c4;
+    return dirichlet(d, (int)(f / d.NF), (int)(f % d.NF));
+}
+
+// out = v - lam (lam = G_x^T v from a reversed pass), 0 in the Dirichlet
+// slots, which the pass fills with parameter adjoints
+__global__ void __launch_bounds__(BSA) k_gm_apply(Dev d, const double *v, const double *lam, double *out, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    for (long s = t; s < n; s += stride) out[s] = lam_fixed(d, s) ? 0. : v[s] - lam[s];
+}
+
+// partial dot products of w with basis vectors V[0..j): block (x, k)
+// sums its grid-stride share of V[k] . w into part[k * gridDim.x + x]
+__global__ void __launch_bounds__(BSA) k_gm_dots(const double *V, long ld, const double *w, long n, double *part)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    const double *v = V + (size_t)blockIdx.y * ld;
+    double acc = 0.;
+    for (long s = t; s < n; s += stride) acc += v[s] * w[s];
+    __shared__ double sh[BSA / 64];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) r += sh[q];
+        part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = r;
+    }
+}
+
+// out[k] = sum of part[k * nb .. k * nb + nb), one block per k, fixed order
+__global__ void __launch_bounds__(BSA) k_gm_dots_final(const double *part, int nb, double *out)
+{
+    const double *p = part + (size_t)blockIdx.x * nb;
+    double acc = 0.;
+    for (int q = threadIdx.x; q < nb; q += blockDim.x) acc += p[q];
+    __shared__ double sh[BSA / 64];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) r += sh[q];
+        out[blockIdx.x] = r;
+    }
+}
+
+// x += sign * sum_{k < j} c[k] V[k]   (c in device memory)
+__global__ void __launch_bounds__(BSA) k_gm_combine(double *x, const double *V, long ld, int j, const double *c,
+                                                    double sign, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    for (long s = t; s < n; s += stride) {
+        double acc = 0.;
+        for (int k = 0; k < j; k++) acc += c[k] * V[(size_t)k * ld + s];
+        x[s] += sign * acc;
+    }
+}
+
+// y = x outside the Dirichlet slots, 0 in them
+__global__ void __launch_bounds__(BSA) k_gm_mask(Dev d, double *y, const double *x, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    for (long s = t; s < n; s += stride) y[s] = lam_fixed(d, s) ? 0. : x[s];
+}
+
+// y = a * x (+ b * z when z is given)
+__global__ void __launch_bounds__(BSA) k_gm_scale(double *y, const double *x, double a, const double *z, double b, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    for (long s = t; s < n; s += stride) y[s] = z ? a * x[s] + b * z[s] : a * x[s];
+}
+
 }  // namespace
 
 // ---- host side --------------------------------------------------------------
@@ -1054,6 +1140,12 @@ struct nim_adj {
     double *seed = nullptr, *prev = nullptr;  // [4N + 5NF] each, like lam (= lc, lf)
     double *cbar = nullptr, *tcbar = nullptr, *fbar = nullptr;
     double *theta = nullptr;                  // [NPAR] accumulated over steps
+    double *thf = nullptr;                    // [NT NZ] per hexagon face, accumulated over steps
+    // GMRES workspace (allocated on first use): basis V [m + 1][nl], w, x,
+    // partial dots, Hessenberg column and combination coefficients
+    void *gm_block = nullptr;
+    double *gm_host = nullptr;  // pinned: coefficients in [0, m + 1), dot results in [m + 1, 2 m + 2)
+    int gm_m = 0;
     unsigned long long *dmax = nullptr;       // [2]
     unsigned long long *h_dmax = nullptr;
     void *block = nullptr;
@@ -1074,15 +1166,22 @@ static int cell_grid(long n) { return (int)((n + BSA - 1) / BSA); }
 // per CU, one atomic each
 static int reduce_grid(long n) { return std::min(cell_grid(n), 2048); }
 
+// doubles of the adjoint's device block: lam, seed, prev, lsrc, lTF, lMF,
+// lold, ltc, scr, cbar, tcbar, fbar, par, theta, dmax, thf
+static size_t adj_doubles(const Dev &d)
+{
+    const size_t N = d.N, NF = d.NF, nl = 4 * N + 5 * NF;
+    return 3 * nl + NSRC * N + 2 * NCF * N + 4 * N + 3 * N + 10 * N + 4 * N + 3 * N + 5 * NF + 2 * NPAR + 8 +
+           (size_t)d.NT * d.NZ;
+}
+
 extern "C" int32_t nim_adj_create(nim_ctx *ctx, nim_adj **out)
 {
     if (!ctx || !out) return nim_internal_fail(NIM_E_ARG, "null argument");
     *out = nullptr;
     const Dev *d = nim_internal_dev(ctx);
     const size_t N = d->N, NF = d->NF, nl = 4 * N + 5 * NF;
-    // lam, seed, prev, lsrc, lTF, lMF, lold, ltc, scr, cbar, tcbar, fbar, par, theta
-    const size_t total =
-        3 * nl + NSRC * N + 2 * NCF * N + 4 * N + 3 * N + 10 * N + 4 * N + 3 * N + 5 * NF + 2 * NPAR + 8;
+    const size_t total = adj_doubles(*d);
     nim_adj *A = new nim_adj();
     A->ctx = ctx;
     if (hipMalloc(&A->block, total * sizeof(double)) != hipSuccess) {
@@ -1112,6 +1211,7 @@ extern "C" int32_t nim_adj_create(nim_ctx *ctx, nim_adj **out)
     A->a.par = take(NPAR);
     A->theta = take(NPAR);
     A->dmax = (unsigned long long *)take(8);
+    A->thf = take((size_t)d->NT * d->NZ);
     hipStream_t s = (hipStream_t)nim_internal_stream(ctx);
     if (hipMemsetAsync(A->block, 0, total * sizeof(double), s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
@@ -1130,6 +1230,8 @@ extern "C" int32_t nim_adj_destroy(nim_adj *A)
     (void)hipStreamSynchronize((hipStream_t)nim_internal_stream(A->ctx));
     (void)hipHostFree(A->h_dmax);
     (void)hipFree(A->block);
+    if (A->gm_block) (void)hipFree(A->gm_block);
+    if (A->gm_host) (void)hipHostFree(A->gm_host);
     delete A;
     return NIM_OK;
 }
@@ -1197,7 +1299,7 @@ extern "C" int32_t nim_adj_seed_nusselt(nim_adj *A, const double weights[3])
     nim_internal_lengths(A->ctx, &R_i, &R_o);
     // out[2] = (out[0] + out[1]) / 2
     const double wc = weights[0] + 0.5 * weights[2], wh = weights[1] + 0.5 * weights[2];
-    hipLaunchKernelGGL(k_adj_nusselt, dim3(1), dim3(BSA), 0, s, d, A->fbar, A->theta, R_i, R_o, wc, wh);
+    hipLaunchKernelGGL(k_adj_nusselt, dim3(1), dim3(BSA), 0, s, d, A->fbar, A->theta, A->thf, R_i, R_o, wc, wh);
     return adj_check("adjoint Nusselt seed launch failed");
 }
 
@@ -1209,6 +1311,26 @@ extern "C" int32_t nim_adj_reverse_pass(nim_adj *A)
     int32_t rc = adj_check("adjoint pass launch failed");
     if (rc) return rc;
     if (hipStreamSynchronize(s) != hipSuccess) return nim_internal_fail(NIM_E_HIP, "adjoint pass failed");
+    return NIM_OK;
+}
+
+// end of a reverse step, after the last reversed pass: the hexagon wall
+// temperatures and scalar parameters of that pass into the gradient, the
+// momentum coefficients' adjoint to the time-centred velocity, and the
+// adjoint of the state before the step
+static int32_t finish_step(nim_adj *A, hipStream_t s)
+{
+    const Dev &d = *nim_internal_dev(A->ctx);
+    const long nl = 4 * (long)d.N + 5 * (long)d.NF;
+    hipLaunchKernelGGL(k_adj_hex, dim3(cell_grid((long)d.NT * d.NZ)), dim3(BSA), 0, s, d, A->a, A->theta, A->thf);
+    hipLaunchKernelGGL(k_adj_par_add, dim3(1), dim3(64), 0, s, A->theta, A->a.par);
+    if (hipMemsetAsync(A->a.ltc, 0, 3 * (size_t)d.N * sizeof(double), s) != hipSuccess)
+        return nim_internal_fail(NIM_E_HIP, "adjoint clear failed");
+    reverse_coefs(A, s, 0, A->theta, P_NU);
+    hipLaunchKernelGGL(k_adj_carry, dim3(grid_for(nl)), dim3(BSA), 0, s, d, A->a, A->cbar, A->tcbar, A->fbar);
+    int32_t rc = adj_check("adjoint step launch failed");
+    if (rc) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) return nim_internal_fail(NIM_E_HIP, "adjoint step failed");
     return NIM_OK;
 }
 
@@ -1252,17 +1374,191 @@ extern "C" int32_t nim_adj_reverse_step(nim_adj *A, double tol, int32_t max_pass
     }
     if (passes) *passes = it;
     if (delta) *delta = lm > 0. ? dm / lm : dm;
-    // parameter adjoints of the last pass, then the momentum coefficients
-    hipLaunchKernelGGL(k_adj_hex, dim3(cell_grid((long)d.NT * d.NZ)), dim3(BSA), 0, s, d, A->a, A->theta);
-    hipLaunchKernelGGL(k_adj_par_add, dim3(1), dim3(64), 0, s, A->theta, A->a.par);
-    if (hipMemsetAsync(A->a.ltc, 0, 3 * (size_t)d.N * sizeof(double), s) != hipSuccess)
-        return nim_internal_fail(NIM_E_HIP, "adjoint clear failed");
-    reverse_coefs(A, s, 0, A->theta, P_NU);
-    hipLaunchKernelGGL(k_adj_carry, dim3(grid_for(nl)), dim3(BSA), 0, s, d, A->a, A->cbar, A->tcbar, A->fbar);
-    int32_t rc = adj_check("adjoint step launch failed");
+    int32_t rc = finish_step(A, s);
     if (rc) return rc;
-    if (hipStreamSynchronize(s) != hipSuccess) return nim_internal_fail(NIM_E_HIP, "adjoint step failed");
     return conv ? NIM_OK : nim_internal_fail(NIM_E_MAXITER, "adjoint fixed point not converged");
+}
+
+// ---- GMRES ------------------------------------------------------------------
+static constexpr int GM_DOT_BLOCKS = 512;
+
+static int32_t gm_alloc(nim_adj *A, int m)
+{
+    if (A->gm_block && A->gm_m == m) return NIM_OK;
+    if (A->gm_block) { (void)hipFree(A->gm_block); A->gm_block = nullptr; }
+    if (A->gm_host) { (void)hipHostFree(A->gm_host); A->gm_host = nullptr; }
+    if (hipHostMalloc((void **)&A->gm_host, 2 * (size_t)(m + 1) * sizeof(double), 0) != hipSuccess) {
+        A->gm_host = nullptr;
+        return nim_internal_fail(NIM_E_HIP, "GMRES host buffer allocation failed");
+    }
+    const Dev &d = *nim_internal_dev(A->ctx);
+    const size_t nl = 4 * (size_t)d.N + 5 * (size_t)d.NF;
+    const size_t doubles = (size_t)(m + 1) * nl + 3 * nl + (size_t)(m + 1) * GM_DOT_BLOCKS + 2 * (size_t)(m + 1);
+    if (hipMalloc(&A->gm_block, doubles * sizeof(double)) != hipSuccess) {
+        A->gm_block = nullptr;
+        return nim_internal_fail(NIM_E_HIP, "GMRES workspace allocation failed");
+    }
+    A->gm_m = m;
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t max_passes, int32_t restart,
+                                              int32_t *passes, double *relres)
+{
+    if (!A || max_passes < 1 || restart < 1 || restart > 200 || !(tol > 0.))
+        return nim_internal_fail(NIM_E_ARG, "bad GMRES reverse-step arguments");
+    const Dev &d = *nim_internal_dev(A->ctx);
+    hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
+    const long nl = 4 * (long)d.N + 5 * (long)d.NF;
+    const int gL = grid_for(nl), m = restart;
+    int32_t rc = gm_alloc(A, m);
+    if (rc) return rc;
+    double *V = (double *)A->gm_block, *w = V + (size_t)(m + 1) * nl, *x = w + nl, *b = x + nl;
+    double *part = b + nl, *dh = part + (size_t)(m + 1) * GM_DOT_BLOCKS, *dc = dh + (m + 1);
+    double *lam = A->a.lc, *hin = A->gm_host, *hout = A->gm_host + (m + 1);
+    const int nbd = (int)std::min<long>(GM_DOT_BLOCKS, (nl + BSA - 1) / BSA);
+    nim_internal_energy_coefs(A->ctx);
+    // the seed (b), as the plain iteration starts from it
+    hipLaunchKernelGGL(k_adj_seed, dim3(gL), dim3(BSA), 0, s, d, A->seed, lam, A->prev, A->cbar, A->tcbar, A->fbar);
+    // dots of vec with V[0..j) into host h
+    auto dots = [&](const double *vec, int j, double *h) -> int32_t {
+        hipLaunchKernelGGL(k_gm_dots, dim3(nbd, j), dim3(BSA), 0, s, (const double *)V, nl, vec, nl, part);
+        hipLaunchKernelGGL(k_gm_dots_final, dim3(j), dim3(BSA), 0, s, (const double *)part, nbd, dh);
+        if (hipMemcpyAsync(hout, dh, j * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return nim_internal_fail(NIM_E_HIP, "GMRES dot products failed");
+        std::copy(hout, hout + j, h);
+        return NIM_OK;
+    };
+    auto norm_of = [&](const double *vec, double *out) -> int32_t {
+        hipLaunchKernelGGL(k_gm_dots, dim3(nbd, 1), dim3(BSA), 0, s, vec, nl, vec, nl, part);
+        hipLaunchKernelGGL(k_gm_dots_final, dim3(1), dim3(BSA), 0, s, (const double *)part, nbd, dh);
+        if (hipMemcpyAsync(hout, dh, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return nim_internal_fail(NIM_E_HIP, "GMRES norm failed");
+        *out = std::sqrt(std::max(hout[0], 0.));
+        return NIM_OK;
+    };
+    // out = (I - G_x^T) v
+    int ops = 0;
+    bool copy_failed = false;
+    auto apply = [&](const double *v, double *out) {
+        if (hipMemcpyAsync(lam, v, nl * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess) copy_failed = true;
+        reverse_pass(A, s);
+        hipLaunchKernelGGL(k_gm_apply, dim3(gL), dim3(BSA), 0, s, d, v, (const double *)lam, out, nl);
+        ops++;
+    };
+    // b = the seed without its Dirichlet slots (they hold no unknown); x = 0
+    hipLaunchKernelGGL(k_gm_mask, dim3(gL), dim3(BSA), 0, s, d, b, (const double *)A->seed, nl);
+    if (hipMemsetAsync(x, 0, nl * sizeof(double), s) != hipSuccess)
+        return nim_internal_fail(NIM_E_HIP, "GMRES clear failed");
+    bool x_zero = true;
+    double bnorm = 0.;
+    if ((rc = norm_of(b, &bnorm))) return rc;
+    double res = bnorm;
+    bool conv = bnorm == 0.;
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), h(m + 1), y(m);
+    while (!conv && ops < max_passes) {
+        // r = b - A x  ->  V[0]
+        if (x_zero) {
+            hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, V, (const double *)b, 1., (const double *)nullptr,
+                               0., nl);
+        } else {
+            apply(x, w);
+            hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, V, (const double *)b, 1., (const double *)w, -1.,
+                               nl);
+        }
+        double beta;
+        if ((rc = norm_of(V, &beta))) return rc;
+        if (!std::isfinite(beta)) return nim_internal_fail(NIM_E_NONFINITE, "non-finite adjoint residual");
+        res = beta;
+        if (beta <= tol * bnorm) { conv = true; break; }
+        hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, V, (const double *)V, 1. / beta,
+                           (const double *)nullptr, 0., nl);
+        std::fill(g.begin(), g.end(), 0.);
+        g[0] = beta;
+        int jdone = 0;
+        for (int j = 0; j < m && ops < max_passes; j++) {
+            double *vj1 = V + (size_t)(j + 1) * nl;
+            apply(V + (size_t)j * nl, w);
+            // classical Gram-Schmidt, twice
+            std::fill(h.begin(), h.end(), 0.);
+            for (int pass = 0; pass < 2; pass++) {
+                std::vector<double> hh(j + 1);
+                if ((rc = dots(w, j + 1, hh.data()))) return rc;  // (synchronises: hin is free)
+                std::copy(hh.begin(), hh.end(), hin);
+                if (hipMemcpyAsync(dc, hin, (j + 1) * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
+                    return nim_internal_fail(NIM_E_HIP, "GMRES copy failed");
+                hipLaunchKernelGGL(k_gm_combine, dim3(gL), dim3(BSA), 0, s, w, (const double *)V, nl, j + 1,
+                                   (const double *)dc, -1., nl);
+                for (int k = 0; k <= j; k++) h[k] += hh[k];
+            }
+            double hn;
+            if ((rc = norm_of(w, &hn))) return rc;
+            if (!std::isfinite(hn)) return nim_internal_fail(NIM_E_NONFINITE, "non-finite adjoint Krylov vector");
+            h[j + 1] = hn;
+            if (hn > 0.)
+                hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, vj1, (const double *)w, 1. / hn,
+                                   (const double *)nullptr, 0., nl);
+            // Givens rotations of the new column
+            for (int k = 0; k < j; k++) {
+                const double t = cs[k] * h[k] + sn[k] * h[k + 1];
+                h[k + 1] = -sn[k] * h[k] + cs[k] * h[k + 1];
+                h[k] = t;
+            }
+            const double r = std::hypot(h[j], h[j + 1]);
+            cs[j] = r > 0. ? h[j] / r : 1.;
+            sn[j] = r > 0. ? h[j + 1] / r : 0.;
+            h[j] = r;
+            h[j + 1] = 0.;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            for (int k = 0; k <= j; k++) H[(size_t)k * m + j] = h[k];
+            jdone = j + 1;
+            res = std::fabs(g[j + 1]);
+            if (res <= tol * bnorm || hn == 0.) break;
+        }
+        // x += V y, H y = g (upper triangular)
+        for (int k = jdone - 1; k >= 0; k--) {
+            double t = g[k];
+            for (int q = k + 1; q < jdone; q++) t -= H[(size_t)k * m + q] * y[q];
+            y[k] = t / H[(size_t)k * m + k];
+        }
+        if (hipStreamSynchronize(s) != hipSuccess) return nim_internal_fail(NIM_E_HIP, "GMRES failed");
+        std::copy(y.begin(), y.begin() + jdone, hin);
+        if (hipMemcpyAsync(dc, hin, jdone * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
+            return nim_internal_fail(NIM_E_HIP, "GMRES copy failed");
+        hipLaunchKernelGGL(k_gm_combine, dim3(gL), dim3(BSA), 0, s, x, (const double *)V, nl, jdone,
+                           (const double *)dc, 1., nl);
+        x_zero = false;
+        // the next cycle starts from the true residual of x and stops there
+        // if it is within the tolerance
+    }
+    if (rc = adj_check("GMRES launch failed"); rc) return rc;
+    // the parameter adjoints of lam = x: one more reversed pass, then the
+    // seed added, as the plain iteration ends
+    if (copy_failed || hipMemcpyAsync(lam, x, nl * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return nim_internal_fail(NIM_E_HIP, "GMRES copy failed");
+    reverse_pass(A, s);
+    hipLaunchKernelGGL(k_adj_add_seed, dim3(gL), dim3(BSA), 0, s, lam, (const double *)A->seed, nl);
+    ops++;
+    if (passes) *passes = ops;
+    if (relres) *relres = bnorm > 0. ? res / bnorm : 0.;
+    if ((rc = finish_step(A, s))) return rc;
+    return conv ? NIM_OK : nim_internal_fail(NIM_E_MAXITER, "adjoint GMRES not converged");
+}
+
+extern "C" int32_t nim_adj_gradient_faces(nim_adj *A, double *out, int64_t count)
+{
+    if (!A || !out) return nim_internal_fail(NIM_E_ARG, "null argument");
+    const Dev &d = *nim_internal_dev(A->ctx);
+    if (count != (int64_t)d.NT * d.NZ)
+        return nim_internal_fail(NIM_E_ARG, "nim_adj_gradient_faces: count must be n_theta * n_z");
+    hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
+    if (hipMemcpyAsync(out, A->thf, count * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return nim_internal_fail(NIM_E_HIP, "adjoint gradient copy failed");
+    return NIM_OK;
 }
 
 extern "C" int32_t nim_adj_gradient(nim_adj *A, double out[4])
@@ -1279,8 +1575,7 @@ extern "C" int32_t nim_adj_clear(nim_adj *A)
 {
     if (!A) return nim_internal_fail(NIM_E_ARG, "null argument");
     const Dev &d = *nim_internal_dev(A->ctx);
-    const size_t N = d.N, NF = d.NF, nl = 4 * N + 5 * NF;
-    const size_t total = 3 * nl + NSRC * N + 2 * NCF * N + 4 * N + 3 * N + 4 * N + 3 * N + 5 * NF + 2 * NPAR + 8;
+    const size_t total = adj_doubles(d);
     hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
     if (hipMemsetAsync(A->block, 0, total * sizeof(double), s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
         return nim_internal_fail(NIM_E_HIP, "adjoint clear failed");

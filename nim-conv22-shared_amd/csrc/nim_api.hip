@@ -74,6 +74,12 @@ __global__ void k_e_d_T_src(Ens e);
 __global__ void k_e_d_T_sweep(Ens e, int color);
 __global__ void k_e_mix_update(Ens e);
 __global__ void k_e_gather(Ens e, Ctl *out);
+__global__ void k_e_coef_A(Ens e);
+__global__ void k_e_coef_F(Ens e, int cast);
+__global__ void k_e_end_step_partial(Ens e);
+__global__ void k_e_end_step_final(Ens e, const int *nred, double *out);
+__global__ void k_e_advance(Ens e);
+__global__ void k_e_nusselt(Ens e, const double *R, double *out);
 }  // namespace nim
 
 using namespace nim;
@@ -1245,7 +1251,7 @@ extern "C" int32_t nim_kernel_times_mixed(nim_ctx *c, int32_t cycles, int32_t in
 
 // ---- ensembles -----------------------------------------------------------
 // launch shapes: blocks per case of each kind of launch
-enum { SH_CELL64, SH_CELL256, SH_COL64, SH_COL256, SH_WALL, SH_UPD, NSHAPE };
+enum { SH_CELL64, SH_CELL256, SH_COL64, SH_COL256, SH_WALL, SH_UPD, SH_NRED, NSHAPE };
 
 struct nim_ens {
     int n = 0;
@@ -1261,6 +1267,13 @@ struct nim_ens {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int gbatch = 0, ginner = -1;
+    // whole-step form (nim_ens_time_step): per-step kernels of all cases in
+    // two captured graphs around the solve, results of all cases in one read
+    int *d_nred = nullptr;           // [n] partial blocks of each case's end-of-step reduction
+    double *d_R = nullptr;           // [2n] R_i, R_o
+    double *d_out = nullptr, *h_out = nullptr;  // [8n] dudt, dTdt, dilatation, Nu[5]
+    hipGraph_t g_pre[2] = {nullptr, nullptr}, g_post = nullptr;
+    hipGraphExec_t x_pre[2] = {nullptr, nullptr}, x_post = nullptr;
     std::vector<void *> allocs;
 };
 
@@ -1281,8 +1294,15 @@ extern "C" int32_t nim_ens_destroy(nim_ens *e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (int q = 0; q < 2; q++) {
+        if (e->x_pre[q]) (void)hipGraphExecDestroy(e->x_pre[q]);
+        if (e->g_pre[q]) (void)hipGraphDestroy(e->g_pre[q]);
+    }
+    if (e->x_post) (void)hipGraphExecDestroy(e->x_post);
+    if (e->g_post) (void)hipGraphDestroy(e->g_post);
     for (void *p : e->allocs) (void)hipFree(p);
     if (e->h_ctl) (void)hipHostFree(e->h_ctl);
+    if (e->h_out) (void)hipHostFree(e->h_out);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return NIM_OK;
@@ -1315,6 +1335,7 @@ extern "C" int32_t nim_ens_create(nim_ctx *const *cases, int32_t ncase, nim_ens 
             case SH_COL64: acc += (cc + 63) / 64; break;
             case SH_COL256: acc += nblk(cc); break;
             case SH_WALL: acc += nblk(2 * c->NT * c->NR); break;
+            case SH_NRED: acc += c->nred; break;  // blocks of 256 cells in storage order
             default: acc += nblk(std::max(c->N, c->NF)); break;
             }
         }
@@ -1332,10 +1353,26 @@ extern "C" int32_t nim_ens_create(nim_ctx *const *cases, int32_t ncase, nim_ens 
         (rc = alloc((void **)&e->d_v64, sizeof(View<double>) * ncase)) ||
         (rc = alloc((void **)&e->d_v32, sizeof(View<float>) * ncase)) ||
         (rc = alloc((void **)&e->d_bstart, sizeof(int) * bs.size())) ||
-        (rc = alloc((void **)&e->d_ctl, sizeof(Ctl) * ncase)))
+        (rc = alloc((void **)&e->d_ctl, sizeof(Ctl) * ncase)) ||
+        (rc = alloc((void **)&e->d_nred, sizeof(int) * ncase)) ||
+        (rc = alloc((void **)&e->d_R, sizeof(double) * 2 * ncase)) ||
+        (rc = alloc((void **)&e->d_out, sizeof(double) * 8 * ncase)))
         return bad(rc);
-    if (hipHostMalloc((void **)&e->h_ctl, sizeof(Ctl) * ncase, 0) != hipSuccess)
+    if (hipHostMalloc((void **)&e->h_ctl, sizeof(Ctl) * ncase, 0) != hipSuccess ||
+        hipHostMalloc((void **)&e->h_out, sizeof(double) * 8 * ncase, 0) != hipSuccess)
         return bad(fail(NIM_E_HIP, "hipHostMalloc failed"));
+    {
+        std::vector<int> nred(ncase);
+        std::vector<double> R(2 * (size_t)ncase);
+        for (int q = 0; q < ncase; q++) {
+            nred[q] = cases[q]->nred;
+            R[2 * q] = cases[q]->prm.R_i;
+            R[2 * q + 1] = cases[q]->prm.R_o;
+        }
+        if (hipMemcpy(e->d_nred, nred.data(), sizeof(int) * ncase, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(e->d_R, R.data(), sizeof(double) * 2 * ncase, hipMemcpyHostToDevice) != hipSuccess)
+            return bad(fail(NIM_E_HIP, "ensemble table upload failed"));
+    }
     if (hipStreamCreate(&e->stream) != hipSuccess) return bad(fail(NIM_E_HIP, "hipStreamCreate failed"));
     if (hipMemcpy(e->d_dev, devs.data(), sizeof(Dev) * ncase, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(e->d_bstart, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -1415,30 +1452,26 @@ static int32_t ens_batch(nim_ens *e, int batch, int inner)
     return NIM_OK;
 }
 
-extern "C" int32_t nim_ens_inner_solve(nim_ens *e, const double tol[5], int32_t max_iter, int32_t min_iter,
-                                       int32_t inner, int32_t *iters, double *err)
+static int32_t ens_upload_views(nim_ens *e)
 {
-    if (!e || !tol) return fail(NIM_E_ARG, "null argument");
-    if (max_iter < 1 || inner < 0 || inner > 1024) return fail(NIM_E_ARG, "bad iteration arguments");
+    if (e->views) return NIM_OK;
+    std::vector<View<double>> v64(e->n);
+    std::vector<View<float>> v32(e->n);
+    for (int q = 0; q < e->n; q++) {
+        v64[q] = e->cs[q]->mx.v64;
+        v32[q] = e->cs[q]->mx.v32;
+    }
+    HIPCHK(hipMemcpy(e->d_v64, v64.data(), sizeof(View<double>) * e->n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_v32, v32.data(), sizeof(View<float>) * e->n, hipMemcpyHostToDevice));
+    e->views = true;
+    return NIM_OK;
+}
+
+// the passes (inner = 0) or mixed cycles of every case on the ensemble
+// stream until all have stopped; control blocks already reset
+static int32_t ens_loop(nim_ens *e, int inner, int32_t *iters, double *err)
+{
     int32_t rc;
-    for (nim_ctx *c : e->cs) {
-        if (inner > 0 && (rc = mixed_prepare(c))) return rc;
-        if ((rc = reset_ctl(c, tol, max_iter, min_iter))) return rc;
-    }
-    if (inner > 0 && !e->views) {
-        std::vector<View<double>> v64(e->n);
-        std::vector<View<float>> v32(e->n);
-        for (int q = 0; q < e->n; q++) {
-            v64[q] = e->cs[q]->mx.v64;
-            v32[q] = e->cs[q]->mx.v32;
-        }
-        HIPCHK(hipMemcpy(e->d_v64, v64.data(), sizeof(View<double>) * e->n, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(e->d_v32, v32.data(), sizeof(View<float>) * e->n, hipMemcpyHostToDevice));
-        e->views = true;
-    }
-    // the cases' own streams carry their per-step work (coefficients,
-    // casts, control reset): finish it before the ensemble stream starts
-    for (nim_ctx *c : e->cs) HIPCHK(hipStreamSynchronize(c->stream));
     int batch = inner > 0 ? std::max(1, (16 + inner) / (inner + 1)) : batch_size();
     for (;;) {
         if ((rc = ens_batch(e, batch, inner))) return rc;
@@ -1463,5 +1496,84 @@ extern "C" int32_t nim_ens_inner_solve(nim_ens *e, const double tol[5], int32_t 
     }
     if (worst == NIM_E_NONFINITE) return fail(worst, "non-finite increment in a case");
     if (worst == NIM_E_MAXITER) return fail(worst, "iteration cap reached in a case");
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_ens_inner_solve(nim_ens *e, const double tol[5], int32_t max_iter, int32_t min_iter,
+                                       int32_t inner, int32_t *iters, double *err)
+{
+    if (!e || !tol) return fail(NIM_E_ARG, "null argument");
+    if (max_iter < 1 || inner < 0 || inner > 1024) return fail(NIM_E_ARG, "bad iteration arguments");
+    int32_t rc;
+    for (nim_ctx *c : e->cs) {
+        if (inner > 0 && (rc = mixed_prepare(c))) return rc;
+        if ((rc = reset_ctl(c, tol, max_iter, min_iter))) return rc;
+    }
+    if (inner > 0 && (rc = ens_upload_views(e))) return rc;
+    // the cases' own streams carry their per-step work (coefficients,
+    // casts, control reset): finish it before the ensemble stream starts
+    for (nim_ctx *c : e->cs) HIPCHK(hipStreamSynchronize(c->stream));
+    return ens_loop(e, inner, iters, err);
+}
+
+// capture `body` (launches on the ensemble stream) into g / x once
+template <class Body>
+static int32_t ens_graph(nim_ens *e, hipGraph_t &g, hipGraphExec_t &x, Body body)
+{
+    if (x) return NIM_OK;
+    if (!use_graph()) {
+        body();
+        HIPCHK(hipGetLastError());
+        return NIM_OK;
+    }
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    HIPCHK(hipStreamEndCapture(e->stream, &g));
+    HIPCHK(hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_ens_time_step(nim_ens *e, const double tol[5], int32_t max_iter, int32_t min_iter,
+                                     int32_t inner, int32_t *iters, double *err, double *out)
+{
+    if (!e || !tol) return fail(NIM_E_ARG, "null argument");
+    if (max_iter < 1 || inner < 0 || inner > 1024) return fail(NIM_E_ARG, "bad iteration arguments");
+    int32_t rc;
+    // the cases' streams must be idle; the mixed loop's buffers and FP32
+    // constants exist (set up once, per case)
+    for (nim_ctx *c : e->cs) {
+        if (inner > 0 && (!c->mx.alloc || c->mx.version != c->const_version) && (rc = mixed_prepare(c))) return rc;
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (inner > 0 && (rc = ens_upload_views(e))) return rc;
+    hipStream_t s = e->stream;
+    // control blocks, then the momentum coefficients of every case (with
+    // their FP32 copies for the mixed loop): one captured graph
+    for (nim_ctx *c : e->cs)
+        hipLaunchKernelGGL(k_solve_reset, dim3(1), dim3(256), 0, s, c->dev, tol[0], tol[1], tol[2], tol[3], tol[4],
+                           max_iter, min_iter, c->p_measure);
+    HIPCHK(hipGetLastError());
+    const int cast = inner > 0;
+    auto pre = [&]() {
+        hipLaunchKernelGGL(k_e_coef_A, dim3(e->total[SH_NRED]), dim3(BS), 0, s, ens_args(e, SH_NRED));
+        hipLaunchKernelGGL(k_e_coef_F, dim3(e->total[SH_NRED]), dim3(BS), 0, s, ens_args(e, SH_NRED), cast);
+    };
+    if ((rc = ens_graph(e, e->g_pre[cast], e->x_pre[cast], pre))) return rc;
+    if (use_graph()) HIPCHK(hipGraphLaunch(e->x_pre[cast], s));
+    if ((rc = ens_loop(e, inner, iters, err))) return rc;
+    // end of step, time shift and Nusselt numbers of every case: one graph
+    auto post = [&]() {
+        hipLaunchKernelGGL(k_e_end_step_partial, dim3(e->total[SH_NRED]), dim3(BS), 0, s, ens_args(e, SH_NRED));
+        hipLaunchKernelGGL(k_e_end_step_final, dim3(e->n), dim3(BS), 0, s, ens_args(e, SH_NRED),
+                           (const int *)e->d_nred, e->d_out);
+        hipLaunchKernelGGL(k_e_advance, dim3(e->total[SH_NRED]), dim3(BS), 0, s, ens_args(e, SH_NRED));
+        hipLaunchKernelGGL(k_e_nusselt, dim3(e->n), dim3(BS), 0, s, ens_args(e, SH_NRED), (const double *)e->d_R,
+                           e->d_out);
+    };
+    if ((rc = ens_graph(e, e->g_post, e->x_post, post))) return rc;
+    if (use_graph()) HIPCHK(hipGraphLaunch(e->x_post, s));
+    HIPCHK(hipMemcpyAsync(e->h_out, e->d_out, sizeof(double) * 8 * e->n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (out) std::memcpy(out, e->h_out, sizeof(double) * 8 * e->n);
     return NIM_OK;
 }

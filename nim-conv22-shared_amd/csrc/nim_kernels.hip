@@ -85,11 +85,8 @@ __device__ __forceinline__ bool sched_cell(const Dev &d, const Sched &s, int lb,
 // exponential-fit constants of every cell: momentum (time-centred velocity,
 // nu; nim.f90:166-290) or energy (mean of current iterate and previous
 // level, alpha; nim.f90:339-463)
-__global__ void __launch_bounds__(256) k_coef_A(Dev d, int energy)
+__device__ __forceinline__ void coef_A_cell(const Dev &d, int energy, int c)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (energy && d.ctl->done) return;
-    if (c >= d.N) return;
     const int N = d.N;
     const double *g = d.gc;
     double ux, uy, uz, kd;
@@ -115,17 +112,21 @@ __global__ void __launch_bounds__(256) k_coef_A(Dev d, int energy)
     }
 }
 
+__global__ void __launch_bounds__(256) k_coef_A(Dev d, int energy)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (energy && d.ctl->done) return;
+    if (c >= d.N) return;
+    coef_A_cell(d, energy, c);
+}
+
 // F4, F1..F3 from the constants (nim.f90:292-329 / 465-502); for the energy
 // equation the same kernel also applies the adiabatic top/bottom wall
 // extrapolation apply_zero_flux (bc.f90:345-390), which reads only interior
 // axial faces
-__global__ void __launch_bounds__(256) k_coef_F(Dev d, int energy)
+__device__ __forceinline__ void coef_F_cell(const Dev &d, int energy, int c, double &eT)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (energy && d.ctl->done) return;
-    bool act = c < d.N;
-    double eT = 0.;
-    if (act) {
+    if (c < d.N) {
         const int N = d.N;
         const double *A = energy ? d.TA : d.MA;
         double *F = energy ? d.TF : d.MF;
@@ -179,6 +180,14 @@ __global__ void __launch_bounds__(256) k_coef_F(Dev d, int energy)
             T[fw] = nv;
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_coef_F(Dev d, int energy)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (energy && d.ctl->done) return;
+    double eT = 0.;
+    coef_F_cell(d, energy, c, eT);
     if (energy) err_commit(d, F_T, eT);
 }
 
@@ -1262,9 +1271,8 @@ __global__ void __launch_bounds__(256) k_wall_pressure_coeffs(Dev d)
 }
 
 // update_time_step (nse.f90:1292-1313)
-__global__ void __launch_bounds__(256) k_advance(Dev d)
+__device__ __forceinline__ void advance_cell(const Dev &d, int c)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= d.N) return;
     const int N = d.N;
     double *s = d.sc;
@@ -1281,11 +1289,13 @@ __global__ void __launch_bounds__(256) k_advance(Dev d)
     s[S_DILAO * N + c] = s[S_DILA * N + c];
 }
 
+__global__ void __launch_bounds__(256) k_advance(Dev d) { advance_cell(d, blockIdx.x * blockDim.x + threadIdx.x); }
+
 // per-block partial results of check_steady_state (nse.f90:1316-1332) and
 // evaluate_D (nse.f90:2504-2537): red[b*6 + {du, dv, dw, dT, sumD2}]
-__global__ void __launch_bounds__(256) k_end_step_partial(Dev d)
+__device__ __forceinline__ void end_step_partial(const Dev &d, int lb)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = lb * blockDim.x + threadIdx.x;
     double m[4] = {0., 0., 0., 0.}, s = 0.;
     if (c < d.N) {
         const int N = d.N;
@@ -1318,13 +1328,15 @@ __global__ void __launch_bounds__(256) k_end_step_partial(Dev d)
         for (int q = 0; q < 5; q++) {
             double v = sh[0][q];
             for (int w = 1; w < (int)(blockDim.x >> 6); w++) v = q < 4 ? fmax(v, sh[w][q]) : v + sh[w][q];
-            d.red[(size_t)blockIdx.x * 6 + q] = v;
+            d.red[(size_t)lb * 6 + q] = v;
         }
     }
 }
 
+__global__ void __launch_bounds__(256) k_end_step_partial(Dev d) { end_step_partial(d, blockIdx.x); }
+
 // combine the partials in a fixed order; out = {dudt_max, dTdt_max, Dilatation}
-__global__ void __launch_bounds__(256) k_end_step_final(Dev d, int nblocks, double *out)
+__device__ __forceinline__ void end_step_final(const Dev &d, int nblocks, double *out)
 {
     int l = threadIdx.x;  // 256 threads
     double m[4] = {0., 0., 0., 0.}, s = 0.;
@@ -1353,9 +1365,14 @@ __global__ void __launch_bounds__(256) k_end_step_final(Dev d, int nblocks, doub
     }
 }
 
+__global__ void __launch_bounds__(256) k_end_step_final(Dev d, int nblocks, double *out)
+{
+    end_step_final(d, nblocks, out);
+}
+
 // find_Nu2 (nse.f90:2699-2798): one block, fixed summation order.
 // out = {Nu_H, Nu_C, Average, Area1, Area2}
-__global__ void __launch_bounds__(256) k_nusselt(Dev d, double R_i, double R_o, double *out)
+__device__ __forceinline__ void nusselt(const Dev &d, double R_i, double R_o, double *out)
 {
     const int NT = d.NT, NR = d.NR, NZ = d.NZ, NB = d.NB;
     const double *T = d.fc + (size_t)F_T * d.NF, *W = d.wall;
@@ -1392,6 +1409,11 @@ __global__ void __launch_bounds__(256) k_nusselt(Dev d, double R_i, double R_o, 
         out[3] = a1;
         out[4] = a2;
     }
+}
+
+__global__ void __launch_bounds__(256) k_nusselt(Dev d, double R_i, double R_o, double *out)
+{
+    nusselt(d, R_i, R_o, out);
 }
 
 
@@ -1477,6 +1499,50 @@ __global__ void __launch_bounds__(256) k_e_d_T_sweep(Ens e, int color)
     ENS_CASE T_sweep<MODE_DELTA>(d, e.v32[q], e.v32[q].TF, full_phase(d, color), lb);
 }
 __global__ void __launch_bounds__(256) k_e_mix_update(Ens e) { ENS_CASE mix_update(d, e.v64[q], e.v32[q], lb); }
+
+// per-step work of every case (no early exit on convergence: it runs
+// around the solve)
+#define ENS_STEP_CASE                                                                                   \
+    int lb;                                                                                             \
+    const int q = ens_case(e, lb);                                                                      \
+    if (lb >= e.bstart[q + 1] - e.bstart[q]) return;                                                    \
+    const Dev &d = e.dev[q];
+
+// momentum coefficients (calculate_velocity_nim_constants_formulation2),
+// with the FP32 copy of the mixed loop when cast is set
+__global__ void __launch_bounds__(256) k_e_coef_A(Ens e)
+{
+    ENS_STEP_CASE
+    const int c = lb * blockDim.x + threadIdx.x;
+    if (c < d.N) coef_A_cell(d, 0, c);
+}
+__global__ void __launch_bounds__(256) k_e_coef_F(Ens e, int cast)
+{
+    ENS_STEP_CASE
+    const int c = lb * blockDim.x + threadIdx.x;
+    double eT = 0.;
+    coef_F_cell(d, 0, c, eT);
+    if (cast && c < d.N)
+        for (int m = 0; m < NCF; m++)
+            const_cast<float *>(e.v32[q].MF)[(size_t)m * d.N + c] = (float)d.MF[(size_t)m * d.N + c];
+}
+// check_steady_state + evaluate_D partials (per-cell launch shape); then one
+// block per case reduces them into out[8 q + 0..2]
+__global__ void __launch_bounds__(256) k_e_end_step_partial(Ens e) { ENS_STEP_CASE end_step_partial(d, lb); }
+__global__ void __launch_bounds__(256) k_e_end_step_final(Ens e, const int *nred, double *out)
+{
+    end_step_final(e.dev[blockIdx.x], nred[blockIdx.x], out + 8 * (size_t)blockIdx.x);
+}
+__global__ void __launch_bounds__(256) k_e_advance(Ens e)
+{
+    ENS_STEP_CASE
+    advance_cell(d, lb * blockDim.x + threadIdx.x);
+}
+// find_Nu2 of every case, one block per case, into out[8 q + 3..7]
+__global__ void __launch_bounds__(256) k_e_nusselt(Ens e, const double *R, double *out)
+{
+    nusselt(e.dev[blockIdx.x], R[2 * blockIdx.x], R[2 * blockIdx.x + 1], out + 8 * (size_t)blockIdx.x + 3);
+}
 
 // the iteration control of every case into one array (one host read)
 __global__ void __launch_bounds__(64) k_e_gather(Ens e, Ctl *out)

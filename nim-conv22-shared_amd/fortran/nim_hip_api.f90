@@ -103,6 +103,12 @@ module nim_hip_api
       integer(c_int32_t), intent(out) :: cycles
       real(c_double), intent(out) :: err(5)
     end function
+    ! what tol(4) bounds: 0 max |dp| (find_error), 1 half the range of dp
+    integer(c_int32_t) function nim_set_p_measure(ctx, mode) bind(C, name="nim_set_p_measure")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: mode
+    end function
     ! FP64 residual measure of the current state
     integer(c_int32_t) function nim_residual(ctx, err) bind(C, name="nim_residual")
       import :: c_int32_t, c_ptr, c_double

@@ -9,7 +9,7 @@ module nim_nse
   use nim_hip_api
   implicit none
   private
-  public :: nim_run_config, nim_setup, nim_solve, nim_write_restart, nim_read_restart, nim_finish
+  public :: nim_run_config, nim_setup, nim_solve, nim_write_restart, nim_read_restart, nim_finish, nim_write_data
 
   type :: nim_run_config
     type(nim_geo_case) :: geo
@@ -27,11 +27,19 @@ module nim_nse
     ! per-cell metrics on the device (nim_device_metrics, bit for bit the
     ! host's build_geometry, about 10x faster at 8M cells); .false.: host
     logical :: device_metrics = .true.
+    ! write the cell fields every step_out steps (STEP_OUT, parameters.f90:138;
+    ! 0: never) as plots_dir/stepNNNNN.vtk, listed in plots.visit
+    integer :: step_out = 0
+    ! what tol(4) bounds: 0 max |dp| (find_error, the reference's), 1 half
+    ! the range of dp (blind to p's constant level)
+    integer :: p_measure = 0
+    character(len=256) :: plots_dir = 'plots'
   end type
 
   type(c_ptr), save :: ctx = c_null_ptr
-  integer, save :: N, NF, NB
+  integer, save :: N, NF, NB, nth, nrr, nzz
   real(c_double), save :: TI, DTs
+  real(c_double), allocatable, save :: node_xyz(:,:)   ! (3, NNODE), for the field output
   integer, save :: STEP
   ! host copies of the state in the reference's layout
   real(c_double), allocatable, target, save :: faces_u(:), faces_v(:), faces_w(:), faces_p(:), faces_T(:)
@@ -115,7 +123,12 @@ contains
     g%G1 = c_loc(G1); g%G2 = c_loc(G2); g%G3 = c_loc(G3); g%faces_area = c_loc(area)
     g%faces_normal = c_loc(nrm); g%faces_mid = c_loc(mid); g%bc_points = c_loc(bcpts); g%bc_a = c_loc(bc_a)
     call check(nim_upload_geometry(ctx, g), 'nim_upload_geometry')
+    nth = nt; nrr = nr; nzz = nz
+    if (allocated(node_xyz)) deallocate(node_xyz)
+    allocate(node_xyz(3, nn))
+    node_xyz = nodes
     call check(nim_build_pressure_coeffs(ctx), 'nim_build_pressure_coeffs')
+    call check(nim_set_p_measure(ctx, int(cfg%p_measure, c_int32_t)), 'nim_set_p_measure')
     allocate(faces_u(NF), faces_v(NF), faces_w(NF), faces_p(NF), faces_T(NF))
     allocate(uxyz(N), vxyz(N), wxyz(N), Txyz(N), uxyz_old(N), vxyz_old(N), wxyz_old(N), Txyz_old(N))
     allocate(uxyzt(N), vxyzt(N), wxyzt(N), Txyzt(N), meshes_p(N), dila(N))
@@ -155,6 +168,10 @@ contains
         call check(rc, 'nim_inner_solve')
       endif
       call check(nim_end_step(ctx, dudt, dTdt, dil), 'nim_end_step')
+      ! write_data every STEP_OUT steps (nse.f90:59)
+      if (cfg%step_out > 0) then
+        if (mod(STEP, cfg%step_out) == 0) call nim_write_data(trim(cfg%plots_dir))
+      endif
       print '(A, I6, A, I6, A, 1p, E11.4, A, E11.4, A, E11.4, A, E11.4, A, E11.4, A, E11.4)', 'step #', STEP, &
             '  | # iters. ', iters, '  | err V=', max(err(1), err(2), err(3)), '  | err p=', err(4), &
             '  | err T=', err(5), '  | D=', dil, '  | dudt=', dudt, '  | dTdt=', dTdt
@@ -170,6 +187,7 @@ contains
           write(tag, '(I0)') STEP
           call nim_write_restart(trim(restart_prefix)//trim(tag))
         endif
+        if (cfg%step_out > 0) call nim_write_data(trim(cfg%plots_dir))   ! nse.f90:66
         done = done + 1
         exit
       endif
@@ -236,6 +254,78 @@ contains
     STEP = STEP + 1
     TI = TI + 2*DTs
     call check(nim_advance_time(ctx), 'nim_advance_time')
+  end subroutine
+
+  ! write_data (postprocess.f90:2-124) without Silo, which is not available
+  ! here: the same mesh (nodes, hexahedra) and the same zone-centred single
+  ! precision fields u, v, w (time-centred), p, T (time-centred) and D
+  ! (dilatation), as a legacy VTK unstructured grid dir/stepNNNNN.vtk with
+  ! the cycle and time of the reference's option list, and the file name
+  ! appended to plots.visit
+  subroutine nim_write_data(dir)
+    character(len=*), intent(in) :: dir
+    character(len=5) :: x1
+    character(len=300) :: name
+    real(c_double) :: dtime
+    integer :: u, c, i, j, k, ip, nd(8)
+    logical :: exist
+    call check(nim_download_state(ctx, state_ptrs()), 'nim_download_state')
+    dtime = STEP*2.*DTs - DTs      ! postprocess.f90:47-48
+    if (STEP == 0) dtime = 0.0
+    write(x1, '(I5.5)') STEP
+    name = trim(dir)//'/step'//x1//'.vtk'
+    call execute_command_line('mkdir -p '//trim(dir))
+    print '(A, 1p, E13.6, A, I0)', 'Printing data at time = ', dtime, ' | Time step = ', STEP
+    open(newunit=u, file=trim(name), status='replace', form='formatted', action='write')
+    write(u, '(A)') '# vtk DataFile Version 3.0'
+    write(u, '(A, I0, A, 1p, E15.8)') 'Navier Stokes Equation Solution, cycle ', STEP, ', time ', dtime
+    write(u, '(A)') 'ASCII'
+    write(u, '(A)') 'DATASET UNSTRUCTURED_GRID'
+    write(u, '(A, I0, A)') 'POINTS ', size(node_xyz, 2), ' float'
+    do i = 1, size(node_xyz, 2)
+      write(u, '(1p, 3E16.8)') real(node_xyz(:, i), 4)
+    enddo
+    ! hexahedra: node i + nth (j + (nrr + 1) k) (0-based), bottom quad r
+    ! first then theta (counter-clockwise seen from +z), then the top quad
+    write(u, '(A, I0, A, I0)') 'CELLS ', N, ' ', 9*N
+    do c = 0, N - 1
+      i = mod(c, nth); j = mod(c / nth, nrr); k = c / (nth*nrr); ip = mod(i + 1, nth)
+      nd = (/ i + nth*(j + (nrr+1)*k), i + nth*(j + 1 + (nrr+1)*k), ip + nth*(j + 1 + (nrr+1)*k), &
+              ip + nth*(j + (nrr+1)*k), i + nth*(j + (nrr+1)*(k+1)), i + nth*(j + 1 + (nrr+1)*(k+1)), &
+              ip + nth*(j + 1 + (nrr+1)*(k+1)), ip + nth*(j + (nrr+1)*(k+1)) /)
+      write(u, '(I0, 8(1X, I0))') 8, nd
+    enddo
+    write(u, '(A, I0)') 'CELL_TYPES ', N
+    do c = 1, N
+      write(u, '(A)') '12'
+    enddo
+    write(u, '(A, I0)') 'CELL_DATA ', N
+    call field('u', uxyzt)
+    call field('v', vxyzt)
+    call field('w', wxyzt)
+    call field('p', meshes_p)
+    call field('T', Txyzt)
+    call field('D', dila)
+    close(u)
+    inquire(file='plots.visit', exist=exist)
+    if (exist) then
+      open(newunit=u, file='plots.visit', status='old', position='append', action='write')
+    else
+      open(newunit=u, file='plots.visit', status='new', action='write')
+    endif
+    write(u, '(A)') trim(name)
+    close(u)
+  contains
+    subroutine field(fname, v)
+      character(len=*), intent(in) :: fname
+      real(c_double), intent(in) :: v(:)
+      integer :: q
+      write(u, '(A, A, A)') 'SCALARS ', fname, ' float 1'
+      write(u, '(A)') 'LOOKUP_TABLE default'
+      do q = 1, size(v)
+        write(u, '(1p, E16.8)') real(v(q), 4)
+      enddo
+    end subroutine
   end subroutine
 
   subroutine nim_finish()

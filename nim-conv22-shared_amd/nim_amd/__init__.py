@@ -131,6 +131,8 @@ def lib():
         L.nim_ens_destroy.argtypes = [vp]
         L.nim_ens_inner_solve.restype = i32
         L.nim_ens_inner_solve.argtypes = [vp, _P, i32, i32, i32, _PI, _P]
+        L.nim_ens_time_step.restype = i32
+        L.nim_ens_time_step.argtypes = [vp, _P, i32, i32, i32, _PI, _P, _P]
         L.nim_adj_create.restype = i32
         L.nim_adj_create.argtypes = [vp, ctypes.POINTER(vp)]
         for n in ("nim_adj_destroy", "nim_adj_reverse_pass", "nim_adj_clear"):
@@ -142,6 +144,10 @@ def lib():
         L.nim_adj_reverse_step.argtypes = [vp, ctypes.c_double, i32, ctypes.POINTER(i32), _P]
         L.nim_adj_gradient.restype = i32
         L.nim_adj_gradient.argtypes = [vp, _P]
+        L.nim_adj_reverse_step_gmres.restype = i32
+        L.nim_adj_reverse_step_gmres.argtypes = [vp, ctypes.c_double, i32, i32, ctypes.POINTER(i32), _P]
+        L.nim_adj_gradient_faces.restype = i32
+        L.nim_adj_gradient_faces.argtypes = [vp, _P, ctypes.c_int64]
         L.nim_adj_reverse_stage.restype = i32
         L.nim_adj_reverse_stage.argtypes = [vp, i32]
         L.nim_adj_get.restype = i32
@@ -163,8 +169,8 @@ def exported_symbols():
             "nim_inner_solve", "nim_set_p_measure", "nim_end_step", "nim_end_step_noalloc", "nim_advance_time", "nim_nusselt", "nim_run_stage",
             "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_kernel_times",
             "nim_inner_solve_mixed", "nim_residual", "nim_kernel_times_mixed", "nim_ens_create", "nim_ens_destroy",
-            "nim_ens_inner_solve", "nim_revolve_plan", "nim_ckpt_create", "nim_ckpt_destroy", "nim_ckpt_store",
-            "nim_ckpt_restore", "nim_adj_create", "nim_adj_destroy", "nim_adj_seed_nusselt", "nim_adj_reverse_step",
+            "nim_ens_inner_solve", "nim_ens_time_step", "nim_revolve_plan", "nim_ckpt_create", "nim_ckpt_destroy", "nim_ckpt_store",
+            "nim_ckpt_restore", "nim_adj_create", "nim_adj_destroy", "nim_adj_seed_nusselt", "nim_adj_reverse_step", "nim_adj_reverse_step_gmres", "nim_adj_gradient_faces",
             "nim_adj_gradient", "nim_adj_reverse_pass", "nim_adj_reverse_stage", "nim_adj_clear", "nim_adj_get",
             "nim_adj_set", "nim_device_metrics", "nim_last_error"]
 
@@ -422,6 +428,22 @@ class Ensemble:
             _check(rc)
         return rc, it, err
 
+    def time_step_batched(self, tol, inner=0, max_iter=100000, min_iter=0, raise_on_cap=True):
+        """One solve_nse step of every case with the per-step work batched
+        over the cases and graph-captured (nim_ens_time_step). Returns (rc,
+        iters[ncase], err[ncase, 5], [(end-step scalars, Nusselt[5])])."""
+        t = np.ascontiguousarray(tol, dtype=np.float64)
+        n = len(self.cases)
+        it = np.zeros(n, dtype=np.int32)
+        err = np.zeros((n, 5))
+        out = np.zeros((n, 8))
+        rc = self.L.nim_ens_time_step(self.h, _dp(t), max_iter, min_iter, inner, it.ctypes.data_as(_PI), _dp(err),
+                                      _dp(out))
+        if rc != NIM_OK and (raise_on_cap or rc != NIM_E_MAXITER):
+            _check(rc)
+        res = [(dict(dudt_max=o[0], dTdt_max=o[1], Dilatation=o[2]), o[3:8].copy()) for o in out]
+        return rc, it, err, res
+
     def time_step(self, tol, inner=0, max_iter=100000, min_iter=0):
         """One solve_nse step of every case."""
         for c in self.cases:
@@ -562,10 +584,26 @@ class Adjoint:
         _check(self.L.nim_adj_reverse_step(self.h, tol, max_passes, ctypes.byref(n), _dp(dl)))
         return n.value, float(dl[0])
 
+    def reverse_step_gmres(self, tol, max_passes=20000, restart=30):
+        """The same as reverse_step with restarted GMRES on the reversed
+        fixed point; returns (reversed passes, relative residual)."""
+        n = ctypes.c_int32()
+        rr = np.zeros(1)
+        _check(self.L.nim_adj_reverse_step_gmres(self.h, tol, max_passes, restart, ctypes.byref(n), _dp(rr)))
+        return n.value, float(rr[0])
+
     def gradient(self):
         out = np.zeros(4)
         _check(self.L.nim_adj_gradient(self.h, _dp(out)))
         return dict(zip(self.PARAMS, out.tolist()))
+
+    def gradient_faces(self):
+        """Accumulated dJ/dT of each hexagon wall face, n_theta * n_z values
+        in the reference's boundary order (face i + n_theta k)."""
+        nt, nr, nz = self.ctx.dims.n_theta, self.ctx.dims.n_r, self.ctx.dims.n_z
+        out = np.zeros(nt * nz)
+        _check(self.L.nim_adj_gradient_faces(self.h, _dp(out), out.size))
+        return out
 
 
 def run_steps(ctx, nsteps, tol=None, passes=None):
@@ -578,44 +616,76 @@ def run_steps(ctx, nsteps, tol=None, passes=None):
     return ctx.nusselt()
 
 
-def _solve(ctx, tol, passes):
+def _solve(ctx, tol, passes, max_iter=200000, mixed_inner=0):
+    """One step's momentum coefficients and inner solve (to tol, or exactly
+    `passes` passes; with mixed_inner > 0 the mixed-precision loop to tol on
+    its FP64 residual); returns the passes (FP32 passes when mixed)."""
     ctx.build_momentum_coeffs()
+    if mixed_inner > 0 and passes is None:
+        rc, cycles, _ = ctx.inner_solve_mixed(tol, inner=mixed_inner, max_cycles=max_iter // mixed_inner + 1)
+        return (cycles - 1) * mixed_inner
     if passes is None:
-        ctx.inner_solve(tol)
-    else:
-        rc, it, _ = ctx.inner_solve(np.zeros(5), max_iter=passes, raise_on_cap=False)
-        if it != passes:
-            raise NimError(rc, "inner solve stopped after %d of %d passes" % (it, passes))
+        rc, it, _ = ctx.inner_solve(tol, max_iter=max_iter)
+        return it
+    rc, it, _ = ctx.inner_solve(np.zeros(5), max_iter=passes, raise_on_cap=False)
+    if it != passes:
+        raise NimError(rc, "inner solve stopped after %d of %d passes" % (it, passes))
+    return it
 
 
-def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, weights=(0.0, 1.0, 0.0)):
+def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, weights=(0.0, 1.0, 0.0),
+                     average=False, solver="gmres", restart=30, max_adj_passes=100000, mixed_inner=0):
     """J = weights . nim_nusselt[:3] after nsteps steps of ctx from its
-    current state (as run_steps) and dJ/dT_hot, dJ/dRa by the discrete
-    adjoint, the run reversed with nslots device snapshots (binomial plan).
-    case supplies Ra, Pr and nu. Returns (J, {"T_hot", "Ra"}, info)."""
+    current state (as run_steps), or with average=True its mean over the
+    states after steps 1 .. nsteps, and its gradient by the discrete adjoint,
+    the run reversed with nslots device snapshots (binomial plan). The
+    reversed fixed point of each step is solved by restarted GMRES (solver
+    "gmres", relative residual adj_tol) or by plain iteration ("plain",
+    change per pass adj_tol). case supplies Ra, Pr and nu. Returns (J,
+    {"T_hot", "Ra", "T_faces"}, info); T_faces holds dJ/dT of each hexagon
+    wall face (n_theta * n_z, face i + n_theta k), whose sum is T_hot.
+    mixed_inner > 0 runs the forward steps with the mixed-precision loop
+    (its fixed point is the FP64 one, at which the adjoint is taken).
+    info holds the forward and adjoint wall times and pass counts."""
     if nsteps < 1 or nslots < 1:
         raise ValueError("nsteps and nslots must be at least 1")
+    if solver not in ("gmres", "plain"):
+        raise ValueError("solver must be 'gmres' or 'plain'")
+    import time
     w = np.asarray(weights, dtype=np.float64)
     adj = Adjoint(ctx)
-    out = {}
+    out = {"J": 0.0}
     passes_adj = []
+    clock = {"forward_s": 0.0, "adjoint_s": 0.0, "forward_passes": 0}
+
+    def solve():
+        t0 = time.perf_counter()
+        clock["forward_passes"] += _solve(ctx, tol, passes, mixed_inner=mixed_inner)
+        clock["forward_s"] += time.perf_counter() - t0
 
     def forward_step():
-        _solve(ctx, tol, passes)
+        solve()
         ctx.advance_time()
 
     def reverse_step(k):
-        _solve(ctx, tol, passes)
-        if k == nsteps - 1:
-            out["J"] = float(np.dot(w, ctx.nusselt()[:3]))
-            adj.seed_nusselt(w)
-        passes_adj.append(adj.reverse_step(adj_tol))
+        solve()
+        t0 = time.perf_counter()
+        if average or k == nsteps - 1:
+            wk = w / nsteps if average else w
+            out["J"] += float(np.dot(wk, ctx.nusselt()[:3]))
+            adj.seed_nusselt(wk)
+        if solver == "gmres":
+            passes_adj.append(adj.reverse_step_gmres(adj_tol, max_adj_passes, restart))
+        else:
+            passes_adj.append(adj.reverse_step(adj_tol, max_adj_passes))
+        clock["adjoint_s"] += time.perf_counter() - t0
 
     try:
         fwd = reverse_run(ctx, nsteps, nslots, forward_step, reverse_step)
         g = adj.gradient()
+        gf = adj.gradient_faces()
     finally:
         adj.close()
     dnu = g["nu"] + g["Rinv"] + g["alpha"] / case.Pr
-    grad = {"T_hot": g["T_hot"], "Ra": dnu * (-case.nu / (2.0 * case.Ra))}
-    return out["J"], grad, dict(partials=g, forward_steps=fwd, adjoint_passes=passes_adj)
+    grad = {"T_hot": g["T_hot"], "Ra": dnu * (-case.nu / (2.0 * case.Ra)), "T_faces": gf}
+    return out["J"], grad, dict(partials=g, forward_steps=fwd, adjoint_passes=passes_adj, **clock)

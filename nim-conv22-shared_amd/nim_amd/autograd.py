@@ -21,8 +21,9 @@ def make_context(case, amp=1e-3, seed=20221):
 
 class NusseltRun(torch.autograd.Function):
     """J(T_hot, Ra) = weights . Nu after `nsteps` steps of `case` with those
-    two values, from the seeded initial state. run holds case, nsteps,
-    nslots, adj_tol, passes (or tol), weights, amp, seed."""
+    two values (or its mean over the steps, run["average"]), from the seeded
+    initial state. run holds case, nsteps, nslots, adj_tol, passes (or tol),
+    weights, average, solver, amp, seed."""
 
     @staticmethod
     def forward(fctx, T_hot, Ra, run):
@@ -31,7 +32,8 @@ class NusseltRun(torch.autograd.Function):
         try:
             J, grad, info = nusselt_gradient(ctx, case, run["nsteps"], run["nslots"], run["adj_tol"],
                                              tol=run.get("tol"), passes=run.get("passes"),
-                                             weights=run.get("weights", (0.0, 1.0, 0.0)))
+                                             weights=run.get("weights", (0.0, 1.0, 0.0)),
+                                             average=run.get("average", False), solver=run.get("solver", "gmres"))
         finally:
             ctx.close()
         fctx.grad = (grad["T_hot"], grad["Ra"])

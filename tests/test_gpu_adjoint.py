@@ -11,7 +11,11 @@ The reference has no adjoint, so parity is with the mathematics:
   - dNu/dT_hot and dNu/dRa over a run of time steps reversed with fewer
     snapshots than steps (binomial checkpointing) against central finite
     differences of the forward run, to 1e-5;
+  - the reversed fixed point solved by GMRES gives the plain iteration's
+    gradient with far fewer reversed passes;
   - the torch.autograd harness returns the same gradient.
+The specification's case (48 x 96 x 24, per-face wall temperatures, the
+time-averaged Nusselt number) is in tests/test_gpu_adjoint_c4.py.
 """
 import numpy as np
 import pytest
@@ -195,7 +199,7 @@ def adjoint_run():
     case = small_case(Ra=5e4)
     ctx = nim_amd.autograd.make_context(case)
     try:
-        J, grad, info = nim_amd.nusselt_gradient(ctx, case, NSTEPS, NSLOTS, 1e-13, passes=PASSES)
+        J, grad, info = nim_amd.nusselt_gradient(ctx, case, NSTEPS, NSLOTS, 1e-13, passes=PASSES, solver="plain")
     finally:
         ctx.close()
     return case, J, grad, info
@@ -216,12 +220,34 @@ def test_gradient_matches_finite_differences(adjoint_run):
     assert abs(grad["Ra"] - fd_R) <= 1e-5 * abs(fd_R), (grad, fd_R, info)
 
 
+def test_gmres_matches_plain_iteration(adjoint_run):
+    """Restarted GMRES on (I - G_x^T) lam = seed against the plain reversed
+    iteration (about 7,000 passes per step here): the same gradient, the
+    per-face wall gradients summing to the scalar one, at a small fraction
+    of the reversed passes."""
+    case, J, grad, info = adjoint_run
+    ctx = nim_amd.autograd.make_context(case)
+    try:
+        Jg, gg, ig = nim_amd.nusselt_gradient(ctx, case, NSTEPS, NSLOTS, 1e-12, passes=PASSES, solver="gmres")
+    finally:
+        ctx.close()
+    plain = sum(n for n, _ in info["adjoint_passes"])
+    krylov = sum(n for n, _ in ig["adjoint_passes"])
+    print("reversed passes: plain", plain, "GMRES", krylov, ig["adjoint_passes"])
+    assert Jg == J
+    assert abs(gg["T_hot"] - grad["T_hot"]) <= 1e-9 * abs(grad["T_hot"])
+    assert abs(gg["Ra"] - grad["Ra"]) <= 1e-9 * abs(grad["Ra"])
+    assert abs(gg["T_faces"].sum() - gg["T_hot"]) <= 1e-12 * abs(gg["T_hot"])
+    np.testing.assert_allclose(gg["T_faces"], grad["T_faces"], rtol=1e-8, atol=1e-12 * abs(grad["T_hot"]))
+    assert krylov * 10 < plain
+
+
 def test_autograd_harness(adjoint_run):
     import torch
     case, J, grad, info = adjoint_run
     T = torch.tensor(case.T_hot, dtype=torch.float64, requires_grad=True)
     Ra = torch.tensor(case.Ra, dtype=torch.float64, requires_grad=True)
-    run = dict(case=case, nsteps=NSTEPS, nslots=NSLOTS, adj_tol=1e-13, passes=PASSES)
+    run = dict(case=case, nsteps=NSTEPS, nslots=NSLOTS, adj_tol=1e-13, passes=PASSES, solver="plain")
     out = nim_amd.autograd.nusselt(T, Ra, run)
     gT, gR = torch.autograd.grad(3.0 * out, (T, Ra))
     assert abs(out.item() - J) <= 1e-12 * abs(J)

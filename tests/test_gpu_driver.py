@@ -155,3 +155,64 @@ def test_device_and_host_metrics_run_the_same_bits(tmp_path):
     a, b = run("true"), run("false")
     for n in RECORDS:
         assert np.array_equal(a[n], b[n]), n
+
+
+def read_vtk(path):
+    """Points, hexahedra and cell scalars of the driver's legacy VTK file."""
+    lines = open(path).read().split("\n")
+    assert lines[0].startswith("# vtk DataFile") and lines[3] == "DATASET UNSTRUCTURED_GRID"
+    n_pts = int(lines[4].split()[1])
+    pts = np.array([list(map(float, l.split())) for l in lines[5:5 + n_pts]])
+    c = 5 + n_pts
+    n_cells = int(lines[c].split()[1])
+    conn = np.array([list(map(int, l.split())) for l in lines[c + 1:c + 1 + n_cells]])
+    fields, q = {}, c + 1 + n_cells
+    while q < len(lines):
+        if lines[q].startswith("SCALARS"):
+            name = lines[q].split()[1]
+            fields[name] = np.array(lines[q + 2:q + 2 + n_cells], dtype=float)
+            q += 2 + n_cells
+        else:
+            q += 1
+    return lines[1], pts, conn, fields
+
+
+def test_field_output_every_step_out(tmp_path):
+    """write_data every STEP_OUT steps (nse.f90:59, postprocess.f90:2-124):
+    with step_out=2 over 3 steps the driver writes plots/step00002.vtk (the
+    mesh and the single-precision cell fields u, v, w, p, T, D of the Silo
+    file, here legacy VTK) after step 2's inner solve, and lists it in
+    plots.visit. Its fields are those of the same two steps run through the
+    C-ABI from Python, to single precision."""
+    import nim_amd
+    from nim_amd import host
+    nml = tmp_path / "out.nml"
+    nml.write_text(NML.format(out=tmp_path / "final", inner=0).replace(" /\n", ", step_out=2 /\n"))
+    p = subprocess.run([EXE, str(nml)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    assert (tmp_path / "plots.visit").read_text().split() == ["plots/step00002.vtk"]
+    assert not (tmp_path / "plots" / "step00001.vtk").exists()
+    title, pts, conn, fields = read_vtk(tmp_path / "plots" / "step00002.vtk")
+    assert "cycle 2" in title
+    case = host.reference_case()
+    geo = R.geometry()
+    xyz = host.build_geometry(case)["node_xyz"].reshape(-1, 3)
+    np.testing.assert_allclose(pts, xyz, rtol=1e-7, atol=1e-7)
+    assert conn.shape == (case.ncells, 9)
+    np.testing.assert_allclose(xyz[conn[:, 1:]].mean(axis=1), host.cell_centroids(host.build_geometry(case)),
+                               atol=1e-12)
+    # the same two steps through the C-ABI
+    ctx = G.gpu_context(geo)
+    ctx.upload_state(host.initial_state(case, amp=1e-3, seed=20221))
+    for l in range(2):
+        ctx.build_momentum_coeffs()
+        ctx.inner_solve(TOL, max_iter=200000, min_iter=2 if l == 0 else 0)
+        ctx.end_step()
+        if l == 0:
+            ctx.advance_time()
+    want = {"u": ctx.get("uxyzt"), "v": ctx.get("vxyzt"), "w": ctx.get("wxyzt"), "p": ctx.get("meshes_p"),
+            "T": ctx.get("Txyzt"), "D": ctx.get("meshes_dila")}
+    ctx.close()
+    assert sorted(fields) == sorted(want)
+    for n, v in want.items():
+        np.testing.assert_allclose(fields[n], v.astype(np.float32), rtol=1e-6, atol=1e-6 * np.abs(v).max(), err_msg=n)

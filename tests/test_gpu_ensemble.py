@@ -75,3 +75,54 @@ def test_ensemble_matches_individual_solves(setups, inner):
     ens.close()
     for c in alone + together:
         c.close()
+
+
+def test_c3_cases_whole_steps_match_solo_steps():
+    """Six cases drawn as the specification's 48-case ensemble draws them
+    (host.ensemble_cases: Ra log-uniform in [1e3, 1e6], Pr in {0.71, 7},
+    hexagon circumradius in [0.2, 0.4], n_r in {16, 24, 32, 48}, n_theta =
+    2 n_r, n_z in {12, 16, 24}, seeded), two whole time steps through
+    nim_ens_time_step (the per-step kernels batched over the cases and
+    graph-captured around the solve) in the FP64 loop and in the mixed loop
+    (8 FP32 passes per residual): every case's pass counts, end-of-step
+    scalars, Nusselt numbers and fields are bitwise those of the same steps
+    on its own context."""
+    drawn = host.ensemble_cases()[:6]
+    dims = {case.dims for case, _ in drawn}
+    assert len(dims) >= 4
+    setups = [(case, host.build_geometry(case, device=True), host.initial_state(case, amp=1e-3, seed=sd))
+              for case, sd in drawn]
+    tol = np.array([1e-8, 1e-8, 1e-8, 5e-5, 1e-8])
+    for inner in (0, 8):
+        print("C3 draw", [(c.dims, round(c.Ra), c.Pr) for c, _ in drawn], "inner", inner, flush=True)
+        alone = contexts(setups)
+        ref = []
+        for c in alone:
+            rows = []
+            for step in range(2):
+                c.build_momentum_coeffs()
+                if inner:
+                    rc, n, _ = c.inner_solve_mixed(tol, inner=inner)
+                else:
+                    rc, n, _ = c.inner_solve(tol, min_iter=2 if step == 0 else 0)
+                scal = c.end_step()
+                c.advance_time()
+                rows.append((n, scal, c.nusselt()))
+            ref.append(rows)
+        together = contexts(setups)
+        ens = nim_amd.Ensemble(together)
+        got = [[] for _ in together]
+        for step in range(2):
+            rc, its, err, res = ens.time_step_batched(tol, inner=inner, min_iter=2 if (step == 0 and not inner) else 0)
+            for q in range(len(together)):
+                got[q].append((int(its[q]), res[q][0], res[q][1]))
+        for q, (a, b) in enumerate(zip(alone, together)):
+            for (n0, s0, nu0), (n1, s1, nu1) in zip(ref[q], got[q]):
+                assert n0 == n1, (q, n0, n1)
+                assert s0 == s1, (q, s0, s1)
+                assert np.array_equal(nu0, nu1), (q, nu0, nu1)
+            for n in NAMES + ["uxyz_old", "Txyz_old", "uxyzt", "Txyzt", "meshes_p", "meshes_dila"]:
+                assert np.array_equal(a.get(n), b.get(n)), (q, n)
+        ens.close()
+        for c in alone + together:
+            c.close()

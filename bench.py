@@ -323,6 +323,45 @@ def adjoint_bench(args):
     return 0
 
 
+def strict_tolerance(ctx, args):
+    """FP64 and mixed steps with every field at args.tol, p measured as half
+    the range of its increments (blind to its constant level), each from the
+    current state of ctx (restored afterwards): passes and seconds per step,
+    the final measures and the speedup. p's slowest relaxation mode (about
+    1 - 1.5e-5 per pass at the 1M-cell case) sets the cost."""
+    start = ctx.download_state()
+    tol = np.full(5, args.tol)
+    ctx.set_p_measure(nim_amd.P_HALFRANGE)
+    out = {}
+    for mixed in (False, True):
+        ctx.upload_state(start)
+        steps = []
+        for _ in range(args.strict_steps):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.build_momentum_coeffs()
+            if mixed:
+                rc, n, err = ctx.inner_solve_mixed(tol, inner=args.inner, max_cycles=args.max_iter // args.inner + 1)
+                n = (n - 1) * args.inner
+            else:
+                rc, n, err = ctx.inner_solve(tol, max_iter=args.max_iter)
+            ctx.end_step()
+            ctx.advance_time()
+            ctx.nusselt()
+            ctx.synchronize()
+            steps.append({"passes": int(n), "s": time.perf_counter() - t0, "err": [float(x) for x in err]})
+            print("strict %s step: %d passes, %.1f s" % ("mixed" if mixed else "fp64", n, steps[-1]["s"]),
+                  file=sys.stderr, flush=True)
+        out["mixed" if mixed else "fp64"] = steps
+    ctx.set_p_measure(nim_amd.P_MAXABS)
+    ctx.upload_state(start)
+    t64 = sum(x["s"] for x in out["fp64"])
+    tmx = sum(x["s"] for x in out["mixed"])
+    out["tol"] = "%g on every field, p as half the range of its increments" % args.tol
+    out["speedup_mixed_vs_fp64"] = t64 / tmx
+    return out
+
+
 def time_reversed_pass(ctx, reps):
     """Wall time (ms) of one reversed pass of the discrete adjoint at the
     context's state, from a random adjoint of the iterate (every stage does
@@ -354,6 +393,10 @@ def main():
     ap.add_argument("--p-measure", choices=("maxabs", "halfrange"), default="maxabs",
                     help="what tol-p bounds: max |dp| (find_error) or half the range of dp (blind to p's constant)")
     ap.add_argument("--max-iter", type=int, default=200000, help="cap on passes (FP64) or cycles (mixed) per step")
+    ap.add_argument("--strict-steps", type=int, default=0,
+                    help="after the timed runs: this many steps of each loop with every field at --tol, p as half "
+                         "the range of its increments, from the FP64 run's final state (slow: about 2e5 passes per "
+                         "step at the 1M-cell case)")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
@@ -452,6 +495,9 @@ def main():
 
     wall, total_iters, res64, nu = run(False)
     final64 = {n: ctx.get(n) for n in FACE_STATE + ("faces_p",)}
+    strict = None
+    if args.strict_steps > 0:
+        strict = strict_tolerance(ctx, args)
 
     N = case.ncells
     cups = N * total_iters / wall
@@ -547,6 +593,7 @@ def main():
         "kernel_ms_per_iteration": {k: v[0] for k, v in kt.items()},
         "residual_last_step": [float(x) for x in res64],
         "mixed": mixed,
+        "strict": strict,
         "adjoint": adjoint,
         "setup_s": t_setup,
     }

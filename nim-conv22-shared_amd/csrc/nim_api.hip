@@ -161,7 +161,12 @@ struct nim_ctx {
     // below, the arrays stay cached from pass to pass and plain loads are
     // faster (the mixed loop 16% at 110,592 cells). NIM_NT=0/1 forces it.
     bool nt = false;
-    std::vector<Sched> sched256;   // FP64 sweeps
+    // FP64 sweeps: colour launches walking the planes alternately from the
+    // bottom and from the top (NIM_KREV=0: always from the bottom), starting
+    // from the bottom (sched256) or from the top (sched256r); Dev copy
+    // dev_rev walks the per-cell kernels from the top
+    std::vector<Sched> sched256, sched256r;
+    Dev dev_rev;
     std::vector<Sched> dsched64, dsched256;  // FP32 correction sweeps (storage order: the radial slabs slow them)
     // FP64 loop: energy coefficients built inside the energy sweep instead
     // of stored by a kernel of their own (default; NIM_FUSE_T=0 turns it off)
@@ -342,6 +347,15 @@ extern "C" int32_t nim_ctx_create(const nim_dims *dims, const nim_params *prm, n
     TRY(dalloc(c, &d.errs, (size_t)2 * NERR * ERR_COPIES * ERR_STRIDE));
     TRY(dalloc(c, &c->d_out, (size_t)16));
     d.gc = gc; d.An = An; d.wall = wall; d.wpts = wpts;
+    {
+        const char *kr = getenv("NIM_KREV");
+        const bool krev = kr ? atoi(kr) != 0 : true;
+        c->dev_rev = d;
+        c->dev_rev.krev = krev && d.rslab;
+        if (!krev)
+            for (auto *v : {&c->sched256, &c->sched256r})
+                for (Sched &sc : *v) sc.rev = 0;
+    }
     if (hipHostMalloc((void **)&c->h_ctl, sizeof(Ctl), 0) != hipSuccess ||
         hipHostMalloc((void **)&c->h_out, 16 * sizeof(double), 0) != hipSuccess) {
         nim_ctx_destroy(c);
@@ -727,7 +741,7 @@ static void launch_T_src(const nim_ctx *c, hipStream_t s)
 // reads exactly what it reads in the four full-domain colour phases, which
 // are the one-slab case. Colour c + 1 comes back to a slab two launches
 // later, while its data are still in the caches.
-static std::vector<Sched> build_schedule(const nim_ctx *c, int cells_per_block, bool rslab)
+static std::vector<Sched> build_schedule(const nim_ctx *c, int cells_per_block, bool rslab, int first_rev = 0)
 {
     const int NZ = c->NZ, ks = c->slab, S = (NZ + ks - 1) / ks;
     const int per_plane = (c->NT / 2) * ((c->NR + 1) / 2);
@@ -753,6 +767,9 @@ static std::vector<Sched> build_schedule(const nim_ctx *c, int cells_per_block, 
         if (rslab && sc.n == 1 && sc.k0[0] == 0 && sc.k1[0] == NZ && nrh % 8 == 0 &&
             (nth * (nrh / 8)) % cells_per_block == 0)
             sc.rsb = nth * (nrh / 8) / cells_per_block;
+        // a full-domain colour launch on radial slabs walks the planes in the
+        // direction opposite to the launch before it
+        sc.rev = sc.rsb ? (first_rev + (int)out.size()) & 1 : 0;
         out.push_back(sc);
     }
     return out;
@@ -763,7 +780,8 @@ static void init_schedules(nim_ctx *c)
     const char *e = getenv("NIM_SLAB");
     int ks = e ? atoi(e) : 0;
     c->slab = ks >= 2 && ks < c->NZ ? ks : c->NZ;
-    c->sched256 = build_schedule(c, BS, c->rslab_sweeps);
+    c->sched256 = build_schedule(c, BS, c->rslab_sweeps, 0);
+    c->sched256r = build_schedule(c, BS, c->rslab_sweeps, 1);
     c->dsched64 = build_schedule(c, 64, false);
     c->dsched256 = build_schedule(c, BS, false);
 }
@@ -780,15 +798,18 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark, bool fused = false)
     hipStream_t s = c->stream;
     const int gS = cell_grid(c, BS);
     switch (stage) {
+    // plane directions along a pass: momentum sources up, momentum sweep
+    // down-up-down-up, pressure sources down, pressure sweep up-down-up-down,
+    // energy sources up, energy sweep down-up-down-up (with radial slabs)
     case 0: hipLaunchKernelGGL(k_mom_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d); mark(0); break;
     case 1:
-        for (const Sched &sc : c->sched256) {
+        for (const Sched &sc : c->sched256r) {
             hipLaunchKernelGGL(c->nt ? k_mom_sweep<true> : k_mom_sweep<false>, dim3(sched_blocks(sc)), dim3(BS), 0,
                                s, d, sc);
             mark(1);
         }
         break;
-    case 2: hipLaunchKernelGGL(k_ppe_src, dim3(gS), dim3(BS), 0, s, d); mark(2); break;
+    case 2: hipLaunchKernelGGL(k_ppe_src, dim3(gS), dim3(BS), 0, s, c->dev_rev); mark(2); break;
     case 3:
         for (const Sched &sc : c->sched256) {
             hipLaunchKernelGGL(c->nt ? k_ppe_sweep<true> : k_ppe_sweep<false>, dim3(sched_blocks(sc)), dim3(BS), 0,
@@ -805,7 +826,7 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark, bool fused = false)
         break;
     case 5: launch_T_src(c, s); mark(6); break;
     case 6:
-        for (const Sched &sc : c->sched256) {
+        for (const Sched &sc : c->sched256r) {
             if (fused)
                 hipLaunchKernelGGL(c->nt ? k_T_sweep_fused<true> : k_T_sweep_fused<false>, dim3(sched_blocks(sc)),
                                    dim3(BS), 0, s, d, sc);

@@ -116,6 +116,7 @@ struct Dev {
     double w_u, w_v, w_w, w_p, w_p2, w_T, v_limit;
     int kfast;          // per-cell kernels: blocks numbered plane-fastest (see cell_of_block)
     int rslab;          // per-cell kernels: one radial slab per XCD (see cell_of_block)
+    int krev;           // with rslab: walk the planes from the top (see cell_of_block)
     const double *gc;   // [NGC][N]
     const double *An;   // [3][NF] face area times unit normal
     double *AA;         // [3][15][N] pressure coefficients AA_px, AA_py, AA_pz
@@ -179,6 +180,7 @@ struct Sched {
     int color[MAX_ITEMS], k0[MAX_ITEMS], k1[MAX_ITEMS];
     int bstart[MAX_ITEMS + 1];
     int rsb;  // one item over all planes, one radial slab per XCD: blocks per slab and plane (0: storage order)
+    int rev;  // with rsb: walk the planes from the top
 };
 
 // An ensemble launch: per-case device descriptors and, for this launch
@@ -211,7 +213,9 @@ __device__ __forceinline__ int xcd_block()
 //    x (rows [x*NR/8, (x+1)*NR/8)) plane after plane, so the planes above and
 //    below a cell come through its own L2 and only the rows at the slab edges
 //    are read twice. Set only when NR % 8 == 0, NT*NR/8 is a multiple of
-//    every block size and the grid is N/bs blocks;
+//    every block size and the grid is N/bs blocks. With krev the planes are
+//    walked from the top, so that a kernel starts on the planes the kernel
+//    before it ended on, whose data its XCD's L2 still holds;
 //  * kfast: blocks numbered plane-fastest: logical block lb covers cells
 //    [t*bs, (t+1)*bs) of plane lb % NZ, t = lb / NZ. Returns N for threads
 //    past the end of a plane.
@@ -219,8 +223,8 @@ __device__ __forceinline__ int cell_of_block(const Dev &d, int lb, int bs)
 {
     if (d.rslab) {
         const int sp = d.NT * (d.NR / NXCD), bps = sp / bs, q = d.NZ * bps;
-        const int x = lb / q, r = lb - x * q, k = r / bps;
-        return k * (d.NT * d.NR) + x * sp + (r - k * bps) * bs + (int)threadIdx.x;
+        const int x = lb / q, r = lb - x * q, kk = r / bps, k = d.krev ? d.NZ - 1 - kk : kk;
+        return k * (d.NT * d.NR) + x * sp + (r - kk * bps) * bs + (int)threadIdx.x;
     }
     if (!d.kfast) return lb * bs + (int)threadIdx.x;
     const int ppl = d.NT * d.NR;

@@ -53,6 +53,9 @@ struct PErr {
 __device__ __forceinline__ void err_commit_p(const Dev &d, const PErr &e)
 {
     err_commit(d, F_P, e.a);
+    // (the extra reductions and atomics only when the solve asked for the
+    // half range: uniform over the launch)
+    if (d.ctl->p_measure != P_HALFRANGE) return;
     const double hi = wave_max(e.hi), lo = wave_max(e.lo);
     if ((threadIdx.x & 63) == 0 && hi > -INFINITY) {
         const int slot = d.ctl->iter & 1, cp = blockIdx.x & (ERR_COPIES - 1);
@@ -67,10 +70,10 @@ __device__ __forceinline__ bool sched_cell(const Dev &d, const Sched &s, int lb,
 {
     if (s.rsb) {
         // XCD x takes rows-of-this-colour [x*nrh/8, (x+1)*nrh/8) of every
-        // plane (a radial slab), plane after plane
+        // plane (a radial slab), plane after plane (from the top with rev)
         const int nth = d.NT >> 1, nrh = (d.NR + 1) >> 1, q = s.bstart[1] / NXCD;
-        const int x = lb / q, r = lb - x * q, kk = r / s.rsb;
-        const int tid = kk * (nth * nrh) + x * (nth * (nrh / NXCD)) + (r - kk * s.rsb) * (int)blockDim.x +
+        const int x = lb / q, r = lb - x * q, kk = r / s.rsb, kp = s.rev ? d.NZ - 1 - kk : kk;
+        const int tid = kp * (nth * nrh) + x * (nth * (nrh / NXCD)) + (r - kk * s.rsb) * (int)blockDim.x +
                         (int)threadIdx.x;
         return color_cell(d, s.color[0], s.k0[0], s.k1[0], tid, i, j, k);
     }

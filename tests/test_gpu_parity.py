@@ -333,3 +333,11 @@ def test_non_temporal_sweep_loads_are_bitwise(monkeypatch):
     (NIM_NT=1, the default from 2^19 cells on) load the same values as with
     plain loads, in the FP64 loop and in the FP32 passes of the mixed loop."""
     _bitwise_under(monkeypatch, "NIM_NT", (128, 64, 4))
+
+
+def test_alternating_plane_direction_is_bitwise(monkeypatch):
+    """With radial slabs, the FP64 loop's kernels walk the planes alternately
+    from the bottom and from the top (NIM_KREV, default on), so that each
+    starts where its predecessor ended; only the order of the cells within
+    a launch changes, so every field keeps its bits."""
+    _bitwise_under(monkeypatch, "NIM_KREV", (128, 64, 4))

@@ -124,6 +124,19 @@ KERNEL_CLASS = {"k_mom_src": "velocity_sources", "k_mom_sweep": "velocity_sweep"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "%s_fp64_hbm_traffic.md")
 
 
+def kernel_source_hash():
+    """SHA-256 (first 16 hex digits) of the device sources: a traffic
+    summary stamped with another value was measured on other kernels."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    for path in sorted(glob.glob(os.path.join(PKG, "csrc", "*.hip")) + glob.glob(os.path.join(PKG, "csrc", "*.h"))):
+        h.update(os.path.basename(path).encode())
+        with open(path, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def _pmc_rows(d):
     """Per-dispatch (kernel, counter value KiB, duration us) of one rocprofv3
     --pmc run (the *_counter_collection.csv under d)."""
@@ -168,20 +181,25 @@ def pmc_summary(d, case, copy_mib):
     out.append("Copy calibration (%d MiB read and written per launch): FETCH_SIZE scale %.4f, WRITE_SIZE scale %.4f."
                % (copy_mib, sf, sw))
     out.append("")
+    out.append("kernel-sources: " + kernel_source_hash())
     out.append("bench-traffic: " + json.dumps({k: round(v) for k, v in traffic.items()}))
     return "\n".join(out)
 
 
 def measured_traffic(case):
     """HBM bytes per launch of each kernel class from the committed counter
-    summary of this case, or None."""
+    summary of this case, and whether it was measured on the current device
+    sources (its kernel-sources stamp); (None, None) without a summary."""
     path = TRAFFIC_FILE % case
     if not os.path.exists(path):
-        return None
+        return None, None
+    traffic, stamp = None, None
     for line in open(path):
         if line.startswith("bench-traffic:"):
-            return json.loads(line.split(":", 1)[1])
-    return None
+            traffic = json.loads(line.split(":", 1)[1])
+        if line.startswith("kernel-sources:"):
+            stamp = line.split(":", 1)[1].strip()
+    return traffic, stamp == kernel_source_hash()
 
 
 def copy_probe(mib):
@@ -284,7 +302,7 @@ def adjoint_bench(args):
     20 snapshots; C5: 200 steps) and each reversed fixed point solved by
     GMRES. The forward steps converge to the bench tolerances."""
     case = host.baseline_case(args.adjoint_case)
-    nsteps = args.adj_steps or (400 if args.adjoint_case == "C4" else 200)
+    nsteps = args.adj_steps or {"C2": 10, "C4": 400, "C5": 200}[args.adjoint_case]
     t0 = time.perf_counter()
     ctx, geo, st = setup(case)
     ctx.set_p_measure(nim_amd.P_HALFRANGE if args.p_measure == "halfrange" else nim_amd.P_MAXABS)
@@ -412,7 +430,7 @@ def main():
     ap.add_argument("--profile-only", action="store_true")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only time reversed passes of the discrete adjoint at the initial state")
-    ap.add_argument("--adjoint-case", choices=("C4", "C5"), default=None,
+    ap.add_argument("--adjoint-case", choices=("C2", "C4", "C5"), default=None,
                     help="the specification's checkpointed adjoint run instead of the throughput bench")
     ap.add_argument("--adj-steps", type=int, default=None, help="time steps of the adjoint run (C4: 400, C5: 200)")
     ap.add_argument("--adj-slots", type=int, default=20, help="snapshots of the adjoint run")
@@ -550,6 +568,7 @@ def main():
             "event_cycle_ms": cyc_ms,
         }
 
+    traffic, traffic_current = measured_traffic(args.case)
     line = {
         "metric": "cell-updates/s & steps/s at 1M cells, FP64 vs mixed at 1e-8 residual; % HBM",
         "value": cups,
@@ -580,7 +599,9 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": (measured_traffic(args.case) or {}).get(dom),
+            "traffic": (traffic or {}).get(dom) if traffic_current else None,
+            "traffic_note": ("counter bytes per launch from the committed summary, measured on these kernel sources"
+                             if traffic_current else "no counter summary for the current kernel sources"),
             "bytes_per_cell": BYTES_PER_CELL[dom],
             "launch_ms": per_launch_s * 1e3,
         },

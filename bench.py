@@ -46,6 +46,11 @@ BYTES_PER_CELL = {
     "energy_coefficients": 4, "unused": 0, "energy_sources": 13 * 8, "energy_sweep": 40 * 8, "check": 0,
 }
 PATH_BYTES_PER_CELL_UPDATE = 1200  # 150 doubles per cell and outer iteration
+# The oracle's natural order against the compiled reference (flang -O3, its
+# modules built at the 48 x 24 x 16 case by oracle/Makefile.ref c1), both on
+# one core of the same host, per outer iteration: 18.2 ms against 11.6 ms.
+# The reference is this factor faster than the oracle's natural order.
+REFERENCE_OVER_ORACLE_SPEED = 1.57
 # Algorithmic FP32 bytes per cell of the correction-pass kernels of the
 # mixed loop (the FP64 counts without the previous time level, plus the
 # residual of each relaxed unknown and, for the pressure sources, the nine
@@ -183,6 +188,35 @@ def pmc_summary(d, case, copy_mib):
     out.append("")
     out.append("kernel-sources: " + kernel_source_hash())
     out.append("bench-traffic: " + json.dumps({k: round(v) for k, v in traffic.items()}))
+    return "\n".join(out)
+
+
+def trace_summary(d, min_us=3.0):
+    """Markdown table of a rocprofv3 --kernel-trace run under d: per kernel
+    the dispatches, those that did work (longer than min_us: the kernels of
+    a converged batch return at once, in about 1.5 us) and their mean
+    duration, and the registers of the dispatch."""
+    import csv
+    import glob
+    rows = {}
+    for path in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+        for r in csv.DictReader(open(path)):
+            n = r["Kernel_Name"].replace("(anonymous namespace)::", "").split("(")[0]
+            n = n.replace("void ", "").replace("nim::", "")
+            us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            e = rows.setdefault(n, {"n": 0, "work": [], "vgpr": r.get("VGPR_Count"), "scratch": r.get("Scratch_Size")})
+            e["n"] += 1
+            if us > min_us:
+                e["work"].append(us)
+    out = ["| kernel | dispatches | working dispatches | mean us (working) | total ms (working) | VGPRs | scratch B |",
+           "|---|---|---|---|---|---|---|"]
+    for n, e in sorted(rows.items(), key=lambda kv: -sum(kv[1]["work"])):
+        if not e["work"]:
+            continue
+        out.append("| %s | %d | %d | %.1f | %.1f | %s | %s |" % (n, e["n"], len(e["work"]), np.mean(e["work"]),
+                                                             sum(e["work"]) / 1e3, e["vgpr"], e["scratch"]))
+    out.append("")
+    out.append("kernel-sources: " + kernel_source_hash())
     return "\n".join(out)
 
 
@@ -439,9 +473,13 @@ def main():
                     help="adjoint run: forward steps with the mixed-precision loop")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--pmc-summary", metavar="DIR", help="print the HBM traffic summary of counter runs under DIR")
+    ap.add_argument("--trace-summary", metavar="DIR", help="print the per-kernel summary of a kernel trace under DIR")
     args = ap.parse_args()
     if args.pmc_summary:
         print(pmc_summary(args.pmc_summary, args.case, args.copy_probe or 1024))
+        return 0
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary))
         return 0
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library runs one case per device and has no multi-GPU "
@@ -626,7 +664,12 @@ def main():
                                           % (args.cpu_iters * 4, N, nth, dtc),
                                 "single_core_natural": {"value": v, "cores": 1,
                                                         "sample": "%d outer iterations, the reference's natural "
-                                                                  "order on one core, %.1f s" % (args.cpu_iters, dt)}}
+                                                                  "order on one core, %.1f s" % (args.cpu_iters, dt)},
+                                "reference_single_core_estimate": {
+                                    "value": v * REFERENCE_OVER_ORACLE_SPEED, "cores": 1,
+                                    "note": "single_core_natural times %.2f, the compiled reference's speed over "
+                                            "the oracle's natural order on one core, measured at the 48 x 24 x 16 "
+                                            "case in the build container" % REFERENCE_OVER_ORACLE_SPEED}}
     ctx.close()
     print(json.dumps(line))
     return 0

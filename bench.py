@@ -347,7 +347,8 @@ def adjoint_bench(args):
     t0 = time.perf_counter()
     inner = (args.inner or 96) if args.mixed_forward else 0
     J, grad, info = nim_amd.nusselt_gradient(ctx, case, nsteps, args.adj_slots, args.adj_tol, tol=tol, average=True,
-                                             solver="gmres", mixed_inner=inner)
+                                             solver="gmres", mixed_inner=inner, restart=args.adj_restart,
+                                             progress=lambda m: print(m, file=sys.stderr, flush=True))
     wall = time.perf_counter() - t0
     ctx.close()
     gf = grad["T_faces"]
@@ -358,11 +359,12 @@ def adjoint_bench(args):
         "value": wall, "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
         "data": "synthetic (seeded 1e-3 noise on cell averages)",
         "config": {"workload": "%s: %d x %d x %d (theta x r x z) = %d cells, Ra=%g, Pr=%g, DT=%g, %d steps, %d "
-                               "snapshots, forward %s to tol %g (p %g on %s), reversed fixed points by GMRES to %g"
+                               "snapshots, forward %s to tol %g (p %g on %s), reversed fixed points by GMRES(%d) to %g"
                                % (args.adjoint_case, nt, nr, nz, case.ncells, case.Ra, case.Pr, case.DT, nsteps,
                                   args.adj_slots, "mixed (%d FP32 passes per FP64 residual)" % inner if inner
                                   else "FP64", args.tol, args.tol_p,
-                                  "max |dp|" if args.p_measure == "maxabs" else "half the range of dp", args.adj_tol),
+                                  "max |dp|" if args.p_measure == "maxabs" else "half the range of dp",
+                                  args.adj_restart, args.adj_tol),
                    "hexagon_faces": nt * nz},
         "J": J, "dJ_dRa": grad["Ra"], "dJ_dT_hot": grad["T_hot"],
         "dJ_dT_faces": {"count": int(gf.size), "sum": float(gf.sum()), "min": float(gf.min()), "max": float(gf.max()),
@@ -469,6 +471,7 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=None, help="time steps of the adjoint run (C4: 400, C5: 200)")
     ap.add_argument("--adj-slots", type=int, default=20, help="snapshots of the adjoint run")
     ap.add_argument("--adj-tol", type=float, default=1e-10, help="relative residual of each reversed fixed point")
+    ap.add_argument("--adj-restart", type=int, default=30, help="GMRES restart length of the reversed fixed points")
     ap.add_argument("--mixed-forward", action="store_true",
                     help="adjoint run: forward steps with the mixed-precision loop")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")

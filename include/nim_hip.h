@@ -353,7 +353,10 @@ int32_t nim_adj_gradient(nim_adj *adj, double out[4]);
  * operator, to a relative residual tol (true residual at each restart);
  * passes returns the reversed passes used, relres the final relative
  * residual. NIM_E_MAXITER after max_passes. Allocates its Krylov basis
- * ((restart + 4) vectors of 4 NMESH + 5 NFACE doubles) on first use. */
+ * ((restart + 4) vectors of 4 NMESH + 5 NFACE doubles) on first use. Each
+ * call after a converged one starts from that call's solution (the
+ * adjoints of consecutive steps differ little); nim_adj_clear forgets it.
+ * NIM_VERBOSE=1 prints the residual at every restart to stderr. */
 int32_t nim_adj_reverse_step_gmres(nim_adj *adj, double tol, int32_t max_passes, int32_t restart,
                                    int32_t *passes, double *relres);
 /* The hexagon wall temperature gradient face by face, accumulated like

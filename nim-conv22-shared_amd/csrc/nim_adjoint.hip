@@ -25,6 +25,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1146,6 +1148,7 @@ struct nim_adj {
     void *gm_block = nullptr;
     double *gm_host = nullptr;  // pinned: coefficients in [0, m + 1), dot results in [m + 1, 2 m + 2)
     int gm_m = 0;
+    bool gm_warm = false;       // x holds the last reverse step's solution (the next one starts from it)
     unsigned long long *dmax = nullptr;       // [2]
     unsigned long long *h_dmax = nullptr;
     void *block = nullptr;
@@ -1399,6 +1402,7 @@ static int32_t gm_alloc(nim_adj *A, int m)
         return nim_internal_fail(NIM_E_HIP, "GMRES workspace allocation failed");
     }
     A->gm_m = m;
+    A->gm_warm = false;
     return NIM_OK;
 }
 
@@ -1448,11 +1452,17 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
         hipLaunchKernelGGL(k_gm_apply, dim3(gL), dim3(BSA), 0, s, d, v, (const double *)lam, out, nl);
         ops++;
     };
-    // b = the seed without its Dirichlet slots (they hold no unknown); x = 0
+    // b = the seed without its Dirichlet slots (they hold no unknown); x
+    // starts from the previous reverse step's solution (consecutive steps'
+    // adjoints differ little), or 0
     hipLaunchKernelGGL(k_gm_mask, dim3(gL), dim3(BSA), 0, s, d, b, (const double *)A->seed, nl);
-    if (hipMemsetAsync(x, 0, nl * sizeof(double), s) != hipSuccess)
+    bool x_zero = !A->gm_warm;
+    if (x_zero && hipMemsetAsync(x, 0, nl * sizeof(double), s) != hipSuccess)
         return nim_internal_fail(NIM_E_HIP, "GMRES clear failed");
-    bool x_zero = true;
+    static const bool verbose = [] {
+        const char *v = getenv("NIM_VERBOSE");
+        return v && atoi(v) != 0;
+    }();
     double bnorm = 0.;
     if ((rc = norm_of(b, &bnorm))) return rc;
     double res = bnorm;
@@ -1472,6 +1482,10 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
         if ((rc = norm_of(V, &beta))) return rc;
         if (!std::isfinite(beta)) return nim_internal_fail(NIM_E_NONFINITE, "non-finite adjoint residual");
         res = beta;
+        if (verbose) {
+            fprintf(stderr, "GMRES: %d reversed passes, relative residual %.3e\n", ops, bnorm > 0. ? beta / bnorm : 0.);
+            fflush(stderr);
+        }
         if (beta <= tol * bnorm) { conv = true; break; }
         hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, V, (const double *)V, 1. / beta,
                            (const double *)nullptr, 0., nl);
@@ -1544,6 +1558,7 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
     ops++;
     if (passes) *passes = ops;
     if (relres) *relres = bnorm > 0. ? res / bnorm : 0.;
+    A->gm_warm = conv;
     if ((rc = finish_step(A, s))) return rc;
     return conv ? NIM_OK : nim_internal_fail(NIM_E_MAXITER, "adjoint GMRES not converged");
 }
@@ -1577,6 +1592,7 @@ extern "C" int32_t nim_adj_clear(nim_adj *A)
     const Dev &d = *nim_internal_dev(A->ctx);
     const size_t total = adj_doubles(d);
     hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
+    A->gm_warm = false;
     if (hipMemsetAsync(A->block, 0, total * sizeof(double), s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
         return nim_internal_fail(NIM_E_HIP, "adjoint clear failed");
     return NIM_OK;

@@ -634,7 +634,8 @@ def _solve(ctx, tol, passes, max_iter=200000, mixed_inner=0):
 
 
 def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, weights=(0.0, 1.0, 0.0),
-                     average=False, solver="gmres", restart=30, max_adj_passes=100000, mixed_inner=0):
+                     average=False, solver="gmres", restart=30, max_adj_passes=100000, mixed_inner=0,
+                     progress=None):
     """J = weights . nim_nusselt[:3] after nsteps steps of ctx from its
     current state (as run_steps), or with average=True its mean over the
     states after steps 1 .. nsteps, and its gradient by the discrete adjoint,
@@ -646,7 +647,9 @@ def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, 
     wall face (n_theta * n_z, face i + n_theta k), whose sum is T_hot.
     mixed_inner > 0 runs the forward steps with the mixed-precision loop
     (its fixed point is the FP64 one, at which the adjoint is taken).
-    info holds the forward and adjoint wall times and pass counts."""
+    info holds the forward and adjoint wall times and pass counts. progress,
+    if given, is called with a line of text after every forward and reverse
+    step."""
     if nsteps < 1 or nslots < 1:
         raise ValueError("nsteps and nslots must be at least 1")
     if solver not in ("gmres", "plain"):
@@ -666,6 +669,8 @@ def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, 
     def forward_step():
         solve()
         ctx.advance_time()
+        if progress:
+            progress("forward step: %d passes so far, %.1f s" % (clock["forward_passes"], clock["forward_s"]))
 
     def reverse_step(k):
         solve()
@@ -679,6 +684,9 @@ def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, 
         else:
             passes_adj.append(adj.reverse_step(adj_tol, max_adj_passes))
         clock["adjoint_s"] += time.perf_counter() - t0
+        if progress:
+            progress("reverse step %d: %d reversed passes, residual %.1e, %.1f s" % (
+                k, passes_adj[-1][0], passes_adj[-1][1], time.perf_counter() - t0))
 
     try:
         fwd = reverse_run(ctx, nsteps, nslots, forward_step, reverse_step)

@@ -43,3 +43,51 @@ def test_passes_match_oracle_at_benchmark_size(name):
     np.testing.assert_allclose(ctx.nusselt()[:3], o.nusselt()[:3], rtol=1e-9)
     ctx.close()
     o.close()
+
+
+def test_converged_step_c1_matches_oracle():
+    """A whole time step at C1 to the benchmark's tolerances (u, v, w, T
+    1e-8, p 5e-5 on max |dp|) with the stopping test and the graph-replayed
+    batches of passes: the same pass count as the oracle in the colour order
+    (within the batch's reading of the flag, which the kernels honour
+    exactly) and the same fields."""
+    case = host.baseline_case("C1")
+    geo = host.build_geometry(case, device=True)
+    st = host.initial_state(case, amp=1e-3, seed=20221)
+    ctx = G.gpu_for_case(case, geo, st)
+    o = G.oracle_for_case(case, geo, st)
+    tol = np.array([1e-8, 1e-8, 1e-8, 5e-5, 1e-8])
+    ctx.build_momentum_coeffs()
+    o.velocity_constants()
+    rc, it_g, err_g = ctx.inner_solve(tol, max_iter=100000, min_iter=2)
+    rc_o, it_o, err_o = o.inner_solve(tol, O.COLORED, max_iter=100000, min_iter=2)
+    print("C1 converged step: passes gpu %d oracle %d" % (it_g, it_o))
+    assert rc_o == 0 and abs(it_g - it_o) <= 2, (it_g, it_o)
+    for f in "uvwpT":
+        assert G.rel_l2(ctx.get("faces_" + f), o.get("faces_" + f)) < 1e-9, f
+    ctx.close()
+    o.close()
+
+
+def test_converged_step_c2_graph_and_direct_launches_agree(monkeypatch):
+    """At C2 a whole step to the benchmark's tolerances (about 5,000
+    passes from the seeded state) replayed from the captured graph of 16
+    passes and launched pass by pass stop at the same pass with the same
+    bits: the stopping test and the early exit of the kernels after it work
+    the same at full size."""
+    case = host.baseline_case("C2")
+    geo = host.build_geometry(case, device=True)
+    st = host.initial_state(case, amp=1e-3, seed=20221)
+    tol = np.array([1e-8, 1e-8, 1e-8, 5e-5, 1e-8])
+    out = {}
+    for graph in ("1", "0"):
+        monkeypatch.setenv("NIM_GRAPH", graph)
+        ctx = G.gpu_for_case(case, geo, st)
+        ctx.build_momentum_coeffs()
+        rc, it, err = ctx.inner_solve(tol, max_iter=100000, min_iter=2)
+        out[graph] = (it, {f: ctx.get(f) for f in ("faces_u", "faces_p", "faces_T", "Txyz")})
+        ctx.close()
+    print("C2 converged step: passes", out["1"][0], out["0"][0])
+    assert out["1"][0] == out["0"][0] > 1000
+    for f in out["1"][1]:
+        assert np.array_equal(out["1"][1][f], out["0"][1][f]), f

@@ -348,7 +348,8 @@ def adjoint_bench(args):
     inner = (args.inner or 96) if args.mixed_forward else 0
     J, grad, info = nim_amd.nusselt_gradient(ctx, case, nsteps, args.adj_slots, args.adj_tol, tol=tol, average=True,
                                              solver="gmres", mixed_inner=inner, restart=args.adj_restart,
-                                             progress=lambda m: print(m, file=sys.stderr, flush=True))
+                                             progress=lambda m: print(m, file=sys.stderr, flush=True),
+                                             basis=args.adj_basis, augment=args.adj_augment)
     wall = time.perf_counter() - t0
     ctx.close()
     gf = grad["T_faces"]
@@ -359,12 +360,13 @@ def adjoint_bench(args):
         "value": wall, "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
         "data": "synthetic (seeded 1e-3 noise on cell averages)",
         "config": {"workload": "%s: %d x %d x %d (theta x r x z) = %d cells, Ra=%g, Pr=%g, DT=%g, %d steps, %d "
-                               "snapshots, forward %s to tol %g (p %g on %s), reversed fixed points by GMRES(%d) to %g"
+                               "snapshots, forward %s to tol %g (p %g on %s), reversed fixed points by GMRES(%d) with %d "
+                               "augmentation vectors to %g, %s Krylov basis"
                                % (args.adjoint_case, nt, nr, nz, case.ncells, case.Ra, case.Pr, case.DT, nsteps,
                                   args.adj_slots, "mixed (%d FP32 passes per FP64 residual)" % inner if inner
                                   else "FP64", args.tol, args.tol_p,
                                   "max |dp|" if args.p_measure == "maxabs" else "half the range of dp",
-                                  args.adj_restart, args.adj_tol),
+                                  args.adj_restart, args.adj_augment, args.adj_tol, args.adj_basis.upper()),
                    "hexagon_faces": nt * nz},
         "J": J, "dJ_dRa": grad["Ra"], "dJ_dT_hot": grad["T_hot"],
         "dJ_dT_faces": {"count": int(gf.size), "sum": float(gf.sum()), "min": float(gf.min()), "max": float(gf.max()),
@@ -471,7 +473,11 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=None, help="time steps of the adjoint run (C4: 400, C5: 200)")
     ap.add_argument("--adj-slots", type=int, default=20, help="snapshots of the adjoint run")
     ap.add_argument("--adj-tol", type=float, default=1e-10, help="relative residual of each reversed fixed point")
-    ap.add_argument("--adj-restart", type=int, default=30, help="GMRES restart length of the reversed fixed points")
+    ap.add_argument("--adj-restart", type=int, default=100, help="GMRES restart length of the reversed fixed points")
+    ap.add_argument("--adj-augment", type=int, default=3,
+                    help="earlier-cycle corrections that augment each GMRES cycle (LGMRES; 0: plain restarts)")
+    ap.add_argument("--adj-basis", choices=("fp32", "fp64"), default="fp64",
+                    help="storage of the GMRES Krylov basis (arithmetic is FP64 either way)")
     ap.add_argument("--mixed-forward", action="store_true",
                     help="adjoint run: forward steps with the mixed-precision loop")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")

@@ -349,16 +349,24 @@ int32_t nim_adj_seed_nusselt(nim_adj *adj, const double weights[3]);
 int32_t nim_adj_reverse_step(nim_adj *adj, double tol, int32_t max_passes, int32_t *passes, double *delta);
 int32_t nim_adj_gradient(nim_adj *adj, double out[4]);
 /* nim_adj_reverse_step with the reversed fixed point (I - G_x^T) lam = seed
- * solved by restarted GMRES(restart) with the reversed pass as the
- * operator, to a relative residual tol (true residual at each restart);
- * passes returns the reversed passes used, relres the final relative
- * residual. NIM_E_MAXITER after max_passes. Allocates its Krylov basis
- * ((restart + 4) vectors of 4 NMESH + 5 NFACE doubles) on first use. Each
- * call after a converged one starts from that call's solution (the
- * adjoints of consecutive steps differ little); nim_adj_clear forgets it.
- * NIM_VERBOSE=1 prints the residual at every restart to stderr. */
+ * solved by restarted GMRES with the reversed pass as the operator, each
+ * cycle of dimension restart, to a relative residual tol (true residual at
+ * each restart); passes returns the reversed passes used, relres the final
+ * relative residual. NIM_E_MAXITER after max_passes. Each cycle after the
+ * first ends its space with the corrections of up to `augment` earlier
+ * cycles (LGMRES; nim_adj_gmres_options, default 3), which cost no pass.
+ * Allocates on first use restart + 1 + 2 augment vectors of 4 NMESH +
+ * 5 NFACE values (FP64, or FP32 by nim_adj_gmres_options) and 5 such
+ * vectors in FP64; all arithmetic is FP64. Each call after a converged one
+ * starts from that call's solution (the adjoints of consecutive steps
+ * differ little); nim_adj_clear forgets it. NIM_VERBOSE=1 prints the
+ * residual at every restart to stderr. */
 int32_t nim_adj_reverse_step_gmres(nim_adj *adj, double tol, int32_t max_passes, int32_t restart,
                                    int32_t *passes, double *relres);
+/* GMRES options: basis_bytes = 8 (FP64, the default) or 4 (FP32 storage of
+ * the basis: half the orthogonalisation traffic, more cycles), augment = 0
+ * (plain restarted GMRES) to 16 correction vectors per cycle. */
+int32_t nim_adj_gmres_options(nim_adj *adj, int32_t basis_bytes, int32_t augment);
 /* The hexagon wall temperature gradient face by face, accumulated like
  * the scalar one (whose value is their sum): dJ/dT of each hexagon face,
  * count = n_theta * n_z values in the reference's boundary order (BCs group

@@ -29,6 +29,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "nim_hip.h"
@@ -1065,29 +1066,68 @@ __device__ __forceinline__ bool lam_fixed(const Dev &d, long s)
 
 // out = v - lam (lam = G_x^T v from a reversed pass), 0 in the Dirichlet
 // slots, which the pass fills with parameter adjoints
-__global__ void __launch_bounds__(BSA) k_gm_apply(Dev d, const double *v, const double *lam, double *out, long n)
+template <typename VT>
+__global__ void __launch_bounds__(BSA) k_gm_apply(Dev d, const VT *v, const double *lam, double *out, long n)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
-    for (long s = t; s < n; s += stride) out[s] = lam_fixed(d, s) ? 0. : v[s] - lam[s];
+    for (long s = t; s < n; s += stride) out[s] = lam_fixed(d, s) ? 0. : (double)v[s] - lam[s];
 }
 
-// partial dot products of w with basis vectors V[0..j): block (x, k)
-// sums its grid-stride share of V[k] . w into part[k * gridDim.x + x]
-__global__ void __launch_bounds__(BSA) k_gm_dots(const double *V, long ld, const double *w, long n, double *part)
+// lam = v (a basis vector, widened to FP64)
+template <typename VT>
+__global__ void __launch_bounds__(BSA) k_gm_load(double *lam, const VT *v, long n)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
-    const double *v = V + (size_t)blockIdx.y * ld;
-    double acc = 0.;
-    for (long s = t; s < n; s += stride) acc += v[s] * w[s];
+    for (long s = t; s < n; s += stride) lam[s] = (double)v[s];
+}
+
+// the sum of acc over the block into *out (thread 0 writes)
+__device__ __forceinline__ void block_sum_to(double acc, double *out)
+{
     __shared__ double sh[BSA / 64];
     acc = wave_sum(acc);
+    __syncthreads();  // sh is reused by consecutive calls
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = 0.;
         for (int q = 0; q < (int)(blockDim.x >> 6); q++) r += sh[q];
-        part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = r;
+        *out = r;
     }
+}
+
+// partial dot products of w with the basis vectors V[0..j), GM_KB of them
+// per block row: block (x, g) sums its grid-stride share of V[k] . w for
+// k in [g GM_KB, (g + 1) GM_KB) into part[k * gridDim.x + x], so that w is
+// read once per GM_KB basis vectors. Products and sums in FP64.
+constexpr int GM_KB = 8;
+template <typename VT>
+__global__ void __launch_bounds__(BSA) k_gm_dots(const VT *V, long ld, const double *w, long n, int j, double *part)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    const int k0 = blockIdx.y * GM_KB, kn = min(GM_KB, j - k0);
+    const VT *v = V + (size_t)k0 * ld;
+    double acc[GM_KB];
+#pragma unroll
+    for (int q = 0; q < GM_KB; q++) acc[q] = 0.;
+    for (long s = t; s < n; s += stride) {
+        const double ws = w[s];
+#pragma unroll
+        for (int q = 0; q < GM_KB; q++)
+            if (q < kn) acc[q] += (double)v[(size_t)q * ld + s] * ws;
+    }
+#pragma unroll
+    for (int q = 0; q < GM_KB; q++)
+        if (q < kn) block_sum_to(acc[q], part + (size_t)(k0 + q) * gridDim.x + blockIdx.x);
+}
+
+// partial sums of w . w, block x into part[x]
+__global__ void __launch_bounds__(BSA) k_gm_norm2(const double *w, long n, double *part)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    double acc = 0.;
+    for (long s = t; s < n; s += stride) acc += w[s] * w[s];
+    block_sum_to(acc, part + blockIdx.x);
 }
 
 // out[k] = sum of part[k * nb .. k * nb + nb), one block per k, fixed order
@@ -1096,25 +1136,18 @@ __global__ void __launch_bounds__(BSA) k_gm_dots_final(const double *part, int n
     const double *p = part + (size_t)blockIdx.x * nb;
     double acc = 0.;
     for (int q = threadIdx.x; q < nb; q += blockDim.x) acc += p[q];
-    __shared__ double sh[BSA / 64];
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = 0.;
-        for (int q = 0; q < (int)(blockDim.x >> 6); q++) r += sh[q];
-        out[blockIdx.x] = r;
-    }
+    block_sum_to(acc, out + blockIdx.x);
 }
 
 // x += sign * sum_{k < j} c[k] V[k]   (c in device memory)
-__global__ void __launch_bounds__(BSA) k_gm_combine(double *x, const double *V, long ld, int j, const double *c,
+template <typename VT>
+__global__ void __launch_bounds__(BSA) k_gm_combine(double *x, const VT *V, long ld, int j, const double *c,
                                                     double sign, long n)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
     for (long s = t; s < n; s += stride) {
         double acc = 0.;
-        for (int k = 0; k < j; k++) acc += c[k] * V[(size_t)k * ld + s];
+        for (int k = 0; k < j; k++) acc += c[k] * (double)V[(size_t)k * ld + s];
         x[s] += sign * acc;
     }
 }
@@ -1133,6 +1166,22 @@ __global__ void __launch_bounds__(BSA) k_gm_scale(double *y, const double *x, do
     for (long s = t; s < n; s += stride) y[s] = z ? a * x[s] + b * z[s] : a * x[s];
 }
 
+// v = a * (x - y), rounded to the basis type
+template <typename VT>
+__global__ void __launch_bounds__(BSA) k_gm_diff_store(VT *v, const double *x, const double *y, double a, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    for (long s = t; s < n; s += stride) v[s] = (VT)(a * (x[s] - y[s]));
+}
+
+// a basis vector: v = a * x, rounded to the basis type
+template <typename VT>
+__global__ void __launch_bounds__(BSA) k_gm_store(VT *v, const double *x, double a, long n)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    for (long s = t; s < n; s += stride) v[s] = (VT)(a * x[s]);
+}
+
 }  // namespace
 
 // ---- host side --------------------------------------------------------------
@@ -1143,11 +1192,15 @@ struct nim_adj {
     double *cbar = nullptr, *tcbar = nullptr, *fbar = nullptr;
     double *theta = nullptr;                  // [NPAR] accumulated over steps
     double *thf = nullptr;                    // [NT NZ] per hexagon face, accumulated over steps
-    // GMRES workspace (allocated on first use): basis V [m + 1][nl], w, x,
-    // partial dots, Hessenberg column and combination coefficients
+    // GMRES workspace (allocated on first use): basis V [m + 1][nl] and the
+    // augmentation pairs Z, AZ [k][nl] each, of gm_vbytes-byte values; then
+    // w, x, b, r_old, dz, partial dots and combination coefficients in FP64
     void *gm_block = nullptr;
     double *gm_host = nullptr;  // pinned: coefficients in [0, m + 1), dot results in [m + 1, 2 m + 2)
-    int gm_m = 0;
+    int gm_m = 0, gm_k = 0;
+    int gm_vbytes = 0;
+    int gm_basis = 8;           // bytes per basis value asked for (nim_adj_gmres_options)
+    int gm_aug = 3;             // augmentation vectors asked for (nim_adj_gmres_options)
     bool gm_warm = false;       // x holds the last reverse step's solution (the next one starts from it)
     unsigned long long *dmax = nullptr;       // [2]
     unsigned long long *h_dmax = nullptr;
@@ -1385,9 +1438,9 @@ extern "C" int32_t nim_adj_reverse_step(nim_adj *A, double tol, int32_t max_pass
 // ---- GMRES ------------------------------------------------------------------
 static constexpr int GM_DOT_BLOCKS = 512;
 
-static int32_t gm_alloc(nim_adj *A, int m)
+static int32_t gm_alloc(nim_adj *A, int m, int k, int vbytes)
 {
-    if (A->gm_block && A->gm_m == m) return NIM_OK;
+    if (A->gm_block && A->gm_m == m && A->gm_k == k && A->gm_vbytes == vbytes) return NIM_OK;
     if (A->gm_block) { (void)hipFree(A->gm_block); A->gm_block = nullptr; }
     if (A->gm_host) { (void)hipHostFree(A->gm_host); A->gm_host = nullptr; }
     if (hipHostMalloc((void **)&A->gm_host, 2 * (size_t)(m + 1) * sizeof(double), 0) != hipSuccess) {
@@ -1396,29 +1449,50 @@ static int32_t gm_alloc(nim_adj *A, int m)
     }
     const Dev &d = *nim_internal_dev(A->ctx);
     const size_t nl = 4 * (size_t)d.N + 5 * (size_t)d.NF;
-    const size_t doubles = (size_t)(m + 1) * nl + 3 * nl + (size_t)(m + 1) * GM_DOT_BLOCKS + 2 * (size_t)(m + 1);
-    if (hipMalloc(&A->gm_block, doubles * sizeof(double)) != hipSuccess) {
+    const size_t vb = ((size_t)(m + 1 + 2 * k) * nl * vbytes + 255) / 256 * 256;
+    const size_t doubles = 5 * nl + (size_t)(m + 1) * GM_DOT_BLOCKS + 2 * (size_t)(m + 1);
+    if (hipMalloc(&A->gm_block, vb + doubles * sizeof(double)) != hipSuccess) {
         A->gm_block = nullptr;
         return nim_internal_fail(NIM_E_HIP, "GMRES workspace allocation failed");
     }
     A->gm_m = m;
+    A->gm_k = k;
+    A->gm_vbytes = vbytes;
     A->gm_warm = false;
     return NIM_OK;
 }
 
-extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t max_passes, int32_t restart,
-                                              int32_t *passes, double *relres)
+// Restarted GMRES on (I - G_x^T) lam = seed, the basis stored as VT, with
+// the approximation space of each cycle augmented by the corrections of up
+// to k earlier cycles (LGMRES: A. H. Baker, E. R. Jessup and T. Manteuffel,
+// SIAM J. Matrix Anal. Appl. 26 (2005) 962-984). A cycle of dimension m takes
+// m - n_aug Krylov vectors (one reversed pass each) and then the n_aug stored
+// corrections z, whose images A z need no pass: A z = r_old - r_new, the
+// residuals before and after the cycle that made z. The flexible Arnoldi
+// relation A [V | Z] = V H keeps H upper Hessenberg, so the Givens
+// least-squares update is GMRES's. Restarted GMRES stagnates here (the first
+// cycle takes the residual down by 1e6, then each restart only halves it);
+// the corrections carry the slowly converging directions across restarts.
+// Everything but the basis storage is FP64: the operator (a reversed pass
+// of the widened vector), the Gram-Schmidt products and sums (classical,
+// twice), the iterate and the residual b - A x that starts each cycle and
+// decides convergence. An FP32 basis halves the orthogonalisation traffic;
+// its rounding (6e-8 relative per stored vector) bounds how far one cycle
+// can reduce the residual.
+template <typename VT>
+static int32_t gmres_solve(nim_adj *A, double tol, int32_t max_passes, int m, int kaug, int32_t *passes,
+                           double *relres)
 {
-    if (!A || max_passes < 1 || restart < 1 || restart > 200 || !(tol > 0.))
-        return nim_internal_fail(NIM_E_ARG, "bad GMRES reverse-step arguments");
     const Dev &d = *nim_internal_dev(A->ctx);
     hipStream_t s = (hipStream_t)nim_internal_stream(A->ctx);
     const long nl = 4 * (long)d.N + 5 * (long)d.NF;
-    const int gL = grid_for(nl), m = restart;
-    int32_t rc = gm_alloc(A, m);
+    const int gL = grid_for(nl);
+    int32_t rc = gm_alloc(A, m, kaug, (int)sizeof(VT));
     if (rc) return rc;
-    double *V = (double *)A->gm_block, *w = V + (size_t)(m + 1) * nl, *x = w + nl, *b = x + nl;
-    double *part = b + nl, *dh = part + (size_t)(m + 1) * GM_DOT_BLOCKS, *dc = dh + (m + 1);
+    VT *V = (VT *)A->gm_block, *Za = V + (size_t)(m + 1) * nl, *AZa = Za + (size_t)kaug * nl;
+    double *w = (double *)((char *)A->gm_block + ((size_t)(m + 1 + 2 * kaug) * nl * sizeof(VT) + 255) / 256 * 256);
+    double *x = w + nl, *b = x + nl, *rold = b + nl, *dz = rold + nl;
+    double *part = dz + nl, *dh = part + (size_t)(m + 1) * GM_DOT_BLOCKS, *dc = dh + (m + 1);
     double *lam = A->a.lc, *hin = A->gm_host, *hout = A->gm_host + (m + 1);
     const int nbd = (int)std::min<long>(GM_DOT_BLOCKS, (nl + BSA - 1) / BSA);
     nim_internal_energy_coefs(A->ctx);
@@ -1426,7 +1500,8 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
     hipLaunchKernelGGL(k_adj_seed, dim3(gL), dim3(BSA), 0, s, d, A->seed, lam, A->prev, A->cbar, A->tcbar, A->fbar);
     // dots of vec with V[0..j) into host h
     auto dots = [&](const double *vec, int j, double *h) -> int32_t {
-        hipLaunchKernelGGL(k_gm_dots, dim3(nbd, j), dim3(BSA), 0, s, (const double *)V, nl, vec, nl, part);
+        hipLaunchKernelGGL(k_gm_dots<VT>, dim3(nbd, (j + GM_KB - 1) / GM_KB), dim3(BSA), 0, s, (const VT *)V, nl, vec,
+                           nl, j, part);
         hipLaunchKernelGGL(k_gm_dots_final, dim3(j), dim3(BSA), 0, s, (const double *)part, nbd, dh);
         if (hipMemcpyAsync(hout, dh, j * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
@@ -1435,7 +1510,7 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
         return NIM_OK;
     };
     auto norm_of = [&](const double *vec, double *out) -> int32_t {
-        hipLaunchKernelGGL(k_gm_dots, dim3(nbd, 1), dim3(BSA), 0, s, vec, nl, vec, nl, part);
+        hipLaunchKernelGGL(k_gm_norm2, dim3(nbd), dim3(BSA), 0, s, vec, nl, part);
         hipLaunchKernelGGL(k_gm_dots_final, dim3(1), dim3(BSA), 0, s, (const double *)part, nbd, dh);
         if (hipMemcpyAsync(hout, dh, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
@@ -1443,13 +1518,21 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
         *out = std::sqrt(std::max(hout[0], 0.));
         return NIM_OK;
     };
-    // out = (I - G_x^T) v
+    // y (device, n values) from host
+    auto upload = [&](const double *y, int n) -> int32_t {
+        if (hipStreamSynchronize(s) != hipSuccess) return nim_internal_fail(NIM_E_HIP, "GMRES failed");
+        std::copy(y, y + n, hin);
+        if (hipMemcpyAsync(dc, hin, n * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
+            return nim_internal_fail(NIM_E_HIP, "GMRES copy failed");
+        return NIM_OK;
+    };
+    // out = (I - G_x^T) v, v a basis vector or x
     int ops = 0;
-    bool copy_failed = false;
-    auto apply = [&](const double *v, double *out) {
-        if (hipMemcpyAsync(lam, v, nl * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess) copy_failed = true;
+    auto apply = [&](const auto *v, double *out) {
+        using T = std::remove_cv_t<std::remove_pointer_t<decltype(v)>>;
+        hipLaunchKernelGGL(k_gm_load<T>, dim3(gL), dim3(BSA), 0, s, lam, v, nl);
         reverse_pass(A, s);
-        hipLaunchKernelGGL(k_gm_apply, dim3(gL), dim3(BSA), 0, s, d, v, (const double *)lam, out, nl);
+        hipLaunchKernelGGL(k_gm_apply<T>, dim3(gL), dim3(BSA), 0, s, d, v, (const double *)lam, out, nl);
         ops++;
     };
     // b = the seed without its Dirichlet slots (they hold no unknown); x
@@ -1467,19 +1550,32 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
     if ((rc = norm_of(b, &bnorm))) return rc;
     double res = bnorm;
     bool conv = bnorm == 0.;
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), h(m + 1), y(m);
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), h(m + 1), y(m), hh(m + 1);
+    // augmentation: `filled` pairs in slots [0, filled) (filled in order, then
+    // the oldest replaced); the correction of the last cycle waits in slot
+    // `pend` for its image, which the next cycle's residual gives before any
+    // pair is used
+    int filled = 0, pend = -1, next = 0;
+    double dz_norm = 0.;
     while (!conv && ops < max_passes) {
-        // r = b - A x  ->  V[0]
+        // r = b - A x (in w), V[0] = r / |r|
         if (x_zero) {
-            hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, V, (const double *)b, 1., (const double *)nullptr,
+            hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, w, (const double *)b, 1., (const double *)nullptr,
                                0., nl);
         } else {
-            apply(x, w);
-            hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, V, (const double *)b, 1., (const double *)w, -1.,
+            apply((const double *)x, w);
+            hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, w, (const double *)b, 1., (const double *)w, -1.,
                                nl);
         }
+        if (pend >= 0) {
+            // A z = (r_old - r) / |dz| for the stored z = dz / |dz|
+            hipLaunchKernelGGL(k_gm_diff_store<VT>, dim3(gL), dim3(BSA), 0, s, AZa + (size_t)pend * nl,
+                               (const double *)rold, (const double *)w, 1. / dz_norm, nl);
+            pend = -1;
+        }
+        const int naug = filled;
         double beta;
-        if ((rc = norm_of(V, &beta))) return rc;
+        if ((rc = norm_of(w, &beta))) return rc;
         if (!std::isfinite(beta)) return nim_internal_fail(NIM_E_NONFINITE, "non-finite adjoint residual");
         res = beta;
         if (verbose) {
@@ -1487,23 +1583,28 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
             fflush(stderr);
         }
         if (beta <= tol * bnorm) { conv = true; break; }
-        hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, V, (const double *)V, 1. / beta,
-                           (const double *)nullptr, 0., nl);
+        if (kaug > 0)
+            hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, rold, (const double *)w, 1.,
+                               (const double *)nullptr, 0., nl);
+        hipLaunchKernelGGL(k_gm_store<VT>, dim3(gL), dim3(BSA), 0, s, V, (const double *)w, 1. / beta, nl);
         std::fill(g.begin(), g.end(), 0.);
         g[0] = beta;
+        const int nk = m - std::min(naug, m - 1);  // Krylov vectors of this cycle, then the corrections
         int jdone = 0;
         for (int j = 0; j < m && ops < max_passes; j++) {
-            double *vj1 = V + (size_t)(j + 1) * nl;
-            apply(V + (size_t)j * nl, w);
+            if (j < nk)
+                apply((const VT *)(V + (size_t)j * nl), w);
+            else
+                hipLaunchKernelGGL(k_gm_load<VT>, dim3(gL), dim3(BSA), 0, s, w,
+                                   (const VT *)(AZa + (size_t)(j - nk) * nl), nl);
             // classical Gram-Schmidt, twice
             std::fill(h.begin(), h.end(), 0.);
             for (int pass = 0; pass < 2; pass++) {
-                std::vector<double> hh(j + 1);
                 if ((rc = dots(w, j + 1, hh.data()))) return rc;  // (synchronises: hin is free)
-                std::copy(hh.begin(), hh.end(), hin);
+                std::copy(hh.begin(), hh.begin() + j + 1, hin);
                 if (hipMemcpyAsync(dc, hin, (j + 1) * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
                     return nim_internal_fail(NIM_E_HIP, "GMRES copy failed");
-                hipLaunchKernelGGL(k_gm_combine, dim3(gL), dim3(BSA), 0, s, w, (const double *)V, nl, j + 1,
+                hipLaunchKernelGGL(k_gm_combine<VT>, dim3(gL), dim3(BSA), 0, s, w, (const VT *)V, nl, j + 1,
                                    (const double *)dc, -1., nl);
                 for (int k = 0; k <= j; k++) h[k] += hh[k];
             }
@@ -1512,8 +1613,8 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
             if (!std::isfinite(hn)) return nim_internal_fail(NIM_E_NONFINITE, "non-finite adjoint Krylov vector");
             h[j + 1] = hn;
             if (hn > 0.)
-                hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, vj1, (const double *)w, 1. / hn,
-                                   (const double *)nullptr, 0., nl);
+                hipLaunchKernelGGL(k_gm_store<VT>, dim3(gL), dim3(BSA), 0, s, V + (size_t)(j + 1) * nl,
+                                   (const double *)w, 1. / hn, nl);
             // Givens rotations of the new column
             for (int k = 0; k < j; k++) {
                 const double t = cs[k] * h[k] + sn[k] * h[k + 1];
@@ -1532,27 +1633,43 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
             res = std::fabs(g[j + 1]);
             if (res <= tol * bnorm || hn == 0.) break;
         }
-        // x += V y, H y = g (upper triangular)
+        if (jdone == 0) break;
+        // H y = g (upper triangular); dz = [V | Z] y; x += dz
         for (int k = jdone - 1; k >= 0; k--) {
             double t = g[k];
             for (int q = k + 1; q < jdone; q++) t -= H[(size_t)k * m + q] * y[q];
             y[k] = t / H[(size_t)k * m + k];
         }
-        if (hipStreamSynchronize(s) != hipSuccess) return nim_internal_fail(NIM_E_HIP, "GMRES failed");
-        std::copy(y.begin(), y.begin() + jdone, hin);
-        if (hipMemcpyAsync(dc, hin, jdone * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
-            return nim_internal_fail(NIM_E_HIP, "GMRES copy failed");
-        hipLaunchKernelGGL(k_gm_combine, dim3(gL), dim3(BSA), 0, s, x, (const double *)V, nl, jdone,
+        if ((rc = upload(y.data(), jdone))) return rc;
+        if (hipMemsetAsync(dz, 0, nl * sizeof(double), s) != hipSuccess)
+            return nim_internal_fail(NIM_E_HIP, "GMRES clear failed");
+        hipLaunchKernelGGL(k_gm_combine<VT>, dim3(gL), dim3(BSA), 0, s, dz, (const VT *)V, nl, std::min(jdone, nk),
                            (const double *)dc, 1., nl);
+        if (jdone > nk)
+            hipLaunchKernelGGL(k_gm_combine<VT>, dim3(gL), dim3(BSA), 0, s, dz, (const VT *)Za, nl, jdone - nk,
+                               (const double *)(dc + nk), 1., nl);
+        hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, x, (const double *)x, 1., (const double *)dz, 1.,
+                           nl);
         x_zero = false;
+        // keep dz / |dz| as a correction (its image follows from the next
+        // cycle's residual), replacing the oldest when all k are in use
+        if (kaug > 0) {
+            if ((rc = norm_of(dz, &dz_norm))) return rc;
+            if (dz_norm > 0. && std::isfinite(dz_norm)) {
+                pend = next;
+                next = (next + 1) % kaug;
+                filled = std::min(filled + 1, kaug);
+                hipLaunchKernelGGL(k_gm_store<VT>, dim3(gL), dim3(BSA), 0, s, Za + (size_t)pend * nl,
+                                   (const double *)dz, 1. / dz_norm, nl);
+            }
+        }
         // the next cycle starts from the true residual of x and stops there
         // if it is within the tolerance
     }
     if (rc = adj_check("GMRES launch failed"); rc) return rc;
     // the parameter adjoints of lam = x: one more reversed pass, then the
     // seed added, as the plain iteration ends
-    if (copy_failed || hipMemcpyAsync(lam, x, nl * sizeof(double), hipMemcpyDeviceToDevice, s) != hipSuccess)
-        return nim_internal_fail(NIM_E_HIP, "GMRES copy failed");
+    hipLaunchKernelGGL(k_gm_load<double>, dim3(gL), dim3(BSA), 0, s, lam, (const double *)x, nl);
     reverse_pass(A, s);
     hipLaunchKernelGGL(k_adj_add_seed, dim3(gL), dim3(BSA), 0, s, lam, (const double *)A->seed, nl);
     ops++;
@@ -1561,6 +1678,25 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
     A->gm_warm = conv;
     if ((rc = finish_step(A, s))) return rc;
     return conv ? NIM_OK : nim_internal_fail(NIM_E_MAXITER, "adjoint GMRES not converged");
+}
+
+extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t max_passes, int32_t restart,
+                                              int32_t *passes, double *relres)
+{
+    if (!A || max_passes < 1 || restart < 1 || restart > 200 || !(tol > 0.))
+        return nim_internal_fail(NIM_E_ARG, "bad GMRES reverse-step arguments");
+    const int k = std::min(A->gm_aug, restart - 1);
+    return A->gm_basis == 8 ? gmres_solve<double>(A, tol, max_passes, restart, k, passes, relres)
+                            : gmres_solve<float>(A, tol, max_passes, restart, k, passes, relres);
+}
+
+extern "C" int32_t nim_adj_gmres_options(nim_adj *A, int32_t basis_bytes, int32_t augment)
+{
+    if (!A || (basis_bytes != 4 && basis_bytes != 8) || augment < 0 || augment > 16)
+        return nim_internal_fail(NIM_E_ARG, "GMRES options: basis of 4 or 8 bytes, 0 to 16 augmentation vectors");
+    A->gm_basis = basis_bytes;
+    A->gm_aug = augment;
+    return NIM_OK;
 }
 
 extern "C" int32_t nim_adj_gradient_faces(nim_adj *A, double *out, int64_t count)

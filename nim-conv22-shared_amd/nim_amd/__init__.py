@@ -146,6 +146,8 @@ def lib():
         L.nim_adj_gradient.argtypes = [vp, _P]
         L.nim_adj_reverse_step_gmres.restype = i32
         L.nim_adj_reverse_step_gmres.argtypes = [vp, ctypes.c_double, i32, i32, ctypes.POINTER(i32), _P]
+        L.nim_adj_gmres_options.restype = i32
+        L.nim_adj_gmres_options.argtypes = [vp, i32, i32]
         L.nim_adj_gradient_faces.restype = i32
         L.nim_adj_gradient_faces.argtypes = [vp, _P, ctypes.c_int64]
         L.nim_adj_reverse_stage.restype = i32
@@ -170,7 +172,7 @@ def exported_symbols():
             "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_kernel_times",
             "nim_inner_solve_mixed", "nim_residual", "nim_kernel_times_mixed", "nim_ens_create", "nim_ens_destroy",
             "nim_ens_inner_solve", "nim_ens_time_step", "nim_revolve_plan", "nim_ckpt_create", "nim_ckpt_destroy", "nim_ckpt_store",
-            "nim_ckpt_restore", "nim_adj_create", "nim_adj_destroy", "nim_adj_seed_nusselt", "nim_adj_reverse_step", "nim_adj_reverse_step_gmres", "nim_adj_gradient_faces",
+            "nim_ckpt_restore", "nim_adj_create", "nim_adj_destroy", "nim_adj_seed_nusselt", "nim_adj_reverse_step", "nim_adj_reverse_step_gmres", "nim_adj_gmres_options", "nim_adj_gradient_faces",
             "nim_adj_gradient", "nim_adj_reverse_pass", "nim_adj_reverse_stage", "nim_adj_clear", "nim_adj_get",
             "nim_adj_set", "nim_device_metrics", "nim_last_error"]
 
@@ -592,6 +594,14 @@ class Adjoint:
         _check(self.L.nim_adj_reverse_step_gmres(self.h, tol, max_passes, restart, ctypes.byref(n), _dp(rr)))
         return n.value, float(rr[0])
 
+    def gmres_options(self, basis="fp64", augment=3):
+        """GMRES's basis storage ("fp64" or "fp32") and the number of
+        earlier-cycle corrections that augment each cycle (0: plain
+        restarted GMRES)."""
+        if basis not in ("fp32", "fp64"):
+            raise ValueError("basis must be 'fp32' or 'fp64'")
+        _check(self.L.nim_adj_gmres_options(self.h, 4 if basis == "fp32" else 8, augment))
+
     def gradient(self):
         out = np.zeros(4)
         _check(self.L.nim_adj_gradient(self.h, _dp(out)))
@@ -634,15 +644,17 @@ def _solve(ctx, tol, passes, max_iter=200000, mixed_inner=0):
 
 
 def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, weights=(0.0, 1.0, 0.0),
-                     average=False, solver="gmres", restart=30, max_adj_passes=100000, mixed_inner=0,
-                     progress=None):
+                     average=False, solver="gmres", restart=100, max_adj_passes=100000, mixed_inner=0,
+                     progress=None, basis="fp64", augment=3):
     """J = weights . nim_nusselt[:3] after nsteps steps of ctx from its
     current state (as run_steps), or with average=True its mean over the
     states after steps 1 .. nsteps, and its gradient by the discrete adjoint,
     the run reversed with nslots device snapshots (binomial plan). The
     reversed fixed point of each step is solved by restarted GMRES (solver
     "gmres", relative residual adj_tol) or by plain iteration ("plain",
-    change per pass adj_tol). case supplies Ra, Pr and nu. Returns (J,
+    change per pass adj_tol; basis "fp64" or "fp32" stores GMRES's Krylov
+    basis, augment earlier-cycle corrections join each cycle). case
+    supplies Ra, Pr and nu. Returns (J,
     {"T_hot", "Ra", "T_faces"}, info); T_faces holds dJ/dT of each hexagon
     wall face (n_theta * n_z, face i + n_theta k), whose sum is T_hot.
     mixed_inner > 0 runs the forward steps with the mixed-precision loop
@@ -657,6 +669,7 @@ def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, 
     import time
     w = np.asarray(weights, dtype=np.float64)
     adj = Adjoint(ctx)
+    adj.gmres_options(basis, augment)
     out = {"J": 0.0}
     passes_adj = []
     clock = {"forward_s": 0.0, "adjoint_s": 0.0, "forward_passes": 0}

@@ -242,6 +242,27 @@ def test_gmres_matches_plain_iteration(adjoint_run):
     assert krylov * 10 < plain
 
 
+@pytest.mark.parametrize("basis,augment", [("fp64", 0), ("fp64", 3), ("fp32", 3)])
+def test_gmres_options_match_plain_iteration(adjoint_run, basis, augment):
+    """Short cycles (GMRES(20)) with and without LGMRES augmentation, the
+    Krylov basis stored in FP64 or FP32 (all arithmetic FP64 either way,
+    each cycle started from the FP64 true residual): the plain iteration's
+    gradient to the solve's tolerance."""
+    case, J, grad, info = adjoint_run
+    ctx = nim_amd.autograd.make_context(case)
+    try:
+        Jg, gg, ig = nim_amd.nusselt_gradient(ctx, case, NSTEPS, NSLOTS, 1e-12, passes=PASSES, solver="gmres",
+                                              restart=20, basis=basis, augment=augment)
+    finally:
+        ctx.close()
+    print(basis, "basis,", augment, "augmentation vectors: reversed passes", ig["adjoint_passes"])
+    assert Jg == J
+    assert abs(gg["T_hot"] - grad["T_hot"]) <= 1e-9 * abs(grad["T_hot"])
+    assert abs(gg["Ra"] - grad["Ra"]) <= 1e-9 * abs(grad["Ra"])
+    np.testing.assert_allclose(gg["T_faces"], grad["T_faces"], rtol=1e-8, atol=1e-12 * abs(grad["T_hot"]))
+    assert all(r <= 1e-12 for _, r in ig["adjoint_passes"])
+
+
 def test_autograd_harness(adjoint_run):
     import torch
     case, J, grad, info = adjoint_run

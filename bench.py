@@ -129,13 +129,17 @@ KERNEL_CLASS = {"k_mom_src": "velocity_sources", "k_mom_sweep": "velocity_sweep"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "%s_fp64_hbm_traffic.md")
 
 
+# the sources of the forward loops' kernels and of their launch shapes
+FORWARD_SOURCES = ("nim_api.hip", "nim_kernels.h", "nim_kernels.hip", "nim_stencil.h")
+
+
 def kernel_source_hash():
-    """SHA-256 (first 16 hex digits) of the device sources: a traffic
-    summary stamped with another value was measured on other kernels."""
-    import glob
+    """SHA-256 (first 16 hex digits) of the forward loops' device sources
+    and launches: a traffic summary stamped with another value was measured
+    on other kernels."""
     import hashlib
     h = hashlib.sha256()
-    for path in sorted(glob.glob(os.path.join(PKG, "csrc", "*.hip")) + glob.glob(os.path.join(PKG, "csrc", "*.h"))):
+    for path in [os.path.join(PKG, "csrc", n) for n in FORWARD_SOURCES]:
         h.update(os.path.basename(path).encode())
         with open(path, "rb") as f:
             h.update(f.read())
@@ -347,9 +351,11 @@ def adjoint_bench(args):
     t0 = time.perf_counter()
     inner = (args.inner or 96) if args.mixed_forward else 0
     J, grad, info = nim_amd.nusselt_gradient(ctx, case, nsteps, args.adj_slots, args.adj_tol, tol=tol, average=True,
-                                             solver="gmres", mixed_inner=inner, restart=args.adj_restart,
+                                             solver=args.adj_solver, mixed_inner=inner, restart=args.adj_restart,
+                                             max_adj_passes=args.adj_max_passes,
                                              progress=lambda m: print(m, file=sys.stderr, flush=True),
-                                             basis=args.adj_basis, augment=args.adj_augment)
+                                             basis=args.adj_basis, augment=args.adj_augment,
+                                             degree=args.adj_degree)
     wall = time.perf_counter() - t0
     ctx.close()
     gf = grad["T_faces"]
@@ -360,13 +366,17 @@ def adjoint_bench(args):
         "value": wall, "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
         "data": "synthetic (seeded 1e-3 noise on cell averages)",
         "config": {"workload": "%s: %d x %d x %d (theta x r x z) = %d cells, Ra=%g, Pr=%g, DT=%g, %d steps, %d "
-                               "snapshots, forward %s to tol %g (p %g on %s), reversed fixed points by GMRES(%d) with %d "
-                               "augmentation vectors to %g, %s Krylov basis"
+                               "snapshots, forward %s to tol %g (p %g on %s), reversed fixed points %s"
                                % (args.adjoint_case, nt, nr, nz, case.ncells, case.Ra, case.Pr, case.DT, nsteps,
                                   args.adj_slots, "mixed (%d FP32 passes per FP64 residual)" % inner if inner
                                   else "FP64", args.tol, args.tol_p,
                                   "max |dp|" if args.p_measure == "maxabs" else "half the range of dp",
-                                  args.adj_restart, args.adj_augment, args.adj_tol, args.adj_basis.upper()),
+                                  ("by GMRES(%d) with %d augmentation vectors and %d reversed passes per "
+                                   "operator application to %g, %s Krylov basis"
+                                   % (args.adj_restart, args.adj_augment, args.adj_degree, args.adj_tol,
+                                      args.adj_basis.upper())
+                                   if args.adj_solver == "gmres" else
+                                   "by plain iteration to a change of %g per pass" % args.adj_tol)),
                    "hexagon_faces": nt * nz},
         "J": J, "dJ_dRa": grad["Ra"], "dJ_dT_hot": grad["T_hot"],
         "dJ_dT_faces": {"count": int(gf.size), "sum": float(gf.sum()), "min": float(gf.min()), "max": float(gf.max()),
@@ -389,16 +399,18 @@ def strict_tolerance(ctx, args):
     tol = np.full(5, args.tol)
     ctx.set_p_measure(nim_amd.P_HALFRANGE)
     out = {}
-    for mixed in (False, True):
+    inners = [int(x) for x in args.strict_inner.split(",")] if args.strict_inner else [args.inner]
+    for inner in [0] + inners:
         ctx.upload_state(start)
         steps = []
+        key = "mixed" if inner == args.inner else "mixed_%d" % inner if inner else "fp64"
         for _ in range(args.strict_steps):
             ctx.synchronize()
             t0 = time.perf_counter()
             ctx.build_momentum_coeffs()
-            if mixed:
-                rc, n, err = ctx.inner_solve_mixed(tol, inner=args.inner, max_cycles=args.max_iter // args.inner + 1)
-                n = (n - 1) * args.inner
+            if inner:
+                rc, n, err = ctx.inner_solve_mixed(tol, inner=inner, max_cycles=args.max_iter // inner + 1)
+                n = (n - 1) * inner
             else:
                 rc, n, err = ctx.inner_solve(tol, max_iter=args.max_iter)
             ctx.end_step()
@@ -406,15 +418,15 @@ def strict_tolerance(ctx, args):
             ctx.nusselt()
             ctx.synchronize()
             steps.append({"passes": int(n), "s": time.perf_counter() - t0, "err": [float(x) for x in err]})
-            print("strict %s step: %d passes, %.1f s" % ("mixed" if mixed else "fp64", n, steps[-1]["s"]),
-                  file=sys.stderr, flush=True)
-        out["mixed" if mixed else "fp64"] = steps
+            print("strict %s step: %d passes, %.1f s" % (key, n, steps[-1]["s"]), file=sys.stderr, flush=True)
+        out[key] = steps
     ctx.set_p_measure(nim_amd.P_MAXABS)
     ctx.upload_state(start)
     t64 = sum(x["s"] for x in out["fp64"])
-    tmx = sum(x["s"] for x in out["mixed"])
     out["tol"] = "%g on every field, p as half the range of its increments" % args.tol
-    out["speedup_mixed_vs_fp64"] = t64 / tmx
+    for inner in inners:
+        key = "mixed" if inner == args.inner else "mixed_%d" % inner
+        out["speedup_%s_vs_fp64" % key] = t64 / sum(x["s"] for x in out[key])
     return out
 
 
@@ -449,6 +461,8 @@ def main():
     ap.add_argument("--p-measure", choices=("maxabs", "halfrange"), default="maxabs",
                     help="what tol-p bounds: max |dp| (find_error) or half the range of dp (blind to p's constant)")
     ap.add_argument("--max-iter", type=int, default=200000, help="cap on passes (FP64) or cycles (mixed) per step")
+    ap.add_argument("--strict-inner", default=None, metavar="K,K,...",
+                    help="strict steps of the mixed loop with each of these FP32 passes per residual (default --inner)")
     ap.add_argument("--strict-steps", type=int, default=0,
                     help="after the timed runs: this many steps of each loop with every field at --tol, p as half "
                          "the range of its increments, from the FP64 run's final state (slow: about 2e5 passes per "
@@ -473,9 +487,15 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=None, help="time steps of the adjoint run (C4: 400, C5: 200)")
     ap.add_argument("--adj-slots", type=int, default=20, help="snapshots of the adjoint run")
     ap.add_argument("--adj-tol", type=float, default=1e-10, help="relative residual of each reversed fixed point")
-    ap.add_argument("--adj-restart", type=int, default=100, help="GMRES restart length of the reversed fixed points")
+    ap.add_argument("--adj-restart", type=int, default=50, help="GMRES restart length of the reversed fixed points")
+    ap.add_argument("--adj-solver", choices=("gmres", "plain"), default="gmres",
+                    help="reversed fixed points by GMRES (relative residual --adj-tol) or by plain iteration "
+                         "(change per pass --adj-tol)")
+    ap.add_argument("--adj-max-passes", type=int, default=100000, help="reversed passes allowed per reverse step")
     ap.add_argument("--adj-augment", type=int, default=3,
                     help="earlier-cycle corrections that augment each GMRES cycle (LGMRES; 0: plain restarts)")
+    ap.add_argument("--adj-degree", type=int, default=8,
+                    help="reversed passes per GMRES operator application (> 1: Neumann-preconditioned cycles)")
     ap.add_argument("--adj-basis", choices=("fp32", "fp64"), default="fp64",
                     help="storage of the GMRES Krylov basis (arithmetic is FP64 either way)")
     ap.add_argument("--mixed-forward", action="store_true",

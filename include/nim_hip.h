@@ -350,23 +350,27 @@ int32_t nim_adj_reverse_step(nim_adj *adj, double tol, int32_t max_passes, int32
 int32_t nim_adj_gradient(nim_adj *adj, double out[4]);
 /* nim_adj_reverse_step with the reversed fixed point (I - G_x^T) lam = seed
  * solved by restarted GMRES with the reversed pass as the operator, each
- * cycle of dimension restart, to a relative residual tol (true residual at
- * each restart); passes returns the reversed passes used, relres the final
- * relative residual. NIM_E_MAXITER after max_passes. Each cycle after the
- * first ends its space with the corrections of up to `augment` earlier
- * cycles (LGMRES; nim_adj_gmres_options, default 3), which cost no pass.
+ * cycle of dimension restart, to a relative residual tol of the true
+ * residual (checked at each restart); passes returns the reversed passes
+ * used, relres the final relative residual. NIM_E_MAXITER after
+ * max_passes. Options (nim_adj_gmres_options): each cycle after the first
+ * ends its space with the corrections of up to `augment` earlier cycles
+ * (LGMRES, default 3; they cost no pass), and with degree d > 1 the cycles
+ * solve the Neumann-preconditioned system (I - (G_x^T)^d) lam =
+ * sum_{q<d} (G_x^T)^q seed, d reversed passes per Krylov vector (default 8).
  * Allocates on first use restart + 1 + 2 augment vectors of 4 NMESH +
- * 5 NFACE values (FP64, or FP32 by nim_adj_gmres_options) and 5 such
- * vectors in FP64; all arithmetic is FP64. Each call after a converged one
- * starts from that call's solution (the adjoints of consecutive steps
- * differ little); nim_adj_clear forgets it. NIM_VERBOSE=1 prints the
- * residual at every restart to stderr. */
+ * 5 NFACE values (FP64, or FP32 by option) and 6 such vectors in FP64; all
+ * arithmetic is FP64. Each call after a converged one starts from that
+ * call's solution (the adjoints of consecutive steps differ little);
+ * nim_adj_clear forgets it. NIM_VERBOSE=1 prints the true residual at
+ * every restart to stderr. */
 int32_t nim_adj_reverse_step_gmres(nim_adj *adj, double tol, int32_t max_passes, int32_t restart,
                                    int32_t *passes, double *relres);
 /* GMRES options: basis_bytes = 8 (FP64, the default) or 4 (FP32 storage of
- * the basis: half the orthogonalisation traffic, more cycles), augment = 0
- * (plain restarted GMRES) to 16 correction vectors per cycle. */
-int32_t nim_adj_gmres_options(nim_adj *adj, int32_t basis_bytes, int32_t augment);
+ * the basis: half the orthogonalisation traffic, slower cycles), augment =
+ * 0 (plain restarted GMRES) to 16 correction vectors per cycle, degree = 1
+ * to 64 reversed passes per operator application. */
+int32_t nim_adj_gmres_options(nim_adj *adj, int32_t basis_bytes, int32_t augment, int32_t degree);
 /* The hexagon wall temperature gradient face by face, accumulated like
  * the scalar one (whose value is their sum): dJ/dT of each hexagon face,
  * count = n_theta * n_z values in the reference's boundary order (BCs group

@@ -1194,13 +1194,14 @@ struct nim_adj {
     double *thf = nullptr;                    // [NT NZ] per hexagon face, accumulated over steps
     // GMRES workspace (allocated on first use): basis V [m + 1][nl] and the
     // augmentation pairs Z, AZ [k][nl] each, of gm_vbytes-byte values; then
-    // w, x, b, r_old, dz, partial dots and combination coefficients in FP64
+    // w, x, b, r_old, dz, z, partial dots and combination coefficients in FP64
     void *gm_block = nullptr;
     double *gm_host = nullptr;  // pinned: coefficients in [0, m + 1), dot results in [m + 1, 2 m + 2)
     int gm_m = 0, gm_k = 0;
     int gm_vbytes = 0;
     int gm_basis = 8;           // bytes per basis value asked for (nim_adj_gmres_options)
     int gm_aug = 3;             // augmentation vectors asked for (nim_adj_gmres_options)
+    int gm_deg = 8;             // reversed passes per operator application (nim_adj_gmres_options)
     bool gm_warm = false;       // x holds the last reverse step's solution (the next one starts from it)
     unsigned long long *dmax = nullptr;       // [2]
     unsigned long long *h_dmax = nullptr;
@@ -1450,7 +1451,7 @@ static int32_t gm_alloc(nim_adj *A, int m, int k, int vbytes)
     const Dev &d = *nim_internal_dev(A->ctx);
     const size_t nl = 4 * (size_t)d.N + 5 * (size_t)d.NF;
     const size_t vb = ((size_t)(m + 1 + 2 * k) * nl * vbytes + 255) / 256 * 256;
-    const size_t doubles = 5 * nl + (size_t)(m + 1) * GM_DOT_BLOCKS + 2 * (size_t)(m + 1);
+    const size_t doubles = 6 * nl + (size_t)(m + 1) * GM_DOT_BLOCKS + 2 * (size_t)(m + 1);
     if (hipMalloc(&A->gm_block, vb + doubles * sizeof(double)) != hipSuccess) {
         A->gm_block = nullptr;
         return nim_internal_fail(NIM_E_HIP, "GMRES workspace allocation failed");
@@ -1480,7 +1481,7 @@ static int32_t gm_alloc(nim_adj *A, int m, int k, int vbytes)
 // its rounding (6e-8 relative per stored vector) bounds how far one cycle
 // can reduce the residual.
 template <typename VT>
-static int32_t gmres_solve(nim_adj *A, double tol, int32_t max_passes, int m, int kaug, int32_t *passes,
+static int32_t gmres_solve(nim_adj *A, double tol, int32_t max_passes, int m, int kaug, int pdeg, int32_t *passes,
                            double *relres)
 {
     const Dev &d = *nim_internal_dev(A->ctx);
@@ -1491,8 +1492,8 @@ static int32_t gmres_solve(nim_adj *A, double tol, int32_t max_passes, int m, in
     if (rc) return rc;
     VT *V = (VT *)A->gm_block, *Za = V + (size_t)(m + 1) * nl, *AZa = Za + (size_t)kaug * nl;
     double *w = (double *)((char *)A->gm_block + ((size_t)(m + 1 + 2 * kaug) * nl * sizeof(VT) + 255) / 256 * 256);
-    double *x = w + nl, *b = x + nl, *rold = b + nl, *dz = rold + nl;
-    double *part = dz + nl, *dh = part + (size_t)(m + 1) * GM_DOT_BLOCKS, *dc = dh + (m + 1);
+    double *x = w + nl, *b = x + nl, *rold = b + nl, *dz = rold + nl, *zb = dz + nl;
+    double *part = zb + nl, *dh = part + (size_t)(m + 1) * GM_DOT_BLOCKS, *dc = dh + (m + 1);
     double *lam = A->a.lc, *hin = A->gm_host, *hout = A->gm_host + (m + 1);
     const int nbd = (int)std::min<long>(GM_DOT_BLOCKS, (nl + BSA - 1) / BSA);
     nim_internal_energy_coefs(A->ctx);
@@ -1526,14 +1527,30 @@ static int32_t gmres_solve(nim_adj *A, double tol, int32_t max_passes, int m, in
             return nim_internal_fail(NIM_E_HIP, "GMRES copy failed");
         return NIM_OK;
     };
-    // out = (I - G_x^T) v, v a basis vector or x
+    // out = (I - (G_x^T)^deg) v, v a basis vector or x: deg reversed passes,
+    // the Dirichlet slots cleared between them
     int ops = 0;
-    auto apply = [&](const auto *v, double *out) {
+    auto apply = [&](const auto *v, double *out, int deg) {
         using T = std::remove_cv_t<std::remove_pointer_t<decltype(v)>>;
         hipLaunchKernelGGL(k_gm_load<T>, dim3(gL), dim3(BSA), 0, s, lam, v, nl);
-        reverse_pass(A, s);
+        for (int q = 0; q < deg; q++) {
+            if (q > 0) hipLaunchKernelGGL(k_gm_mask, dim3(gL), dim3(BSA), 0, s, d, lam, (const double *)lam, nl);
+            reverse_pass(A, s);
+        }
         hipLaunchKernelGGL(k_gm_apply<T>, dim3(gL), dim3(BSA), 0, s, d, v, (const double *)lam, out, nl);
-        ops++;
+        ops += deg;
+    };
+    // the Neumann preconditioner: out = sum_{q < deg} (G_x^T)^q r
+    auto precondition = [&](const double *r, double *out, int deg) {
+        hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, out, r, 1., (const double *)nullptr, 0., nl);
+        hipLaunchKernelGGL(k_gm_load<double>, dim3(gL), dim3(BSA), 0, s, lam, r, nl);
+        for (int q = 1; q < deg; q++) {
+            reverse_pass(A, s);
+            hipLaunchKernelGGL(k_gm_mask, dim3(gL), dim3(BSA), 0, s, d, lam, (const double *)lam, nl);
+            hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, out, (const double *)out, 1.,
+                               (const double *)lam, 1., nl);
+            ops++;
+        }
     };
     // b = the seed without its Dirichlet slots (they hold no unknown); x
     // starts from the previous reverse step's solution (consecutive steps'
@@ -1548,7 +1565,7 @@ static int32_t gmres_solve(nim_adj *A, double tol, int32_t max_passes, int m, in
     }();
     double bnorm = 0.;
     if ((rc = norm_of(b, &bnorm))) return rc;
-    double res = bnorm;
+    double res = bnorm, true_res = bnorm;
     bool conv = bnorm == 0.;
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), h(m + 1), y(m), hh(m + 1);
     // augmentation: `filled` pairs in slots [0, filled) (filled in order, then
@@ -1558,42 +1575,61 @@ static int32_t gmres_solve(nim_adj *A, double tol, int32_t max_passes, int m, in
     int filled = 0, pend = -1, next = 0;
     double dz_norm = 0.;
     while (!conv && ops < max_passes) {
-        // r = b - A x (in w), V[0] = r / |r|
+        // the true residual r = b - (I - G_x^T) x (in w) decides convergence
         if (x_zero) {
             hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, w, (const double *)b, 1., (const double *)nullptr,
                                0., nl);
         } else {
-            apply((const double *)x, w);
+            apply((const double *)x, w, 1);
             hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, w, (const double *)b, 1., (const double *)w, -1.,
                                nl);
         }
+        if ((rc = norm_of(w, &true_res))) return rc;
+        if (!std::isfinite(true_res)) return nim_internal_fail(NIM_E_NONFINITE, "non-finite adjoint residual");
+        res = true_res;
+        if (verbose) {
+            fprintf(stderr, "GMRES: %d reversed passes, relative residual %.3e\n", ops,
+                    bnorm > 0. ? true_res / bnorm : 0.);
+            fflush(stderr);
+        }
+        if (true_res <= tol * bnorm) { conv = true; break; }
+        // the cycle's system: M (I - G_x^T) lam = M b with M the Neumann
+        // polynomial of degree pdeg - 1, i.e. (I - (G_x^T)^pdeg) lam = M b;
+        // its residual z = M r starts the cycle, V[0] = z / |z|
+        double *z = w;
+        if (pdeg > 1) {
+            precondition(w, zb, pdeg);
+            z = zb;
+        }
         if (pend >= 0) {
-            // A z = (r_old - r) / |dz| for the stored z = dz / |dz|
+            // the operator's image of the stored correction dz / |dz|:
+            // (z_old - z) / |dz|
             hipLaunchKernelGGL(k_gm_diff_store<VT>, dim3(gL), dim3(BSA), 0, s, AZa + (size_t)pend * nl,
-                               (const double *)rold, (const double *)w, 1. / dz_norm, nl);
+                               (const double *)rold, (const double *)z, 1. / dz_norm, nl);
             pend = -1;
         }
         const int naug = filled;
         double beta;
-        if ((rc = norm_of(w, &beta))) return rc;
-        if (!std::isfinite(beta)) return nim_internal_fail(NIM_E_NONFINITE, "non-finite adjoint residual");
-        res = beta;
-        if (verbose) {
-            fprintf(stderr, "GMRES: %d reversed passes, relative residual %.3e\n", ops, bnorm > 0. ? beta / bnorm : 0.);
-            fflush(stderr);
+        if (pdeg > 1) {
+            if ((rc = norm_of(z, &beta))) return rc;
+            if (!std::isfinite(beta)) return nim_internal_fail(NIM_E_NONFINITE, "non-finite adjoint residual");
+        } else {
+            beta = true_res;
         }
-        if (beta <= tol * bnorm) { conv = true; break; }
+        // the cycle stops where its residual estimate stands to beta as the
+        // tolerance does to the true residual
+        const double cycle_tol = tol * bnorm * (beta / true_res);
         if (kaug > 0)
-            hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, rold, (const double *)w, 1.,
+            hipLaunchKernelGGL(k_gm_scale, dim3(gL), dim3(BSA), 0, s, rold, (const double *)z, 1.,
                                (const double *)nullptr, 0., nl);
-        hipLaunchKernelGGL(k_gm_store<VT>, dim3(gL), dim3(BSA), 0, s, V, (const double *)w, 1. / beta, nl);
+        hipLaunchKernelGGL(k_gm_store<VT>, dim3(gL), dim3(BSA), 0, s, V, (const double *)z, 1. / beta, nl);
         std::fill(g.begin(), g.end(), 0.);
         g[0] = beta;
         const int nk = m - std::min(naug, m - 1);  // Krylov vectors of this cycle, then the corrections
         int jdone = 0;
         for (int j = 0; j < m && ops < max_passes; j++) {
             if (j < nk)
-                apply((const VT *)(V + (size_t)j * nl), w);
+                apply((const VT *)(V + (size_t)j * nl), w, pdeg);
             else
                 hipLaunchKernelGGL(k_gm_load<VT>, dim3(gL), dim3(BSA), 0, s, w,
                                    (const VT *)(AZa + (size_t)(j - nk) * nl), nl);
@@ -1631,7 +1667,7 @@ static int32_t gmres_solve(nim_adj *A, double tol, int32_t max_passes, int m, in
             for (int k = 0; k <= j; k++) H[(size_t)k * m + j] = h[k];
             jdone = j + 1;
             res = std::fabs(g[j + 1]);
-            if (res <= tol * bnorm || hn == 0.) break;
+            if (res <= cycle_tol || hn == 0.) break;
         }
         if (jdone == 0) break;
         // H y = g (upper triangular); dz = [V | Z] y; x += dz
@@ -1674,7 +1710,7 @@ static int32_t gmres_solve(nim_adj *A, double tol, int32_t max_passes, int m, in
     hipLaunchKernelGGL(k_adj_add_seed, dim3(gL), dim3(BSA), 0, s, lam, (const double *)A->seed, nl);
     ops++;
     if (passes) *passes = ops;
-    if (relres) *relres = bnorm > 0. ? res / bnorm : 0.;
+    if (relres) *relres = bnorm > 0. ? true_res / bnorm : 0.;
     A->gm_warm = conv;
     if ((rc = finish_step(A, s))) return rc;
     return conv ? NIM_OK : nim_internal_fail(NIM_E_MAXITER, "adjoint GMRES not converged");
@@ -1686,16 +1722,18 @@ extern "C" int32_t nim_adj_reverse_step_gmres(nim_adj *A, double tol, int32_t ma
     if (!A || max_passes < 1 || restart < 1 || restart > 200 || !(tol > 0.))
         return nim_internal_fail(NIM_E_ARG, "bad GMRES reverse-step arguments");
     const int k = std::min(A->gm_aug, restart - 1);
-    return A->gm_basis == 8 ? gmres_solve<double>(A, tol, max_passes, restart, k, passes, relres)
-                            : gmres_solve<float>(A, tol, max_passes, restart, k, passes, relres);
+    return A->gm_basis == 8 ? gmres_solve<double>(A, tol, max_passes, restart, k, A->gm_deg, passes, relres)
+                            : gmres_solve<float>(A, tol, max_passes, restart, k, A->gm_deg, passes, relres);
 }
 
-extern "C" int32_t nim_adj_gmres_options(nim_adj *A, int32_t basis_bytes, int32_t augment)
+extern "C" int32_t nim_adj_gmres_options(nim_adj *A, int32_t basis_bytes, int32_t augment, int32_t degree)
 {
-    if (!A || (basis_bytes != 4 && basis_bytes != 8) || augment < 0 || augment > 16)
-        return nim_internal_fail(NIM_E_ARG, "GMRES options: basis of 4 or 8 bytes, 0 to 16 augmentation vectors");
+    if (!A || (basis_bytes != 4 && basis_bytes != 8) || augment < 0 || augment > 16 || degree < 1 || degree > 64)
+        return nim_internal_fail(NIM_E_ARG, "GMRES options: basis of 4 or 8 bytes, 0 to 16 augmentation vectors, "
+                                            "1 to 64 passes per operator application");
     A->gm_basis = basis_bytes;
     A->gm_aug = augment;
+    A->gm_deg = degree;
     return NIM_OK;
 }
 

@@ -147,7 +147,7 @@ def lib():
         L.nim_adj_reverse_step_gmres.restype = i32
         L.nim_adj_reverse_step_gmres.argtypes = [vp, ctypes.c_double, i32, i32, ctypes.POINTER(i32), _P]
         L.nim_adj_gmres_options.restype = i32
-        L.nim_adj_gmres_options.argtypes = [vp, i32, i32]
+        L.nim_adj_gmres_options.argtypes = [vp, i32, i32, i32]
         L.nim_adj_gradient_faces.restype = i32
         L.nim_adj_gradient_faces.argtypes = [vp, _P, ctypes.c_int64]
         L.nim_adj_reverse_stage.restype = i32
@@ -586,7 +586,7 @@ class Adjoint:
         _check(self.L.nim_adj_reverse_step(self.h, tol, max_passes, ctypes.byref(n), _dp(dl)))
         return n.value, float(dl[0])
 
-    def reverse_step_gmres(self, tol, max_passes=20000, restart=30):
+    def reverse_step_gmres(self, tol, max_passes=20000, restart=50):
         """The same as reverse_step with restarted GMRES on the reversed
         fixed point; returns (reversed passes, relative residual)."""
         n = ctypes.c_int32()
@@ -594,13 +594,14 @@ class Adjoint:
         _check(self.L.nim_adj_reverse_step_gmres(self.h, tol, max_passes, restart, ctypes.byref(n), _dp(rr)))
         return n.value, float(rr[0])
 
-    def gmres_options(self, basis="fp64", augment=3):
-        """GMRES's basis storage ("fp64" or "fp32") and the number of
+    def gmres_options(self, basis="fp64", augment=3, degree=8):
+        """GMRES's basis storage ("fp64" or "fp32"), the number of
         earlier-cycle corrections that augment each cycle (0: plain
-        restarted GMRES)."""
+        restarted GMRES) and the reversed passes per operator application
+        (degree > 1: the Neumann-preconditioned system)."""
         if basis not in ("fp32", "fp64"):
             raise ValueError("basis must be 'fp32' or 'fp64'")
-        _check(self.L.nim_adj_gmres_options(self.h, 4 if basis == "fp32" else 8, augment))
+        _check(self.L.nim_adj_gmres_options(self.h, 4 if basis == "fp32" else 8, augment, degree))
 
     def gradient(self):
         out = np.zeros(4)
@@ -644,8 +645,8 @@ def _solve(ctx, tol, passes, max_iter=200000, mixed_inner=0):
 
 
 def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, weights=(0.0, 1.0, 0.0),
-                     average=False, solver="gmres", restart=100, max_adj_passes=100000, mixed_inner=0,
-                     progress=None, basis="fp64", augment=3):
+                     average=False, solver="gmres", restart=50, max_adj_passes=100000, mixed_inner=0,
+                     progress=None, basis="fp64", augment=3, degree=8):
     """J = weights . nim_nusselt[:3] after nsteps steps of ctx from its
     current state (as run_steps), or with average=True its mean over the
     states after steps 1 .. nsteps, and its gradient by the discrete adjoint,
@@ -653,8 +654,9 @@ def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, 
     reversed fixed point of each step is solved by restarted GMRES (solver
     "gmres", relative residual adj_tol) or by plain iteration ("plain",
     change per pass adj_tol; basis "fp64" or "fp32" stores GMRES's Krylov
-    basis, augment earlier-cycle corrections join each cycle). case
-    supplies Ra, Pr and nu. Returns (J,
+    basis, augment earlier-cycle corrections join each cycle, degree
+    reversed passes make one operator application). case supplies Ra, Pr
+    and nu. Returns (J,
     {"T_hot", "Ra", "T_faces"}, info); T_faces holds dJ/dT of each hexagon
     wall face (n_theta * n_z, face i + n_theta k), whose sum is T_hot.
     mixed_inner > 0 runs the forward steps with the mixed-precision loop
@@ -669,7 +671,7 @@ def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, 
     import time
     w = np.asarray(weights, dtype=np.float64)
     adj = Adjoint(ctx)
-    adj.gmres_options(basis, augment)
+    adj.gmres_options(basis, augment, degree)
     out = {"J": 0.0}
     passes_adj = []
     clock = {"forward_s": 0.0, "adjoint_s": 0.0, "forward_passes": 0}

@@ -242,20 +242,21 @@ def test_gmres_matches_plain_iteration(adjoint_run):
     assert krylov * 10 < plain
 
 
-@pytest.mark.parametrize("basis,augment", [("fp64", 0), ("fp64", 3), ("fp32", 3)])
-def test_gmres_options_match_plain_iteration(adjoint_run, basis, augment):
+@pytest.mark.parametrize("basis,augment,degree", [("fp64", 0, 1), ("fp64", 3, 1), ("fp32", 3, 1), ("fp64", 3, 4)])
+def test_gmres_options_match_plain_iteration(adjoint_run, basis, augment, degree):
     """Short cycles (GMRES(20)) with and without LGMRES augmentation, the
     Krylov basis stored in FP64 or FP32 (all arithmetic FP64 either way,
-    each cycle started from the FP64 true residual): the plain iteration's
-    gradient to the solve's tolerance."""
+    each cycle started from the FP64 true residual), one or four reversed
+    passes per operator application (the Neumann-preconditioned system):
+    the plain iteration's gradient to the solve's tolerance."""
     case, J, grad, info = adjoint_run
     ctx = nim_amd.autograd.make_context(case)
     try:
         Jg, gg, ig = nim_amd.nusselt_gradient(ctx, case, NSTEPS, NSLOTS, 1e-12, passes=PASSES, solver="gmres",
-                                              restart=20, basis=basis, augment=augment)
+                                              restart=20, basis=basis, augment=augment, degree=degree)
     finally:
         ctx.close()
-    print(basis, "basis,", augment, "augmentation vectors: reversed passes", ig["adjoint_passes"])
+    print(basis, "basis,", augment, "augmentation vectors, degree", degree, ": reversed passes", ig["adjoint_passes"])
     assert Jg == J
     assert abs(gg["T_hot"] - grad["T_hot"]) <= 1e-9 * abs(grad["T_hot"])
     assert abs(gg["Ra"] - grad["Ra"]) <= 1e-9 * abs(grad["Ra"])

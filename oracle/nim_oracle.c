@@ -44,6 +44,11 @@
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
+
+/* meshes smaller than this run the loops on one thread: on a few thousand
+ * cells a parallel region costs more than its loop, and several test
+ * processes sharing the host's cores make spinning teams far slower still */
+#define ORA_PAR_MIN 16384
 #endif
 
 enum { ORA_NATURAL = 0, ORA_COLORED = 1 };
@@ -284,7 +289,7 @@ void ora_nim_direction(double u, double k, double gp, double gm, double vlim, do
 static void nim_face_coeffs(ora_ctx *c, const double *A, double *F1, double *F2, double *F3, double *F4)
 {
     const double DT = c->prm.DT;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN)
     for (int i = 0; i < c->nmesh; i++) {
         F8(F4, i, 2) = 0.5 * (1. / A3(A, i, 1, 5) + 1. / A3(A, i, 2, 5) + 1. / A3(A, i, 3, 5) - 1. / DT);
         F8(F4, i, 3) = -(A3(A, i, 1, 6) / A3(A, i, 1, 5));
@@ -314,7 +319,7 @@ static void nim_face_coeffs(ora_ctx *c, const double *A, double *F1, double *F2,
 /* calculate_velocity_nim_constants_formulation2 (nim.f90:160-330) */
 void ora_velocity_constants(ora_ctx *c)
 {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN)
     for (int i = 0; i < c->nmesh; i++) {
         c->meshes_u[i] = c->uxyzt[i] * XI(c, i, 1, 1) + c->vxyzt[i] * XI(c, i, 1, 2) + c->wxyzt[i] * XI(c, i, 1, 3);
         c->meshes_v[i] = c->uxyzt[i] * XI(c, i, 2, 1) + c->vxyzt[i] * XI(c, i, 2, 2) + c->wxyzt[i] * XI(c, i, 2, 3);
@@ -333,7 +338,7 @@ void ora_velocity_constants(ora_ctx *c)
 /* calculate_temperature_nim_constants_formulation2 (nim.f90:334-503) */
 void ora_temperature_constants(ora_ctx *c)
 {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN)
     for (int i = 0; i < c->nmesh; i++) {
         double ut = 0.5 * (c->uxyz[i] + c->uxyz_old[i]);
         double vt = 0.5 * (c->vxyz[i] + c->vxyz_old[i]);
@@ -537,12 +542,12 @@ static double cross_diffusion(const ora_ctx *c, const double *f, int n, double c
 void ora_velocity_sources(ora_ctx *c)
 {
     const double gb = c->prm.g_beta;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN)
     for (int i = 0; i < c->nmesh; i++) c->meshes_bz[i] = -(gb * 0.5 * (c->Txyz[i] + c->Txyz_old[i]));
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN)
     for (int f = 0; f < c->nface; f++) c->faces_bz[f] = -(gb * c->faces_T[f]);
     const double rinv = 1 / c->prm.rho, Rinv = 1 / c->prm.Re;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN)
     for (int n = 0; n < c->nmesh; n++) {
         double pf[6];
         gather6(c, c->faces_p, n, pf);
@@ -576,7 +581,7 @@ void ora_velocity_sources(ora_ctx *c)
 void ora_pressure_sources(ora_ctx *c)
 {
     const double zero6[6] = {0, 0, 0, 0, 0, 0};
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN)
     for (int n = 0; n < c->nmesh; n++) {
         double uf[6], vf[6], wf[6], bzf[6];
         gather6(c, c->faces_u, n, uf);
@@ -599,7 +604,7 @@ void ora_pressure_sources(ora_ctx *c)
  * parameters.f90:179) and calculate_R_D_T (2000-2134) */
 void ora_temperature_sources(ora_ctx *c)
 {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN)
     for (int n = 0; n < c->nmesh; n++) c->R_D_T[n] = cross_diffusion(c, c->faces_T, n, c->prm.alpha);
 }
 
@@ -818,7 +823,7 @@ void ora_velocity_sweep(ora_ctx *c, int order)
         for (int i = 0; i < c->nmesh; i++) velocity_unit(c, i);
     } else {
         for (int col = 0; col < 4; col++) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN)
             for (int i = 0; i < c->nmesh; i++)
                 if (cell_color(c, i) == col) velocity_unit(c, i);
         }
@@ -831,7 +836,7 @@ void ora_temperature_sweep(ora_ctx *c, int order)
         for (int i = 0; i < c->nmesh; i++) temperature_unit(c, i);
     } else {
         for (int col = 0; col < 4; col++) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN)
             for (int i = 0; i < c->nmesh; i++)
                 if (cell_color(c, i) == col) temperature_unit(c, i);
         }
@@ -857,7 +862,7 @@ void ora_pressure_stage(ora_ctx *c, int order)
         first[c->bc_mesh[b] * 4 + slot] = b;
     }
     for (int col = 0; col < 4; col++) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN)
         for (int i = 0; i < c->nmesh; i++) {
             if (cell_color(c, i) != col) continue;
             for (int s = 0; s < 4; s++)
@@ -881,7 +886,7 @@ void ora_set_p_measure(ora_ctx *c, int mode) { c->p_measure = mode; }
 void ora_find_error(ora_ctx *c, double err[5])
 {
     double eu = 0, ev = 0, ew = 0, ep = 0, eT = 0, pmax = -INFINITY, pmin = INFINITY;
-#pragma omp parallel for schedule(static) reduction(max : eu, ev, ew, ep, eT, pmax) reduction(min : pmin)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN) reduction(max : eu, ev, ew, ep, eT, pmax) reduction(min : pmin)
     for (int i = 0; i < c->nface; i++) {
         const double dp = c->faces_p[i] - c->faces_p_old[i];
         eu = fmax(eu, fabs(c->faces_u[i] - c->faces_u_old[i]));
@@ -898,7 +903,7 @@ void ora_find_error(ora_ctx *c, double err[5])
         c->faces_T_old[i] = c->faces_T[i];
     }
     double cu = 0, cv = 0, cw = 0, cT = 0;
-#pragma omp parallel for schedule(static) reduction(max : cu, cv, cw, cT)
+#pragma omp parallel for schedule(static) if (c->nmesh >= ORA_PAR_MIN) reduction(max : cu, cv, cw, cT)
     for (int i = 0; i < c->nmesh; i++) {
         cu = fmax(cu, fabs(c->uxyz[i] - c->uxyz_old_con[i]));
         cv = fmax(cv, fabs(c->vxyz[i] - c->vxyz_old_con[i]));

@@ -3,18 +3,25 @@
 Workload (default): the ~1M-cell case of the specification, 128 radial x 256
 azimuthal x 32 axial cells, cylinder R = 1, H = 1 around a hexagonal prism of
 circumradius 0.3, Ra = 1e5, Pr = 0.71, DT = 5e-4, seeded 1e-3 noise, FP64,
-inner convergence to a maximum increment of 1e-8 on u, v, w, p and T. A step
-is one time step of solve_nse (nse.f90:49-73): momentum coefficients, the
-inner iterations to convergence, the end-of-step reductions, the time shift
-and the Nusselt numbers. Inputs are resident on the device before the timed
-region.
+inner convergence to 1e-8 on every field: the maximum increment of u, v, w
+and T, and half the range of the pressure increments (blind to p's
+undetermined constant level). A step is one time step of solve_nse
+(nse.f90:49-73): momentum coefficients, the inner iterations to convergence,
+the end-of-step reductions, the time shift and the Nusselt numbers. Inputs
+are resident on the device before the timed region.
+
+The inner loop is the reference's pass, with Anderson acceleration every 8
+passes (nim_inner_solve_accel): the same fixed point and stopping test, in
+1/50 of the passes at this tolerance (--solver plain times the plain passes).
 
 Prints one JSON line. value = cell-updates/s of the FP64 path = cells x
 inner passes / wall time of the timed steps. The `mixed` object runs the
 same steps from the same initial state with the mixed-precision loop (FP64
-residual, FP32 correction passes, FP64 update) to the same tolerances on the
-FP64 residual: its steps/s, speedup, final residuals of both loops and the
-per-field difference from the FP64 result after the same steps.
+residual, FP32 correction passes, FP64 update, accelerated every cycle) to
+the same tolerances on the FP64 residual: its steps/s, speedup, final
+residuals of both loops and the per-field difference from the FP64 result
+after the same steps. `plain_fp64` times the plain FP64 passes at the
+reference's tolerance ratio (p 5e-5 on max |dp|) from the same state.
 """
 import argparse
 import json
@@ -456,10 +463,25 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     # the reference tests p 5000x more loosely than u, v, w, T
     # (parameters.f90:146-150); its max increment relaxes with a slow
-    # constant mode (p is defined up to a constant)
-    ap.add_argument("--tol-p", type=float, default=5e-5)
-    ap.add_argument("--p-measure", choices=("maxabs", "halfrange"), default="maxabs",
-                    help="what tol-p bounds: max |dp| (find_error) or half the range of dp (blind to p's constant)")
+    # constant mode (p is defined up to a constant). The accelerated loops
+    # (the default) hold p to --tol as well, as half the range of its
+    # increments; the plain loops default to the reference's ratio.
+    ap.add_argument("--tol-p", type=float, default=None,
+                    help="pressure tolerance (default: --tol with --solver accel, 5e-5 otherwise)")
+    ap.add_argument("--p-measure", choices=("maxabs", "halfrange"), default=None,
+                    help="what tol-p bounds: max |dp| (find_error) or half the range of dp (blind to p's constant); "
+                         "default halfrange with --solver accel, maxabs otherwise")
+    ap.add_argument("--solver", choices=("accel", "plain"), default="accel",
+                    help="the timed loops: Anderson-accelerated (default) or plain passes as the reference")
+    ap.add_argument("--accel-depth", type=int, default=16, help="history of the FP64 loop's acceleration")
+    ap.add_argument("--accel-period", type=int, default=8, help="FP64 passes per acceleration step")
+    ap.add_argument("--mixed-depth", type=int, default=10, help="history of the mixed loop's acceleration")
+    ap.add_argument("--mixed-period", type=int, default=1, help="mixed cycles per acceleration step")
+    ap.add_argument("--accel-basis", choices=("fp32", "fp64"), default="fp32",
+                    help="storage of the acceleration history (the iterate and the arithmetic stay FP64)")
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="with --solver accel: timed steps of the plain FP64 loop at the reference's tolerance "
+                         "ratio (p 5e-5 on max |dp|), after --warmup steps, for comparison (0: none)")
     ap.add_argument("--max-iter", type=int, default=200000, help="cap on passes (FP64) or cycles (mixed) per step")
     ap.add_argument("--strict-inner", default=None, metavar="K,K,...",
                     help="strict steps of the mixed loop with each of these FP32 passes per residual (default --inner)")
@@ -471,7 +493,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--inner", type=int, default=None,
-                    help="FP32 passes per FP64 residual of the mixed loop (default 96; 8 for the ensemble)")
+                    help="FP32 passes per FP64 residual of the mixed loop (default 16 accelerated, 96 plain; 8 for "
+                         "the ensemble)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the reversed-pass timing")
     ap.add_argument("--ensemble", action="store_true", help="the 48-case ensemble instead of one case")
@@ -516,12 +539,17 @@ def main():
         return 2
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    accel = args.solver == "accel" and not (args.ensemble or args.adjoint_case)
+    if args.tol_p is None:
+        args.tol_p = args.tol if accel else 5e-5
+    if args.p_measure is None:
+        args.p_measure = "halfrange" if accel else "maxabs"
     if args.ensemble:
         args.inner = args.inner or 8
         return ensemble_bench(args)
     if args.adjoint_case:
         return adjoint_bench(args)
-    args.inner = args.inner or 96
+    args.inner = args.inner or (16 if accel else 96)
 
     case = host.baseline_case(args.case)
     t_setup = time.perf_counter()
@@ -543,26 +571,44 @@ def main():
                           "launches_per_iteration": {k: v[1] for k, v in kt.items()}}))
         return 0
     tol = np.array([args.tol, args.tol, args.tol, args.tol_p, args.tol])
+    p_measure = nim_amd.P_HALFRANGE if args.p_measure == "halfrange" else nim_amd.P_MAXABS
     s_init = ctx.download_state()
+    vbytes = 4 if args.accel_basis == "fp32" else 8
 
-    def run(mixed):
-        """warm-up and timed steps from the initial state; returns wall time
-        of the timed steps, their inner passes, the FP64 residual measure
-        after the last inner solve (not timed) and the last Nusselt numbers"""
+    def run(form, steps, warmup, tol=tol, p_measure=p_measure):
+        """warm-up and timed steps from the initial state; form: "fp64" or
+        "mixed" (accelerated with --solver accel, plain otherwise) or
+        "plain" (plain FP64 passes). Returns wall time of the timed steps,
+        their inner passes (FP64 or FP32), the FP64 residual measure after
+        the last inner solve (not timed) and the last Nusselt numbers."""
         ctx.upload_state(s_init)
+        ctx.set_p_measure(p_measure)
+        mixed = form == "mixed"
+        acc = accel and form != "plain"
+        if acc:
+            if mixed:
+                ctx.accel_options(depth=args.mixed_depth, period=args.mixed_period, basis_bytes=vbytes)
+            else:
+                ctx.accel_options(depth=args.accel_depth, period=args.accel_period, basis_bytes=vbytes)
         passes, wall, t_res, res = 0, 0., 0., None
-        for s in range(args.warmup + args.steps):
-            timed, first, last = s >= args.warmup, s == 0, s == args.warmup + args.steps - 1
-            if timed and s == args.warmup:
+        for s in range(warmup + steps):
+            timed, first, last = s >= warmup, s == 0, s == warmup + steps - 1
+            if timed and s == warmup:
                 ctx.synchronize()
                 t0 = time.perf_counter()
             ctx.build_momentum_coeffs()
             if mixed:
-                rc, n, err = ctx.inner_solve_mixed(tol, inner=args.inner, max_cycles=args.max_iter)
+                if acc:
+                    rc, n, err = ctx.inner_solve_accel(tol, inner=args.inner, max_iter=args.max_iter // args.inner + 1)
+                else:
+                    rc, n, err = ctx.inner_solve_mixed(tol, inner=args.inner, max_cycles=args.max_iter)
+                # the last cycle's FP64 residual met the test: no FP32 passes after it
                 n = (n - 1) * args.inner
+            elif acc:
+                rc, n, err = ctx.inner_solve_accel(tol, max_iter=args.max_iter, min_iter=2 if first else 0)
             else:
                 rc, n, err = ctx.inner_solve(tol, max_iter=args.max_iter, min_iter=2 if first else 0)
-            print("%s step: %d passes, err %s" % ("mixed" if mixed else "fp64", n, " ".join("%.2e" % e for e in err)),
+            print("%s step: %d passes, err %s" % (form, n, " ".join("%.2e" % e for e in err)),
                   file=sys.stderr, flush=True)
             if last:
                 ctx.synchronize()
@@ -578,8 +624,21 @@ def main():
         wall = time.perf_counter() - t0 - t_res
         return wall, passes, res, nu
 
-    wall, total_iters, res64, nu = run(False)
+    head = "fp64"
+    wall, total_iters, res64, nu = run(head, args.steps, args.warmup)
     final64 = {n: ctx.get(n) for n in FACE_STATE + ("faces_p",)}
+    plain = None
+    if accel and args.plain_steps > 0:
+        tol_ref = np.array([args.tol, args.tol, args.tol, 5e-5, args.tol])
+        wall_p, passes_p, res_p, _ = run("plain", args.plain_steps, args.warmup, tol=tol_ref,
+                                           p_measure=nim_amd.P_MAXABS)
+        plain = {"steps_per_s": args.plain_steps / wall_p, "cell_updates_per_s": case.ncells * passes_p / wall_p,
+                 "passes_per_step": passes_p / args.plain_steps, "steps": args.plain_steps, "warmup": args.warmup,
+                 "tol": "%g on u, v, w, T, p 5e-5 on max |dp| (the reference's ratio, parameters.f90:146-150)"
+                        % args.tol,
+                 "residual_last_step": [float(x) for x in res_p],
+                 "note": "the plain FP64 loop (the reference's iteration) at the reference's tolerance ratio; with p "
+                         "at 1e-8 too it takes 1.4e5 - 1.8e5 passes per step here (strict_tolerance)"}
     strict = None
     if args.strict_steps > 0:
         strict = strict_tolerance(ctx, args)
@@ -608,7 +667,7 @@ def main():
 
     mixed = None
     if not args.no_mixed:
-        wall_m, passes_m, res_m, nu_m = run(True)
+        wall_m, passes_m, res_m, nu_m = run("mixed", args.steps, args.warmup)
         diff = {n[6:]: float(np.linalg.norm(ctx.get(n) - final64[n]) / np.linalg.norm(final64[n])) for n in FACE_STATE}
         pm, p64 = ctx.get("faces_p"), final64["faces_p"]
         diff["p (mean removed)"] = float(np.linalg.norm((pm - pm.mean()) - (p64 - p64.mean()))
@@ -625,6 +684,9 @@ def main():
             "fp32_passes_per_step": passes_m / args.steps,
             "inner_passes_per_fp64_residual": args.inner,
             "ms_per_step": wall_m / args.steps * 1e3,
+            "p_measure": ("half the range of each cycle's FP64 update of p per FP32 pass (the FP64 residual "
+                          "of p keeps the pressure drift's range)") if args.p_measure == "halfrange"
+            else "max |w r / a| of the FP64 residual",
             "residual_last_step": {"mixed": [float(x) for x in res_m], "fp64": [float(x) for x in res64]},
             "rel_l2_vs_fp64_after_same_steps": diff,
             "nusselt_last": [float(x) for x in nu_m[:3]],
@@ -634,6 +696,11 @@ def main():
             "kernel_ms_per_cycle": {k: v[0] for k, v in ktm.items()},
             "event_cycle_ms": cyc_ms,
         }
+        if accel:
+            mixed["acceleration"] = {"depth": args.mixed_depth, "cycles_per_step": args.mixed_period,
+                                     "history": args.accel_basis}
+        if plain:
+            mixed["speedup_vs_plain_fp64_reference_tolerances"] = (args.steps / wall_m) / plain["steps_per_s"]
 
     traffic, traffic_current = measured_traffic(args.case)
     line = {
@@ -650,11 +717,14 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded 1e-3 noise on cell averages)",
         "config": {"workload": "%s: %d x %d x %d (theta x r x z) = %d cells, Ra=%g, Pr=%g, DT=%g, FP64, "
-                               "tol %g (p %g on %s)" % (args.case, case.dims[0], case.dims[1], case.dims[2], N, case.Ra,
-                                                        case.Pr, case.DT, args.tol, args.tol_p,
-                                                        "max |dp|" if args.p_measure == "maxabs"
-                                                        else "half the range of dp"),
-                   "cells": N, "precision": "fp64", "sweep_order": "4-colour"},
+                               "tol %g (p %g on %s)%s" % (args.case, case.dims[0], case.dims[1], case.dims[2], N, case.Ra,
+                                                          case.Pr, case.DT, args.tol, args.tol_p,
+                                                          "max |dp|" if args.p_measure == "maxabs"
+                                                          else "half the range of dp",
+                                                          ", Anderson-accelerated passes" if accel else ""),
+                   "cells": N, "precision": "fp64", "sweep_order": "4-colour",
+                   "solver": ("Anderson acceleration every %d passes over the last %d steps, %s history"
+                              % (args.accel_period, args.accel_depth, args.accel_basis)) if accel else "plain passes"},
         "steps_per_s": args.steps / wall,
         "inner_iterations": total_iters,
         "inner_iterations_per_step": total_iters / args.steps,
@@ -680,6 +750,7 @@ def main():
         },
         "kernel_ms_per_iteration": {k: v[0] for k, v in kt.items()},
         "residual_last_step": [float(x) for x in res64],
+        "plain_fp64": plain,
         "mixed": mixed,
         "strict": strict,
         "adjoint": adjoint,

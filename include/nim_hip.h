@@ -161,7 +161,14 @@ int32_t nim_inner_solve(nim_ctx *ctx, const double tol[5], int32_t max_iter, int
  * Neumann-type walls only), and the relaxation moves p's constant level by a
  * nearly uniform amount every pass, which max |dp| counts and the half range
  * does not. The iteration is the same; only the stopping test differs. Read
- * at every solve call; the oracle has the same measure. */
+ * at every solve call; the oracle has the same measure. In the mixed loops
+ * (nim_inner_solve_mixed, nim_inner_solve_accel with inner > 0) the FP64
+ * residual measures p as max |w r / a|, and under NIM_P_HALFRANGE p is
+ * measured by half the range of each cycle's FP64 update of p divided by the
+ * cycle's FP32 passes (max |w r / a| before the first update): the residual
+ * of the drifting rest state is the drift times the sweep's splitting applied
+ * to a constant, which keeps a range, while its update is uniform.
+ * nim_residual and ensembles take the half range of w r / a. */
 enum { NIM_P_MAXABS = 0, NIM_P_HALFRANGE = 1 };
 int32_t nim_set_p_measure(nim_ctx *ctx, int32_t mode);
 
@@ -177,6 +184,30 @@ int32_t nim_set_p_measure(nim_ctx *ctx, int32_t mode);
  * Returns NIM_E_MAXITER after max_cycles. */
 int32_t nim_inner_solve_mixed(nim_ctx *ctx, const double tol[5], int32_t max_cycles, int32_t min_cycles,
                               int32_t inner, int32_t *cycles, double err[5]);
+
+/* Accelerated form of the inner loop (integrate_nse, nse.f90:117-176; the
+ * reference has no counterpart). inner = 0: the passes of nim_inner_solve;
+ * inner > 0: the cycles of nim_inner_solve_mixed with `inner` FP32 passes.
+ * After every `period` passes (or cycles) the iterate (cell values of
+ * u, v, w, T and the five face fields) is replaced by the Anderson
+ * extrapolation over the last `depth` such steps (Walker and Ni 2011), with
+ * the pressure faces' constant level left out of the least-squares problem.
+ * The stopping test, the fixed point and the returned counts are those of
+ * the plain loop: the solve ends on a pass whose increments (or a cycle whose
+ * FP64 residual) meet tol[] after at least min_iter passes (cycles). It
+ * removes the slow pressure mode that holds the plain loop: at the 1M-cell
+ * case p to 1e-8 takes 1e5 passes or more without it. The workspace, made
+ * on the first call and kept with the context, is one FP64 vector and
+ * 2 + 2 depth history vectors of 4 N + 5 NF values. Returns NIM_E_MAXITER
+ * after max_iter passes (cycles). */
+int32_t nim_inner_solve_accel(nim_ctx *ctx, const double tol[5], int32_t max_iter, int32_t min_iter,
+                              int32_t inner, int32_t *iters, double err[5]);
+
+/* Options of nim_inner_solve_accel: depth in [1, 16] (default 16), period
+ * in [1, 256] passes or cycles per extrapolation (default 8), and the bytes
+ * of each stored history value, 4 (default: FP32 history, half the traffic;
+ * the iterate and the extrapolation's arithmetic stay FP64) or 8. */
+int32_t nim_accel_options(nim_ctx *ctx, int32_t depth, int32_t period, int32_t basis_bytes);
 
 /* FP64 residual measure of the current state, max |w r / a| per field
  * (u, v, w, p, T), as nim_inner_solve_mixed tests it; no reference

@@ -50,7 +50,7 @@ template <bool NTL> __global__ void k_d_ppe_sweep(Dev d, View<float> v, Sched sc
 __global__ void k_d_zero_flux(Dev d, View<float> v);
 __global__ void k_d_T_src(Dev d, View<float> v);
 template <bool NTL> __global__ void k_d_T_sweep(Dev d, View<float> v, Sched sc);
-__global__ void k_mix_update(Dev d, View<double> x, View<float> dx);
+__global__ void k_mix_update(Dev d, View<double> x, View<float> dx, double inv_inner);
 __global__ void k_cast(const double *in, float *out, long n);
 __global__ void k_e_mom_src(Ens e);
 __global__ void k_e_mom_sweep(Ens e, int color);
@@ -72,7 +72,7 @@ __global__ void k_e_d_ppe_sweep(Ens e, int color);
 __global__ void k_e_d_zero_flux(Ens e);
 __global__ void k_e_d_T_src(Ens e);
 __global__ void k_e_d_T_sweep(Ens e, int color);
-__global__ void k_e_mix_update(Ens e);
+__global__ void k_e_mix_update(Ens e, double inv_inner);
 __global__ void k_e_gather(Ens e, Ctl *out);
 __global__ void k_e_coef_A(Ens e);
 __global__ void k_e_coef_F(Ens e, int cast);
@@ -182,6 +182,7 @@ struct nim_ctx {
         hipGraphExec_t gexec = nullptr;
         int gbatch = 0, ginner = 0;
     } mx;
+    void *accel = nullptr;  // Anderson acceleration workspace (nim_accel.hip), made on first use
 };
 
 extern "C" const char *nim_last_error(void) { return g_err.c_str(); }
@@ -204,6 +205,7 @@ void nim_internal_lengths(nim_ctx *c, double *R_i, double *R_o)
 }
 
 static void init_schedules(nim_ctx *c);
+void nim_internal_accel_free(void *ws);  // nim_accel.hip
 
 // The device code addresses per-cell slot groups as slot * N + c and the
 // face fields as field * NF + f in 32-bit integers. The largest products are
@@ -385,6 +387,7 @@ extern "C" int32_t nim_ctx_destroy(nim_ctx *c)
     if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->mx.gexec) (void)hipGraphExecDestroy(c->mx.gexec);
     if (c->mx.graph) (void)hipGraphDestroy(c->mx.graph);
+    nim_internal_accel_free(c->accel);
     for (void *p : c->allocs) (void)hipFree(p);
     if (c->h_ctl) (void)hipHostFree(c->h_ctl);
     if (c->h_out) (void)hipHostFree(c->h_out);
@@ -848,10 +851,18 @@ static void launch_iteration(nim_ctx *c)
     hipLaunchKernelGGL(k_check, dim3(1), dim3(64), 0, c->stream, c->dev);
 }
 
-static int32_t reset_ctl(nim_ctx *c, const double tol[5], int max_iter, int min_iter)
+// the pressure measure of a solve: the context's (nim_set_p_measure); in the
+// mixed loops the half range is taken of each cycle's pressure update, since
+// the FP64 residual of the drifting rest state keeps a range (nim_kernels.h)
+static int p_measure_of(const nim_ctx *c, bool mixed)
+{
+    return mixed && c->p_measure == P_HALFRANGE ? (int)P_HALFRANGE_UPDATE : c->p_measure;
+}
+
+static int32_t reset_ctl(nim_ctx *c, const double tol[5], int max_iter, int min_iter, bool mixed = false)
 {
     hipLaunchKernelGGL(k_solve_reset, dim3(1), dim3(256), 0, c->stream, c->dev, tol[0], tol[1], tol[2], tol[3],
-                       tol[4], max_iter, min_iter, c->p_measure);
+                       tol[4], max_iter, min_iter, p_measure_of(c, mixed));
     HIPCHK(hipGetLastError());
     return NIM_OK;
 }
@@ -1026,17 +1037,17 @@ static void launch_delta(nim_ctx *c, Mark mark)
 static void launch_residual(nim_ctx *c) { launch_residual(c, [](int) {}); }
 static void launch_delta(nim_ctx *c) { launch_delta(c, [](int) {}); }
 
-static void launch_update(nim_ctx *c)
+static void launch_update(nim_ctx *c, int inner)
 {
     hipLaunchKernelGGL(k_mix_update, dim3(nblk(std::max(c->N, c->NF))), dim3(BS), 0, c->stream, c->dev, c->mx.v64,
-                       c->mx.v32);
+                       c->mx.v32, 1. / inner);
 }
 
 static void launch_cycle(nim_ctx *c, int inner)
 {
     launch_residual(c);
     for (int q = 0; q < inner; q++) launch_delta(c);
-    launch_update(c);
+    launch_update(c, inner);
 }
 
 static int32_t launch_mixed_batch(nim_ctx *c, int batch, int inner)
@@ -1070,7 +1081,7 @@ extern "C" int32_t nim_inner_solve_mixed(nim_ctx *c, const double tol[5], int32_
     if (inner < 1 || inner > 1024) return fail(NIM_E_ARG, "inner passes must be in [1, 1024]");
     int32_t rc = mixed_prepare(c);
     if (rc) return rc;
-    if ((rc = reset_ctl(c, tol, max_cycles, min_cycles))) return rc;
+    if ((rc = reset_ctl(c, tol, max_cycles, min_cycles, true))) return rc;
     // cycles between host reads of the control block: about 16 passes
     const char *e = getenv("NIM_MIXED_BATCH");
     int batch = e ? atoi(e) : (16 + inner) / (inner + 1);
@@ -1089,6 +1100,32 @@ extern "C" int32_t nim_inner_solve_mixed(nim_ctx *c, const double tol[5], int32_
     if (st == 4) return fail(NIM_E_NONFINITE, "non-finite residual");
     return NIM_OK;
 }
+
+// ---- for the acceleration driver (nim_accel.hip) ---------------------------
+int32_t nim_internal_reset(nim_ctx *c, const double tol[5], int max_iter, int min_iter, bool mixed)
+{
+    if (!c->has_geo || !c->has_pcoef) return fail(NIM_E_STATE, "geometry or pressure coefficients missing");
+    return reset_ctl(c, tol, max_iter, min_iter, mixed);
+}
+// n FP64 passes (one captured graph)
+int32_t nim_internal_passes(nim_ctx *c, int n) { return launch_batch(c, n); }
+// n mixed cycles of `inner` FP32 passes each (one captured graph)
+int32_t nim_internal_cycles(nim_ctx *c, int n, int inner) { return launch_mixed_batch(c, n, inner); }
+int32_t nim_internal_mixed_prepare(nim_ctx *c)
+{
+    if (!c->has_geo || !c->has_pcoef) return fail(NIM_E_STATE, "geometry or pressure coefficients missing");
+    return mixed_prepare(c);
+}
+// the iteration control block after the work queued so far (synchronises
+// the stream; *out points at the context's pinned copy)
+int32_t nim_internal_read_ctl(nim_ctx *c, const Ctl **out)
+{
+    HIPCHK(hipMemcpyAsync(c->h_ctl, c->dev.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *out = c->h_ctl;
+    return NIM_OK;
+}
+void **nim_internal_accel_slot(nim_ctx *c) { return &c->accel; }
 
 extern "C" int32_t nim_residual(nim_ctx *c, double err[5])
 {
@@ -1247,7 +1284,7 @@ extern "C" int32_t nim_kernel_times_mixed(nim_ctx *c, int32_t cycles, int32_t in
     int32_t rc = mixed_prepare(c);
     if (rc) return rc;
     const double tol[5] = {0, 0, 0, 0, 0};
-    if ((rc = reset_ctl(c, tol, 1 << 30, 1 << 30))) return rc;
+    if ((rc = reset_ctl(c, tol, 1 << 30, 1 << 30, true))) return rc;
     for (int q = 0; q < NIM_MIXED_CLASSES; q++) {
         ms[q] = 0.;
         if (launches) launches[q] = 0;
@@ -1258,7 +1295,7 @@ extern "C" int32_t nim_kernel_times_mixed(nim_ctx *c, int32_t cycles, int32_t in
         log.start();
         launch_residual(c, mark);
         for (int p = 0; p < inner; p++) launch_delta(c, mark);
-        launch_update(c);
+        launch_update(c, inner);
         mark(11);
         HIPCHK(hipGetLastError());
         if ((rc = log.collect(ms, launches, it == 0))) return rc;
@@ -1443,7 +1480,7 @@ static void ens_cycle(nim_ens *e, int inner)
         hipLaunchKernelGGL(k_e_d_T_src, g256, dim3(BS), 0, s, c256);
         for (int q = 0; q < 4; q++) hipLaunchKernelGGL(k_e_d_T_sweep, gk256, dim3(BS), 0, s, k256, q);
     }
-    hipLaunchKernelGGL(k_e_mix_update, dim3(e->total[SH_UPD]), dim3(BS), 0, s, ens_args(e, SH_UPD));
+    hipLaunchKernelGGL(k_e_mix_update, dim3(e->total[SH_UPD]), dim3(BS), 0, s, ens_args(e, SH_UPD), 1. / inner);
 }
 
 static int32_t ens_batch(nim_ens *e, int batch, int inner)

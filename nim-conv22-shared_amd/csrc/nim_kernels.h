@@ -101,8 +101,19 @@ __device__ __host__ inline size_t err_slot(int slot, int field, int copy)
     return ((size_t)(slot * NERR + field) * ERR_COPIES + copy) * ERR_STRIDE;
 }
 
-// pressure convergence measure (Ctl::p_measure)
-enum { P_MAXABS = 0, P_HALFRANGE = 1 };
+// pressure convergence measure (Ctl::p_measure):
+//  P_MAXABS            max |dp| of the pass (or max |w r / a| of the FP64
+//                      residual in the mixed loops)
+//  P_HALFRANGE         half the range of the pass's increments dp, or of
+//                      w r / a in the mixed loops
+//  P_HALFRANGE_UPDATE  mixed loops: half the range of the last cycle's FP64
+//                      update of p divided by its FP32 passes (the residual
+//                      pass gives only max |w r / a|, used before the first
+//                      update). The pressure level drifts by the same amount
+//                      every pass, so the residual of the drifting rest state
+//                      is that amount times the sweep's splitting applied to
+//                      a constant, which has a range; the update's range is 0
+enum { P_MAXABS = 0, P_HALFRANGE = 1, P_HALFRANGE_UPDATE = 2 };
 
 struct Ctl {
     int32_t done, iter, status, max_iter, min_iter, p_measure, pad[2];

@@ -1048,8 +1048,11 @@ __global__ void __launch_bounds__(256) k_d_zero_flux(Dev d, View<float> v)
 }
 
 // outer update of the defect correction in FP64: x0 += dx, dx = 0, for the
-// u, v, w, T cells and the u, v, w, p, T faces
-__device__ __forceinline__ void mix_update(const Dev &d, const View<double> &x, const View<float> &dx, int lb)
+// u, v, w, T cells and the u, v, w, p, T faces. With P_HALFRANGE_UPDATE it
+// also reduces the largest p update and the largest negated one, each times
+// inv_inner (per FP32 pass), into the slot the next cycle's check reads
+__device__ __forceinline__ void mix_update(const Dev &d, const View<double> &x, const View<float> &dx, int lb,
+                                           double inv_inner)
 {
     int t = lb * blockDim.x + threadIdx.x;
     if (t < d.N)
@@ -1058,18 +1061,32 @@ __device__ __forceinline__ void mix_update(const Dev &d, const View<double> &x, 
             x.sc[s] += (double)dx.sc[s];
             dx.sc[s] = 0.f;
         }
+    double hi = -INFINITY, lo = -INFINITY;
     if (t < d.NF)
         for (int q = 0; q < 5; q++) {
             size_t s = (size_t)q * d.NF + t;
-            x.fc[s] += (double)dx.fc[s];
+            const double v = (double)dx.fc[s];
+            x.fc[s] += v;
             dx.fc[s] = 0.f;
+            if (q == F_P) {
+                hi = v * inv_inner;
+                lo = -hi;
+            }
         }
+    if (d.ctl->p_measure != P_HALFRANGE_UPDATE) return;
+    hi = wave_max(hi);
+    lo = wave_max(lo);
+    if ((threadIdx.x & 63) == 0 && hi > -INFINITY) {
+        const int slot = d.ctl->iter & 1, cp = blockIdx.x & (ERR_COPIES - 1);
+        atomicMax(d.errs + err_slot(slot, E_PHI, cp), order_key(hi));
+        atomicMax(d.errs + err_slot(slot, E_PLO, cp), order_key(lo));
+    }
 }
 
-__global__ void __launch_bounds__(256) k_mix_update(Dev d, View<double> x, View<float> dx)
+__global__ void __launch_bounds__(256) k_mix_update(Dev d, View<double> x, View<float> dx, double inv_inner)
 {
     if (d.ctl->done) return;
-    mix_update(d, x, dx, blockIdx.x);
+    mix_update(d, x, dx, blockIdx.x, inv_inner);
 }
 
 // FP32 copy of an FP64 block
@@ -1130,7 +1147,7 @@ __device__ __forceinline__ void check(const Dev &d)
         for (int q = 0; q < 5; q++) fin = fin && isfinite(e[q]);
         // half the range of the increments: blind to a change of p's
         // constant level (no pressure value this pass: the largest magnitude)
-        if (ctl->p_measure == P_HALFRANGE && fin && khi && klo) e[3] = (order_value(khi) + order_value(klo)) / 2;
+        if (ctl->p_measure != P_MAXABS && fin && khi && klo) e[3] = (order_value(khi) + order_value(klo)) / 2;
         for (int q = 0; q < 5; q++) {
             ctl->err[q] = e[q];
             conv = conv && !(e[q] > ctl->tol[q]);
@@ -1501,7 +1518,10 @@ __global__ void __launch_bounds__(256) k_e_d_T_sweep(Ens e, int color)
 {
     ENS_CASE T_sweep<MODE_DELTA>(d, e.v32[q], e.v32[q].TF, full_phase(d, color), lb);
 }
-__global__ void __launch_bounds__(256) k_e_mix_update(Ens e) { ENS_CASE mix_update(d, e.v64[q], e.v32[q], lb); }
+__global__ void __launch_bounds__(256) k_e_mix_update(Ens e, double inv_inner)
+{
+    ENS_CASE mix_update(d, e.v64[q], e.v32[q], lb, inv_inner);
+}
 
 // per-step work of every case (no early exit on convergence: it runs
 // around the solve)

@@ -112,6 +112,10 @@ def lib():
         L.nim_inner_solve_mixed.argtypes = [vp, _P, i32, i32, i32, ctypes.POINTER(i32), _P]
         L.nim_residual.restype = i32
         L.nim_residual.argtypes = [vp, _P]
+        L.nim_inner_solve_accel.restype = i32
+        L.nim_inner_solve_accel.argtypes = [vp, _P, i32, i32, i32, ctypes.POINTER(i32), _P]
+        L.nim_accel_options.restype = i32
+        L.nim_accel_options.argtypes = [vp, i32, i32, i32]
         L.nim_kernel_times_mixed.restype = i32
         L.nim_kernel_times_mixed.argtypes = [vp, i32, i32, _P, _PI]
         L.nim_revolve_plan.restype = i32
@@ -170,7 +174,7 @@ def exported_symbols():
             "nim_build_pressure_coeffs", "nim_upload_state", "nim_download_state", "nim_build_momentum_coeffs",
             "nim_inner_solve", "nim_set_p_measure", "nim_end_step", "nim_end_step_noalloc", "nim_advance_time", "nim_nusselt", "nim_run_stage",
             "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_kernel_times",
-            "nim_inner_solve_mixed", "nim_residual", "nim_kernel_times_mixed", "nim_ens_create", "nim_ens_destroy",
+            "nim_inner_solve_mixed", "nim_inner_solve_accel", "nim_accel_options", "nim_residual", "nim_kernel_times_mixed", "nim_ens_create", "nim_ens_destroy",
             "nim_ens_inner_solve", "nim_ens_time_step", "nim_revolve_plan", "nim_ckpt_create", "nim_ckpt_destroy", "nim_ckpt_store",
             "nim_ckpt_restore", "nim_adj_create", "nim_adj_destroy", "nim_adj_seed_nusselt", "nim_adj_reverse_step", "nim_adj_reverse_step_gmres", "nim_adj_gmres_options", "nim_adj_gradient_faces",
             "nim_adj_gradient", "nim_adj_reverse_pass", "nim_adj_reverse_stage", "nim_adj_clear", "nim_adj_get",
@@ -355,6 +359,32 @@ class Context:
         if rc != NIM_OK and (raise_on_cap or rc != NIM_E_MAXITER):
             _check(rc)
         return rc, cy.value, err
+
+    def accel_options(self, depth=16, period=8, basis_bytes=4):
+        """Anderson acceleration of inner_solve_accel: history depth, passes
+        (or cycles) per extrapolation, bytes per stored history value."""
+        _check(self.L.nim_accel_options(self.h, int(depth), int(period), int(basis_bytes)))
+
+    def inner_solve_accel(self, tol, inner=0, max_iter=1000000, min_iter=0, raise_on_cap=True):
+        """The inner loop with Anderson acceleration: FP64 passes (inner = 0)
+        or mixed cycles of `inner` FP32 passes, the plain loop's stopping
+        test. Returns (rc, passes or cycles, err)."""
+        t = np.ascontiguousarray(tol, dtype=np.float64)
+        it = ctypes.c_int32()
+        err = np.zeros(5)
+        rc = self.L.nim_inner_solve_accel(self.h, _dp(t), max_iter, min_iter, inner, ctypes.byref(it), _dp(err))
+        if rc != NIM_OK and (raise_on_cap or rc != NIM_E_MAXITER):
+            _check(rc)
+        return rc, it.value, err
+
+    def time_step_accel(self, tol, inner=0, max_iter=1000000, min_iter=0):
+        """One step of solve_nse with the accelerated inner loop."""
+        self.build_momentum_coeffs()
+        rc, iters, err = self.inner_solve_accel(tol, inner, max_iter, min_iter)
+        scal = self.end_step()
+        self.advance_time()
+        nu = self.nusselt()
+        return iters, err, scal, nu
 
     def residual(self):
         """FP64 residual measure max |w r / a| of the current state per

@@ -103,6 +103,25 @@ module nim_hip_api
       integer(c_int32_t), intent(out) :: cycles
       real(c_double), intent(out) :: err(5)
     end function
+    ! the same loop (inner = 0: FP64 passes; inner > 0: mixed cycles) with
+    ! Anderson acceleration of its passes: the plain loop's stopping test and
+    ! fixed point (no reference counterpart)
+    integer(c_int32_t) function nim_inner_solve_accel(ctx, tol, max_iter, min_iter, inner, iters, err) &
+        bind(C, name="nim_inner_solve_accel")
+      import :: c_int32_t, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: tol(5)
+      integer(c_int32_t), value :: max_iter, min_iter, inner
+      integer(c_int32_t), intent(out) :: iters
+      real(c_double), intent(out) :: err(5)
+    end function
+    ! history depth, passes (cycles) per extrapolation, bytes per history value
+    integer(c_int32_t) function nim_accel_options(ctx, depth, period, basis_bytes) &
+        bind(C, name="nim_accel_options")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: depth, period, basis_bytes
+    end function
     ! what tol(4) bounds: 0 max |dp| (find_error), 1 half the range of dp
     integer(c_int32_t) function nim_set_p_measure(ctx, mode) bind(C, name="nim_set_p_measure")
       import :: c_int32_t, c_ptr

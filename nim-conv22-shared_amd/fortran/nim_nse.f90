@@ -33,6 +33,11 @@ module nim_nse
     ! what tol(4) bounds: 0 max |dp| (find_error, the reference's), 1 half
     ! the range of dp (blind to p's constant level)
     integer :: p_measure = 0
+    ! > 0: Anderson acceleration of the inner loop's passes (FP64) or cycles
+    ! (mixed) every accel_period of them over the last `accel` steps (1..16);
+    ! the same stopping test and fixed point (nim_inner_solve_accel)
+    integer :: accel = 0
+    integer :: accel_period = 8
     character(len=256) :: plots_dir = 'plots'
   end type
 
@@ -129,6 +134,8 @@ contains
     node_xyz = nodes
     call check(nim_build_pressure_coeffs(ctx), 'nim_build_pressure_coeffs')
     call check(nim_set_p_measure(ctx, int(cfg%p_measure, c_int32_t)), 'nim_set_p_measure')
+    if (cfg%accel > 0) call check(nim_accel_options(ctx, int(cfg%accel, c_int32_t), &
+                                  int(cfg%accel_period, c_int32_t), 4_c_int32_t), 'nim_accel_options')
     allocate(faces_u(NF), faces_v(NF), faces_w(NF), faces_p(NF), faces_T(NF))
     allocate(uxyz(N), vxyz(N), wxyz(N), Txyz(N), uxyz_old(N), vxyz_old(N), wxyz_old(N), Txyz_old(N))
     allocate(uxyzt(N), vxyzt(N), wxyzt(N), Txyzt(N), meshes_p(N), dila(N))
@@ -157,7 +164,17 @@ contains
       call check(nim_build_momentum_coeffs(ctx), 'nim_build_momentum_coeffs')
       ! the reference's *_old arrays start at zero, so its first pass always
       ! sees the Dirichlet wall values as an increment: at least 2 passes
-      if (cfg%mixed_inner > 0) then
+      if (cfg%accel > 0) then
+        if (cfg%mixed_inner > 0) then
+          rc = nim_inner_solve_accel(ctx, cfg%tol, int(cfg%max_iter / cfg%mixed_inner + 1, c_int32_t), 0_c_int32_t, &
+                                     int(cfg%mixed_inner, c_int32_t), iters, err)
+          iters = (iters - 1) * int(cfg%mixed_inner, c_int32_t)
+        else
+          rc = nim_inner_solve_accel(ctx, cfg%tol, int(cfg%max_iter, c_int32_t), &
+                                     merge(2_c_int32_t, 0_c_int32_t, l == 1), 0_c_int32_t, iters, err)
+        endif
+        call check(rc, 'nim_inner_solve_accel')
+      else if (cfg%mixed_inner > 0) then
         rc = nim_inner_solve_mixed(ctx, cfg%tol, int(cfg%max_iter / cfg%mixed_inner + 1, c_int32_t), 0_c_int32_t, &
                                    int(cfg%mixed_inner, c_int32_t), iters, err)
         call check(rc, 'nim_inner_solve_mixed')

@@ -216,3 +216,49 @@ def test_field_output_every_step_out(tmp_path):
     assert sorted(fields) == sorted(want)
     for n, v in want.items():
         np.testing.assert_allclose(fields[n], v.astype(np.float32), rtol=1e-6, atol=1e-6 * np.abs(v).max(), err_msg=n)
+
+
+# R_i left at the driver's default, the reference's 1*(3.14159265359/3),
+# as host.reference_case() has it
+NML_ACCEL = """&case n_theta=36, n_r=6, n_z=12, R_o=2., height=2.,
+      blend_in=1.25, blend_out=1.75, Ra=1e5, Pr=0.7, DT=0.01, nsteps=3,
+      tol=1e-8,1e-8,1e-8,1e-8,1e-8, noise=1e-3, seed=20221, check_steady=.false.,
+      final_out='{out}', mixed_inner={inner}, p_measure=1, accel={depth}, accel_period={period} /
+"""
+
+
+@pytest.mark.parametrize("inner", [0, 16])
+def test_driver_accelerated_steps_match_the_library(tmp_path, inner):
+    """The driver with accel > 0 (every field at 1e-8, p by its half range)
+    runs the same accelerated solves as the library called from Python on
+    the same mesh, metrics and seeded state: the same passes per step and the
+    same bits."""
+    import nim_amd
+    from nim_amd import host
+    depth, period = (16, 8) if inner == 0 else (10, 1)
+    out = tmp_path / "final"
+    nml = tmp_path / "case.nml"
+    nml.write_text(NML_ACCEL.format(out=out, inner=inner, depth=depth, period=period))
+    p = subprocess.run([EXE, str(nml)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    iters = [int(l.split("# iters.")[1].split("|")[0]) for l in p.stdout.splitlines() if l.startswith("step #")]
+    st = read_restart(out)
+    case = host.reference_case()
+    geo = host.build_geometry(case, device=True)
+    ctx = G.gpu_for_case(case, geo, host.initial_state(case, amp=1e-3, seed=20221))
+    ctx.set_p_measure(nim_amd.P_HALFRANGE)
+    ctx.accel_options(depth=depth, period=period)
+    tol = np.full(5, 1e-8)
+    py_iters = []
+    for l in range(3):
+        ctx.build_momentum_coeffs()
+        rc, n, err = ctx.inner_solve_accel(tol, inner=inner, max_iter=200000 // max(inner, 1) + 1,
+                                           min_iter=2 if (l == 0 and inner == 0) else 0)
+        py_iters.append((n - 1) * inner if inner else n)
+        ctx.end_step()
+        ctx.advance_time()
+    print("driver passes", iters, "library passes", py_iters)
+    assert iters == py_iters
+    for n in ["faces_" + f for f in "uvwpT"] + [f + "xyz" for f in "uvwT"]:
+        np.testing.assert_array_equal(st[n], ctx.get(n), err_msg=n)
+    ctx.close()

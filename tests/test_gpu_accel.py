@@ -193,3 +193,38 @@ def test_cap_and_converged_entry(rest):
     with pytest.raises(nim_amd.NimError):
         ctx.accel_options(depth=17)
     ctx.close()
+
+
+def test_c2_accelerated_step_both_loops():
+    """At the benchmark's 1M-cell case, one step from the seeded state with
+    every field at 1e-8: the accelerated FP64 and mixed loops converge in a
+    few thousand passes (the plain loop needs 1.4e5 for p) and end within the
+    distance that the stopping tests leave (p is determined to about 1e-8
+    over the slow mode's rate, 1.5e-5 per pass); T and the Nusselt numbers,
+    which the pressure mode hardly reaches, agree tightly."""
+    case = host.baseline_case("C2")
+    geo = host.build_geometry(case, device=True)
+    st = host.initial_state(case, amp=1e-3, seed=20221)
+    ctx = G.gpu_for_case(case, geo, st)
+    ctx.set_p_measure(nim_amd.P_HALFRANGE)
+    tol = np.full(5, 1e-8)
+    out = {}
+    for form, inner, depth, period in (("fp64", 0, 16, 8), ("mixed", 16, 10, 1)):
+        ctx.upload_state(st)
+        ctx.accel_options(depth=depth, period=period)
+        ctx.build_momentum_coeffs()
+        rc, n, err = ctx.inner_solve_accel(tol, inner=inner, max_iter=20000 // max(inner, 1),
+                                           min_iter=2 if not inner else 0)
+        passes = (n - 1) * inner if inner else n
+        assert rc == 0 and np.all(err <= tol), (form, err)
+        assert passes < 10000, (form, passes)
+        out[form] = ({f: ctx.get(f) for f in FIELDS}, ctx.nusselt()[:3], passes)
+    g = gaps(out["mixed"][0].__getitem__, out["fp64"][0].__getitem__)
+    print("passes fp64 %d mixed %d; mixed against fp64:" % (out["fp64"][2], out["mixed"][2]), g)
+    assert g["faces_p"] < 5e-2, g
+    for f in ("faces_u", "faces_v", "faces_w", "uxyz", "vxyz", "wxyz"):
+        assert g[f] < 1e-3, (f, g[f])
+    for f in ("faces_T", "Txyz"):
+        assert g[f] < 1e-8, (f, g[f])
+    np.testing.assert_allclose(out["mixed"][1], out["fp64"][1], rtol=1e-6)
+    ctx.close()

@@ -288,6 +288,18 @@ def ensemble_bench(args):
                 for c in ctxs:
                     c.synchronize()
                 t_start = time.perf_counter()
+            if mode.endswith("-steps"):
+                # whole steps: per-step work batched over the cases in
+                # captured graphs (nim_ens_time_step)
+                inner = args.inner if mode.startswith("mixed") else 0
+                rc, its, err, _ = ens.time_step_batched(tol, inner=inner, max_iter=20000,
+                                                         min_iter=2 if (s == 0 and inner == 0) else 0)
+                n_pass = (its - 1) * args.inner if inner else its
+                if s >= args.warmup:
+                    passes += sum(int(n) * c.ncells for n, (c, _) in zip(n_pass, cases))
+                print("%s step %d: passes per case min %d max %d" % (mode, s, int(n_pass.min()), int(n_pass.max())),
+                      file=sys.stderr, flush=True)
+                continue
             for c in ctxs:
                 c.build_momentum_coeffs()
             if mode == "sequential":
@@ -330,6 +342,7 @@ def ensemble_bench(args):
                                "residual" % (args.tol, args.tol_p, args.inner),
                    "cells_total": ncell, "cells_min": min(sizes), "cells_max": max(sizes)},
         "batched_mixed": res.get("mixed"), "batched_fp64": res.get("fp64"), "sequential_mixed": res.get("sequential"),
+        "batched_steps_mixed": res.get("mixed-steps"), "batched_steps_fp64": res.get("fp64-steps"),
         "batching_speedup": (res["mixed"]["steps_per_s"] / res["sequential"]["steps_per_s"]
                              if "mixed" in res and "sequential" in res else None),
         "mixed_vs_fp64": (res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"]
@@ -499,7 +512,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the reversed-pass timing")
     ap.add_argument("--ensemble", action="store_true", help="the 48-case ensemble instead of one case")
     ap.add_argument("--ens-modes", default="mixed,fp64,sequential",
-                    help="ensemble runs: batched mixed, batched fp64, mixed one case after another")
+                    help="ensemble runs: batched mixed, batched fp64, mixed one case after another; mixed-steps, "
+                         "fp64-steps: whole steps with the per-step work batched and graph-captured")
     # counter passes: only the per-kernel event pass (few dispatches), or a
     # plain copy of known size that calibrates FETCH_SIZE / WRITE_SIZE
     ap.add_argument("--profile-only", action="store_true")

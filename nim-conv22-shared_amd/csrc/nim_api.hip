@@ -23,6 +23,8 @@ __global__ void k_mom_src(Dev d);
 __global__ void k_ppe_src(Dev d);
 __global__ void k_T_src(Dev d);
 template <int TI> __global__ void k_T_src_tile(Dev d);
+template <int TI> __global__ void k_mom_src_tile(Dev d);
+template <int TI> __global__ void k_ppe_src_tile(Dev d);
 template <bool NTL> __global__ void k_mom_sweep(Dev d, Sched sc);
 __global__ void k_T_sweep(Dev d, Sched sc);
 template <bool NTL> __global__ void k_T_sweep_fused(Dev d, Sched sc);
@@ -154,7 +156,7 @@ struct nim_ctx {
     // sweep schedules (64- and 256-cell blocks) and their slab thickness
     int slab = 0;
     bool rslab_sweeps = true;  // sweeps: one radial slab per XCD (NIM_RSLAB=0: storage order)
-    int tile = 0;              // energy sources from LDS tiles of tile x 256/tile cells (NIM_TILE=1; 0: per cell)
+    int tile = 0;              // FP64 source kernels from LDS tiles of tile x 256/tile cells (NIM_TILE=1; 0: per cell)
     // sweeps load what a pass reads once (coefficients, metric differences,
     // FP32 residuals) non-temporally: from NT_MIN_CELLS cells on, where a
     // pass's arrays overflow the caches (1M cells: the FP32 pass 5% faster);
@@ -722,6 +724,13 @@ static int cell_grid(const nim_ctx *c, int bs)
 }
 static int cell_blocks3(const nim_ctx *c) { return cell_grid(c, 64); }
 
+// blocks of a tiled source launch: one per tile of a plane
+static int tile_blocks(const nim_ctx *c)
+{
+    const int TJ = 256 / c->tile;
+    return (c->NT / c->tile) * ((c->NR + TJ - 1) / TJ) * c->NZ;
+}
+
 // energy sources (update_temperature_sources): from LDS tiles where the
 // azimuthal count allows, else one thread per cell
 static void launch_T_src(const nim_ctx *c, hipStream_t s)
@@ -804,7 +813,12 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark, bool fused = false)
     // plane directions along a pass: momentum sources up, momentum sweep
     // down-up-down-up, pressure sources down, pressure sweep up-down-up-down,
     // energy sources up, energy sweep down-up-down-up (with radial slabs)
-    case 0: hipLaunchKernelGGL(k_mom_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d); mark(0); break;
+    case 0:
+        if (c->tile == 64) hipLaunchKernelGGL(k_mom_src_tile<64>, dim3(tile_blocks(c)), dim3(BS), 0, s, d);
+        else if (c->tile == 32) hipLaunchKernelGGL(k_mom_src_tile<32>, dim3(tile_blocks(c)), dim3(BS), 0, s, d);
+        else hipLaunchKernelGGL(k_mom_src, dim3(cell_grid(c, BS)), dim3(BS), 0, s, d);
+        mark(0);
+        break;
     case 1:
         for (const Sched &sc : c->sched256r) {
             hipLaunchKernelGGL(c->nt ? k_mom_sweep<true> : k_mom_sweep<false>, dim3(sched_blocks(sc)), dim3(BS), 0,
@@ -812,7 +826,12 @@ static void launch_stage(nim_ctx *c, int stage, Mark mark, bool fused = false)
             mark(1);
         }
         break;
-    case 2: hipLaunchKernelGGL(k_ppe_src, dim3(gS), dim3(BS), 0, s, c->dev_rev); mark(2); break;
+    case 2:
+        if (c->tile == 64) hipLaunchKernelGGL(k_ppe_src_tile<64>, dim3(tile_blocks(c)), dim3(BS), 0, s, d);
+        else if (c->tile == 32) hipLaunchKernelGGL(k_ppe_src_tile<32>, dim3(tile_blocks(c)), dim3(BS), 0, s, d);
+        else hipLaunchKernelGGL(k_ppe_src, dim3(gS), dim3(BS), 0, s, c->dev_rev);
+        mark(2);
+        break;
     case 3:
         for (const Sched &sc : c->sched256) {
             hipLaunchKernelGGL(c->nt ? k_ppe_sweep<true> : k_ppe_sweep<false>, dim3(sched_blocks(sc)), dim3(BS), 0,

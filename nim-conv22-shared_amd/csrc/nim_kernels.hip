@@ -402,12 +402,9 @@ __global__ void __launch_bounds__(256) k_d_mom_src(Dev d, View<float> v)
 // and the corrections use the exact expansion R_b_p(x0 + dx) - R_b_p(x0)
 // (e.g. (a0 + da)^2 - a0^2 = 2 a0 da + da^2), which is small in dx.
 template <int MODE, typename T>
-__device__ __forceinline__ void ppe_src(const Dev &d, const View<T> &v, int lb)
+__device__ __forceinline__ T ppe_rbp(const Dev &d, const View<T> &v, int c, int i, int j, int k)
 {
-    int c = cell_of_block(d, lb, blockDim.x);
-    if (c >= d.N) return;
     const int N = d.N;
-    int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     int f[6];
     cell_faces(d, i, j, k, f);
     T uf[6], vf[6], wf[6], bzf[6];
@@ -443,7 +440,18 @@ __device__ __forceinline__ void ppe_src(const Dev &d, const View<T> &v, int lb)
         }
         Rbp = -(ux * ux + vy * vy + wz * wz) - T(2) * (uy * vx + vz * wy + wx * uz) - bzz + Dila / (T(2) * T(d.DT));
     }
+    return Rbp;
+}
+
+template <int MODE, typename T>
+__device__ __forceinline__ void ppe_src(const Dev &d, const View<T> &v, int lb)
+{
+    int c = cell_of_block(d, lb, blockDim.x);
+    if (c >= d.N) return;
+    const int N = d.N;
+    int i = iof(d, c % d.NT), j = (c / d.NT) % d.NR, k = c / (d.NT * d.NR);
     T RDp = cross_diffusion(d, v.gc, v.fc + (size_t)F_P * d.NF, i, j, k, c, T(d.rinv));
+    const T Rbp = ppe_rbp<MODE>(d, v, c, i, j, k);
     v.src[R_P * N + c] = RDp + Rbp;
 }
 
@@ -499,6 +507,60 @@ __global__ void __launch_bounds__(256) k_T_src_tile(Dev d)
 }
 template __global__ void k_T_src_tile<64>(Dev);
 template __global__ void k_T_src_tile<32>(Dev);
+
+// the momentum sources (mom_src_cell) with the cross-diffusion of u, v, w
+// from LDS-staged tiles of one plane (three tiles, 46 KB at 64 x 4 cells)
+template <int TI>
+__global__ void __launch_bounds__(256) k_mom_src_tile(Dev d)
+{
+    if (d.ctl->done) return;
+    __shared__ FaceTile<TI> s[3];
+    int i0, j0, k;
+    tile_of_block<TI>(d, xcd_block(), i0, j0, k);
+    for (int q = 0; q < 3; q++) load_tile<TI>(d, d.fc + (size_t)q * d.NF, s[q], i0, j0, k);
+    __syncthreads();
+    const int li = threadIdx.x % TI, jr = threadIdx.x / TI, i = i0 + li, j = j0 + jr;
+    if (j >= d.NR) return;
+    const int c = cidx(d, i, j, k), N = d.N;
+    int f[6];
+    cell_faces(d, i, j, k, f);
+    const double *P = d.fc + (size_t)F_P * d.NF;
+    double pf[6];
+#pragma unroll
+    for (int p = 0; p < 6; p++) pf[p] = P[f[p]];
+    const double invV = d.gc[G_INVV * N + c];
+    const double bw = -(d.g_beta * 0.5 * (d.sc[3 * N + c] + d.sc[(size_t)S_OLD * N + 3 * N + c]));
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double RB = (q == 2 ? bw : 0.) + d.rinv * dphi(d, d.An, pf, f, q, invV);
+        const double RD = cross_diffusion_tile<TI>(d, s[q], d.fc + (size_t)q * d.NF, i, j, k, c, li, jr, j0, d.Rinv);
+        d.src[q * N + c] = RD + RB;
+    }
+}
+template __global__ void k_mom_src_tile<64>(Dev);
+template __global__ void k_mom_src_tile<32>(Dev);
+
+// the pressure sources (ppe_src) with the cross-diffusion of p from an
+// LDS-staged tile
+template <int TI>
+__global__ void __launch_bounds__(256) k_ppe_src_tile(Dev d)
+{
+    if (d.ctl->done) return;
+    __shared__ FaceTile<TI> s;
+    int i0, j0, k;
+    tile_of_block<TI>(d, xcd_block(), i0, j0, k);
+    const double *F = d.fc + (size_t)F_P * d.NF;
+    load_tile<TI>(d, F, s, i0, j0, k);
+    __syncthreads();
+    const int li = threadIdx.x % TI, jr = threadIdx.x / TI, i = i0 + li, j = j0 + jr;
+    if (j >= d.NR) return;
+    const int c = cidx(d, i, j, k);
+    const double Rbp = ppe_rbp<MODE_GS>(d, view64(d), c, i, j, k);
+    const double RDp = cross_diffusion_tile<TI>(d, s, F, i, j, k, c, li, jr, j0, d.rinv);
+    d.src[R_P * d.N + c] = RDp + Rbp;
+}
+template __global__ void k_ppe_src_tile<64>(Dev);
+template __global__ void k_ppe_src_tile<32>(Dev);
 
 __global__ void __launch_bounds__(256) k_d_T_src(Dev d, View<float> v)
 {

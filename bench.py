@@ -369,9 +369,15 @@ def adjoint_bench(args):
     plan, fwd_plan = nim_amd.revolve_plan(nsteps, args.adj_slots)
     ctx.synchronize()
     t0 = time.perf_counter()
-    inner = (args.inner or 96) if args.mixed_forward else 0
+    accel = args.solver == "accel"
+    inner = (args.inner or (16 if accel else 96)) if args.mixed_forward else 0
+    if accel:
+        ctx.accel_options(depth=args.mixed_depth if inner else args.accel_depth,
+                          period=args.mixed_period if inner else args.accel_period,
+                          basis_bytes=4 if args.accel_basis == "fp32" else 8)
     J, grad, info = nim_amd.nusselt_gradient(ctx, case, nsteps, args.adj_slots, args.adj_tol, tol=tol, average=True,
                                              solver=args.adj_solver, mixed_inner=inner, restart=args.adj_restart,
+                                             accel=accel,
                                              max_adj_passes=args.adj_max_passes,
                                              progress=lambda m: print(m, file=sys.stderr, flush=True),
                                              basis=args.adj_basis, augment=args.adj_augment,
@@ -386,10 +392,10 @@ def adjoint_bench(args):
         "value": wall, "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f64",
         "data": "synthetic (seeded 1e-3 noise on cell averages)",
         "config": {"workload": "%s: %d x %d x %d (theta x r x z) = %d cells, Ra=%g, Pr=%g, DT=%g, %d steps, %d "
-                               "snapshots, forward %s to tol %g (p %g on %s), reversed fixed points %s"
+                               "snapshots, forward %s%s to tol %g (p %g on %s), reversed fixed points %s"
                                % (args.adjoint_case, nt, nr, nz, case.ncells, case.Ra, case.Pr, case.DT, nsteps,
                                   args.adj_slots, "mixed (%d FP32 passes per FP64 residual)" % inner if inner
-                                  else "FP64", args.tol, args.tol_p,
+                                  else "FP64", ", Anderson-accelerated" if accel else "", args.tol, args.tol_p,
                                   "max |dp|" if args.p_measure == "maxabs" else "half the range of dp",
                                   ("by GMRES(%d) with %d augmentation vectors and %d reversed passes per "
                                    "operator application to %g, %s Krylov basis"
@@ -553,7 +559,7 @@ def main():
         return 2
     if args.copy_probe:
         return copy_probe(args.copy_probe)
-    accel = args.solver == "accel" and not (args.ensemble or args.adjoint_case)
+    accel = args.solver == "accel" and not args.ensemble
     if args.tol_p is None:
         args.tol_p = args.tol if accel else 5e-5
     if args.p_measure is None:

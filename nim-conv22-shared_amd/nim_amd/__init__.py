@@ -657,11 +657,16 @@ def run_steps(ctx, nsteps, tol=None, passes=None):
     return ctx.nusselt()
 
 
-def _solve(ctx, tol, passes, max_iter=200000, mixed_inner=0):
+def _solve(ctx, tol, passes, max_iter=200000, mixed_inner=0, accel=False):
     """One step's momentum coefficients and inner solve (to tol, or exactly
     `passes` passes; with mixed_inner > 0 the mixed-precision loop to tol on
-    its FP64 residual); returns the passes (FP32 passes when mixed)."""
+    its FP64 residual; with accel the accelerated loop, nim_inner_solve_accel,
+    with the context's acceleration options); returns the passes (FP32
+    passes when mixed)."""
     ctx.build_momentum_coeffs()
+    if accel and passes is None:
+        rc, n, _ = ctx.inner_solve_accel(tol, inner=mixed_inner, max_iter=max_iter // max(mixed_inner, 1) + 1)
+        return (n - 1) * mixed_inner if mixed_inner > 0 else n
     if mixed_inner > 0 and passes is None:
         rc, cycles, _ = ctx.inner_solve_mixed(tol, inner=mixed_inner, max_cycles=max_iter // mixed_inner + 1)
         return (cycles - 1) * mixed_inner
@@ -676,7 +681,7 @@ def _solve(ctx, tol, passes, max_iter=200000, mixed_inner=0):
 
 def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, weights=(0.0, 1.0, 0.0),
                      average=False, solver="gmres", restart=50, max_adj_passes=100000, mixed_inner=0,
-                     progress=None, basis="fp64", augment=3, degree=8):
+                     progress=None, basis="fp64", augment=3, degree=8, accel=False):
     """J = weights . nim_nusselt[:3] after nsteps steps of ctx from its
     current state (as run_steps), or with average=True its mean over the
     states after steps 1 .. nsteps, and its gradient by the discrete adjoint,
@@ -690,7 +695,8 @@ def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, 
     {"T_hot", "Ra", "T_faces"}, info); T_faces holds dJ/dT of each hexagon
     wall face (n_theta * n_z, face i + n_theta k), whose sum is T_hot.
     mixed_inner > 0 runs the forward steps with the mixed-precision loop
-    (its fixed point is the FP64 one, at which the adjoint is taken).
+    (its fixed point is the FP64 one, at which the adjoint is taken); accel
+    runs them with the accelerated loop (the context's accel_options).
     info holds the forward and adjoint wall times and pass counts. progress,
     if given, is called with a line of text after every forward and reverse
     step."""
@@ -708,7 +714,7 @@ def nusselt_gradient(ctx, case, nsteps, nslots, adj_tol, tol=None, passes=None, 
 
     def solve():
         t0 = time.perf_counter()
-        clock["forward_passes"] += _solve(ctx, tol, passes, mixed_inner=mixed_inner)
+        clock["forward_passes"] += _solve(ctx, tol, passes, mixed_inner=mixed_inner, accel=accel)
         clock["forward_s"] += time.perf_counter() - t0
 
     def forward_step():

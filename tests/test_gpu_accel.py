@@ -228,3 +228,32 @@ def test_c2_accelerated_step_both_loops():
         assert g[f] < 1e-8, (f, g[f])
     np.testing.assert_allclose(out["mixed"][1], out["fp64"][1], rtol=1e-6)
     ctx.close()
+
+
+def test_ragged_iterate_length():
+    """10 x 4 x 5 cells: the iterate's length 4 N + 5 NF is 2 mod 4 (a
+    partial group of four at its end) and the pressure faces start inside a
+    group; the accelerated loop reaches the plain loop's rest state there."""
+    case = host.Case((10, 4, 5), 0.3, 1.0, 1.0, Ra=1e4, Pr=0.71, DT=1e-3, v_limit=0.3)
+    N, NF, _ = nim_amd.counts(*case.dims)
+    assert (4 * N + 5 * NF) % 4 == 2 and (4 * N + 3 * NF) % 4 == 2
+    geo = host.build_geometry(case, device=False)
+    st = host.initial_state(case, amp=1e-3, seed=20221)
+    out = {}
+    for form in ("plain", "accel"):
+        ctx = G.gpu_for_case(case, geo, st)
+        ctx.set_p_measure(nim_amd.P_HALFRANGE)
+        ctx.build_momentum_coeffs()
+        if form == "plain":
+            rc, n, err = ctx.inner_solve(np.zeros(5), max_iter=6000, raise_on_cap=False)
+            assert n == 6000 and max(err[:3]) < 1e-15 and err[3] < 1e-13, err
+        else:
+            ctx.accel_options(depth=10, period=4)
+            rc, n, err = ctx.inner_solve_accel(np.full(5, 1e-13), max_iter=6000)
+            assert rc == 0 and n < 2000, (n, err)
+        out[form] = {f: ctx.get(f) for f in FIELDS}
+        ctx.close()
+    g = gaps(out["accel"].__getitem__, out["plain"].__getitem__)
+    print("accelerated against 6,000 plain passes:", g)
+    for f, v in g.items():
+        assert v < (1e-9 if f == "faces_p" else 1e-10), (f, v)

@@ -257,3 +257,38 @@ def test_ragged_iterate_length():
     print("accelerated against 6,000 plain passes:", g)
     for f, v in g.items():
         assert v < (1e-9 if f == "faces_p" else 1e-10), (f, v)
+
+
+def test_high_rayleigh_ensemble_case():
+    """The highest-Ra case of the 48-case ensemble draw (32 x 16 x 16, Ra
+    7.5e5): the accelerated and plain loops at 1e-10 reach the same state
+    from the case's seeded start, the accelerated loop in a fraction of the
+    passes, with the mixed accelerated loop at 1e-8 within its stopping
+    slack."""
+    case, seed = max(host.ensemble_cases(), key=lambda t: t[0].Ra)
+    geo = host.build_geometry(case, device=True)
+    st = host.initial_state(case, seed=seed)
+    out = {}
+    for form, tol in (("plain", 1e-10), ("fp64", 1e-10), ("mixed", 1e-8)):
+        ctx = G.gpu_for_case(case, geo, st)
+        ctx.set_p_measure(nim_amd.P_HALFRANGE)
+        ctx.build_momentum_coeffs()
+        t = np.full(5, tol)
+        if form == "plain":
+            rc, n, err = ctx.inner_solve(t, max_iter=400000, min_iter=2)
+        elif form == "fp64":
+            rc, n, err = ctx.inner_solve_accel(t, max_iter=100000, min_iter=2, raise_on_cap=False)
+        else:
+            ctx.accel_options(depth=10, period=1)
+            rc, n, err = ctx.inner_solve_accel(t, inner=16, max_iter=10000, raise_on_cap=False)
+            n = (n - 1) * 16
+        print(form, rc, n, err)
+        assert rc == 0 and np.all(err <= t), (form, err)
+        out[form] = (n, {f: ctx.get(f) for f in FIELDS})
+        ctx.close()
+    g = gaps(out["fp64"][1].__getitem__, out["plain"][1].__getitem__)
+    gm = gaps(out["mixed"][1].__getitem__, out["fp64"][1].__getitem__)
+    print("passes", {k: v[0] for k, v in out.items()}, "fp64 against plain", g, "mixed against fp64", gm)
+    assert out["fp64"][0] < out["plain"][0] / 4
+    assert g["faces_p"] < 1e-4 and max(g[f] for f in FIELDS if f != "faces_p") < 1e-5, g
+    assert gm["faces_p"] < 1e-1 and max(gm[f] for f in FIELDS if f != "faces_p") < 1e-3, gm

@@ -395,6 +395,17 @@ static void ls_solve(const double G[MAXW][MAXW], const double *r, int n, double 
     for (int i = 0; i < n; i++) gamma[i] = D[i] * y[i];
 }
 
+// the least-squares solve alone, for host tests: G row-major n x n
+extern "C" int32_t nim_internal_ls_solve(const double *G, const double *r, int32_t n, double *gamma)
+{
+    if (!G || !r || !gamma || n < 1 || n > MAXW) return NIM_E_ARG;
+    double g[MAXW][MAXW];
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) g[i][j] = G[i * n + j];
+    ls_solve(g, r, n, gamma);
+    return NIM_OK;
+}
+
 static bool verbose()
 {
     const char *e = getenv("NIM_VERBOSE");

@@ -136,7 +136,8 @@ def cpu_baseline(case, geo, st, iters, colored_iters):
 KERNEL_CLASS = {"k_mom_src": "velocity_sources", "k_mom_sweep": "velocity_sweep", "k_ppe_src": "pressure_sources",
                 "k_ppe_sweep": "pressure_sweep", "k_T_coef": "energy_coefficients", "k_T_src": "energy_sources",
                 "k_T_sweep": "energy_sweep", "k_check": "check", "k_zero_flux": "energy_coefficients",
-                "k_T_sweep_fused": "energy_sweep", "k_T_src_tile": "energy_sources"}
+                "k_T_sweep_fused": "energy_sweep", "k_T_src_tile": "energy_sources",
+                "k_mom_src2": "velocity_sources", "k_ppe_src2": "pressure_sources", "k_T_src2": "energy_sources"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "%s_fp64_hbm_traffic.md")
 
 

@@ -243,6 +243,21 @@ __device__ __forceinline__ int cell_of_block(const Dev &d, int lb, int bs)
     return cp < ppl ? cp + (lb % d.NZ) * ppl : d.N;
 }
 
+// First cell of this thread's pair in the two-cells-per-thread source
+// kernels: cell_of_block's map over blocks of 2 bs cells, the thread taking
+// the block's storage positions 2t and 2t + 1. The host uses it only
+// without kfast and, with rslab, where NT*NR/8 is a multiple of 2 bs.
+__device__ __forceinline__ int pair_of_block(const Dev &d, int lb, int bs)
+{
+    const int bc = 2 * bs;
+    if (d.rslab) {
+        const int sp = d.NT * (d.NR / NXCD), bps = sp / bc, q = d.NZ * bps;
+        const int x = lb / q, r = lb - x * q, kk = r / bps, k = d.krev ? d.NZ - 1 - kk : kk;
+        return k * (d.NT * d.NR) + x * sp + (r - kk * bps) * bc + 2 * (int)threadIdx.x;
+    }
+    return lb * bc + 2 * (int)threadIdx.x;
+}
+
 __device__ __forceinline__ int ip1(const Dev &d, int i) { return i + 1 == d.NT ? 0 : i + 1; }
 __device__ __forceinline__ int im1(const Dev &d, int i) { return i == 0 ? d.NT - 1 : i - 1; }
 // storage position of azimuthal index i within its row: even i first, then

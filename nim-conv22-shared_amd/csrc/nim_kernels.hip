@@ -394,6 +394,44 @@ __global__ void __launch_bounds__(256) k_d_mom_src(Dev d, View<float> v)
     mom_src_cell(d, v, xcd_block());
 }
 
+// mom_src_cell for two cells of one half-row per thread (CellPair): the
+// same arithmetic per cell, with one 16-byte load per pair of values
+template <typename T>
+__device__ __forceinline__ void mom_src_pair(const Dev &d, const View<T> &v, int lb)
+{
+    const int c = pair_of_block(d, lb, blockDim.x);
+    if (c >= d.N) return;
+    const int N = d.N;
+    const CellPair p = cell_pair(d, c);
+    T pa[6], pb[6];
+    faces2(d, p, v.fc + (size_t)F_P * d.NF, pa, pb);
+    T invA, invB;
+    ld2(v.gc + (size_t)G_INVV * N + c, invA, invB);
+    T GA[3], GOA[6], GB[3], GOB[6];
+    cross_metrics2(d, v.gc, c, GA, GOA, GB, GOB);
+    T sa, sb, oa = T(0), ob = T(0);
+    ld2(v.sc + 3 * (size_t)N + c, sa, sb);
+    if (v.old) ld2(v.old + 3 * (size_t)N + c, oa, ob);
+    const T bwa = -(T(d.g_beta) * T(0.5) * (sa + oa)), bwb = -(T(d.g_beta) * T(0.5) * (sb + ob));
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        T Aa[6], Ab[6];
+        faces2(d, p, v.An + (size_t)q * d.NF, Aa, Ab);
+        const T RBa = (q == 2 ? bwa : T(0)) + T(d.rinv) * dphi_vals(pa, Aa, invA);
+        const T RBb = (q == 2 ? bwb : T(0)) + T(d.rinv) * dphi_vals(pb, Ab, invB);
+        T RDa, RDb;
+        cross_diffusion2(d, p, v.fc + (size_t)q * d.NF, GA, GOA, GB, GOB, T(d.Rinv), RDa, RDb);
+        st2(v.src + (size_t)q * N + c, RDa + RBa, RDb + RBb);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_mom_src2(Dev d)
+{
+    if (d.ctl->done) return;
+    mom_src_pair(d, view64(d), xcd_block());
+}
+
+
 // update_pressure_sources (nse.f90:228-235): calculate_R_b_p
 // (nse.f90:2362-2407; faces_bx = faces_by = 0, faces_bz = -g_beta*faces_T)
 // and calculate_R_D_p2; stores R_D_p + R_b_p, the combination the pressure
@@ -473,6 +511,71 @@ __global__ void __launch_bounds__(256) k_d_ppe_src(Dev d, View<float> v)
     ppe_src<MODE_DELTA>(d, v, xcd_block());
 }
 
+// ppe_rbp (GS and RES) of one cell from its loaded face values (u, v, w,
+// T) and area components (A[comp][face]); the same operations as ppe_rbp
+template <int MODE, typename T>
+__device__ __forceinline__ T rbp_vals(const Dev &d, const View<T> &v, int c, const T uf[6], const T vf[6],
+                                      const T wf[6], const T tf[6], const T A[3][6], T invV)
+{
+    static_assert(MODE != MODE_DELTA, "corrections use ppe_rbp");
+    T bzf[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) bzf[q] = -(T(d.g_beta) * tf[q]);
+    T ux = dphi_vals(uf, A[0], invV), uy = dphi_vals(uf, A[1], invV), uz = dphi_vals(uf, A[2], invV);
+    T vx = dphi_vals(vf, A[0], invV), vy = dphi_vals(vf, A[1], invV), vz = dphi_vals(vf, A[2], invV);
+    T wx = dphi_vals(wf, A[0], invV), wy = dphi_vals(wf, A[1], invV), wz = dphi_vals(wf, A[2], invV);
+    T bzz = dphi_vals(bzf, A[2], invV);
+    T Dila = ux + vy + wz;
+    const int N = d.N;
+    if (MODE == MODE_GS) v.sc[S_DILA * N + c] = Dila;
+    if (MODE == MODE_RES) {
+        float *gr = v.grad;
+        gr[0 * N + c] = (float)ux; gr[1 * N + c] = (float)uy; gr[2 * N + c] = (float)uz;
+        gr[3 * N + c] = (float)vx; gr[4 * N + c] = (float)vy; gr[5 * N + c] = (float)vz;
+        gr[6 * N + c] = (float)wx; gr[7 * N + c] = (float)wy; gr[8 * N + c] = (float)wz;
+    }
+    return -(ux * ux + vy * vy + wz * wz) - T(2) * (uy * vx + vz * wy + wx * uz) - bzz + Dila / (T(2) * T(d.DT));
+}
+
+// ppe_src (GS and RES) for two cells of one half-row per thread (CellPair)
+template <int MODE, typename T>
+__device__ __forceinline__ void ppe_src_pair(const Dev &d, const View<T> &v, int lb)
+{
+    const int c = pair_of_block(d, lb, blockDim.x);
+    if (c >= d.N) return;
+    const int N = d.N;
+    const CellPair p = cell_pair(d, c);
+    T GA[3], GOA[6], GB[3], GOB[6];
+    cross_metrics2(d, v.gc, c, GA, GOA, GB, GOB);
+    T RDa, RDb;
+    cross_diffusion2(d, p, v.fc + (size_t)F_P * d.NF, GA, GOA, GB, GOB, T(d.rinv), RDa, RDb);
+    T ua[6], ub[6], va[6], vb[6], wa[6], wb[6], ta[6], tb[6], Aa[3][6], Ab[3][6];
+    faces2(d, p, v.fc, ua, ub);
+    faces2(d, p, v.fc + (size_t)d.NF, va, vb);
+    faces2(d, p, v.fc + 2 * (size_t)d.NF, wa, wb);
+    faces2(d, p, v.fc + 4 * (size_t)d.NF, ta, tb);
+#pragma unroll
+    for (int q = 0; q < 3; q++) faces2(d, p, v.An + (size_t)q * d.NF, Aa[q], Ab[q]);
+    T invA, invB;
+    ld2(v.gc + (size_t)G_INVV * N + c, invA, invB);
+    const T Ra = rbp_vals<MODE>(d, v, c, ua, va, wa, ta, Aa, invA);
+    const T Rb = rbp_vals<MODE>(d, v, c + 1, ub, vb, wb, tb, Ab, invB);
+    st2(v.src + (size_t)R_P * N + c, RDa + Ra, RDb + Rb);
+}
+
+__global__ void __launch_bounds__(256) k_ppe_src2(Dev d)
+{
+    if (d.ctl->done) return;
+    ppe_src_pair<MODE_GS>(d, view64(d), xcd_block());
+}
+
+__global__ void __launch_bounds__(256) k_res_ppe_src2(Dev d, View<double> v)
+{
+    if (d.ctl->done) return;
+    ppe_src_pair<MODE_RES>(d, v, xcd_block());
+}
+
+
 // update_temperature_sources (nse.f90:249-256): R_b_T = q = 0, R_D_T
 template <typename T>
 __device__ __forceinline__ void T_src(const Dev &d, const View<T> &v, int lb)
@@ -488,6 +591,27 @@ __global__ void __launch_bounds__(256) k_T_src(Dev d)
     if (d.ctl->done) return;
     T_src(d, view64(d), xcd_block());
 }
+
+// T_src for two cells of one half-row per thread (CellPair)
+template <typename T>
+__device__ __forceinline__ void T_src_pair(const Dev &d, const View<T> &v, int lb)
+{
+    const int c = pair_of_block(d, lb, blockDim.x);
+    if (c >= d.N) return;
+    const CellPair p = cell_pair(d, c);
+    T GA[3], GOA[6], GB[3], GOB[6];
+    cross_metrics2(d, v.gc, c, GA, GOA, GB, GOB);
+    T ra, rb;
+    cross_diffusion2(d, p, v.fc + (size_t)F_T * d.NF, GA, GOA, GB, GOB, T(d.alpha), ra, rb);
+    st2(v.src + (size_t)R_T * d.N + c, ra + T(0), rb + T(0));
+}
+
+__global__ void __launch_bounds__(256) k_T_src2(Dev d)
+{
+    if (d.ctl->done) return;
+    T_src_pair(d, view64(d), xcd_block());
+}
+
 
 // the same from an LDS-staged tile of 256 cells (NT a multiple of TI)
 template <int TI>

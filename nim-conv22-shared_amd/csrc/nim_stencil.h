@@ -254,6 +254,194 @@ __device__ __forceinline__ void cell_faces(const Dev &d, int i, int j, int k, in
 }
 
 // ---------------------------------------------------------------------
+// two cells per thread (source kernels, NIM_PAIR)
+// ---------------------------------------------------------------------
+// A thread takes storage positions q and q + 1 (q even) of one half-row,
+// the azimuthal indices i and i + 2: every field value of the two cells at
+// the same offset is one aligned 16-byte (FP64) or 8-byte (FP32) load. Their
+// azimuthal neighbours i - 1, i + 1, i + 3 are three consecutive positions
+// of the other half-row, covered by two aligned pairs. Needs n_theta % 4 ==
+// 0 (even half-rows) and even block offsets, which n_theta % 4 == 0 gives.
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+
+template <typename T>
+__device__ __forceinline__ void ld2(const T *p, T &a, T &b)
+{
+    const typename Vec2<T>::type v = *reinterpret_cast<const typename Vec2<T>::type *>(p);
+    a = v.x;
+    b = v.y;
+}
+
+template <typename T>
+__device__ __forceinline__ void st2(T *p, T a, T b)
+{
+    typename Vec2<T>::type v;
+    v.x = a;
+    v.y = b;
+    *reinterpret_cast<typename Vec2<T>::type *>(p) = v;
+}
+
+struct CellPair {
+    int i, j, k;  // indices of the first cell (the second is i + 2)
+    int q;        // storage position of the first cell in its row
+    int lo, hi;   // other-half positions of the pairs holding i - 1, i + 1, i + 3
+    bool odd;     // the pair lies in the odd half-row
+};
+
+// c is the first cell of the pair (an even storage position)
+__device__ __forceinline__ CellPair cell_pair(const Dev &d, int c)
+{
+    CellPair p;
+    const int h = d.NT >> 1;
+    p.q = c % d.NT;
+    p.j = (c / d.NT) % d.NR;
+    p.k = c / (d.NT * d.NR);
+    p.i = iof(d, p.q);
+    p.odd = p.q >= h;
+    const int qq = p.odd ? p.q - h : p.q, ob = p.odd ? 0 : h;
+    // even half, i = 2qq: i - 1 sits at ob + qq - 1 (the second of the pair
+    // before ob + qq), i + 1 and i + 3 at ob + qq and ob + qq + 1;
+    // odd half, i = 2qq + 1: i - 1, i + 1 at ob + qq, ob + qq + 1, i + 3 at
+    // ob + qq + 2; both wrap within the half-row
+    p.lo = ob + (p.odd ? qq : (qq - 2 + h) % h);
+    p.hi = ob + (p.odd ? (qq + 2) % h : qq);
+    return p;
+}
+
+// first storage index of row (j, k) of face position pos (1..6; 4 shares
+// the azimuthal block of 3) or of the cells (pos 0)
+__device__ __forceinline__ int row_base(const Dev &d, int pos, int j, int k)
+{
+    switch (pos) {
+    case 0: return d.NT * (j + d.NR * k);
+    case 1: return d.NT * (j + (d.NR + 1) * k);
+    case 2: return d.NT * (j + 1 + (d.NR + 1) * k);
+    case 3:
+    case 4: return d.oT + d.NT * (j + d.NR * k);
+    case 5: return d.oZ + d.NT * (j + d.NR * k);
+    default: return d.oZ + d.NT * (j + d.NR * (k + 1));
+    }
+}
+
+// values of row rb at the azimuthal neighbours i - 1, i + 1, i + 3
+template <typename T>
+__device__ __forceinline__ void other3(const CellPair &p, const T *F, int rb, T &m1, T &p1, T &p3)
+{
+    T la, lb, ha, hb;
+    ld2(F + rb + p.lo, la, lb);
+    ld2(F + rb + p.hi, ha, hb);
+    m1 = p.odd ? la : lb;
+    p1 = p.odd ? lb : ha;
+    p3 = p.odd ? ha : hb;
+}
+
+// face value at position pos (1..6) of row (j, k) for both cells
+template <typename T>
+__device__ __forceinline__ void face2(const Dev &d, const CellPair &p, const T *F, int pos, int j, int k, T &a, T &b)
+{
+    if (pos == 4) {
+        T m1;
+        other3(p, F, row_base(d, 4, j, k), m1, a, b);
+        return;
+    }
+    ld2(F + row_base(d, pos, j, k) + p.q, a, b);
+}
+
+// tder for both cells (same arithmetic per cell)
+template <typename T>
+__device__ __forceinline__ void tder2(const Dev &d, const CellPair &p, const T *F, int fpos, int dir, T &a, T &b)
+{
+    const int j = p.j, k = p.k;
+    if (dir == 0) {
+        T m1, p1, p3;
+        other3(p, F, row_base(d, fpos, j, k), m1, p1, p3);
+        a = (p1 - m1) / T(4);
+        b = (p3 - p1) / T(4);
+        return;
+    }
+    T f0a, f0b, f1a, f1b, f2a = T(0), f2b = T(0);
+    bool lo, hi;
+    if (dir == 1) {
+        lo = j == 0; hi = j == d.NR - 1;
+        const int j0 = lo ? 0 : hi ? j : j + 1, j1 = lo ? 1 : j - 1, j2 = lo ? 2 : j - 2;
+        face2(d, p, F, fpos, j0, k, f0a, f0b);
+        face2(d, p, F, fpos, j1, k, f1a, f1b);
+        if (lo || hi) face2(d, p, F, fpos, j2, k, f2a, f2b);
+    } else {
+        lo = k == 0; hi = k == d.NZ - 1;
+        const int k0 = lo ? 0 : hi ? k : k + 1, k1 = lo ? 1 : k - 1, k2 = lo ? 2 : k - 2;
+        face2(d, p, F, fpos, j, k0, f0a, f0b);
+        face2(d, p, F, fpos, j, k1, f1a, f1b);
+        if (lo || hi) face2(d, p, F, fpos, j, k2, f2a, f2b);
+    }
+    a = three_point(f0a, f1a, f2a, lo, hi);
+    b = three_point(f0b, f1b, f2b, lo, hi);
+}
+
+// cross_diffusion of both cells, each with its metrics (G, GO)
+template <typename T>
+__device__ __forceinline__ void cross_diffusion2(const Dev &d, const CellPair &p, const T *F, const T GA[3],
+                                                 const T GOA[6], const T GB[3], const T GOB[6], T coef, T &ra, T &rb)
+{
+    T t2a, t2b, t1a, t1b, t6a, t6b, t5a, t5b;
+    tder2(d, p, F, 2, 0, t2a, t2b);
+    tder2(d, p, F, 1, 0, t1a, t1b);
+    tder2(d, p, F, 6, 0, t6a, t6b);
+    tder2(d, p, F, 5, 0, t5a, t5b);
+    const T fxya = T(0.5) * (t2a - t1a), fxyb = T(0.5) * (t2b - t1b);
+    const T fzya = T(0.5) * (t6a - t5a), fzyb = T(0.5) * (t6b - t5b);
+    tder2(d, p, F, 4, 1, t2a, t2b);
+    tder2(d, p, F, 3, 1, t1a, t1b);
+    tder2(d, p, F, 6, 1, t6a, t6b);
+    tder2(d, p, F, 5, 1, t5a, t5b);
+    const T fyxa = T(0.5) * (t2a - t1a), fyxb = T(0.5) * (t2b - t1b);
+    const T fzxa = T(0.5) * (t6a - t5a), fzxb = T(0.5) * (t6b - t5b);
+    tder2(d, p, F, 2, 2, t2a, t2b);
+    tder2(d, p, F, 1, 2, t1a, t1b);
+    tder2(d, p, F, 4, 2, t6a, t6b);
+    tder2(d, p, F, 3, 2, t5a, t5b);
+    const T fxza = T(0.5) * (t2a - t1a), fxzb = T(0.5) * (t2b - t1b);
+    const T fyza = T(0.5) * (t6a - t5a), fyzb = T(0.5) * (t6b - t5b);
+    T fa[6], fb[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) face2(d, p, F, q + 1, p.j, p.k, fa[q], fb[q]);
+    const T fxa = (fa[1] - fa[0]) / T(2), fxb = (fb[1] - fb[0]) / T(2);
+    const T fya = (fa[3] - fa[2]) / T(2), fyb = (fb[3] - fb[2]) / T(2);
+    const T fza = (fa[5] - fa[4]) / T(2), fzb = (fb[5] - fb[4]) / T(2);
+    ra = cross_combine(coef, GA, GOA, fxa, fya, fza, fxya, fyxa, fxza, fzxa, fyza, fzya);
+    rb = cross_combine(coef, GB, GOB, fxb, fyb, fzb, fxyb, fyxb, fxzb, fzxb, fyzb, fzyb);
+}
+
+// the cross metrics of both cells (c the first)
+template <typename T>
+__device__ __forceinline__ void cross_metrics2(const Dev &d, const T *g, int c, T GA[3], T GOA[6], T GB[3], T GOB[6])
+{
+    const int N = d.N;
+#pragma unroll
+    for (int m = 0; m < 3; m++) ld2(g + (size_t)(G_G + m) * N + c, GA[m], GB[m]);
+#pragma unroll
+    for (int m = 0; m < 6; m++) ld2(g + (size_t)(G_GO + m) * N + c, GOA[m], GOB[m]);
+}
+
+// the six face values of a face field for both cells
+template <typename T>
+__device__ __forceinline__ void faces2(const Dev &d, const CellPair &p, const T *F, T a[6], T b[6])
+{
+#pragma unroll
+    for (int q = 0; q < 6; q++) face2(d, p, F, q + 1, p.j, p.k, a[q], b[q]);
+}
+
+// dphi from loaded values: the face values phi and area components A of one cell
+template <typename T>
+__device__ __forceinline__ T dphi_vals(const T phi[6], const T A[6], T invV)
+{
+    T s = phi[1] * A[1] - phi[0] * A[0] + phi[3] * A[3] - phi[2] * A[2] + phi[5] * A[5] - phi[4] * A[4];
+    return invV * s;
+}
+
+// ---------------------------------------------------------------------
 // forward-mode dual numbers (value, derivative)
 // ---------------------------------------------------------------------
 struct Dual {

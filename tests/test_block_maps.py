@@ -13,7 +13,12 @@ assume of them (no GPU needed):
   [x n_r / 8, (x + 1) n_r / 8), under the round-robin dispatch that
   xcd_block() renumbers;
 - the LDS tiles of the energy sources (load_tile / cross_diffusion_tile,
-  nim_stencil.h): every LDS entry a cell's derivatives read was staged.
+  nim_stencil.h): every LDS entry a cell's derivatives read was staged;
+- the two-cells-per-thread source kernels (pair_of_block, nim_kernels.h;
+  cell_pair / other3, nim_stencil.h): every cell exactly once, the two cells
+  of a thread at azimuthal indices i and i + 2 of one row, every pair load
+  aligned and inside its row, and the values picked from the two
+  other-half pairs those at i - 1, i + 1 and i + 3.
 """
 import itertools
 
@@ -180,3 +185,70 @@ def test_tile_reads_hit_staged_entries(dims, TI):
                     for kk in (0, 1):
                         for e in [(jm, li + 2), (jm, li), (ja, li + 1), (jb, li + 1), (jc, li + 1), (jm, li + 1)]:
                             assert (kk,) + e in Z
+
+
+def ipos(NT, i):
+    return (i & 1) * (NT >> 1) + (i >> 1)
+
+
+def iof(NT, p):
+    h = NT >> 1
+    return 2 * p if p < h else 2 * (p - h) + 1
+
+
+def pair_of_block(NT, NR, NZ, lb, tid, bs, rslab, krev):
+    bc = 2 * bs
+    if rslab:
+        sp = NT * (NR // NXCD)
+        bps = sp // bc
+        q = NZ * bps
+        x, r = lb // q, lb % q
+        kk = r // bps
+        k = NZ - 1 - kk if krev else kk
+        return k * NT * NR + x * sp + (r - kk * bps) * bc + 2 * tid
+    return lb * bc + 2 * tid
+
+
+def cell_pair(NT, NR, c):
+    h = NT >> 1
+    q = c % NT
+    odd = q >= h
+    qq = q - h if odd else q
+    ob = 0 if odd else h
+    lo = ob + (qq if odd else (qq - 2 + h) % h)
+    hi = ob + ((qq + 2) % h if odd else qq)
+    return iof(NT, q), (c // NT) % NR, c // (NT * NR), q, lo, hi, odd
+
+
+@pytest.mark.parametrize("dims", [(256, 128, 2), (48, 24, 3), (36, 6, 4), (12, 4, 4), (128, 64, 3), (20, 5, 3)])
+@pytest.mark.parametrize("krev", [False, True])
+def test_pair_map_and_neighbours(dims, krev):
+    """pair_of_block under the round-robin dispatch, with radial slabs where
+    the host enables them (n_r % 8 == 0 and slab-planes of whole 512-cell
+    blocks) and in storage order otherwise; cell_pair's positions."""
+    NT, NR, NZ = dims
+    assert NT % 4 == 0
+    N, bs, h = NT * NR * NZ, 256, NT >> 1
+    rslab = NR % NXCD == 0 and (NT * (NR // NXCD)) % (2 * bs) == 0
+    nb = (N + 2 * bs - 1) // (2 * bs)
+    seen = []
+    for b in range(nb):
+        lb = xcd_block(b, nb) if rslab else b
+        for t in range(bs):
+            c = pair_of_block(NT, NR, NZ, lb, t, bs, rslab, krev and rslab)
+            if c >= N:
+                continue
+            assert c % 2 == 0 and c + 1 < N
+            i, j, k, q, lo, hi, odd = cell_pair(NT, NR, c)
+            # both cells in one row, azimuthal indices i and i + 2
+            assert (c + 1) // NT == c // NT and iof(NT, q + 1) == i + 2 and (q < h) == (q + 1 < h)
+            # aligned pairs of the other half-row
+            for x in (lo, hi):
+                assert x % 2 == 0 and 0 <= x and x + 1 < NT and (x < h) == odd
+            la, lb_, ha, hb = lo, lo + 1, hi, hi + 1
+            m1, p1, p3 = (la, lb_, ha) if odd else (lb_, ha, hb)
+            assert m1 == ipos(NT, (i - 1) % NT)
+            assert p1 == ipos(NT, (i + 1) % NT)
+            assert p3 == ipos(NT, (i + 3) % NT)
+            seen.extend((c, c + 1))
+    assert sorted(seen) == list(range(N))

@@ -327,6 +327,17 @@ def test_tiled_sources_are_bitwise(monkeypatch, dims):
     _bitwise_under(monkeypatch, "NIM_TILE", dims)
 
 
+@pytest.mark.parametrize("dims", [(128, 64, 4), (20, 6, 5), (36, 6, 12)])
+def test_paired_sources_are_bitwise(monkeypatch, dims):
+    """The FP64 source kernels with two cells of a half-row per thread
+    (NIM_PAIR, the default where n_theta % 4 == 0) compute each cell as the
+    per-cell kernels do, in the FP64 loop and in the mixed loop's FP64
+    residual; on radial slabs at 128 x 64 x 4 (1,024 cells per slab and plane, two
+    512-cell blocks), in storage order with a partial last block at
+    20 x 6 x 5 and at the reference's 36 x 6 x 12."""
+    _bitwise_under(monkeypatch, "NIM_PAIR", dims)
+
+
 @pytest.mark.gpu
 def test_non_temporal_sweep_loads_are_bitwise(monkeypatch):
     """The sweeps with non-temporal loads of what a pass reads once

@@ -26,15 +26,14 @@ def test_library_exports_all_declared_symbols():
     assert sorted(nim_amd.exported_symbols()) == names
 
 
-def test_counts_and_face_map_match_reference_numbering():
-    geo = R.geometry()
-    nt, nr, nz = R.REF_DIMS
-    nmesh, nface, nbc = nim_amd.counts(nt, nr, nz)
-    assert (nmesh, nface, nbc) == (int(geo["dims"][3]), int(geo["dims"][4]), int(geo["dims"][5]))
+def _check_face_map(nt, nr, nz, meshes_faces):
+    """nim_face_map is a permutation, and the device slot of every
+    cell-local face holds the reference face meshes_faces (1-based) names."""
+    nmesh, nface, _ = nim_amd.counts(nt, nr, nz)
     fm = nim_amd.face_map(nt, nr, nz)
     assert sorted(fm.tolist()) == list(range(nface))
     # device slot of each cell-local face must be the reference's meshes_faces
-    mf = geo["meshes_faces"].reshape(nmesh, 6) - 1
+    mf = np.asarray(meshes_faces).reshape(nmesh, 6) - 1
     oT = nt * (nr + 1) * nz
     oZ = oT + nmesh
 
@@ -47,6 +46,24 @@ def test_counts_and_face_map_match_reference_numbering():
                  oT + P((i + 1) % nt) + nt * (j + nr * k), oZ + P(i) + nt * (j + nr * k),
                  oZ + P(i) + nt * (j + nr * (k + 1))]
         assert list(fm[slots]) == list(mf[c]), c
+
+
+def test_counts_and_face_map_match_reference_numbering():
+    geo = R.geometry()
+    nt, nr, nz = R.REF_DIMS
+    nmesh, nface, nbc = nim_amd.counts(nt, nr, nz)
+    assert (nmesh, nface, nbc) == (int(geo["dims"][3]), int(geo["dims"][4]), int(geo["dims"][5]))
+    _check_face_map(nt, nr, nz, geo["meshes_faces"])
+
+
+def test_face_map_at_the_48_x_24_x_16_case():
+    """The same at the 48 x 24 x 16 case, against the host geometry's
+    meshes_faces, which tests/golden/ref_c1.json pins bit for bit to the
+    reference built at that size (test_oracle_reference_c1.py)."""
+    from nim_amd import host
+    case = host.baseline_case("C1")
+    geo = host.build_geometry(case)
+    _check_face_map(*case.dims, geo["meshes_faces"])
 
 
 def test_unsupported_dimensions_rejected():

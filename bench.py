@@ -498,7 +498,7 @@ def main():
     ap.add_argument("--solver", choices=("accel", "plain"), default="accel",
                     help="the timed loops: Anderson-accelerated (default) or plain passes as the reference")
     ap.add_argument("--accel-depth", type=int, default=16, help="history of the FP64 loop's acceleration")
-    ap.add_argument("--accel-period", type=int, default=8, help="FP64 passes per acceleration step")
+    ap.add_argument("--accel-period", type=int, default=6, help="FP64 passes per acceleration step")
     ap.add_argument("--mixed-depth", type=int, default=10, help="history of the mixed loop's acceleration")
     ap.add_argument("--mixed-period", type=int, default=1, help="mixed cycles per acceleration step")
     ap.add_argument("--accel-basis", choices=("fp32", "fp64"), default="fp32",

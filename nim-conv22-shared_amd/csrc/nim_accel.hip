@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(BSX) k_acc_update(Dev d, double *x, VT *cprev,
 
 // ---- host side --------------------------------------------------------------
 struct nim_accel_ws {
-    int depth = 16, period = 8, vbytes = 4;  // asked for (nim_accel_options)
+    int depth = 16, period = 6, vbytes = 4;  // asked for (nim_accel_options)
     int am = 0, avb = 0;                     // allocated for
     long nl = 0;
     double *x = nullptr, *part = nullptr, *out = nullptr;

@@ -37,7 +37,7 @@ module nim_nse
     ! (mixed) every accel_period of them over the last `accel` steps (1..16);
     ! the same stopping test and fixed point (nim_inner_solve_accel)
     integer :: accel = 0
-    integer :: accel_period = 8
+    integer :: accel_period = 6
     character(len=256) :: plots_dir = 'plots'
   end type
 

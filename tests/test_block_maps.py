@@ -220,7 +220,8 @@ def cell_pair(NT, NR, c):
     return iof(NT, q), (c // NT) % NR, c // (NT * NR), q, lo, hi, odd
 
 
-@pytest.mark.parametrize("dims", [(256, 128, 2), (48, 24, 3), (36, 6, 4), (12, 4, 4), (128, 64, 3), (20, 5, 3)])
+@pytest.mark.parametrize("dims", [(256, 128, 2), (48, 24, 3), (36, 6, 4), (12, 4, 4), (128, 64, 3), (20, 5, 3), (4, 3, 4),
+                                  (8, 8, 4), (512, 64, 1)])
 @pytest.mark.parametrize("krev", [False, True])
 def test_pair_map_and_neighbours(dims, krev):
     """pair_of_block under the round-robin dispatch, with radial slabs where

@@ -10,9 +10,10 @@ undetermined constant level). A step is one time step of solve_nse
 the end-of-step reductions, the time shift and the Nusselt numbers. Inputs
 are resident on the device before the timed region.
 
-The inner loop is the reference's pass, with Anderson acceleration every 8
-passes (nim_inner_solve_accel): the same fixed point and stopping test, in
-1/50 of the passes at this tolerance (--solver plain times the plain passes).
+The inner loop is the reference's pass, with Anderson acceleration of the
+pressure faces every 2 passes (nim_inner_solve_accel, nim_accel_scope): the
+same fixed point and stopping test, in 1/100 of the passes at this tolerance
+(--solver plain times the plain passes).
 
 Prints one JSON line. value = cell-updates/s of the FP64 path = cells x
 inner passes / wall time of the timed steps. The `mixed` object runs the
@@ -131,6 +132,9 @@ def cpu_baseline(case, geo, st, iters, colored_iters):
     o.close()
     return (case.ncells * iters / dt, dt), (case.ncells * colored_iters / dtc, dtc, nthreads)
 
+
+# --accel-scope names -> nim_accel_scope values
+ACCEL_SCOPES = {"pressure": 1, "iterate": 0}
 
 # kernel symbol -> kernel class of nim_kernel_times
 KERNEL_CLASS = {"k_mom_src": "velocity_sources", "k_mom_sweep": "velocity_sweep", "k_ppe_src": "pressure_sources",
@@ -375,11 +379,12 @@ def adjoint_bench(args):
     ctx.synchronize()
     t0 = time.perf_counter()
     accel = args.solver == "accel"
-    inner = (args.inner or (16 if accel else 96)) if args.mixed_forward else 0
+    inner = (args.inner or (4 if accel else 96)) if args.mixed_forward else 0
     if accel:
         ctx.accel_options(depth=args.mixed_depth if inner else args.accel_depth,
                           period=args.mixed_period if inner else args.accel_period,
                           basis_bytes=4 if args.accel_basis == "fp32" else 8)
+        ctx.accel_scope(ACCEL_SCOPES[args.accel_scope])
     J, grad, info = nim_amd.nusselt_gradient(ctx, case, nsteps, args.adj_slots, args.adj_tol, tol=tol, average=True,
                                              solver=args.adj_solver, mixed_inner=inner, restart=args.adj_restart,
                                              accel=accel,
@@ -497,8 +502,10 @@ def main():
                          "default halfrange with --solver accel, maxabs otherwise")
     ap.add_argument("--solver", choices=("accel", "plain"), default="accel",
                     help="the timed loops: Anderson-accelerated (default) or plain passes as the reference")
-    ap.add_argument("--accel-depth", type=int, default=16, help="history of the FP64 loop's acceleration")
-    ap.add_argument("--accel-period", type=int, default=6, help="FP64 passes per acceleration step")
+    ap.add_argument("--accel-depth", type=int, default=12, help="history of the FP64 loop's acceleration")
+    ap.add_argument("--accel-period", type=int, default=2, help="FP64 passes per acceleration step")
+    ap.add_argument("--accel-scope", choices=tuple(ACCEL_SCOPES), default="pressure",
+                    help="what the acceleration extrapolates: the pressure faces or every unknown of the iterate")
     ap.add_argument("--mixed-depth", type=int, default=10, help="history of the mixed loop's acceleration")
     ap.add_argument("--mixed-period", type=int, default=1, help="mixed cycles per acceleration step")
     ap.add_argument("--accel-basis", choices=("fp32", "fp64"), default="fp32",
@@ -517,7 +524,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--inner", type=int, default=None,
-                    help="FP32 passes per FP64 residual of the mixed loop (default 16 accelerated, 96 plain; 8 for "
+                    help="FP32 passes per FP64 residual of the mixed loop (default 4 accelerated, 96 plain; 8 for "
                          "the ensemble)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the reversed-pass timing")
@@ -574,7 +581,7 @@ def main():
         return ensemble_bench(args)
     if args.adjoint_case:
         return adjoint_bench(args)
-    args.inner = args.inner or (16 if accel else 96)
+    args.inner = args.inner or (4 if accel else 96)
 
     case = host.baseline_case(args.case)
     t_setup = time.perf_counter()
@@ -615,6 +622,7 @@ def main():
                 ctx.accel_options(depth=args.mixed_depth, period=args.mixed_period, basis_bytes=vbytes)
             else:
                 ctx.accel_options(depth=args.accel_depth, period=args.accel_period, basis_bytes=vbytes)
+            ctx.accel_scope(ACCEL_SCOPES[args.accel_scope])
         passes, wall, t_res, res = 0, 0., 0., None
         for s in range(warmup + steps):
             timed, first, last = s >= warmup, s == 0, s == warmup + steps - 1
@@ -723,7 +731,7 @@ def main():
         }
         if accel:
             mixed["acceleration"] = {"depth": args.mixed_depth, "cycles_per_step": args.mixed_period,
-                                     "history": args.accel_basis}
+                                     "history": args.accel_basis, "scope": args.accel_scope}
         if plain:
             mixed["speedup_vs_plain_fp64_reference_tolerances"] = (args.steps / wall_m) / plain["steps_per_s"]
 
@@ -748,8 +756,9 @@ def main():
                                                           else "half the range of dp",
                                                           ", Anderson-accelerated passes" if accel else ""),
                    "cells": N, "precision": "fp64", "sweep_order": "4-colour",
-                   "solver": ("Anderson acceleration every %d passes over the last %d steps, %s history"
-                              % (args.accel_period, args.accel_depth, args.accel_basis)) if accel else "plain passes"},
+                   "solver": ("Anderson acceleration of the %s every %d passes over the last %d steps, %s history"
+                              % ("pressure faces" if args.accel_scope == "pressure" else "whole iterate",
+                                 args.accel_period, args.accel_depth, args.accel_basis)) if accel else "plain passes"},
         "steps_per_s": args.steps / wall,
         "inner_iterations": total_iters,
         "inner_iterations_per_step": total_iters / args.steps,

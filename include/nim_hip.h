@@ -198,16 +198,25 @@ int32_t nim_inner_solve_mixed(nim_ctx *ctx, const double tol[5], int32_t max_cyc
  * removes the slow pressure mode that holds the plain loop: at the 1M-cell
  * case p to 1e-8 takes 1e5 passes or more without it. The workspace, made
  * on the first call and kept with the context, is one FP64 vector and
- * 2 + 2 depth history vectors of 4 N + 5 NF values. Returns NIM_E_MAXITER
+ * 2 + 2 depth history vectors of NF values (of 4 N + 5 NF values with
+ * NIM_ACCEL_ITERATE, nim_accel_scope). Returns NIM_E_MAXITER
  * after max_iter passes (cycles). */
 int32_t nim_inner_solve_accel(nim_ctx *ctx, const double tol[5], int32_t max_iter, int32_t min_iter,
                               int32_t inner, int32_t *iters, double err[5]);
 
-/* Options of nim_inner_solve_accel: depth in [1, 16] (default 16), period
- * in [1, 256] passes or cycles per extrapolation (default 6), and the bytes
+/* Options of nim_inner_solve_accel: depth in [1, 16] (default 12), period
+ * in [1, 256] passes or cycles per extrapolation (default 2), and the bytes
  * of each stored history value, 4 (default: FP32 history, half the traffic;
  * the iterate and the extrapolation's arithmetic stay FP64) or 8. */
 int32_t nim_accel_options(nim_ctx *ctx, int32_t depth, int32_t period, int32_t basis_bytes);
+
+/* The part of the iterate nim_inner_solve_accel extrapolates; no reference
+ * counterpart. NIM_ACCEL_PRESSURE (default): the pressure faces only, the
+ * unknowns whose slow mode holds the plain loop, NF values; the other
+ * unknowns take the plain passes' values. NIM_ACCEL_ITERATE: every cell
+ * value and face, 4 N + 5 NF values. The workspace is sized for the scope. */
+enum { NIM_ACCEL_ITERATE = 0, NIM_ACCEL_PRESSURE = 1 };
+int32_t nim_accel_scope(nim_ctx *ctx, int32_t scope);
 
 /* FP64 residual measure of the current state, max |w r / a| per field
  * (u, v, w, p, T), as nim_inner_solve_mixed tests it; no reference

@@ -65,7 +65,14 @@ struct Coefs {
     double g[MAXW];
 };
 
-__device__ __forceinline__ long it_len(const Dev &d) { return 4L * d.N + 5L * d.NF; }
+// The extrapolated part of the iterate: values [off, off + n) of
+// [sc[0, 4N) | fc[0, 5NF)], the pressure faces among them at [p0, p1)
+// relative to off. off is 0 (the whole iterate) or lies in fc at an even
+// face index (the pressure faces, off = 4N + 3NF, NF even), so that groups
+// of four values starting at multiples of 4 stay in one array and aligned.
+struct Span {
+    long off, n, p0, p1;
+};
 
 // address of element s of the iterate [sc[0, 4N) | fc[0, 5NF)]; for s a
 // multiple of 4 the four values s .. s + 3 lie in the same array (4N is a
@@ -131,15 +138,16 @@ __device__ __forceinline__ void block_sum(double acc, double *out)
     }
 }
 
-// x = the current iterate; one thread per group of four values
-__global__ void __launch_bounds__(BSX) k_acc_save(Dev d, double *x)
+// x = the extrapolated part of the current iterate; one thread per group of
+// four values
+__global__ void __launch_bounds__(BSX) k_acc_save(Dev d, Span sp, double *x)
 {
-    const long n = it_len(d), ng = (n + 3) >> 2, stride = (long)gridDim.x * BSX;
+    const long n = sp.n, ng = (n + 3) >> 2, stride = (long)gridDim.x * BSX;
     for (long gi = (long)blockIdx.x * BSX + threadIdx.x; gi < ng; gi += stride) {
         const long s = gi << 2;
         const int k = (int)(n - s < 4 ? n - s : 4);
         double v[4];
-        ld4(it_ptr(d, s), v, k);
+        ld4(it_ptr(d, sp.off + s), v, k);
         st4(x + s, v, k);
     }
 }
@@ -168,10 +176,10 @@ enum { ROW_A = 0, ROW_B = MAXW, ROW_SDF = 2 * MAXW, ROW_SF = 2 * MAXW + 1, NROW 
 //    fprev = f.
 // One thread per group of four values (vector loads of every array).
 template <typename VT>
-__global__ void __launch_bounds__(BSX) k_acc_diff(Dev d, double *x, VT *fprev, VT *cprev, VT *dF, VT *dG, long ld,
-                                                  int cur, int cnt, double *part)
+__global__ void __launch_bounds__(BSX) k_acc_diff(Dev d, Span sp, double *x, VT *fprev, VT *cprev, VT *dF, VT *dG,
+                                                  long ld, int cur, int cnt, double *part)
 {
-    const long n = it_len(d), ng = (n + 3) >> 2, p0 = 4L * d.N + 3L * d.NF, p1 = p0 + d.NF;
+    const long n = sp.n, ng = (n + 3) >> 2, p0 = sp.p0, p1 = sp.p1;
     const long stride = (long)gridDim.x * BSX;
     double a[MAXW], b[MAXW], sdf = 0., sf = 0.;
 #pragma unroll
@@ -180,7 +188,7 @@ __global__ void __launch_bounds__(BSX) k_acc_diff(Dev d, double *x, VT *fprev, V
         const long s = gi << 2;
         const int k = (int)(n - s < 4 ? n - s : 4);
         double g[4], xv[4], f[4];
-        ld4(it_ptr(d, s), g, k);
+        ld4(it_ptr(d, sp.off + s), g, k);
         ld4(x + s, xv, k);
         for (int e = 0; e < 4; e++) f[e] = g[e] - xv[e];
         VT fv[4];
@@ -238,14 +246,14 @@ __global__ void __launch_bounds__(BSX) k_acc_diff(Dev d, double *x, VT *fprev, V
 // corr = dG gamma over the cnt columns in use; x = g - corr, written to x
 // and the state; cprev = corr (= g - x, for the next step's dG)
 template <typename VT>
-__global__ void __launch_bounds__(BSX) k_acc_update(Dev d, double *x, VT *cprev, const VT *dG, long ld, int cnt,
-                                                    Coefs c)
+__global__ void __launch_bounds__(BSX) k_acc_update(Dev d, Span sp, double *x, VT *cprev, const VT *dG, long ld,
+                                                    int cnt, Coefs c)
 {
-    const long n = it_len(d), ng = (n + 3) >> 2, stride = (long)gridDim.x * BSX;
+    const long n = sp.n, ng = (n + 3) >> 2, stride = (long)gridDim.x * BSX;
     for (long gi = (long)blockIdx.x * BSX + threadIdx.x; gi < ng; gi += stride) {
         const long s = gi << 2;
         const int k = (int)(n - s < 4 ? n - s : 4);
-        double *st = it_ptr(d, s);
+        double *st = it_ptr(d, sp.off + s);
         double g[4], corr[4] = {0., 0., 0., 0.};
         ld4(st, g, k);
 #pragma unroll
@@ -271,7 +279,8 @@ __global__ void __launch_bounds__(BSX) k_acc_update(Dev d, double *x, VT *cprev,
 
 // ---- host side --------------------------------------------------------------
 struct nim_accel_ws {
-    int depth = 16, period = 6, vbytes = 4;  // asked for (nim_accel_options)
+    int depth = 12, period = 2, vbytes = 4;  // asked for (nim_accel_options)
+    int scope = NIM_ACCEL_PRESSURE;          // extrapolated part (nim_accel_scope)
     int am = 0, avb = 0;                     // allocated for
     long nl = 0;
     double *x = nullptr, *part = nullptr, *out = nullptr;
@@ -298,13 +307,21 @@ static nim_accel_ws *accel_ws(nim_ctx *c)
     return (nim_accel_ws *)*slot;
 }
 
+// the extrapolated part of the iterate for a scope
+static Span accel_span(const Dev &d, int scope)
+{
+    const long pf = 4L * d.N + 3L * d.NF;
+    if (scope == NIM_ACCEL_PRESSURE) return Span{pf, (long)d.NF, 0, (long)d.NF};
+    return Span{0, 4L * d.N + 5L * d.NF, pf, pf + d.NF};
+}
+
 // device workspace for the current options: x (FP64), fprev, cprev and
-// the 2 m difference columns (vbytes-byte values), the partial sums and the
-// results
+// the 2 m difference columns (vbytes-byte values) over the extrapolated
+// part, the partial sums and the results
 static int32_t accel_alloc(nim_ctx *c, nim_accel_ws *w)
 {
     const Dev *d = nim_internal_dev(c);
-    const long nl = 4L * d->N + 5L * d->NF;
+    const long nl = accel_span(*d, w->scope).n;
     if (w->block && w->am == w->depth && w->avb == w->vbytes && w->nl == nl) return NIM_OK;
     if (w->block) {
         (void)hipStreamSynchronize((hipStream_t)nim_internal_stream(c));
@@ -420,11 +437,12 @@ static int32_t accel_solve(nim_ctx *c, nim_accel_ws *w, const double tol[5], int
     hipStream_t s = (hipStream_t)nim_internal_stream(c);
     const long ld = (w->nl + 63) / 64 * 64;
     const double inv_nf = 1. / d.NF;
+    const Span sp = accel_span(d, w->scope);
     VT *dF = (VT *)w->dF, *dG = (VT *)w->dG, *fprev = (VT *)w->fprev, *cprev = (VT *)w->cprev;
     const int m = w->depth;
     int32_t rc = nim_internal_reset(c, tol, max_iter, min_iter, inner > 0);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_acc_save, dim3(NBX), dim3(BSX), 0, s, d, w->x);
+    hipLaunchKernelGGL(k_acc_save, dim3(NBX), dim3(BSX), 0, s, d, sp, w->x);
     int cnt = 0, cur = -1, steps = 0;
     bool first = true;
     const Ctl *ctl = nullptr;
@@ -435,7 +453,7 @@ static int32_t accel_solve(nim_ctx *c, nim_accel_ws *w, const double tol[5], int
         // least-squares problem, read back with the control block (the state
         // is not touched, so a converged pass or cycle stays as it is)
         const int ncur = first ? -1 : (cur + 1) % m, ncnt = first ? 0 : std::min(cnt + 1, m);
-        hipLaunchKernelGGL(k_acc_diff<VT>, dim3(NBX), dim3(BSX), 0, s, d, w->x, fprev, cprev, dF, dG, ld, ncur,
+        hipLaunchKernelGGL(k_acc_diff<VT>, dim3(NBX), dim3(BSX), 0, s, d, sp, w->x, fprev, cprev, dF, dG, ld, ncur,
                            ncnt, w->part);
         if (ncnt) {
             hipLaunchKernelGGL(k_acc_final, dim3(NROW), dim3(BSX), 0, s, w->part, NBX, w->out);
@@ -465,7 +483,7 @@ static int32_t accel_solve(nim_ctx *c, nim_accel_ws *w, const double tol[5], int
         ls_solve(w->gram, rhs, cnt, gam);
         Coefs cf{};
         for (int q = 0; q < cnt; q++) cf.g[q] = gam[q];
-        hipLaunchKernelGGL(k_acc_update<VT>, dim3(NBX), dim3(BSX), 0, s, d, w->x, cprev, dG, ld, cnt, cf);
+        hipLaunchKernelGGL(k_acc_update<VT>, dim3(NBX), dim3(BSX), 0, s, d, sp, w->x, cprev, dG, ld, cnt, cf);
         if (hipGetLastError() != hipSuccess) return nim_internal_fail(NIM_E_HIP, "acceleration launch failed");
         if (verbose() && ++steps % 50 == 0)
             fprintf(stderr, "accel: %d %s, errors %.2e %.2e %.2e %.2e %.2e\n", ctl->iter, inner ? "cycles" : "passes",
@@ -489,6 +507,15 @@ extern "C" int32_t nim_accel_options(nim_ctx *c, int32_t depth, int32_t period, 
     w->depth = depth;
     w->period = period;
     w->vbytes = basis_bytes;
+    return NIM_OK;
+}
+
+extern "C" int32_t nim_accel_scope(nim_ctx *c, int32_t scope)
+{
+    if (!c) return nim_internal_fail(NIM_E_ARG, "null context");
+    if (scope != NIM_ACCEL_ITERATE && scope != NIM_ACCEL_PRESSURE)
+        return nim_internal_fail(NIM_E_ARG, "acceleration scope: NIM_ACCEL_ITERATE or NIM_ACCEL_PRESSURE");
+    accel_ws(c)->scope = scope;
     return NIM_OK;
 }
 

@@ -17,7 +17,8 @@
 ! that many FP32 correction passes per residual). accel=m > 0 extrapolates
 ! the inner loop's passes (or mixed cycles) every accel_period of them over
 ! the last m steps (Anderson acceleration; the same stopping test and fixed
-! point, far fewer passes for p at tight tolerances). device_metrics=.false.
+! point, far fewer passes for p at tight tolerances) over the pressure faces
+! (accel_scope=1) or the whole iterate (0). device_metrics=.false.
 ! computes the per-cell metrics on the host instead of the device (the same
 ! bits either way).
 program nim3d_hip
@@ -27,7 +28,7 @@ program nim3d_hip
   implicit none
   type(nim_run_config) :: cfg
   integer :: n_theta = 36, n_r = 6, n_z = 12, nsteps = 3, restart_every = 0, done, u, ios, mixed_inner = 0
-  integer :: step_out = 0, p_measure = 0, accel = 0, accel_period = 6
+  integer :: step_out = 0, p_measure = 0, accel = 0, accel_period = 2, accel_scope = 1
   real(c_double) :: R_i = 1.0d0*(3.14159265359d0/3.0d0), R_o = 2.0d0, height = 2.0d0
   real(c_double) :: blend_in = 1.25d0, blend_out = 1.75d0, grading = 1.5d0, hex_order = -1.0d0
   real(c_double) :: Ra = 1.0d5, Pr = 0.7d0, DT = 0.01d0, noise = 1.0d-3, tol(5) = (/1d-6, 1d-6, 1d-6, 5d-3, 1d-6/)
@@ -39,7 +40,7 @@ program nim3d_hip
   namelist /case/ n_theta, n_r, n_z, R_i, R_o, height, blend_in, blend_out, grading, hex_order, Ra, Pr, DT, &
                   nsteps, tol, noise, seed, T_hot, T_cold, restart_every, restart_prefix, restart_in, &
                   check_steady, final_out, mixed_inner, device_metrics, step_out, plots_dir, &
-                  p_measure, accel, accel_period
+                  p_measure, accel, accel_period, accel_scope
 
   if (command_argument_count() >= 1) call get_command_argument(1, nml)
   open(newunit=u, file=trim(nml), status='old', iostat=ios)
@@ -53,7 +54,7 @@ program nim3d_hip
   cfg%T_hot = T_hot; cfg%T_cold = T_cold; cfg%check_steady = check_steady; cfg%mixed_inner = mixed_inner
   cfg%device_metrics = device_metrics
   cfg%step_out = step_out; cfg%plots_dir = plots_dir; cfg%p_measure = p_measure
-  cfg%accel = accel; cfg%accel_period = accel_period
+  cfg%accel = accel; cfg%accel_period = accel_period; cfg%accel_scope = accel_scope
   call nim_setup(cfg)
   if (len_trim(restart_in) > 0) call nim_read_restart(trim(restart_in))
   done = nim_solve(cfg, nsteps, restart_every, trim(restart_prefix))

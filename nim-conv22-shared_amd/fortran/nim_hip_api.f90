@@ -122,6 +122,12 @@ module nim_hip_api
       type(c_ptr), value :: ctx
       integer(c_int32_t), value :: depth, period, basis_bytes
     end function
+    ! extrapolated part: 0 the whole iterate, 1 the pressure faces (default)
+    integer(c_int32_t) function nim_accel_scope(ctx, scope) bind(C, name="nim_accel_scope")
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: scope
+    end function
     ! what tol(4) bounds: 0 max |dp| (find_error), 1 half the range of dp
     integer(c_int32_t) function nim_set_p_measure(ctx, mode) bind(C, name="nim_set_p_measure")
       import :: c_int32_t, c_ptr

@@ -35,9 +35,11 @@ module nim_nse
     integer :: p_measure = 0
     ! > 0: Anderson acceleration of the inner loop's passes (FP64) or cycles
     ! (mixed) every accel_period of them over the last `accel` steps (1..16);
-    ! the same stopping test and fixed point (nim_inner_solve_accel)
+    ! the same stopping test and fixed point (nim_inner_solve_accel); of the
+    ! pressure faces (accel_scope 1) or of the whole iterate (0)
     integer :: accel = 0
-    integer :: accel_period = 6
+    integer :: accel_period = 2
+    integer :: accel_scope = 1
     character(len=256) :: plots_dir = 'plots'
   end type
 
@@ -134,8 +136,11 @@ contains
     node_xyz = nodes
     call check(nim_build_pressure_coeffs(ctx), 'nim_build_pressure_coeffs')
     call check(nim_set_p_measure(ctx, int(cfg%p_measure, c_int32_t)), 'nim_set_p_measure')
-    if (cfg%accel > 0) call check(nim_accel_options(ctx, int(cfg%accel, c_int32_t), &
-                                  int(cfg%accel_period, c_int32_t), 4_c_int32_t), 'nim_accel_options')
+    if (cfg%accel > 0) then
+      call check(nim_accel_options(ctx, int(cfg%accel, c_int32_t), int(cfg%accel_period, c_int32_t), &
+                                   4_c_int32_t), 'nim_accel_options')
+      call check(nim_accel_scope(ctx, int(cfg%accel_scope, c_int32_t)), 'nim_accel_scope')
+    endif
     allocate(faces_u(NF), faces_v(NF), faces_w(NF), faces_p(NF), faces_T(NF))
     allocate(uxyz(N), vxyz(N), wxyz(N), Txyz(N), uxyz_old(N), vxyz_old(N), wxyz_old(N), Txyz_old(N))
     allocate(uxyzt(N), vxyzt(N), wxyzt(N), Txyzt(N), meshes_p(N), dila(N))

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(PKG, "libnim_hip.so")
 
 NIM_OK, NIM_E_ARG, NIM_E_HIP, NIM_E_MAXITER, NIM_E_NONFINITE, NIM_E_STATE = range(6)
 P_MAXABS, P_HALFRANGE = 0, 1  # pressure convergence measures (nim_set_p_measure)
+ACCEL_ITERATE, ACCEL_PRESSURE = 0, 1  # extrapolated part of the iterate (nim_accel_scope)
 
 STAGES = {
     "velocity_sources": 0, "velocity_sweep": 1, "pressure_sources": 2, "pressure_stage": 3,
@@ -116,6 +117,8 @@ def lib():
         L.nim_inner_solve_accel.argtypes = [vp, _P, i32, i32, i32, ctypes.POINTER(i32), _P]
         L.nim_accel_options.restype = i32
         L.nim_accel_options.argtypes = [vp, i32, i32, i32]
+        L.nim_accel_scope.restype = i32
+        L.nim_accel_scope.argtypes = [vp, i32]
         L.nim_kernel_times_mixed.restype = i32
         L.nim_kernel_times_mixed.argtypes = [vp, i32, i32, _P, _PI]
         L.nim_revolve_plan.restype = i32
@@ -174,7 +177,7 @@ def exported_symbols():
             "nim_build_pressure_coeffs", "nim_upload_state", "nim_download_state", "nim_build_momentum_coeffs",
             "nim_inner_solve", "nim_set_p_measure", "nim_end_step", "nim_end_step_noalloc", "nim_advance_time", "nim_nusselt", "nim_run_stage",
             "nim_get_array", "nim_set_array", "nim_stream", "nim_synchronize", "nim_kernel_times",
-            "nim_inner_solve_mixed", "nim_inner_solve_accel", "nim_accel_options", "nim_residual", "nim_kernel_times_mixed", "nim_ens_create", "nim_ens_destroy",
+            "nim_inner_solve_mixed", "nim_inner_solve_accel", "nim_accel_options", "nim_accel_scope", "nim_residual", "nim_kernel_times_mixed", "nim_ens_create", "nim_ens_destroy",
             "nim_ens_inner_solve", "nim_ens_time_step", "nim_revolve_plan", "nim_ckpt_create", "nim_ckpt_destroy", "nim_ckpt_store",
             "nim_ckpt_restore", "nim_adj_create", "nim_adj_destroy", "nim_adj_seed_nusselt", "nim_adj_reverse_step", "nim_adj_reverse_step_gmres", "nim_adj_gmres_options", "nim_adj_gradient_faces",
             "nim_adj_gradient", "nim_adj_reverse_pass", "nim_adj_reverse_stage", "nim_adj_clear", "nim_adj_get",
@@ -360,10 +363,15 @@ class Context:
             _check(rc)
         return rc, cy.value, err
 
-    def accel_options(self, depth=16, period=6, basis_bytes=4):
+    def accel_options(self, depth=12, period=2, basis_bytes=4):
         """Anderson acceleration of inner_solve_accel: history depth, passes
         (or cycles) per extrapolation, bytes per stored history value."""
         _check(self.L.nim_accel_options(self.h, int(depth), int(period), int(basis_bytes)))
+
+    def accel_scope(self, scope):
+        """What inner_solve_accel extrapolates: ACCEL_PRESSURE (the pressure
+        faces, the default) or ACCEL_ITERATE (every cell value and face)."""
+        _check(self.L.nim_accel_scope(self.h, int(scope)))
 
     def inner_solve_accel(self, tol, inner=0, max_iter=1000000, min_iter=0, raise_on_cap=True):
         """The inner loop with Anderson acceleration: FP64 passes (inner = 0)

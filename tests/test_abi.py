@@ -112,13 +112,15 @@ def test_argument_errors_before_any_device_use():
 
 
 def test_accelerated_solve_arguments_checked_before_device_use():
-    """nim_inner_solve_accel and nim_accel_options refuse a null context (and
-    the solve a null tolerance) with NIM_E_ARG, without a device."""
+    """nim_inner_solve_accel, nim_accel_options and nim_accel_scope refuse a
+    null context (and the solve a null tolerance) with NIM_E_ARG, without a
+    device."""
     L = nim_amd.lib()
     tol = np.full(5, 1e-8)
     err = np.zeros(5)
     it = ctypes.c_int32()
     P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     assert L.nim_accel_options(None, 16, 8, 4) == nim_amd.NIM_E_ARG
+    assert L.nim_accel_scope(None, nim_amd.ACCEL_PRESSURE) == nim_amd.NIM_E_ARG
     assert L.nim_inner_solve_accel(None, P(tol), 100, 0, 0, ctypes.byref(it), P(err)) == nim_amd.NIM_E_ARG
     assert b"null" in L.nim_last_error()

@@ -443,6 +443,12 @@ static int32_t accel_solve(nim_ctx *c, nim_accel_ws *w, const double tol[5], int
     int32_t rc = nim_internal_reset(c, tol, max_iter, min_iter, inner > 0);
     if (rc) return rc;
     hipLaunchKernelGGL(k_acc_save, dim3(NBX), dim3(BSX), 0, s, d, sp, w->x);
+    // blocks of the difference kernel: each thread takes about GPT groups of
+    // four values, so that the block reductions of its 2 m + 2 sums stay
+    // small beside the streaming (at most NBX blocks; at the 1M-cell case's
+    // pressure faces 389 blocks instead of 2048, 1.7% less time per pass)
+    constexpr long GPT = 8;
+    const int nbd = (int)std::min<long>(NBX, std::max<long>(64, ((sp.n + 3) / 4 + GPT * BSX - 1) / (GPT * BSX)));
     int cnt = 0, cur = -1, steps = 0;
     bool first = true;
     const Ctl *ctl = nullptr;
@@ -453,10 +459,10 @@ static int32_t accel_solve(nim_ctx *c, nim_accel_ws *w, const double tol[5], int
         // least-squares problem, read back with the control block (the state
         // is not touched, so a converged pass or cycle stays as it is)
         const int ncur = first ? -1 : (cur + 1) % m, ncnt = first ? 0 : std::min(cnt + 1, m);
-        hipLaunchKernelGGL(k_acc_diff<VT>, dim3(NBX), dim3(BSX), 0, s, d, sp, w->x, fprev, cprev, dF, dG, ld, ncur,
+        hipLaunchKernelGGL(k_acc_diff<VT>, dim3(nbd), dim3(BSX), 0, s, d, sp, w->x, fprev, cprev, dF, dG, ld, ncur,
                            ncnt, w->part);
         if (ncnt) {
-            hipLaunchKernelGGL(k_acc_final, dim3(NROW), dim3(BSX), 0, s, w->part, NBX, w->out);
+            hipLaunchKernelGGL(k_acc_final, dim3(NROW), dim3(BSX), 0, s, w->part, nbd, w->out);
             if (hipMemcpyAsync(w->h_out, w->out, NROW * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess)
                 return nim_internal_fail(NIM_E_HIP, "acceleration copy failed");
         }

@@ -211,7 +211,8 @@ def test_cap_and_converged_entry(rest):
 def test_c2_accelerated_step_both_loops():
     """At the benchmark's 1M-cell case, one step from the seeded state: the
     accelerated FP64 loop at 1e-11 over the pressure faces and over the whole
-    iterate reach the same fixed point; at 1e-8 on every field the FP64 and
+    iterate reach the same fixed point (to what their stopping test leaves:
+    2.8e-5 in p, 1.2e-6 in v measured); at 1e-8 on every field the FP64 and
     mixed loops of both scopes converge in a few thousand passes (the plain
     loop needs 1.4e5 for p) and end within the distance that the stopping
     test leaves from that point (p is determined to about 1e-8 over the slow
@@ -244,7 +245,9 @@ def test_c2_accelerated_step_both_loops():
     ref = out["ref"][0]
     g = gaps(out["ref_iterate"][0].__getitem__, ref.__getitem__)
     print("references (%d, %d passes) against each other:" % (out["ref"][2], out["ref_iterate"][2]), g)
-    assert g["faces_p"] < 1e-4 and max(g[f] for f in FIELDS if f != "faces_p") < 1e-6, g
+    # each stops within about 1e-3 of the 1e-8 slack below (p 1.5e-5, v 7e-7),
+    # possibly on opposite sides
+    assert g["faces_p"] < 1e-4 and max(g[f] for f in FIELDS if f != "faces_p") < 5e-6, g
     for form in ("fp64", "mixed", "fp64_iterate", "mixed_iterate"):
         g = gaps(out[form][0].__getitem__, ref.__getitem__)
         print(form, out[form][2], "passes, against the fixed point:", g)

@@ -223,22 +223,24 @@ def test_field_output_every_step_out(tmp_path):
 NML_ACCEL = """&case n_theta=36, n_r=6, n_z=12, R_o=2., height=2.,
       blend_in=1.25, blend_out=1.75, Ra=1e5, Pr=0.7, DT=0.01, nsteps=3,
       tol=1e-8,1e-8,1e-8,1e-8,1e-8, noise=1e-3, seed=20221, check_steady=.false.,
-      final_out='{out}', mixed_inner={inner}, p_measure=1, accel={depth}, accel_period={period} /
+      final_out='{out}', mixed_inner={inner}, p_measure=1, accel={depth}, accel_period={period},
+      accel_scope={scope} /
 """
 
 
-@pytest.mark.parametrize("inner", [0, 16])
-def test_driver_accelerated_steps_match_the_library(tmp_path, inner):
-    """The driver with accel > 0 (every field at 1e-8, p by its half range)
-    runs the same accelerated solves as the library called from Python on
-    the same mesh, metrics and seeded state: the same passes per step and the
-    same bits."""
+@pytest.mark.parametrize("inner,scope", [(0, 1), (16, 1), (0, 0)])
+def test_driver_accelerated_steps_match_the_library(tmp_path, inner, scope):
+    """The driver with accel > 0 (every field at 1e-8, p by its half range),
+    extrapolating the pressure faces (accel_scope=1) or the whole iterate
+    (0), runs the same accelerated solves as the library called from Python
+    on the same mesh, metrics and seeded state: the same passes per step and
+    the same bits."""
     import nim_amd
     from nim_amd import host
-    depth, period = (16, 8) if inner == 0 else (10, 1)
+    depth, period = ((12, 2) if scope else (16, 8)) if inner == 0 else (10, 1)
     out = tmp_path / "final"
     nml = tmp_path / "case.nml"
-    nml.write_text(NML_ACCEL.format(out=out, inner=inner, depth=depth, period=period))
+    nml.write_text(NML_ACCEL.format(out=out, inner=inner, depth=depth, period=period, scope=scope))
     p = subprocess.run([EXE, str(nml)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
     iters = [int(l.split("# iters.")[1].split("|")[0]) for l in p.stdout.splitlines() if l.startswith("step #")]
@@ -248,6 +250,7 @@ def test_driver_accelerated_steps_match_the_library(tmp_path, inner):
     ctx = G.gpu_for_case(case, geo, host.initial_state(case, amp=1e-3, seed=20221))
     ctx.set_p_measure(nim_amd.P_HALFRANGE)
     ctx.accel_options(depth=depth, period=period)
+    ctx.accel_scope(scope)
     tol = np.full(5, 1e-8)
     py_iters = []
     for l in range(3):

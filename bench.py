@@ -11,7 +11,7 @@ the end-of-step reductions, the time shift and the Nusselt numbers. Inputs
 are resident on the device before the timed region.
 
 The inner loop is the reference's pass, with Anderson acceleration of the
-pressure faces every 2 passes (nim_inner_solve_accel, nim_accel_scope): the
+pressure faces after every pass (nim_inner_solve_accel, nim_accel_scope): the
 same fixed point and stopping test, in 1/100 of the passes at this tolerance
 (--solver plain times the plain passes).
 
@@ -503,7 +503,7 @@ def main():
     ap.add_argument("--solver", choices=("accel", "plain"), default="accel",
                     help="the timed loops: Anderson-accelerated (default) or plain passes as the reference")
     ap.add_argument("--accel-depth", type=int, default=12, help="history of the FP64 loop's acceleration")
-    ap.add_argument("--accel-period", type=int, default=2, help="FP64 passes per acceleration step")
+    ap.add_argument("--accel-period", type=int, default=1, help="FP64 passes per acceleration step")
     ap.add_argument("--accel-scope", choices=tuple(ACCEL_SCOPES), default="pressure",
                     help="what the acceleration extrapolates: the pressure faces or every unknown of the iterate")
     ap.add_argument("--mixed-depth", type=int, default=10, help="history of the mixed loop's acceleration")

@@ -205,7 +205,7 @@ int32_t nim_inner_solve_accel(nim_ctx *ctx, const double tol[5], int32_t max_ite
                               int32_t inner, int32_t *iters, double err[5]);
 
 /* Options of nim_inner_solve_accel: depth in [1, 16] (default 12), period
- * in [1, 256] passes or cycles per extrapolation (default 2), and the bytes
+ * in [1, 256] passes or cycles per extrapolation (default 1), and the bytes
  * of each stored history value, 4 (default: FP32 history, half the traffic;
  * the iterate and the extrapolation's arithmetic stay FP64) or 8. */
 int32_t nim_accel_options(nim_ctx *ctx, int32_t depth, int32_t period, int32_t basis_bytes);

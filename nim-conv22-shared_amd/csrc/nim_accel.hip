@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(BSX) k_acc_update(Dev d, Span sp, double *x, V
 
 // ---- host side --------------------------------------------------------------
 struct nim_accel_ws {
-    int depth = 12, period = 2, vbytes = 4;  // asked for (nim_accel_options)
+    int depth = 12, period = 1, vbytes = 4;  // asked for (nim_accel_options)
     int scope = NIM_ACCEL_PRESSURE;          // extrapolated part (nim_accel_scope)
     int am = 0, avb = 0;                     // allocated for
     long nl = 0;

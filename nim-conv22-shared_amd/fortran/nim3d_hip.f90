@@ -28,7 +28,7 @@ program nim3d_hip
   implicit none
   type(nim_run_config) :: cfg
   integer :: n_theta = 36, n_r = 6, n_z = 12, nsteps = 3, restart_every = 0, done, u, ios, mixed_inner = 0
-  integer :: step_out = 0, p_measure = 0, accel = 0, accel_period = 2, accel_scope = 1
+  integer :: step_out = 0, p_measure = 0, accel = 0, accel_period = 1, accel_scope = 1
   real(c_double) :: R_i = 1.0d0*(3.14159265359d0/3.0d0), R_o = 2.0d0, height = 2.0d0
   real(c_double) :: blend_in = 1.25d0, blend_out = 1.75d0, grading = 1.5d0, hex_order = -1.0d0
   real(c_double) :: Ra = 1.0d5, Pr = 0.7d0, DT = 0.01d0, noise = 1.0d-3, tol(5) = (/1d-6, 1d-6, 1d-6, 5d-3, 1d-6/)

@@ -38,7 +38,7 @@ module nim_nse
     ! the same stopping test and fixed point (nim_inner_solve_accel); of the
     ! pressure faces (accel_scope 1) or of the whole iterate (0)
     integer :: accel = 0
-    integer :: accel_period = 2
+    integer :: accel_period = 1
     integer :: accel_scope = 1
     character(len=256) :: plots_dir = 'plots'
   end type

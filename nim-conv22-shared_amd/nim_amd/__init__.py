@@ -363,7 +363,7 @@ class Context:
             _check(rc)
         return rc, cy.value, err
 
-    def accel_options(self, depth=12, period=2, basis_bytes=4):
+    def accel_options(self, depth=12, period=1, basis_bytes=4):
         """Anderson acceleration of inner_solve_accel: history depth, passes
         (or cycles) per extrapolation, bytes per stored history value."""
         _check(self.L.nim_accel_options(self.h, int(depth), int(period), int(basis_bytes)))

@@ -121,7 +121,7 @@ def c1_steps():
             ctx.accel_scope(nim_amd.ACCEL_ITERATE)
             rc, n, err = ctx.inner_solve_accel(t, max_iter=200000, min_iter=2)
         else:
-            ctx.accel_options(depth=12, period=2, basis_bytes=4)
+            ctx.accel_options(depth=12, period=1, basis_bytes=4)
             rc, n, err = ctx.inner_solve_accel(t, max_iter=200000, min_iter=2)
         assert rc == 0 and (form == "mixed_budget" or np.all(err <= t)), (form, err)
         if form == "ref":
@@ -226,8 +226,8 @@ def test_c2_accelerated_step_both_loops():
     ctx.set_p_measure(nim_amd.P_HALFRANGE)
     P, I = nim_amd.ACCEL_PRESSURE, nim_amd.ACCEL_ITERATE
     out = {}
-    for form, tol, inner, depth, period, scope in (("ref", 1e-11, 0, 12, 2, P), ("ref_iterate", 1e-11, 0, 16, 6, I),
-                                                    ("fp64", 1e-8, 0, 12, 2, P), ("mixed", 1e-8, 4, 10, 1, P),
+    for form, tol, inner, depth, period, scope in (("ref", 1e-11, 0, 12, 1, P), ("ref_iterate", 1e-11, 0, 16, 6, I),
+                                                    ("fp64", 1e-8, 0, 12, 1, P), ("mixed", 1e-8, 4, 10, 1, P),
                                                     ("fp64_iterate", 1e-8, 0, 16, 6, I),
                                                     ("mixed_iterate", 1e-8, 16, 10, 1, I)):
         ctx.upload_state(st)

@@ -237,7 +237,7 @@ def test_driver_accelerated_steps_match_the_library(tmp_path, inner, scope):
     the same bits."""
     import nim_amd
     from nim_amd import host
-    depth, period = ((12, 2) if scope else (16, 8)) if inner == 0 else (10, 1)
+    depth, period = ((12, 1) if scope else (16, 8)) if inner == 0 else (10, 1)
     out = tmp_path / "final"
     nml = tmp_path / "case.nml"
     nml.write_text(NML_ACCEL.format(out=out, inner=inner, depth=depth, period=period, scope=scope))

@@ -26,6 +26,12 @@
 // problem therefore measures f's pressure faces with their mean removed, so
 // that it does not chase that drift.
 //
+// By default only the pressure faces are extrapolated (nim_accel_scope): the
+// slow mode is the pressure's, the other unknowns relax in a few passes once
+// p is right, and at a sixth of the iterate's values the extrapolation is
+// cheap enough to follow every pass. The other unknowns keep the values of
+// the passes.
+//
 // Device work per acceleration step, one grid-stride pass over the iterate
 // each: f and the new difference columns with all dot products of the
 // least-squares problem's new Gram row; the update x = g - dG gamma written
@@ -74,9 +80,8 @@ struct Span {
     long off, n, p0, p1;
 };
 
-// address of element s of the iterate [sc[0, 4N) | fc[0, 5NF)]; for s a
-// multiple of 4 the four values s .. s + 3 lie in the same array (4N is a
-// multiple of 4) and are aligned for vector access
+// address of element s of the iterate [sc[0, 4N) | fc[0, 5NF)] (a Span's
+// value s is element off + s)
 __device__ __forceinline__ double *it_ptr(const Dev &d, long s)
 {
     const long c4 = 4L * d.N;

@@ -237,15 +237,31 @@ __global__ void __launch_bounds__(BSX) k_acc_diff(Dev d, Span sp, double *x, VT 
             }
     }
     if (cnt == 0) return;
-    const int nb = gridDim.x;
+    // the block's 2 cnt + 2 sums through one LDS exchange: each wave's sums
+    // by shuffles, then thread r adds row r over the waves in wave order
+    // (block_sum's order, one barrier instead of two per row)
+    __shared__ double sh[BSX / 64][NROW];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double v;
 #pragma unroll
     for (int q = 0; q < MAXW; q++)
         if (q < cnt) {
-            block_sum(a[q], part + (size_t)(ROW_A + q) * nb + blockIdx.x);
-            block_sum(b[q], part + (size_t)(ROW_B + q) * nb + blockIdx.x);
+            v = wave_sum(a[q]);
+            if (lane == 0) sh[wv][ROW_A + q] = v;
+            v = wave_sum(b[q]);
+            if (lane == 0) sh[wv][ROW_B + q] = v;
         }
-    block_sum(sdf, part + (size_t)ROW_SDF * nb + blockIdx.x);
-    block_sum(sf, part + (size_t)ROW_SF * nb + blockIdx.x);
+    v = wave_sum(sdf);
+    if (lane == 0) sh[wv][ROW_SDF] = v;
+    v = wave_sum(sf);
+    if (lane == 0) sh[wv][ROW_SF] = v;
+    __syncthreads();
+    const int r = threadIdx.x;
+    if ((r < ROW_A + cnt) || (r >= ROW_B && r < ROW_B + cnt) || r == ROW_SDF || r == ROW_SF) {
+        double t = 0.;
+        for (int w = 0; w < BSX / 64; w++) t += sh[w][r];
+        part[(size_t)r * gridDim.x + blockIdx.x] = t;
+    }
 }
 
 // corr = dG gamma over the cnt columns in use; x = g - corr, written to x

@@ -379,58 +379,58 @@ static int32_t accel_alloc(nim_ctx *c, nim_accel_ws *w)
 }
 
 // Least squares min || f - dF gamma || from the Gram matrix G = dF^T dF and
-// r = dF^T f: G scaled to unit diagonal, eigen-decomposed by cyclic Jacobi
-// rotations, directions with eigenvalues below 1e-14 of the largest dropped
-// (the columns of a converging history become nearly dependent).
+// r = dF^T f: G scaled to unit diagonal, factored by Cholesky with diagonal
+// pivoting (the column with the largest remaining norm first); a column
+// whose part outside the span of those before it has a squared norm below
+// 1e-14 of its own is dropped (the columns of a converging history become
+// nearly dependent), and gamma solves the problem over the columns kept: a
+// least-squares minimiser. n^3 / 3 operations, about 1 us at n = 12, while
+// the device waits for it after every pass.
 static void ls_solve(const double G[MAXW][MAXW], const double *r, int n, double *gamma)
 {
-    double A[MAXW][MAXW], Q[MAXW][MAXW], D[MAXW], y[MAXW];
+    double A[MAXW][MAXW], D[MAXW], y[MAXW];
+    int perm[MAXW];
     for (int i = 0; i < n; i++) D[i] = G[i][i] > 0 ? 1. / std::sqrt(G[i][i]) : 0.;
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) {
-            A[i][j] = D[i] * G[i][j] * D[j];
-            Q[i][j] = i == j ? 1. : 0.;
-        }
-    for (int sweep = 0; sweep < 60; sweep++) {
-        double off = 0.;
-        for (int i = 0; i < n; i++)
-            for (int j = i + 1; j < n; j++) off += A[i][j] * A[i][j];
-        if (off < 1e-30) break;
-        for (int p = 0; p < n; p++)
-            for (int q = p + 1; q < n; q++) {
-                if (std::fabs(A[p][q]) < 1e-300) continue;
-                const double th = (A[q][q] - A[p][p]) / (2. * A[p][q]);
-                const double t = (th >= 0 ? 1. : -1.) / (std::fabs(th) + std::sqrt(th * th + 1.));
-                const double cs = 1. / std::sqrt(t * t + 1.), sn = t * cs;
-                for (int k = 0; k < n; k++) {
-                    const double akp = A[k][p], akq = A[k][q];
-                    A[k][p] = cs * akp - sn * akq;
-                    A[k][q] = sn * akp + cs * akq;
-                }
-                for (int k = 0; k < n; k++) {
-                    const double apk = A[p][k], aqk = A[q][k];
-                    A[p][k] = cs * apk - sn * aqk;
-                    A[q][k] = sn * apk + cs * aqk;
-                }
-                for (int k = 0; k < n; k++) {
-                    const double qkp = Q[k][p], qkq = Q[k][q];
-                    Q[k][p] = cs * qkp - sn * qkq;
-                    Q[k][q] = sn * qkp + cs * qkq;
-                }
-            }
+    for (int i = 0; i < n; i++) {
+        perm[i] = i;
+        for (int j = 0; j < n; j++) A[i][j] = D[i] * G[i][j] * D[j];
     }
-    double lmax = 0.;
-    for (int i = 0; i < n; i++) lmax = std::max(lmax, A[i][i]);
-    for (int i = 0; i < n; i++) y[i] = 0.;
+    // A = P L L^T P^T over the first `rank` pivots; L in the lower triangle
+    int rank = 0;
     for (int k = 0; k < n; k++) {
-        const double lk = A[k][k];
-        if (!(lk > 1e-14 * lmax)) continue;
-        double proj = 0.;
-        for (int i = 0; i < n; i++) proj += Q[i][k] * D[i] * r[i];
-        proj /= lk;
-        for (int i = 0; i < n; i++) y[i] += proj * Q[i][k];
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (A[i][i] > A[p][p]) p = i;
+        // columns of unit norm: the remaining diagonal is the squared norm
+        // of a column's part outside the span of the pivots before it
+        if (!(A[p][p] > 1e-14)) break;
+        if (p != k) {
+            std::swap(perm[k], perm[p]);
+            for (int j = 0; j < n; j++) std::swap(A[k][j], A[p][j]);
+            for (int i = 0; i < n; i++) std::swap(A[i][k], A[i][p]);
+        }
+        const double lkk = std::sqrt(A[k][k]);
+        A[k][k] = lkk;
+        for (int i = k + 1; i < n; i++) A[i][k] /= lkk;
+        for (int i = k + 1; i < n; i++)
+            for (int j = k + 1; j <= i; j++) A[i][j] -= A[i][k] * A[j][k];
+        for (int i = k + 1; i < n; i++)
+            for (int j = i + 1; j < n; j++) A[i][j] = A[j][i];
+        rank = k + 1;
     }
-    for (int i = 0; i < n; i++) gamma[i] = D[i] * y[i];
+    // L L^T z = (D r) permuted, over the kept pivots
+    for (int k = 0; k < rank; k++) {
+        double t = D[perm[k]] * r[perm[k]];
+        for (int j = 0; j < k; j++) t -= A[k][j] * y[j];
+        y[k] = t / A[k][k];
+    }
+    for (int k = rank - 1; k >= 0; k--) {
+        double t = y[k];
+        for (int j = k + 1; j < rank; j++) t -= A[j][k] * y[j];
+        y[k] = t / A[k][k];
+    }
+    for (int i = 0; i < n; i++) gamma[i] = 0.;
+    for (int k = 0; k < rank; k++) gamma[perm[k]] = D[perm[k]] * y[k];
 }
 
 // the least-squares solve alone, for host tests: G row-major n x n
